@@ -1,0 +1,232 @@
+/*
+ * fame6d.h - C ABI of the MI355X-native hot path of the null-space-free 6D
+ * Fourier projection eigensolver (scalar-permittivity mass operator, Method-C
+ * explicit reduced inverse, thick-restart Lanczos, Taylor reconstruction).
+ *
+ * Every entry point replaces one function of the reference MATLAB code; the
+ * reference file:line is cited above each declaration (paths relative to the
+ * reference repository root).
+ *
+ * Conventions
+ *   - Grid vectors are the column-major (2N)^6 arrays of the reference
+ *     (linear index k1 + 2N*k2 + ... , first index fastest), NF = (2N)^6.
+ *   - Complex vectors are interleaved (re, im): complex128 for FAME6D_FP64,
+ *     complex64 for FAME6D_FP32.  Multi-component vectors are stacked
+ *     component-first ([x1; x2; x3]); several right-hand sides are columns
+ *     with leading dimension equal to the row count.
+ *   - Vector arguments are DEVICE pointers unless the name ends in _host.
+ *   - Work is enqueued on the stream given at fame6d_mass_create; calls that
+ *     return host scalars synchronise that stream.  A handle is not re-entrant:
+ *     use one handle per stream.
+ *   - The caller owns input/output buffers; handles own their workspaces.
+ *   - Return value 0 = success, otherwise a FAME6D_E* code; the message of the
+ *     last failure on the calling thread is fame6d_last_error().
+ */
+#ifndef FAME6D_H
+#define FAME6D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; map the reference error identifiers. */
+#define FAME6D_OK 0
+#define FAME6D_ESIZE 1      /* rank1_exact_fft_scatter:BadSize (exp1/rank1_exact_fft_scatter.m:5-7) */
+#define FAME6D_ENONFINITE 2 /* NaN/Inf guards (exp1/make_explicit_krinv_compact.m:27,37,67) */
+#define FAME6D_ENOCONV 3    /* PCG failure / EXP1:NotConverged (make_explicit_krinv_compact.m:55-58, run_method_C_fig1bc.m:15-17) */
+#define FAME6D_EBUDGET 4    /* EXP1:IterationLimit / TimeLimit (exp1/make_run_budget.m:28-46) */
+#define FAME6D_ENOMEM 5     /* EXP1:MemoryLimit (exp1/classify_exp1_exception.m:22-36) */
+#define FAME6D_EHIP 6       /* HIP runtime failure */
+#define FAME6D_EARG 7       /* invalid argument */
+#define FAME6D_EGAMMA 8     /* zero projected mode, Gamma point (exp1/assemble_K_and_nullspace_free_basis.m:36-42) */
+#define FAME6D_EMASSNORM 9  /* recover_original_eigenvectors_exp2:BadMassNorm (exp2/recover_original_eigenvectors_exp2.m:36-41) */
+
+/* Arithmetic of an operation. */
+#define FAME6D_FP64 0  /* complex128 vectors, FP64 FFTs (cfg.gpu.precision = 'double') */
+#define FAME6D_FP32 1  /* complex64 vectors, FP32 FFTs with FP32 accumulation */
+#define FAME6D_MIXED 2 /* FP32 inner PCG + FP64 iterative refinement; FP64 Lanczos */
+
+typedef struct fame6d_mass fame6d_mass;
+typedef struct fame6d_qsys fame6d_qsys;
+
+const char* fame6d_last_error(void);
+int fame6d_version(void);
+
+/* ----------------------------------------------------------------- mass backend */
+
+typedef struct {
+  int N;
+  int nGrid;
+  long long NF;
+  double eps_min, eps_max, eps_mean;
+  int tile_a64, tile_b64, tile_a32, tile_b32; /* autotuned tile parameters */
+} fame6d_mass_info_t;
+
+/* build_mass_backend_large_scalar (exp1/build_mass_backend_large_scalar.m:1-49) with
+ * the 'batchfft-scalar' operator M_s = F diag(eps) F^{-1} (exp1/applyMij_fft.m:11),
+ * which equals the 'rank1-exact-scalar' operator (exp1/applyMij_rank1_exact.m:4-5).
+ * eps_host: NF real permittivity samples, column-major. stream: hipStream_t or NULL. */
+int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass** out);
+int fame6d_mass_destroy(fame6d_mass* m);
+int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
+
+/* Project-written tile autotuner: times every compiled tile variant of the three
+ * passes at this N and precision and keeps the fastest (no reference counterpart;
+ * the reference has one implicit cuFFT plan). */
+int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
+/* Set tile variants explicitly (variant indices 0/1 for passes A/C and B). */
+int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
+
+/* mass.MscalarOp: applyMij_rank1_exact_lowmem (exp1/applyMij_rank1_exact_lowmem.m:8-22),
+ * applyMij_fft (exp1/applyMij_fft.m:9-23).  x, y: NF x nrhs.  y may alias x. */
+int fame6d_mass_apply_scalar(fame6d_mass* m, int precision, const void* x, void* y, int nrhs);
+/* mass.Mop: applyM_component_first_rank1_scalar (exp1/applyM_component_first_rank1_scalar.m:4-19).
+ * U, MU: 3NF x nrhs. */
+int fame6d_mass_apply_3comp(fame6d_mass* m, int precision, const void* U, void* MU, int nrhs);
+/* mass.Mcols12op: applyM_cols12_rank1_scalar (exp1/applyM_cols12_rank1_scalar.m:3-11).
+ * x12: 2NF x nrhs -> Y: 3NF x nrhs with block 3 = 0. */
+int fame6d_mass_apply_cols12(fame6d_mass* m, int precision, const void* x12, void* Y, int nrhs);
+/* mass.Mcol3op: applyM_col3_rank1_scalar (exp1/applyM_col3_rank1_scalar.m:3-9).
+ * x: NF x nrhs -> Y: 3NF x nrhs with blocks 1-2 = 0. */
+int fame6d_mass_apply_col3(fame6d_mass* m, int precision, const void* x, void* Y, int nrhs);
+
+/* ----------------------------------------------------------------- q-system */
+
+typedef struct {
+  long long NF;
+  double min_abs_det_v1; /* build_experiment1_q_system_C_compact.m:15-19 (warn if < 1e-12) */
+  double min_qq;         /* min |q_xi|^2 (assemble_K_and_nullspace_free_basis.m:33-42) */
+  int n_singular;        /* modes on the singular branch (assemble_K_and_nullspace_free_basis.m:75-91) */
+  double k_lift[6];      /* P (P^T P)^{-1} q (build_experiment1_q_system_C_compact.m:9) */
+} fame6d_qsys_info_t;
+
+/* build_experiment1_q_system_C_compact (exp1/build_experiment1_q_system_C_compact.m:7-38)
+ * + assemble_K_and_nullspace_free_basis (exp1/assemble_K_and_nullspace_free_basis.m:18-105).
+ * q_host: Bloch vector (3); P_host: 6x3 projection, row-major (P[l*3+c]); T = 2*pi. */
+int fame6d_qsys_create(fame6d_mass* m, const double* q_host, const double* P_host, fame6d_qsys** out);
+int fame6d_qsys_destroy(fame6d_qsys* s);
+int fame6d_qsys_info(const fame6d_qsys* s, fame6d_qsys_info_t* info);
+/* Modewise diagonals as host arrays (each NF, may be NULL): V11,V12,V21,V22,V31,V32,W1,W2,invC. */
+int fame6d_qsys_diagonals(const fame6d_qsys* s, double* V11, double* V12, double* V21, double* V22,
+                          double* V31, double* V32, double* W1, double* W2, double* invC);
+
+/* Parameters of the FAME6D_MIXED Mhat solve: relative tolerance of each FP32 inner PCG
+ * and the maximum number of FP64 refinement cycles (no reference counterpart; the
+ * reference solves in FP64 only, cfg.gpu.precision = 'double'). */
+int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles);
+
+/* apply_Mhat_C_scalar_compact (exp1/apply_Mhat_C_scalar_compact.m:8-21): x, y NF. */
+int fame6d_mhat_apply(fame6d_qsys* s, int precision, const void* x, void* y);
+
+/* apply_V_blockdiag (exp1/apply_V_blockdiag.m:9-16): x 2NF -> y 3NF (FP64). */
+int fame6d_apply_v(fame6d_qsys* s, const void* x, void* y);
+/* Kop (exp1/assemble_K_and_nullspace_free_basis.m:110-119): x 3NF -> y 3NF (FP64). */
+int fame6d_apply_k(fame6d_qsys* s, const void* x, void* y);
+
+typedef struct {
+  long long calls;
+  long long total_iterations;  /* Mhat CG iterations (FP32 ones in MIXED) */
+  long long total_fp64_iterations;
+  int max_iterations;
+  int last_iterations;
+  int last_flag;
+  double last_relres;
+  double max_relres;
+  long long refinement_cycles; /* MIXED only */
+  long long mass_applies_fp64; /* scalar M_s applies issued */
+  long long mass_applies_fp32;
+  double seconds;
+} fame6d_krinv_stats_t;
+
+/* make_explicit_krinv_compact applyOne (exp1/make_explicit_krinv_compact.m:24-80):
+ * y = K_r^{-1} q with PCG on Mhat (tol, maxit; pcg call at :43).  q, y: 2NF complex128.
+ * precision FAME6D_MIXED runs the Mhat solve in FP32 with FP64 iterative refinement. */
+int fame6d_krinv_apply(fame6d_qsys* s, int precision, const void* q, void* y, double tol, int maxit,
+                       fame6d_krinv_stats_t* stats);
+
+/* ----------------------------------------------------------------- eigensolver */
+
+typedef struct {
+  int converged;
+  int num_converged;
+  int outer_steps;      /* total Lanczos steps (Lanczos_hybrid_identity_gpuop.m:75) */
+  int restarts;
+  int hit_max_restart;
+  double max_ritz_residual;
+  double seconds;
+  fame6d_krinv_stats_t krinv;
+} fame6d_lanczos_stats_t;
+
+/* run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:4-34) driving
+ * Lanczos_hybrid_identity_gpuop (exp1/Lanczos_hybrid_identity_gpuop.m:11-130) with the
+ * basis kept on the device.  v0_host: 2NF complex128 start vector (normalised inside).
+ * lambda_host: k ascending eigenvalues 1/theta.  Y: optional device 2NF x k complex128
+ * reduced Ritz vectors (ordered as lambda). */
+int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart,
+                         double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host,
+                         void* Y, fame6d_lanczos_stats_t* stats);
+
+/* The same for nq Bloch points sharing one mass backend, solved together: every
+ * Lanczos step issues one batched K_r^{-1} over all still-active points, so each
+ * kernel launch covers all of them (q_host: 3*nq; v0_host: nq x 2NF; lambda_host:
+ * nq x k; Y: optional device nq x (2NF x k)).  No reference counterpart: the
+ * reference loops over Bloch points sequentially (exp1/run_fig1a_BC_method.m:37-80). */
+int fame6d_eigs_method_c_batch(fame6d_mass* m, int nq, const double* q_host, const double* P_host,
+                               int precision, int k, int p, double tol, int max_restart,
+                               double tol_cg, int maxit_cg, const void* v0_host,
+                               double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
+
+typedef struct {
+  int iterations[64];
+  double relres[64];
+  double mass_norm2[64];
+} fame6d_recover_stats_t;
+
+/* recover_original_eigenvectors_exp2 (exp2/recover_original_eigenvectors_exp2.m:18-51):
+ * X(:,j) = M^{-1} V Y(:,j) by PCG on M (tol, maxit), M-normalised; MX = M X.
+ * Y: 2NF x k, X, MX: 3NF x k (complex128, device). k <= 64. */
+int fame6d_recover_eigenvectors(fame6d_qsys* s, const void* Y, int k, double tol, int maxit, void* X,
+                                void* MX, fame6d_recover_stats_t* stats);
+
+/* compute_fourier_diagnostics_exp2 (exp2/compute_fourier_diagnostics_exp2.m:10-26). */
+int fame6d_fourier_diagnostics(fame6d_qsys* s, const void* X, const void* MX,
+                               const double* lambda_host, int k, double* res_gevp_host,
+                               double* div_host, double* rayleigh_host);
+
+/* ----------------------------------------------------------------- Taylor */
+
+/* Moments of reconstruct_component_taylor_batch_exp4 (exp3/reconstruct_component_taylor_batch_exp4.m:14-25):
+ * moments(alpha, c) = i^|alpha|/alpha! * sum_j q_j^alpha exp(i q_j . r_c) coeff_j
+ * for |alpha| <= p (ordering of build_multi_index_list_3d_exp3.m:6-12).
+ * qmodes: NF x 3 real (column-major, device), coeff: NF complex128 (device),
+ * centers_host: C x 3 (row-major), moments: L x C complex128 (device, column-major). */
+int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff, int p, int C,
+                          const double* centers_host, void* moments, void* stream);
+/* Evaluation loop of the same function (:33-38): u(pt) = sum_alpha (r - r_c)^alpha
+ * moments(alpha, c(pt)).  points: M x 3 real (column-major, device), center_id: M int32
+ * (device), u: M complex128 (device). */
+int fame6d_taylor_evaluate(long long M, const double* points, const int* center_id, int p, int C,
+                           const double* centers_host, const void* moments, void* u, void* stream);
+/* reconstruct_field_direct_blocked_exp3 (exp2/reconstruct_field_direct_blocked_exp3.m:10-15):
+ * u = exp(i R q^T) coeff. */
+int fame6d_direct_reconstruct(long long NF, const double* qmodes, const void* coeff, long long M,
+                              const double* points, void* u, void* stream);
+
+/* ----------------------------------------------------------------- adjoint */
+
+/* Adjoint of y = M_s x with respect to the permittivity samples: for each rhs,
+ * geps += NF * Re(conj(ifftn(gy)) .* ifftn(x))  (derivative of applyMij_fft.m:11 in eps;
+ * no reference counterpart).  x, gy: NF x nrhs complex128; geps: NF real (device). */
+int fame6d_mass_adjoint_eps(fame6d_mass* m, const void* x, const void* gy, int nrhs, double* geps);
+/* Hellmann-Feynman eigenvalue gradient: geps += sum_j glambda_j * (-lambda_j * NF *
+ * sum_c |ifftn(X_c(:,j))|^2) for M-normalised X (3NF x k, device). */
+int fame6d_eig_grad_eps(fame6d_mass* m, const void* X, int k, const double* lambda_host,
+                        const double* glambda_host, double* geps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FAME6D_H */

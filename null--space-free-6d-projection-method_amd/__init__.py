@@ -1,0 +1,16 @@
+"""MI355X-native hot path of the null-space-free 6D Fourier projection eigensolver.
+
+The compute path is libfame6d.so (hand-written HIP kernels for gfx950 behind the C ABI
+of include/fame6d.h); this package is its host-side mirror of the reference's
+operator interface (the `mass` struct of build_mass_backend_large_scalar, the
+q-system, K_r^{-1}, Lanczos, recovery and Taylor reconstruction).  There is no
+CPU fallback: every operation raises when the library or the GPU is missing.
+"""
+from ._lib import FP32, FP64, MIXED, Fame6dError, load  # noqa: F401
+from .mass import (MassBackend, P_DEFAULT, build_mass_backend_large_scalar, cosine_material,  # noqa: F401
+                   smooth_scalar_material, start_vector)
+from .eigs import (QSystem, build_experiment1_q_system_C_compact, compute_fourier_diagnostics,  # noqa: F401
+                   make_explicit_krinv_compact, recover_original_eigenvectors, run_method_C,
+                   run_method_C_batch)
+from .taylor import (block_partition, build_multi_index_list_3d, direct_reconstruct,  # noqa: F401
+                     taylor_evaluate, taylor_moments, yee_points)

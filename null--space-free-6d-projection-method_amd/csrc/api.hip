@@ -1,0 +1,493 @@
+// C ABI of libfame6d (declared in include/fame6d.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+#include "krylov_kernels.hpp"
+
+namespace fame6d {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
+                   double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
+                   fame6d_lanczos_stats_t* stats);
+void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double tol, int maxit,
+                          cpx<double>* X, cpx<double>* MX, fame6d_recover_stats_t* stats);
+void fourier_diagnostics(fame6d_qsys* S, const cpx<double>* X, const cpx<double>* MX,
+                         const double* lambda, int k, double* res, double* div, double* ray);
+
+// geps[j] += sum_g w[g] * Re(conj(a_g[j]) * b_g[j])   (real grid vector, groups of length n)
+__global__ void k_eps_grad(long long n, int G, const cpx<double>* a, const cpx<double>* b, long long ld,
+                           const double* w, double* geps) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < G; ++g) {
+      const cpx<double> u = a[g * ld + i], v = b[g * ld + i];
+      s += w[g] * (u.re * v.re + u.im * v.im);
+    }
+    geps[i] += s;
+  }
+}
+
+// Unnormalised inverse 6D DFT of G grid vectors: t = F^{-1}_unnorm x (passes A and B mode 1).
+static void ifft6_unnorm(fame6d_mass* m, const cpx<double>* x, long long ldx, cpx<double>* t, int G) {
+  PassAArgs<double> a{};
+  a.x = x;
+  a.ldx = ldx;
+  a.t = t;
+  a.ldt = m->NF;
+  launch_pass_a<double>(m, PRE_COPY, G, a, m->stream);
+  launch_pass_b<double>(m, 1, 1, G, t, m->NF, no_skip(), m->stream);
+}
+
+template <class T>
+static void cols12(fame6d_mass* m, const cpx<T>* x, cpx<T>* Y, int nrhs) {
+  const long long NF = m->NF;
+  for (int j = 0; j < nrhs; ++j) {
+    mass_apply<T>(m, x + 2 * NF * j, NF, Y + 3 * NF * j, NF, 2, m->stream);
+    k_fill_zero<T><<<grid_for(NF), 256, 0, m->stream>>>(NF, Y + 3 * NF * j + 2 * NF);
+  }
+  F6_CHECK_LAUNCH();
+}
+
+template <class T>
+static void col3(fame6d_mass* m, const cpx<T>* x, cpx<T>* Y, int nrhs) {
+  const long long NF = m->NF;
+  if (nrhs <= 0) return;
+  for (int j = 0; j < nrhs; ++j)
+    k_fill_zero<T><<<grid_for(2 * NF), 256, 0, m->stream>>>(2 * NF, Y + 3 * NF * j);
+  F6_CHECK_LAUNCH();
+  mass_apply<T>(m, x, NF, Y + 2 * NF, 3 * NF, nrhs, m->stream);
+}
+
+__global__ void k_apply_k_api(const double* q1, const double* q2, const double* q3, const cpx<double>* x,
+                              cpx<double>* y, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double a = q1[i], b = q2[i], c = q3[i];
+    const cpx<double> x1 = x[i], x2 = x[n + i], x3 = x[2 * n + i];
+    y[i] = (b * b + c * c) * x1 - (a * b) * x2 - (a * c) * x3;
+    y[n + i] = (a * a + c * c) * x2 - (a * b) * x1 - (b * c) * x3;
+    y[2 * n + i] = (a * a + b * b) * x3 - (a * c) * x1 - (b * c) * x2;
+  }
+}
+
+}  // namespace fame6d
+
+using namespace fame6d;
+
+#define F6_API_BEGIN try {
+#define F6_API_END                              \
+  return FAME6D_OK;                             \
+  }                                             \
+  catch (const Error& e) {                      \
+    set_last_error(e.what());                   \
+    return e.code;                              \
+  }                                             \
+  catch (const std::exception& e) {             \
+    set_last_error(e.what());                   \
+    return FAME6D_EARG;                         \
+  }
+
+#define F6_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) throw Error(code, msg); \
+  } while (0)
+
+extern "C" {
+
+const char* fame6d_last_error(void) { return g_last_error.c_str(); }
+int fame6d_version(void) { return 10000; }
+
+int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass** out) {
+  F6_API_BEGIN
+  F6_REQUIRE(out != nullptr && eps_host != nullptr, FAME6D_EARG, "null argument");
+  *out = nullptr;
+  F6_REQUIRE(N >= 1 && N <= 6, FAME6D_EARG, "N must be in 1..6 (nGrid = 2N in {2,...,12})");
+  auto* m = new fame6d_mass();
+  try {
+    m->N = N;
+    m->n = 2 * N;
+    long long NF = 1;
+    for (int i = 0; i < 6; ++i) NF *= m->n;
+    m->NF = NF;
+    m->stream = static_cast<hipStream_t>(stream);
+    F6_HIP(hipGetDevice(&m->device));
+    m->eps_host.assign(eps_host, eps_host + NF);
+    double mn = INFINITY, mx = -INFINITY, sum = 0;
+    for (long long i = 0; i < NF; ++i) {
+      const double e = eps_host[i];
+      if (!std::isfinite(e)) throw Error(FAME6D_ENONFINITE, "permittivity contains NaN/Inf");
+      mn = std::min(mn, e);
+      mx = std::max(mx, e);
+      sum += e;
+    }
+    if (!(mn > 0))
+      throw Error(FAME6D_EARG, "permittivity must be positive: M_s = F diag(eps) F^{-1} is then Hermitian "
+                               "positive definite, which the PCG solves require");
+    m->eps_min = mn;
+    m->eps_max = mx;
+    m->eps_mean = sum / (double)NF;
+    std::vector<double> e64(NF);
+    std::vector<float> e32(NF);
+    for (long long i = 0; i < NF; ++i) {
+      e64[i] = eps_host[i] / (double)NF;
+      e32[i] = (float)e64[i];
+    }
+    m->eps64.ensure(sizeof(double) * NF);
+    m->eps32.ensure(sizeof(float) * NF);
+    F6_HIP(hipMemcpyAsync(m->eps64.ptr, e64.data(), sizeof(double) * NF, hipMemcpyHostToDevice, m->stream));
+    F6_HIP(hipMemcpyAsync(m->eps32.ptr, e32.data(), sizeof(float) * NF, hipMemcpyHostToDevice, m->stream));
+    F6_HIP(hipStreamSynchronize(m->stream));
+  } catch (...) {
+    m->eps64.release();
+    m->eps32.release();
+    delete m;
+    throw;
+  }
+  *out = m;
+  F6_API_END
+}
+
+int fame6d_mass_destroy(fame6d_mass* m) {
+  F6_API_BEGIN
+  if (!m) return FAME6D_OK;
+  (void)hipStreamSynchronize(m->stream);
+  m->eps64.release();
+  m->eps32.release();
+  m->work64.release();
+  m->work32.release();
+  m->io64.release();
+  m->partials.release();
+  m->scal.release();
+  delete m;
+  F6_API_END
+}
+
+int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && info, FAME6D_EARG, "null argument");
+  info->N = m->N;
+  info->nGrid = m->n;
+  info->NF = m->NF;
+  info->eps_min = m->eps_min;
+  info->eps_max = m->eps_max;
+  info->eps_mean = m->eps_mean;
+  int th;
+  tile_variant_desc(m->n, false, false, m->tile64.a, &info->tile_a64, &th);
+  tile_variant_desc(m->n, false, true, m->tile64.b, &info->tile_b64, &th);
+  tile_variant_desc(m->n, true, false, m->tile32.a, &info->tile_a32, &th);
+  tile_variant_desc(m->n, true, true, m->tile32.b, &info->tile_b32, &th);
+  F6_API_END
+}
+
+int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
+  F6_API_BEGIN
+  F6_REQUIRE(m, FAME6D_EARG, "null handle");
+  F6_REQUIRE(va >= 0 && va < 2 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant must be 0 or 1");
+  TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
+  t.a = va;
+  t.b = vb;
+  F6_API_END
+}
+
+int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps) {
+  F6_API_BEGIN
+  F6_REQUIRE(m, FAME6D_EARG, "null handle");
+  F6_REQUIRE(precision == FAME6D_FP64 || precision == FAME6D_FP32, FAME6D_EARG, "precision");
+  if (reps < 1) reps = 1;
+  const bool f32 = precision == FAME6D_FP32;
+  const long long NF = m->NF;
+  const size_t esz = f32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  DevBuf buf;
+  buf.ensure(esz * NF * 2);
+  if (f32) k_fill_zero<float><<<grid_for(NF * 2), 256, 0, m->stream>>>(NF * 2, buf.as<cpx<float>>());
+  else k_fill_zero<double><<<grid_for(NF * 2), 256, 0, m->stream>>>(NF * 2, buf.as<cpx<double>>());
+  F6_CHECK_LAUNCH();
+  hipEvent_t e0, e1;
+  F6_HIP(hipEventCreate(&e0));
+  F6_HIP(hipEventCreate(&e1));
+  TileChoice& tc = f32 ? m->tile32 : m->tile64;
+  float best = 1e30f;
+  TileChoice bestc = tc;
+  for (int va = 0; va < 2; ++va) {
+    for (int vb = 0; vb < 2; ++vb) {
+      tc.a = va;
+      tc.b = vb;
+      auto run = [&]() {
+        if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 3, m->stream);
+        else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
+      };
+      // one warm-up (and the workspace allocation) outside the timed region
+      if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 1, m->stream);
+      else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
+      F6_HIP(hipStreamSynchronize(m->stream));
+      F6_HIP(hipEventRecord(e0, m->stream));
+      for (int r = 0; r < reps; ++r) {
+        if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 1, m->stream);
+        else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
+      }
+      F6_HIP(hipEventRecord(e1, m->stream));
+      F6_HIP(hipEventSynchronize(e1));
+      (void)run;
+      float ms = 0;
+      F6_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bestc = tc;
+      }
+    }
+  }
+  tc = bestc;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  buf.release();
+  F6_API_END
+}
+
+int fame6d_mass_apply_scalar(fame6d_mass* m, int precision, const void* x, void* y, int nrhs) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && x && y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nrhs >= 0, FAME6D_ESIZE, "nrhs must be >= 0");
+  if (precision == FAME6D_FP64)
+    mass_apply<double>(m, (const cpx<double>*)x, m->NF, (cpx<double>*)y, m->NF, nrhs, m->stream);
+  else if (precision == FAME6D_FP32)
+    mass_apply<float>(m, (const cpx<float>*)x, m->NF, (cpx<float>*)y, m->NF, nrhs, m->stream);
+  else
+    throw Error(FAME6D_EARG, "precision must be FAME6D_FP64 or FAME6D_FP32");
+  F6_API_END
+}
+
+int fame6d_mass_apply_3comp(fame6d_mass* m, int precision, const void* U, void* MU, int nrhs) {
+  // [U1;U2;U3] columns are 3*nrhs contiguous NF blocks with the same layout in MU
+  return fame6d_mass_apply_scalar(m, precision, U, MU, 3 * nrhs);
+}
+
+int fame6d_mass_apply_cols12(fame6d_mass* m, int precision, const void* x12, void* Y, int nrhs) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && x12 && Y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nrhs >= 0, FAME6D_ESIZE, "nrhs must be >= 0");
+  F6_REQUIRE(x12 != Y, FAME6D_EARG, "x12 and Y must not alias");
+  if (precision == FAME6D_FP64) cols12<double>(m, (const cpx<double>*)x12, (cpx<double>*)Y, nrhs);
+  else if (precision == FAME6D_FP32) cols12<float>(m, (const cpx<float>*)x12, (cpx<float>*)Y, nrhs);
+  else throw Error(FAME6D_EARG, "precision must be FAME6D_FP64 or FAME6D_FP32");
+  F6_API_END
+}
+
+int fame6d_mass_apply_col3(fame6d_mass* m, int precision, const void* x, void* Y, int nrhs) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && x && Y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nrhs >= 0, FAME6D_ESIZE, "nrhs must be >= 0");
+  F6_REQUIRE(x != Y, FAME6D_EARG, "x and Y must not alias");
+  if (precision == FAME6D_FP64) col3<double>(m, (const cpx<double>*)x, (cpx<double>*)Y, nrhs);
+  else if (precision == FAME6D_FP32) col3<float>(m, (const cpx<float>*)x, (cpx<float>*)Y, nrhs);
+  else throw Error(FAME6D_EARG, "precision must be FAME6D_FP64 or FAME6D_FP32");
+  F6_API_END
+}
+
+int fame6d_qsys_create(fame6d_mass* m, const double* q_host, const double* P_host, fame6d_qsys** out) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && q_host && P_host && out, FAME6D_EARG, "null argument");
+  *out = qsys_build(m, 1, q_host, P_host);
+  F6_API_END
+}
+
+int fame6d_qsys_destroy(fame6d_qsys* s) {
+  F6_API_BEGIN
+  if (!s) return FAME6D_OK;
+  (void)hipStreamSynchronize(s->m->stream);
+  for (DevBuf* b : {&s->Vd, &s->Wd, &s->Wf, &s->Qd, &s->kr64, &s->kr32, &s->st, &s->part, &s->part2,
+                    &s->ints, &s->sc})
+    b->release();
+  delete s;
+  F6_API_END
+}
+
+int fame6d_qsys_info(const fame6d_qsys* s, fame6d_qsys_info_t* info) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && info, FAME6D_EARG, "null argument");
+  info->NF = s->NF;
+  info->min_abs_det_v1 = s->min_abs_det[0];
+  info->min_qq = s->min_qq[0];
+  info->n_singular = s->n_singular[0];
+  for (int l = 0; l < 6; ++l) info->k_lift[l] = s->klift[l];
+  F6_API_END
+}
+
+int fame6d_qsys_diagonals(const fame6d_qsys* s, double* V11, double* V12, double* V21, double* V22,
+                          double* V31, double* V32, double* W1, double* W2, double* invC) {
+  F6_API_BEGIN
+  F6_REQUIRE(s, FAME6D_EARG, "null handle");
+  const size_t bytes = sizeof(double) * s->NF;
+  double* outs[6] = {V11, V12, V21, V22, V31, V32};
+  for (int i = 0; i < 6; ++i)
+    if (outs[i]) F6_HIP(hipMemcpy(outs[i], s->V(i), bytes, hipMemcpyDeviceToHost));
+  double* wo[3] = {W1, W2, invC};
+  for (int i = 0; i < 3; ++i)
+    if (wo[i]) F6_HIP(hipMemcpy(wo[i], s->W64(i), bytes, hipMemcpyDeviceToHost));
+  F6_API_END
+}
+
+int fame6d_mhat_apply(fame6d_qsys* s, int precision, const void* x, void* y) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && x && y, FAME6D_EARG, "null argument");
+  if (precision == FAME6D_FP64)
+    mhat_apply<double>(s, 1, nullptr, (const cpx<double>*)x, (cpx<double>*)y, no_skip(), s->m->stream);
+  else if (precision == FAME6D_FP32)
+    mhat_apply<float>(s, 1, nullptr, (const cpx<float>*)x, (cpx<float>*)y, no_skip(), s->m->stream);
+  else
+    throw Error(FAME6D_EARG, "precision must be FAME6D_FP64 or FAME6D_FP32");
+  F6_API_END
+}
+
+int fame6d_apply_v(fame6d_qsys* s, const void* x, void* y) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && x && y, FAME6D_EARG, "null argument");
+  const long long NF = s->NF;
+  k_apply_v<<<dim3(grid_for(NF), 1), 256, 0, s->m->stream>>>((const cpx<double>*)x, 2 * NF, NF, s->V(0), s->V(1),
+                                                              s->V(2), s->V(3), s->V(4), s->V(5),
+                                                              (cpx<double>*)y, 3 * NF, no_skip(), nullptr);
+  F6_CHECK_LAUNCH();
+  F6_API_END
+}
+
+int fame6d_apply_k(fame6d_qsys* s, const void* x, void* y) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && x && y && x != y, FAME6D_EARG, "null or aliased argument");
+  k_apply_k_api<<<grid_for(s->NF), 256, 0, s->m->stream>>>(s->Qm(0), s->Qm(1), s->Qm(2),
+                                                            (const cpx<double>*)x, (cpx<double>*)y, s->NF);
+  F6_CHECK_LAUNCH();
+  F6_API_END
+}
+
+int fame6d_krinv_apply(fame6d_qsys* s, int precision, const void* q, void* y, double tol, int maxit,
+                       fame6d_krinv_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && q && y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(tol > 0 && maxit >= 0, FAME6D_EARG, "tol must be > 0 and maxit >= 0");
+  const fame6d_krinv_stats_t st0 = s->stats;
+  const int slot = 0;
+  krinv_groups(s, precision, 1, &slot, (const cpx<double>*)q, (cpx<double>*)y, tol, maxit, nullptr);
+  if (stats) {
+    *stats = s->stats;
+    stats->calls -= st0.calls;
+    stats->total_iterations -= st0.total_iterations;
+    stats->total_fp64_iterations -= st0.total_fp64_iterations;
+    stats->refinement_cycles -= st0.refinement_cycles;
+    stats->mass_applies_fp64 -= st0.mass_applies_fp64;
+    stats->mass_applies_fp32 -= st0.mass_applies_fp32;
+    stats->seconds -= st0.seconds;
+  }
+  F6_API_END
+}
+
+int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && inner_tol > 0 && inner_tol < 1 && max_cycles > 0, FAME6D_EARG, "bad mixed parameters");
+  s->mixed_inner_tol = inner_tol;
+  s->mixed_max_cycles = max_cycles;
+  F6_API_END
+}
+
+int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart,
+                         double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host, void* Y,
+                         fame6d_lanczos_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
+  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
+                lambda_host, (cpx<double>*)Y, stats);
+  F6_API_END
+}
+
+int fame6d_eigs_method_c_batch(fame6d_mass* m, int nq, const double* q_host, const double* P_host,
+                               int precision, int k, int p, double tol, int max_restart, double tol_cg,
+                               int maxit_cg, const void* v0_host, double* lambda_host, void* Y,
+                               fame6d_lanczos_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && q_host && P_host && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nq >= 1 && nq <= 64, FAME6D_EARG, "nq must be in 1..64");
+  fame6d_qsys* S = qsys_build(m, nq, q_host, P_host);
+  try {
+    eigs_method_c(S, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
+                  lambda_host, (cpx<double>*)Y, stats);
+  } catch (...) {
+    fame6d_qsys_destroy(S);
+    throw;
+  }
+  fame6d_qsys_destroy(S);
+  F6_API_END
+}
+
+int fame6d_recover_eigenvectors(fame6d_qsys* s, const void* Y, int k, double tol, int maxit, void* X,
+                                void* MX, fame6d_recover_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && Y && X && MX, FAME6D_EARG, "null argument");
+  recover_eigenvectors(s, (const cpx<double>*)Y, k, tol, maxit, (cpx<double>*)X, (cpx<double>*)MX, stats);
+  F6_API_END
+}
+
+int fame6d_fourier_diagnostics(fame6d_qsys* s, const void* X, const void* MX, const double* lambda_host,
+                               int k, double* res_gevp_host, double* div_host, double* rayleigh_host) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && X && MX && lambda_host, FAME6D_EARG, "null argument");
+  fourier_diagnostics(s, (const cpx<double>*)X, (const cpx<double>*)MX, lambda_host, k, res_gevp_host,
+                      div_host, rayleigh_host);
+  F6_API_END
+}
+
+int fame6d_mass_adjoint_eps(fame6d_mass* m, const void* x, const void* gy, int nrhs, double* geps) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && x && gy && geps, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nrhs >= 0, FAME6D_ESIZE, "nrhs must be >= 0");
+  const long long NF = m->NF;
+  DevBuf t;
+  t.ensure(sizeof(cpx<double>) * NF * 2);
+  DevBuf w;
+  w.ensure(sizeof(double) * 8);
+  // NF * Re(conj(ifftn gy) ifftn x) = (1/NF) Re(conj(F^-1_u gy) F^-1_u x)
+  const double wv = 1.0 / (double)NF;
+  F6_HIP(hipMemcpyAsync(w.ptr, &wv, sizeof(double), hipMemcpyHostToDevice, m->stream));
+  for (int j = 0; j < nrhs; ++j) {
+    ifft6_unnorm(m, (const cpx<double>*)gy + NF * j, NF, t.as<cpx<double>>(), 1);
+    ifft6_unnorm(m, (const cpx<double>*)x + NF * j, NF, t.as<cpx<double>>() + NF, 1);
+    k_eps_grad<<<grid_for(NF), 256, 0, m->stream>>>(NF, 1, t.as<cpx<double>>(), t.as<cpx<double>>() + NF, NF,
+                                                    w.as<double>(), geps);
+    F6_CHECK_LAUNCH();
+  }
+  F6_HIP(hipStreamSynchronize(m->stream));
+  t.release();
+  w.release();
+  F6_API_END
+}
+
+int fame6d_eig_grad_eps(fame6d_mass* m, const void* X, int k, const double* lambda_host,
+                        const double* glambda_host, double* geps) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && X && lambda_host && glambda_host && geps, FAME6D_EARG, "null argument");
+  const long long NF = m->NF;
+  DevBuf t, w;
+  t.ensure(sizeof(cpx<double>) * NF * 3);
+  w.ensure(sizeof(double) * 3);
+  // d lambda / d eps_j = -lambda * NF * sum_c |ifftn(x_c)_j|^2 = -(lambda/NF) sum_c |F^-1_u x_c|^2
+  for (int j = 0; j < k; ++j) {
+    const double c = -glambda_host[j] * lambda_host[j] / (double)NF;
+    const double wv[3] = {c, c, c};
+    F6_HIP(hipMemcpyAsync(w.ptr, wv, sizeof(wv), hipMemcpyHostToDevice, m->stream));
+    ifft6_unnorm(m, (const cpx<double>*)X + 3 * NF * j, NF, t.as<cpx<double>>(), 3);
+    k_eps_grad<<<grid_for(NF), 256, 0, m->stream>>>(NF, 3, t.as<cpx<double>>(), t.as<cpx<double>>(), NF,
+                                                    w.as<double>(), geps);
+    F6_CHECK_LAUNCH();
+    F6_HIP(hipStreamSynchronize(m->stream));
+  }
+  t.release();
+  w.release();
+  F6_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Library-internal declarations shared by the translation units of libfame6d.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "mass_kernels.hpp"
+#include "../../include/fame6d.h"
+
+namespace fame6d {
+
+// Error carrying a FAME6D_* status code; converted to a return code at the C-ABI.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define F6_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::fame6d::Error(FAME6D_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define F6_CHECK_LAUNCH() F6_HIP(hipGetLastError())
+
+// Growable device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    if (ptr) F6_HIP(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&ptr, need);
+    if (e != hipSuccess) throw Error(FAME6D_ENOMEM, "hipMalloc of " + std::to_string(need) + " bytes failed");
+    bytes = need;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class U>
+  U* as() const {
+    return static_cast<U*>(ptr);
+  }
+};
+
+// Tile variant indices of the three passes (chosen by the autotuner or defaults).
+struct TileChoice {
+  int a = 0;  // pass A / pass C variant
+  int b = 0;  // pass B variant
+};
+
+}  // namespace fame6d
+
+// Opaque handle of build_mass_backend_large_scalar (exp1/build_mass_backend_large_scalar.m:1-49).
+struct fame6d_mass {
+  int N = 0, n = 0;
+  long long NF = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double eps_min = 0, eps_max = 0, eps_mean = 0;
+  fame6d::DevBuf eps64, eps32;  // eps / NF
+  fame6d::DevBuf work64, work32;   // pass intermediates
+  fame6d::DevBuf io64;             // staging for component wrappers
+  fame6d::DevBuf partials;         // reduction partials
+  fame6d::DevBuf scal;             // small device scalars
+  fame6d::TileChoice tile64, tile32;
+  std::vector<double> eps_host;    // unscaled samples (for adjoint / info)
+};
+
+namespace fame6d {
+
+// Pass launchers (mass.hip).  Vectors of group g live at base + g*ld.
+template <class T>
+void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>& a, hipStream_t s);
+// mode 0: fused inverse / eps / forward over dims 4-6; mode 1: inverse only.
+template <class T>
+void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
+                   Skip skip, hipStream_t s);
+template <class T>
+void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s);
+// Number of workgroups pass C uses per group (size of its partials per group).
+int pass_c_blocks(const fame6d_mass* m, bool fp32);
+int num_tile_variants(int n, bool fp32, bool pass_b);
+void tile_variant_desc(int n, bool fp32, bool pass_b, int v, int* param, int* threads);
+
+// Plain scalar apply y = M_s x for nrhs columns (applyMij_fft semantics).
+template <class T>
+void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long long ldy, int nrhs,
+                hipStream_t s);
+
+template <class T>
+DevBuf& work_buf(fame6d_mass* m);
+
+// Per-solve results of a (batched) PCG on Mhat.
+struct PcgOut {
+  int flag = 0;
+  int iter = 0;
+  double relres = 0;
+  int checks = 0;  // true-residual checks (one Mhat apply each)
+};
+
+}  // namespace fame6d
+
+// Opaque handle of build_experiment1_q_system_C_compact for G Bloch points
+// (exp1/build_experiment1_q_system_C_compact.m:7-38).
+struct fame6d_qsys {
+  fame6d_mass* m = nullptr;
+  int G = 0;
+  long long NF = 0;
+  double P[18];
+  std::vector<double> q, klift, min_abs_det, min_qq;
+  std::vector<int> n_singular;
+  // diagonal blocks, each G*NF: V11 V12 V21 V22 V31 V32 | W1 W2 invC | q1 q2 q3
+  fame6d::DevBuf Vd, Wd, Wf, Qd;
+  // Method-C workspaces (grown to the number of active groups)
+  fame6d::DevBuf kr64, kr32, st, part, part2, ints, sc;
+  double mixed_inner_tol = 3e-5;
+  int mixed_max_cycles = 30;
+  int chunk_hint = 0, chunk_hint32 = 0;
+  fame6d_krinv_stats_t stats{};
+  const double* V(int i) const { return Vd.as<double>() + (size_t)i * G * NF; }
+  const double* W64(int i) const { return Wd.as<double>() + (size_t)i * G * NF; }
+  const float* W32(int i) const { return Wf.as<float>() + (size_t)i * G * NF; }
+  const double* Qm(int i) const { return Qd.as<double>() + (size_t)i * G * NF; }
+};
+
+namespace fame6d {
+
+// Builds the q-system of G Bloch points on the host (FP64) and uploads it.
+fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P);
+// y = Mhat x for nact groups in q-system slots qmap (device ints, nullptr = identity).
+template <class T>
+void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
+                hipStream_t s);
+// Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).
+void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
+                  const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,
+                  std::vector<PcgOut>* outs);
+template <class T>
+void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, const cpx<T>* b,
+                long long ldb, long long n, double* out_re_host, double* out_im_host);
+
+inline int grid_for(long long n, int threads = 256) {
+  long long b = (n + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return static_cast<int>(b);
+}
+
+}  // namespace fame6d

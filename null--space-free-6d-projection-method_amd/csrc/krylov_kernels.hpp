@@ -1,0 +1,400 @@
+// Modewise (diagonal) kernels of the null-space-free q-system, deterministic
+// wave64 reductions, the device-side conjugate-gradient scalar recurrences and
+// the tall-skinny kernels of the Lanczos basis.
+#pragma once
+
+#include "common.hpp"
+#include "mass_kernels.hpp"
+
+namespace fame6d {
+
+constexpr int kRedThreads = 256;
+constexpr int kMaxRedBlocks = 1024;
+
+__host__ inline int red_blocks(long long n) {
+  long long b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
+  if (b < 1) b = 1;
+  if (b > kMaxRedBlocks) b = kMaxRedBlocks;
+  return static_cast<int>(b);
+}
+
+// ------------------------------------------------------------------ modewise
+// All modewise kernels run over grid.y = groups (Bloch points solved together).
+// Group g reads its 2NF/NF vectors at base + g*ld and its diagonals in q-system
+// slot qslot(qmap, g) (diagonals of slot s at base + s*n).  V, W, invC are real
+// because q_xi = P^T (xi + k_lift) is real for real P, xi and Bloch vector.
+#define F6_MODEWISE_LOOP                                                              \
+  const int g = blockIdx.y;                                                           \
+  if (skipped(skip, g)) return;                                                       \
+  const long long qs = qslot(qmap, g) * n;                                            \
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;         \
+       i += (long long)gridDim.x * blockDim.x)
+
+// r = V1^{-*} q   (solveV1H_blockdiag.m:7-8)
+template <class T>
+__global__ void k_v1h_solve(const cpx<double>* q, long long ldq, long long n, const double* V11,
+                            const double* V12, const double* V21, const double* V22, cpx<T>* r,
+                            long long ldr, Skip skip, const int* qmap) {
+  F6_MODEWISE_LOOP {
+    const long long j = qs + i;
+    const double det = V11[j] * V22[j] - V12[j] * V21[j];
+    const cpx<double> x1 = q[g * ldq + i], x2 = q[g * ldq + n + i];
+    const cpx<double> y1 = (1.0 / det) * (V22[j] * x1 - V21[j] * x2);
+    const cpx<double> y2 = (1.0 / det) * (V11[j] * x2 - V12[j] * x1);
+    r[g * ldr + i] = cvt<T>(y1);
+    r[g * ldr + n + i] = cvt<T>(y2);
+  }
+}
+
+// bhat = -invC .* (W1 .* mr1 + W2 .* mr2)   (make_explicit_krinv_compact.m:34-35)
+template <class T>
+__global__ void k_bhat(const cpx<T>* mr, long long ldm, long long n, const T* W1, const T* W2,
+                       const T* invC, cpx<T>* bhat, long long ldb, Skip skip, const int* qmap) {
+  F6_MODEWISE_LOOP {
+    const long long j = qs + i;
+    const cpx<T> s = W1[j] * mr[g * ldm + i] + W2[j] * mr[g * ldm + n + i];
+    bhat[g * ldb + i] = (-invC[j]) * s;
+  }
+}
+
+// z = invC .* u; out = [W1 .* z; W2 .* z]   (make_explicit_krinv_compact.m:60-62)
+template <class T>
+__global__ void k_zw(const cpx<T>* u, long long ldu, long long n, const T* W1, const T* W2,
+                     const T* invC, cpx<T>* out, long long ldo, Skip skip, const int* qmap) {
+  F6_MODEWISE_LOOP {
+    const long long j = qs + i;
+    const cpx<T> z = invC[j] * u[g * ldu + i];
+    out[g * ldo + i] = W1[j] * z;
+    out[g * ldo + n + i] = W2[j] * z;
+  }
+}
+
+// y = V1^{-1} [mr1 + s1; mr2 + s2]   (make_explicit_krinv_compact.m:61-64, solveV1_blockdiag.m:7-8)
+template <class T>
+__global__ void k_v1_solve(const cpx<T>* mr, const cpx<T>* s, long long ld, long long n,
+                           const double* V11, const double* V12, const double* V21,
+                           const double* V22, cpx<double>* y, long long ldy, Skip skip,
+                           const int* qmap) {
+  F6_MODEWISE_LOOP {
+    const long long j = qs + i;
+    const double det = V11[j] * V22[j] - V12[j] * V21[j];
+    const cpx<double> m1 = cvt<double>(mr[g * ld + i]) + cvt<double>(s[g * ld + i]);
+    const cpx<double> m2 = cvt<double>(mr[g * ld + n + i]) + cvt<double>(s[g * ld + n + i]);
+    y[g * ldy + i] = (1.0 / det) * (V22[j] * m1 - V12[j] * m2);
+    y[g * ldy + n + i] = (1.0 / det) * (V11[j] * m2 - V21[j] * m1);
+  }
+}
+
+// y(3NF) = V x(2NF)   (apply_V_blockdiag.m:12-16)
+static __global__ void k_apply_v(const cpx<double>* x, long long ldx, long long n, const double* V11,
+                          const double* V12, const double* V21, const double* V22,
+                          const double* V31, const double* V32, cpx<double>* y, long long ldy,
+                          Skip skip, const int* qmap) {
+  F6_MODEWISE_LOOP {
+    const long long j = qs + i;
+    const cpx<double> x1 = x[g * ldx + i], x2 = x[g * ldx + n + i];
+    y[g * ldy + i] = V11[j] * x1 + V12[j] * x2;
+    y[g * ldy + n + i] = V21[j] * x1 + V22[j] * x2;
+    y[g * ldy + 2 * n + i] = V31[j] * x1 + V32[j] * x2;
+  }
+}
+
+// r = b - y (groups of length n; the true residual of a CG check)
+template <class T>
+__global__ void k_residual(const cpx<T>* b, const cpx<T>* y, cpx<T>* r, long long ld, long long n,
+                           Skip skip) {
+  const int g = blockIdx.y;
+  if (skipped(skip, g)) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    r[g * ld + i] = b[g * ld + i] - y[g * ld + i];
+  }
+}
+
+// ------------------------------------------------------------------ reductions
+// Block-level partial sums of conj(a).*b over groups of length n: one partial
+// per (group, block) for the real part and one for the imaginary part.
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partials(const cpx<T>* a, long long lda,
+                                                              const cpx<T>* b, long long ldb,
+                                                              long long n, double* pre,
+                                                              double* pim) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.y;
+  const cpx<T>* ag = a + g * lda;
+  const cpx<T>* bg = b + g * ldb;
+  double sr = 0.0, si = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const cpx<T> u = ag[i], v = bg[i];
+    sr += static_cast<double>(u.re) * v.re + static_cast<double>(u.im) * v.im;
+    si += static_cast<double>(u.re) * v.im - static_cast<double>(u.im) * v.re;
+  }
+  sr = block_sum<kRedThreads>(sr, red);
+  __syncthreads();
+  si = block_sum<kRedThreads>(si, red);
+  if (threadIdx.x == 0) {
+    pre[(long long)g * gridDim.x + blockIdx.x] = sr;
+    if (pim != nullptr) pim[(long long)g * gridDim.x + blockIdx.x] = si;
+  }
+}
+
+// Fixed-order sum of nb partials per group; out[g] = sum.
+static __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* partials, int nb,
+                                                              double* out) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[(long long)g * nb + i];
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) out[g] = s;
+}
+
+// ------------------------------------------------------------------ CG scalars
+// Follows MATLAB pcg without preconditioner and x0 = 0 (call sites
+// make_explicit_krinv_compact.m:43): rho = r'r, p = r + (rho/rho_old) p,
+// alpha = rho/(p'q), stop on the recurrence residual ||r|| <= tol*||b||
+// (the host then verifies with the true residual b - A x).
+__device__ __forceinline__ double sum_partials(const double* partials, int nb, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
+  return block_sum<kRedThreads>(s, red);
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_cg_init(CgState* st, const double* partials,
+                                                         int nb, double tol, int maxit) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  const double s = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    CgState c{};
+    c.n2b = sqrt(s);
+    c.tolb = tol * c.n2b;
+    c.rho = s;
+    c.normr = c.n2b;
+    c.iter = 0;
+    c.maxit = maxit;
+    c.flag = 0;
+    c.done = (!(c.n2b > 0.0) || c.normr <= c.tolb || maxit <= 0) ? 1 : 0;
+    if (!isfinite(c.n2b)) c.flag = -1;
+    if (maxit <= 0 && c.done && c.n2b > 0.0 && c.normr > c.tolb) c.flag = 1;
+    st[g] = c;
+  }
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_cg_alpha(CgState* st, const double* partials,
+                                                          int nb) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (st[g].done) return;
+  const double pq = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    st[g].pq = pq;
+    const double alpha = st[g].rho / pq;
+    if (!(pq > 0.0) || isinf(pq) || isinf(alpha) || isnan(alpha)) {
+      st[g].flag = 4;
+      st[g].done = 1;
+    } else {
+      st[g].alpha = alpha;
+    }
+  }
+}
+
+// x += alpha p; r -= alpha q; partials of ||r||^2
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_cg_update(cpx<T>* x, const cpx<T>* p, cpx<T>* r,
+                                                           const cpx<T>* q, long long ld,
+                                                           long long n, const CgState* st,
+                                                           double* partials) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.y;
+  if (st[g].done) return;
+  const T alpha = static_cast<T>(st[g].alpha);
+  cpx<T>* xg = x + g * ld;
+  const cpx<T>* pg = p + g * ld;
+  cpx<T>* rg = r + g * ld;
+  const cpx<T>* qg = q + g * ld;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    xg[i] = xg[i] + alpha * pg[i];
+    const cpx<T> rn = rg[i] - alpha * qg[i];
+    rg[i] = rn;
+    s += static_cast<double>(rn.re) * rn.re + static_cast<double>(rn.im) * rn.im;
+  }
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) partials[(long long)g * gridDim.x + blockIdx.x] = s;
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, const double* partials,
+                                                         int nb) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (st[g].done) return;
+  const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    CgState c = st[g];
+    c.iter += 1;
+    c.normr = sqrt(rho_new);
+    if (!isfinite(rho_new)) {
+      c.flag = -1;
+      c.done = 1;
+    } else if (c.normr <= c.tolb || c.check_every) {
+      c.flag = 0;  // provisional: the host checks the true residual
+      c.done = 1;
+    } else if (c.iter >= c.maxit) {
+      c.flag = 1;
+      c.done = 1;
+    } else {
+      const double beta = rho_new / c.rho;
+      if (beta == 0.0 || isinf(beta) || rho_new == 0.0) {
+        c.flag = 4;
+        c.done = 1;
+      }
+      c.beta = beta;
+    }
+    c.rho_prev = c.rho;
+    c.rho = rho_new;
+    st[g] = c;
+  }
+}
+
+// After a failed true-residual check the iteration continues from r = b - A x
+// (MATLAB pcg "moresteps"): beta uses the true residual, and every later
+// iteration is checked against the true residual as well.
+static __global__ __launch_bounds__(kRedThreads) void k_cg_resume(CgState* st, const double* partials,
+                                                           int nb, const int* resume) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (!resume[g]) return;
+  const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    st[g].beta = rho_new / st[g].rho_prev;
+    st[g].rho = rho_new;
+    st[g].normr = sqrt(rho_new);
+    st[g].done = (st[g].iter >= st[g].maxit) ? 1 : 0;
+    st[g].flag = st[g].done ? 1 : 0;
+    st[g].check_every = 1;
+  }
+}
+
+// ------------------------------------------------------------------ vectors
+template <class T>
+__global__ void k_axpby(long long n, cpx<double> a, const cpx<T>* x, cpx<double> b,
+                        const cpx<T>* y, cpx<T>* z) {
+  // z = a*x + b*y
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    z[i] = cvt<T>(a * cvt<double>(x[i]) + b * cvt<double>(y[i]));
+  }
+}
+
+template <class To, class Ti>
+__global__ void k_convert(long long n, const cpx<Ti>* x, double scale, cpx<To>* y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cpx<Ti> v = x[i];
+    y[i] = mk(static_cast<To>(scale * v.re), static_cast<To>(scale * v.im));
+  }
+}
+
+// acc += scale * x  (x possibly lower precision)
+template <class Ti>
+__global__ void k_accumulate(long long n, const cpx<Ti>* x, double scale, cpx<double>* acc) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    acc[i] = acc[i] + scale * cvt<double>(x[i]);
+  }
+}
+
+template <class T>
+__global__ void k_fill_zero(long long n, cpx<T>* x) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] = mk<T>(0, 0);
+  }
+}
+
+template <class T>
+__global__ void k_fill_zero_real(long long n, T* x) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    x[i] = T(0);
+  }
+}
+
+// ------------------------------------------------------------------ Lanczos basis
+// Partials of h_k = Q(:,k)' r for k in [k0, k0+KB) (basis columns contiguous, ld = n).
+template <int KB>
+__global__ __launch_bounds__(kRedThreads) void k_gemv_c_partials(const cpx<double>* Q, long long ld,
+                                                                 int k0, int kcount,
+                                                                 const cpx<double>* r,
+                                                                 long long n, double* pre,
+                                                                 double* pim) {
+  __shared__ double red[kRedThreads];
+  double ar[KB], ai[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) ar[k] = ai[k] = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const cpx<double> v = r[i];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      if (k < kcount) {
+        const cpx<double> q = Q[(long long)(k0 + k) * ld + i];
+        ar[k] += q.re * v.re + q.im * v.im;
+        ai[k] += q.re * v.im - q.im * v.re;
+      }
+    }
+  }
+  for (int k = 0; k < kcount; ++k) {
+    const double s = block_sum<kRedThreads>(ar[k], red);
+    __syncthreads();
+    const double t = block_sum<kRedThreads>(ai[k], red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      pre[(long long)(k0 + k) * gridDim.x + blockIdx.x] = s;
+      pim[(long long)(k0 + k) * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// r -= Q(:, 0:j) h   (h complex, on device)
+static __global__ void k_gemv_n_sub(const cpx<double>* Q, long long ld, int j, const double* hre,
+                             const double* him, cpx<double>* r, long long n) {
+  __shared__ cpx<double> hs[128];
+  for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);
+  __syncthreads();
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    cpx<double> acc = r[i];
+    for (int k = 0; k < j; ++k) acc = acc - Q[(long long)k * ld + i] * hs[k];
+    r[i] = acc;
+  }
+}
+
+// In-place restart Q(:, 0:kk) = Q(:, 0:m) U with U (m x kk, column-major, real).
+// Each workgroup owns kRows rows and stages them in LDS.
+constexpr int kRestartRows = 32;
+static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long long ld, long long n, int m,
+                                                 int kk, const double* U) {
+  __shared__ cpx<double> tile[kRestartRows * 64];
+  __shared__ double Us[64 * 64];
+  for (int t = threadIdx.x; t < m * kk; t += blockDim.x) Us[t] = U[t];
+  const long long r0 = (long long)blockIdx.x * kRestartRows;
+  for (int t = threadIdx.x; t < kRestartRows * m; t += blockDim.x) {
+    const int rr = t % kRestartRows, c = t / kRestartRows;
+    const long long i = r0 + rr;
+    tile[c * kRestartRows + rr] = (i < n) ? Q[(long long)c * ld + i] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kRestartRows * kk; t += blockDim.x) {
+    const int rr = t % kRestartRows, c = t / kRestartRows;
+    const long long i = r0 + rr;
+    cpx<double> acc = mk(0.0, 0.0);
+    for (int l = 0; l < m; ++l) acc = acc + Us[c * m + l] * tile[l * kRestartRows + rr];
+    if (i < n) Q[(long long)c * ld + i] = acc;
+  }
+}
+
+}  // namespace fame6d

@@ -1,0 +1,581 @@
+// Thick-restart Lanczos for the largest eigenvalues theta of K_r^{-1}
+// (lambda = 1/theta), eigenvector recovery and Fourier diagnostics.
+//
+// Restates Lanczos_hybrid_identity_gpuop (exp1/Lanczos_hybrid_identity_gpuop.m:11-130):
+// the recurrence, the extra projections, two full re-orthogonalisation passes
+// per step, the projected matrix H = [diag(d) c'; c tridiag], the Ritz residual
+// |beta * U(end,:)|, the convergence threshold tol*max(eps^(2/3), |theta|) and
+// the restart size k0 + min(nconv, floor((p-k0)/2)).  The basis stays on the
+// device; each re-orthogonalisation pass is classical Gram-Schmidt (one
+// Q^H r product and one Q h update) instead of the reference's column loop,
+// which is the same projection up to rounding.  Several Bloch points step
+// together and share one batched K_r^{-1} per step.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "internal.hpp"
+#include "krylov_kernels.hpp"
+
+namespace fame6d {
+
+// y += c * x with c = c_re[0] + i*c_im[0] read from device memory, times scale.
+__global__ void k_axpy_dev(long long n, cpx<double>* y, const cpx<double>* x, const double* c_re,
+                           const double* c_im, double scale) {
+  const cpx<double> c = mk(scale * c_re[0], c_im ? scale * c_im[0] : 0.0);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = y[i] + c * x[i];
+}
+
+// y = x * (1/sqrt(s[0]))
+__global__ void k_scale_rsqrt_dev(long long n, cpx<double>* y, const cpx<double>* x, const double* s) {
+  const double f = 1.0 / sqrt(s[0]);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = f * x[i];
+}
+
+// Kop: y = K x with K_xi = |q|^2 I - q q^T (assemble_K_and_nullspace_free_basis.m:110-119)
+__global__ void k_apply_k(const double* q1, const double* q2, const double* q3, const cpx<double>* x,
+                          cpx<double>* y, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double a = q1[i], b = q2[i], c = q3[i];
+    const cpx<double> x1 = x[i], x2 = x[n + i], x3 = x[2 * n + i];
+    y[i] = (b * b + c * c) * x1 - (a * b) * x2 - (a * c) * x3;
+    y[n + i] = (a * a + c * c) * x2 - (a * b) * x1 - (b * c) * x3;
+    y[2 * n + i] = (a * a + b * b) * x3 - (a * c) * x1 - (b * c) * x2;
+  }
+}
+
+// D0 .* (U0^* x) = sum_c q_c x_c  (compute_fourier_diagnostics_exp2.m:16-17; real q)
+__global__ void k_div(const double* q1, const double* q2, const double* q3, const cpx<double>* x,
+                      cpx<double>* y, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = q1[i] * x[i] + q2[i] * x[n + i] + q3[i] * x[2 * n + i];
+}
+
+// ------------------------------------------------------------------ small dense eig
+// Cyclic Jacobi for a real symmetric m x m matrix (column-major); eigenvectors
+// in the columns of V.
+static void jacobi_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
+  auto a = [&](int r, int c) -> double& { return A[(size_t)c * m + r]; };
+  double fro = 0;
+  for (double v : A) fro += v * v;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0;
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < m; ++r)
+        if (r != c) off += a(r, c) * a(r, c);
+    if (off <= 1e-32 * fro || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p) {
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = a(p, q);
+        if (apq == 0.0) continue;
+        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; ++k) {  // columns p, q
+          const double akp = a(k, p), akq = a(k, q);
+          a(k, p) = c * akp - s * akq;
+          a(k, q) = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {  // rows p, q
+          const double apk = a(p, k), aqk = a(q, k);
+          a(p, k) = c * apk - s * aqk;
+          a(q, k) = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double vkp = V[(size_t)p * m + k], vkq = V[(size_t)q * m + k];
+          V[(size_t)p * m + k] = c * vkp - s * vkq;
+          V[(size_t)q * m + k] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  w.resize(m);
+  for (int i = 0; i < m; ++i) w[i] = a(i, i);
+}
+
+// ------------------------------------------------------------------ device basis ops
+struct Basis {
+  cpx<double>* Q = nullptr;  // n x p
+  cpx<double>* v = nullptr;  // next Lanczos vector
+  cpx<double>* r = nullptr;  // residual work
+  double* sc = nullptr;      // device scalars: [0] alpha, [1] beta2, [2..3] corr, [8..] h_re, [136..] h_im
+  double* part = nullptr;    // partials
+};
+
+constexpr int kScH = 8, kScHim = 8 + 128;
+
+// dst_re/dst_im = sum(conj(a) .* b)
+static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, double* part,
+                    double* dst_re, double* dst_im, hipStream_t s) {
+  const int nb = red_blocks(n);
+  double* pim = part + nb;
+  k_dot_partials<double><<<dim3(nb, 1), kRedThreads, 0, s>>>(a, n, b, n, n, part, pim);
+  k_reduce_final<<<1, kRedThreads, 0, s>>>(part, nb, dst_re);
+  if (dst_im) k_reduce_final<<<1, kRedThreads, 0, s>>>(pim, nb, dst_im);
+  F6_CHECK_LAUNCH();
+}
+
+// r -= Q(:,0:j) (Q(:,0:j)^H r)  -- one classical Gram-Schmidt pass over j columns
+static void cgs_pass(long long n, const Basis& B, int j, hipStream_t s) {
+  const int nb = red_blocks(n);
+  double* pre = B.part;
+  double* pim = B.part + (size_t)nb * 128;
+  constexpr int KB = 8;
+  for (int k0 = 0; k0 < j; k0 += KB) {
+    const int kc = std::min(KB, j - k0);
+    k_gemv_c_partials<KB><<<nb, kRedThreads, 0, s>>>(B.Q, n, k0, kc, B.r, n, pre, pim);
+  }
+  k_reduce_final<<<j, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH);
+  k_reduce_final<<<j, kRedThreads, 0, s>>>(pim, nb, B.sc + kScHim);
+  k_gemv_n_sub<<<grid_for(n), 256, 0, s>>>(B.Q, n, j, B.sc + kScH, B.sc + kScHim, B.r, n);
+  F6_CHECK_LAUNCH();
+}
+
+// Host-side state of one Lanczos run.
+struct LzState {
+  int slot = 0;
+  Basis B;
+  std::vector<double> Alpha, Beta, d, c, Us, res;
+  std::vector<int> ind;
+  double beta = 0;
+  bool justRestarted = false;
+  int sizeV = 1, jj = 1, k = 0, kk = 0, mm = 1, qcols = 0;
+  bool done = false, converged = false, hitmax = false, breakdown = false;
+  int nconv = 0;
+  double maxres = 0;
+  std::vector<double> theta;
+};
+
+// Builds H, solves it, tests convergence and restarts (Lanczos_hybrid_identity_gpuop.m:79-120).
+// Returns true when the run is finished.
+static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, long long n2,
+                         DevBuf& ubuf, hipStream_t s) {
+  const int nd = (int)L.d.size(), na = (int)L.Alpha.size();
+  const int m = nd + na;
+  if (m == 0) throw Error(FAME6D_ENOCONV, "Lanczos produced no projected data.");
+  std::vector<double> H((size_t)m * m, 0.0);
+  for (int i = 0; i < nd; ++i) H[(size_t)i * m + i] = L.d[i];
+  for (int i = 0; i < na; ++i) H[(size_t)(nd + i) * m + nd + i] = L.Alpha[i];
+  for (int i = 0; i + 1 < na; ++i) {
+    H[(size_t)(nd + i + 1) * m + nd + i] = L.Beta[i];
+    H[(size_t)(nd + i) * m + nd + i + 1] = L.Beta[i];
+  }
+  if (nd > 0) {
+    for (int i = 0; i < L.k; ++i) {
+      H[(size_t)L.k * m + i] = L.c[i];  // H(1:k, k+1) = c'
+      H[(size_t)i * m + L.k] = L.c[i];  // H(k+1, 1:k) = c
+    }
+  }
+  L.Alpha.clear();
+  L.Beta.clear();
+  std::vector<double> w, U;
+  jacobi_eig(m, H, w, U);
+  std::vector<int> ind(m);
+  std::iota(ind.begin(), ind.end(), 0);
+  std::stable_sort(ind.begin(), ind.end(), [&](int a, int b) { return w[a] > w[b]; });
+  std::vector<double> ds(m), res(m);
+  for (int i = 0; i < m; ++i) {
+    ds[i] = w[ind[i]];
+    res[i] = std::fabs(L.beta * U[(size_t)ind[i] * m + (m - 1)]);
+  }
+  L.Us = U;
+  L.ind = ind;
+  L.res = res;
+  L.qcols = m;
+  const int nw = std::min(k0, m);
+  const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+  int nconv = 0;
+  double mx = 0;
+  for (int i = 0; i < nw; ++i) {
+    const double thr = tol * std::max(eps23, std::fabs(ds[i]));
+    if (res[i] < thr) ++nconv;
+    mx = std::max(mx, res[i]);
+  }
+  L.nconv = nconv;
+  L.maxres = mx;
+  L.d = ds;
+  if (nconv >= k0) {
+    L.converged = true;
+    return true;
+  }
+  if (L.mm >= maxRestart) {
+    L.hitmax = true;
+    return true;
+  }
+  int k = k0 + std::min(nconv, (p - k0) / 2);
+  if (k == 1 && p > 3) k = p / 2;
+  if (k > m) k = m;
+  // Q(:,1:k) = Q(:,1:m) * Ukeep ; d = d(1:k) ; c = beta * Ukeep(end,:)
+  std::vector<double> Uk((size_t)m * k);
+  L.c.assign(k, 0.0);
+  for (int j = 0; j < k; ++j) {
+    for (int i = 0; i < m; ++i) Uk[(size_t)j * m + i] = U[(size_t)ind[j] * m + i];
+    L.c[j] = L.beta * U[(size_t)ind[j] * m + (m - 1)];
+  }
+  ubuf.ensure(sizeof(double) * Uk.size());
+  F6_HIP(hipMemcpyAsync(ubuf.ptr, Uk.data(), sizeof(double) * Uk.size(), hipMemcpyHostToDevice, s));
+  const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
+  k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n2, n2, m, k, ubuf.as<double>());
+  F6_CHECK_LAUNCH();
+  F6_HIP(hipStreamSynchronize(s));  // ubuf is reused by the next restart
+  L.d.resize(k);
+  L.k = k;
+  L.justRestarted = true;
+  L.sizeV = k + 1;
+  L.jj = k + 1;
+  L.mm += 1;
+  return false;
+}
+
+// Eigensolve of all G Bloch points of S.  v0: G x 2NF host start vectors.
+void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
+                   double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
+                   fame6d_lanczos_stats_t* stats) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const int G = S->G;
+  const long long NF = m->NF, n2 = 2 * NF;
+  if (k0 < 1 || p <= k0 || p > 64) throw Error(FAME6D_EARG, "need 1 <= k < p <= 64");
+  if (G > 64) throw Error(FAME6D_EARG, "at most 64 Bloch points");
+  if (precision != FAME6D_FP64 && precision != FAME6D_MIXED)
+    throw Error(FAME6D_EARG, "eigensolver precision must be FAME6D_FP64 or FAME6D_MIXED");
+  const auto t0 = std::chrono::steady_clock::now();
+  const fame6d_krinv_stats_t st0 = S->stats;
+
+  // device memory: per group Q (n2 x p) + v + r; scalars; partials; packed in/out
+  const int nb = red_blocks(n2);
+  DevBuf basis, scal, part, io, ubuf;
+  basis.ensure(sizeof(cpx<double>) * (size_t)n2 * (p + 2) * G);
+  scal.ensure(sizeof(double) * 512 * G);
+  part.ensure(sizeof(double) * (size_t)nb * 256 * G);
+  io.ensure(sizeof(cpx<double>) * (size_t)n2 * 2 * G);
+  std::vector<LzState> L(G);
+  for (int g = 0; g < G; ++g) {
+    L[g].slot = g;
+    cpx<double>* b0 = basis.as<cpx<double>>() + (size_t)n2 * (p + 2) * g;
+    L[g].B.Q = b0;
+    L[g].B.v = b0 + (size_t)n2 * p;
+    L[g].B.r = b0 + (size_t)n2 * (p + 1);
+    L[g].B.sc = scal.as<double>() + 512 * g;
+    L[g].B.part = part.as<double>() + (size_t)nb * 256 * g;
+    L[g].k = k0;
+  }
+  cpx<double>* qin = io.as<cpx<double>>();
+  cpx<double>* qout = qin + (size_t)n2 * G;
+  const int gr = grid_for(n2);
+
+  // v = v0 / ||v0||  (Lanczos_hybrid_identity_gpuop.m:14-16)
+  for (int g = 0; g < G; ++g) {
+    F6_HIP(hipMemcpyAsync(L[g].B.v, v0 + (size_t)n2 * g, sizeof(cpx<double>) * n2, hipMemcpyHostToDevice, s));
+    dot_dev(n2, L[g].B.v, L[g].B.v, L[g].B.part, L[g].B.sc + 1, nullptr, s);
+  }
+  {
+    std::vector<double> nv(G);
+    for (int g = 0; g < G; ++g)
+      F6_HIP(hipMemcpyAsync(&nv[g], L[g].B.sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    F6_HIP(hipStreamSynchronize(s));
+    for (int g = 0; g < G; ++g) {
+      if (!(std::isfinite(nv[g]) && nv[g] > 0)) throw Error(FAME6D_EARG, "Invalid Lanczos start vector.");
+      k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, L[g].B.v, L[g].B.v, L[g].B.sc + 1);
+    }
+    F6_CHECK_LAUNCH();
+  }
+
+  int steps = 0;
+  while (true) {
+    std::vector<int> act;
+    for (int g = 0; g < G; ++g)
+      if (!L[g].done) act.push_back(g);
+    if (act.empty()) break;
+    const int na = (int)act.size();
+    std::vector<int> slots(na);
+    for (int i = 0; i < na; ++i) {
+      LzState& Lg = L[act[i]];
+      cpx<double>* Qj = Lg.B.Q + (size_t)n2 * (Lg.jj - 1);
+      F6_HIP(hipMemcpyAsync(Qj, Lg.B.v, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      slots[i] = Lg.slot;
+    }
+    krinv_groups(S, precision, na, slots.data(), qin, qout, tol_cg, maxit_cg, nullptr);
+    ++steps;
+    for (int i = 0; i < na; ++i) {
+      LzState& Lg = L[act[i]];
+      const Basis& B = Lg.B;
+      const int j = Lg.jj - 1;
+      const cpx<double>* w = qout + (size_t)n2 * i;
+      cpx<double>* Qj = B.Q + (size_t)n2 * j;
+      // alpha = real(Aq' * Q(:,jj))
+      dot_dev(n2, Qj, w, B.part, B.sc + 0, nullptr, s);
+      F6_HIP(hipMemcpyAsync(B.r, w, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      if (Lg.jj == 1) {
+        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 0, nullptr, -1.0);
+      } else if (Lg.justRestarted) {
+        cgs_pass(n2, B, Lg.jj, s);
+        Lg.justRestarted = false;
+      } else {
+        const cpx<double>* Qm1 = B.Q + (size_t)n2 * (j - 1);
+        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 0, nullptr, -1.0);
+        k_axpby<double><<<gr, 256, 0, s>>>(n2, mk(1.0, 0.0), B.r, mk(-Lg.beta, 0.0), Qm1, B.r);
+        dot_dev(n2, Qm1, B.r, B.part, B.sc + 2, B.sc + 3, s);
+        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qm1, B.sc + 2, B.sc + 3, -1.0);
+      }
+      dot_dev(n2, Qj, B.r, B.part, B.sc + 2, B.sc + 3, s);
+      k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 2, B.sc + 3, -1.0);
+      cgs_pass(n2, B, Lg.jj, s);
+      cgs_pass(n2, B, Lg.jj, s);
+      dot_dev(n2, B.r, B.r, B.part, B.sc + 1, nullptr, s);
+      F6_CHECK_LAUNCH();
+    }
+    std::vector<double> ab(2 * na);
+    for (int i = 0; i < na; ++i)
+      F6_HIP(hipMemcpyAsync(&ab[2 * i], L[act[i]].B.sc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    F6_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < na; ++i) {
+      LzState& Lg = L[act[i]];
+      const double alpha = ab[2 * i], beta2 = ab[2 * i + 1];
+      if (!std::isfinite(alpha) || !std::isfinite(beta2))
+        throw Error(FAME6D_ENONFINITE, "Lanczos inverse action returned invalid data.");
+      bool end_cycle = false;
+      if (beta2 <= 0 || std::sqrt(beta2) < 1e-14) {
+        Lg.beta = std::sqrt(std::max(beta2, 0.0));
+        Lg.breakdown = true;
+        end_cycle = true;
+      } else {
+        Lg.beta = std::sqrt(beta2);
+        k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, Lg.B.v, Lg.B.r, Lg.B.sc + 1);
+        F6_CHECK_LAUNCH();
+        Lg.Alpha.push_back(alpha);
+        Lg.Beta.push_back(Lg.beta);
+        Lg.kk += 1;
+        if (Lg.jj >= p) end_cycle = true;
+        else Lg.jj += 1;
+      }
+      if (end_cycle) {
+        if (restart_step(Lg, k0, p, tol, maxRestart, n2, ubuf, s)) Lg.done = true;
+      }
+    }
+  }
+
+  // outputs: theta = d(1:k0) (descending), lambda = sort(1./theta) ascending;
+  // Ritz vectors Y(:,i) = Q(:,1:qcols) * U(:, ind(i)) in the same order as lambda.
+  for (int g = 0; g < G; ++g) {
+    LzState& Lg = L[g];
+    if ((int)Lg.d.size() < k0) throw Error(FAME6D_ENOCONV, "Lanczos ended without k inverse Ritz values.");
+    std::vector<std::pair<double, int>> lam(k0);
+    for (int i = 0; i < k0; ++i) {
+      const double th = Lg.d[i];
+      if (!std::isfinite(th) || th <= 0) throw Error(FAME6D_ENOCONV, "Method C invalid inverse Ritz values.");
+      lam[i] = {1.0 / th, i};
+    }
+    std::stable_sort(lam.begin(), lam.end());
+    for (int i = 0; i < k0; ++i) lambda[(size_t)g * k0 + i] = lam[i].first;
+    if (Y) {
+      const int mq = Lg.qcols;
+      std::vector<double> Uy((size_t)mq * k0);
+      for (int i = 0; i < k0; ++i) {
+        const int col = Lg.ind[lam[i].second];
+        for (int r = 0; r < mq; ++r) Uy[(size_t)i * mq + r] = Lg.Us[(size_t)col * mq + r];
+      }
+      ubuf.ensure(sizeof(double) * Uy.size());
+      F6_HIP(hipMemcpyAsync(ubuf.ptr, Uy.data(), sizeof(double) * Uy.size(), hipMemcpyHostToDevice, s));
+      // out-of-place product through a k0-column staging area inside the basis
+      const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
+      k_restart<<<blocks, 256, 0, s>>>(Lg.B.Q, n2, n2, mq, k0, ubuf.as<double>());
+      F6_CHECK_LAUNCH();
+      F6_HIP(hipMemcpyAsync(Y + (size_t)n2 * k0 * g, Lg.B.Q, sizeof(cpx<double>) * n2 * k0,
+                            hipMemcpyDeviceToDevice, s));
+      F6_HIP(hipStreamSynchronize(s));
+    }
+    if (stats) {
+      fame6d_lanczos_stats_t& o = stats[g];
+      o.converged = Lg.converged ? 1 : 0;
+      o.num_converged = Lg.nconv;
+      o.outer_steps = Lg.kk;
+      o.restarts = Lg.mm;
+      o.hit_max_restart = Lg.hitmax ? 1 : 0;
+      o.max_ritz_residual = Lg.maxres;
+    }
+  }
+  F6_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    fame6d_krinv_stats_t d = S->stats;
+    d.calls -= st0.calls;
+    d.total_iterations -= st0.total_iterations;
+    d.total_fp64_iterations -= st0.total_fp64_iterations;
+    d.refinement_cycles -= st0.refinement_cycles;
+    d.mass_applies_fp64 -= st0.mass_applies_fp64;
+    d.mass_applies_fp32 -= st0.mass_applies_fp32;
+    d.seconds -= st0.seconds;
+    for (int g = 0; g < G; ++g) {
+      stats[g].seconds = secs;
+      stats[g].krinv = d;
+    }
+  }
+  basis.release();
+  scal.release();
+  part.release();
+  io.release();
+  ubuf.release();
+  for (int g = 0; g < G; ++g)
+    if (!L[g].converged)
+      throw Error(FAME6D_ENOCONV, "Method C outer Lanczos did not converge: " +
+                                      std::to_string(L[g].nconv) + "/" + std::to_string(k0) + ".");
+}
+
+// ------------------------------------------------------------------ recovery
+// pcg on M = I3 (x) M_s for one 3NF right-hand side (host-stepped MATLAB pcg).
+static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cpx<double>* r,
+                        cpx<double>* p, cpx<double>* q, double tol, int maxit) {
+  hipStream_t s = m->stream;
+  const long long NF = m->NF, n3 = 3 * NF;
+  const int gr = grid_for(n3);
+  auto dot = [&](const cpx<double>* a, const cpx<double>* c) {
+    double re;
+    dot_groups<double>(m, 1, a, n3, c, n3, n3, &re, nullptr);
+    return re;
+  };
+  PcgOut o;
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, x);
+  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+  const double n2b = std::sqrt(dot(b, b));
+  if (!std::isfinite(n2b)) throw Error(FAME6D_ENONFINITE, "recovery right-hand side contains NaN/Inf");
+  if (n2b == 0) return o;
+  const double tolb = tol * n2b;
+  double rho = 1, normr = n2b;
+  int moresteps = 0;
+  const int maxmsteps = (int)std::min<long long>({NF * 3 / 50, 5, std::max<long long>(n3 - maxit, 0)});
+  o.flag = 1;
+  for (int ii = 1; ii <= maxit; ++ii) {
+    const double rho1 = rho;
+    rho = dot(r, r);
+    if (rho == 0 || std::isinf(rho)) {
+      o.flag = 4;
+      o.iter = ii - 1;
+      break;
+    }
+    if (ii == 1) {
+      F6_HIP(hipMemcpyAsync(p, r, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+    } else {
+      const double beta = rho / rho1;
+      k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), r, mk(beta, 0.0), p, p);
+    }
+    mass_apply<double>(m, p, NF, q, NF, 3, s);
+    const double pq = dot(p, q);
+    if (!(pq > 0) || std::isinf(pq)) {
+      o.flag = 4;
+      o.iter = ii - 1;
+      break;
+    }
+    const double alpha = rho / pq;
+    k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), x, mk(alpha, 0.0), p, x);
+    k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), r, mk(-alpha, 0.0), q, r);
+    normr = std::sqrt(dot(r, r));
+    o.iter = ii;
+    if (normr <= tolb || moresteps) {
+      mass_apply<double>(m, x, NF, q, NF, 3, s);
+      k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), b, mk(-1.0, 0.0), q, r);
+      const double act = std::sqrt(dot(r, r));
+      normr = act;
+      if (act <= tolb) {
+        o.flag = 0;
+        break;
+      }
+      moresteps += 1;
+      if (moresteps >= maxmsteps) {
+        o.flag = 3;
+        break;
+      }
+    }
+  }
+  o.relres = normr / n2b;
+  return o;
+}
+
+void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double tol, int maxit,
+                          cpx<double>* X, cpx<double>* MX, fame6d_recover_stats_t* stats) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF, n3 = 3 * NF;
+  if (k < 0 || k > 64) throw Error(FAME6D_EARG, "k must be in [0, 64]");
+  DevBuf ws;
+  ws.ensure(sizeof(cpx<double>) * (size_t)n3 * 4);
+  cpx<double>* rhs = ws.as<cpx<double>>();
+  cpx<double>* r = rhs + n3;
+  cpx<double>* p = rhs + 2 * n3;
+  cpx<double>* q = rhs + 3 * n3;
+  const int gr = grid_for(NF);
+  for (int j = 0; j < k; ++j) {
+    k_apply_v<<<dim3(gr, 1), 256, 0, s>>>(Y + (size_t)2 * NF * j, 2 * NF, NF, S->V(0), S->V(1), S->V(2),
+                                          S->V(3), S->V(4), S->V(5), rhs, n3, no_skip(), nullptr);
+    F6_CHECK_LAUNCH();
+    cpx<double>* xj = X + (size_t)n3 * j;
+    cpx<double>* mxj = MX + (size_t)n3 * j;
+    PcgOut o = pcg_mass3(m, rhs, xj, r, p, q, tol, maxit);
+    if (!std::isfinite(o.relres) || (o.flag != 0 && o.relres > 10 * tol))
+      throw Error(FAME6D_ENOCONV, "Recovery CG failed for eigenvector " + std::to_string(j + 1));
+    mass_apply<double>(m, xj, NF, mxj, NF, 3, s);
+    double nj2;
+    dot_groups<double>(m, 1, xj, n3, mxj, n3, n3, &nj2, nullptr);
+    if (!std::isfinite(nj2) || nj2 <= 0)
+      throw Error(FAME6D_EMASSNORM, "Nonpositive/nonfinite M-norm for eigenvector " + std::to_string(j + 1));
+    const double sc = 1.0 / std::sqrt(nj2);
+    k_axpby<double><<<grid_for(n3), 256, 0, s>>>(n3, mk(sc, 0.0), xj, mk(0.0, 0.0), xj, xj);
+    k_axpby<double><<<grid_for(n3), 256, 0, s>>>(n3, mk(sc, 0.0), mxj, mk(0.0, 0.0), mxj, mxj);
+    F6_CHECK_LAUNCH();
+    if (stats) {
+      stats->iterations[j] = o.iter;
+      stats->relres[j] = o.relres;
+      stats->mass_norm2[j] = nj2;
+    }
+  }
+  F6_HIP(hipStreamSynchronize(s));
+  ws.release();
+}
+
+void fourier_diagnostics(fame6d_qsys* S, const cpx<double>* X, const cpx<double>* MX,
+                         const double* lambda, int k, double* res, double* div, double* ray) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF, n3 = 3 * NF;
+  DevBuf ws;
+  ws.ensure(sizeof(cpx<double>) * (size_t)n3 * 2);
+  cpx<double>* Kx = ws.as<cpx<double>>();
+  cpx<double>* t = Kx + n3;
+  for (int j = 0; j < k; ++j) {
+    const cpx<double>* x = X + (size_t)n3 * j;
+    const cpx<double>* mx = MX + (size_t)n3 * j;
+    k_apply_k<<<grid_for(NF), 256, 0, s>>>(S->Qm(0), S->Qm(1), S->Qm(2), x, Kx, NF);
+    F6_CHECK_LAUNCH();
+    double xKx, xMx, nK, nM;
+    dot_groups<double>(m, 1, x, n3, Kx, n3, n3, &xKx, nullptr);
+    dot_groups<double>(m, 1, x, n3, mx, n3, n3, &xMx, nullptr);
+    dot_groups<double>(m, 1, Kx, n3, Kx, n3, n3, &nK, nullptr);
+    dot_groups<double>(m, 1, mx, n3, mx, n3, n3, &nM, nullptr);
+    nK = std::sqrt(nK);
+    nM = std::sqrt(nM);
+    if (ray) ray[j] = xKx / xMx;
+    k_axpby<double><<<grid_for(n3), 256, 0, s>>>(n3, mk(1.0, 0.0), Kx, mk(-lambda[j], 0.0), mx, t);
+    double nr;
+    dot_groups<double>(m, 1, t, n3, t, n3, n3, &nr, nullptr);
+    const double denom = nK + std::fabs(lambda[j]) * nM;
+    if (res) res[j] = std::sqrt(nr) / std::max(denom, 2.220446049250313e-16);
+    k_div<<<grid_for(NF), 256, 0, s>>>(S->Qm(0), S->Qm(1), S->Qm(2), mx, t, NF);
+    F6_CHECK_LAUNCH();
+    double nd;
+    dot_groups<double>(m, 1, t, NF, t, NF, NF, &nd, nullptr);
+    if (div) div[j] = std::sqrt(nd) / std::max(nM, 2.220446049250313e-16);
+  }
+  ws.release();
+}
+
+}  // namespace fame6d

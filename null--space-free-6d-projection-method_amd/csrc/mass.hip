@@ -1,0 +1,190 @@
+// Instantiation and launch of the three mass-operator passes for every grid
+// size nGrid = 2N (N = 1..6) in FP64 and FP32, with the tile variants the
+// autotuner chooses between.
+#include "internal.hpp"
+
+#include <type_traits>
+
+namespace fame6d {
+
+// Tile variant tables.  Pass A/C: BPW = S-blocks per workgroup (threads = BPW*n^2).
+// Pass B: A = consecutive inner positions per workgroup (threads = A*n^2,
+// LDS = A*n^3 complex values).  Every entry divides S = n^3.
+template <int n, class T>
+struct Tiles;
+#define F6_TILES(N_, T_, BPW0, BPW1, A0, A1)                 \
+  template <>                                                 \
+  struct Tiles<N_, T_> {                                      \
+    static constexpr int bpw[2] = {BPW0, BPW1};               \
+    static constexpr int a[2] = {A0, A1};                     \
+  };
+F6_TILES(2, double, 8, 8, 8, 8)
+F6_TILES(2, float, 8, 8, 8, 8)
+F6_TILES(4, double, 16, 8, 16, 32)
+F6_TILES(4, float, 16, 8, 16, 32)
+F6_TILES(6, double, 8, 6, 8, 24)
+F6_TILES(6, float, 8, 6, 8, 24)
+F6_TILES(8, double, 4, 8, 8, 4)
+F6_TILES(8, float, 4, 8, 8, 16)
+F6_TILES(10, double, 4, 2, 4, 8)
+F6_TILES(10, float, 4, 2, 8, 4)
+F6_TILES(12, double, 2, 2, 4, 2)
+F6_TILES(12, float, 2, 2, 4, 6)
+#undef F6_TILES
+
+template <class F>
+void with_n(int n, F&& f) {
+  switch (n) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 10: f(std::integral_constant<int, 10>{}); break;
+    case 12: f(std::integral_constant<int, 12>{}); break;
+    default: throw Error(FAME6D_EARG, "unsupported grid size nGrid=" + std::to_string(n));
+  }
+}
+
+template <class T>
+static const TileChoice& tile_of(const fame6d_mass* m) {
+  if constexpr (std::is_same<T, double>::value) return m->tile64;
+  else return m->tile32;
+}
+
+template <class T>
+void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>& a, hipStream_t s) {
+  if (ngroups <= 0) return;
+  const int v = tile_of<T>(m).a;
+  with_n(m->n, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    constexpr int S = n * n * n;
+    auto go = [&](auto bc) {
+      constexpr int BPW = decltype(bc)::value;
+      dim3 grid(S / BPW, ngroups), block(BPW * n * n);
+      switch (op) {
+        case PRE_COPY: k_pass_a<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
+        case PRE_MHAT: k_pass_a<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
+        case PRE_MHAT_CG: k_pass_a<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
+        default: throw Error(FAME6D_EARG, "bad pass-A op");
+      }
+    };
+    if (v == 0) go(std::integral_constant<int, Tiles<n, T>::bpw[0]>{});
+    else go(std::integral_constant<int, Tiles<n, T>::bpw[1]>{});
+  });
+  F6_CHECK_LAUNCH();
+}
+
+template <class T>
+void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
+                   Skip skip, hipStream_t s) {
+  if (ngroups <= 0) return;
+  PassBArgs<T> a{t, ldt, (const T*)(std::is_same<T, double>::value ? m->eps64.ptr : m->eps32.ptr), skip};
+  const int v = tile_of<T>(m).b;
+  with_n(m->n, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    constexpr int S = n * n * n;
+    auto go = [&](auto ac) {
+      constexpr int A = decltype(ac)::value;
+      dim3 grid(S / A, ngroups), block(A * n * n);
+      if (mode == 0 && nr == 1) k_pass_b<T, n, A, 1, 0><<<grid, block, 0, s>>>(a);
+      else if (mode == 0 && nr == 3) k_pass_b<T, n, A, 3, 0><<<grid, block, 0, s>>>(a);
+      else if (mode == 1 && nr == 1) k_pass_b<T, n, A, 1, 1><<<grid, block, 0, s>>>(a);
+      else throw Error(FAME6D_EARG, "bad pass-B configuration");
+    };
+    if (v == 0) go(std::integral_constant<int, Tiles<n, T>::a[0]>{});
+    else go(std::integral_constant<int, Tiles<n, T>::a[1]>{});
+  });
+  F6_CHECK_LAUNCH();
+}
+
+template <class T>
+void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s) {
+  if (ngroups <= 0) return;
+  const int v = tile_of<T>(m).a;
+  with_n(m->n, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    constexpr int S = n * n * n;
+    auto go = [&](auto bc) {
+      constexpr int BPW = decltype(bc)::value;
+      dim3 grid(S / BPW, ngroups), block(BPW * n * n);
+      if (op == POST_COPY) k_pass_c<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
+      else if (op == POST_MHAT) k_pass_c<T, n, BPW, POST_MHAT, 3><<<grid, block, 0, s>>>(a);
+      else throw Error(FAME6D_EARG, "bad pass-C op");
+    };
+    if (v == 0) go(std::integral_constant<int, Tiles<n, T>::bpw[0]>{});
+    else go(std::integral_constant<int, Tiles<n, T>::bpw[1]>{});
+  });
+  F6_CHECK_LAUNCH();
+}
+
+int pass_c_blocks(const fame6d_mass* m, bool fp32) {
+  int out = 0;
+  const int v = fp32 ? m->tile32.a : m->tile64.a;
+  with_n(m->n, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    constexpr int S = n * n * n;
+    const int bpw = fp32 ? Tiles<n, float>::bpw[v] : Tiles<n, double>::bpw[v];
+    out = S / bpw;
+  });
+  return out;
+}
+
+int num_tile_variants(int n, bool fp32, bool pass_b) {
+  (void)n;
+  (void)fp32;
+  (void)pass_b;
+  return 2;
+}
+
+void tile_variant_desc(int nn, bool fp32, bool pass_b, int v, int* param, int* threads) {
+  with_n(nn, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    int p;
+    if (pass_b) p = fp32 ? Tiles<n, float>::a[v] : Tiles<n, double>::a[v];
+    else p = fp32 ? Tiles<n, float>::bpw[v] : Tiles<n, double>::bpw[v];
+    *param = p;
+    *threads = p * n * n;
+  });
+}
+
+template <class T>
+DevBuf& work_buf(fame6d_mass* m) {
+  if constexpr (std::is_same<T, double>::value) return m->work64;
+  else return m->work32;
+}
+
+template <class T>
+void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long long ldy, int nrhs,
+                hipStream_t s) {
+  if (nrhs <= 0) return;
+  DevBuf& w = work_buf<T>(m);
+  w.ensure(sizeof(cpx<T>) * (size_t)m->NF * (size_t)nrhs);
+  cpx<T>* t = w.as<cpx<T>>();
+  PassAArgs<T> a{};
+  a.x = x;
+  a.ldx = ldx;
+  a.t = t;
+  a.ldt = m->NF;
+  launch_pass_a<T>(m, PRE_COPY, nrhs, a, s);
+  launch_pass_b<T>(m, 1, 0, nrhs, t, m->NF, no_skip(), s);
+  PassCArgs<T> c{};
+  c.t = t;
+  c.ldt = m->NF;
+  c.y = y;
+  c.ldy = ldy;
+  launch_pass_c<T>(m, POST_COPY, nrhs, c, s);
+}
+
+#define F6_INST(T)                                                                                 \
+  template void launch_pass_a<T>(const fame6d_mass*, int, int, const PassAArgs<T>&, hipStream_t);  \
+  template void launch_pass_b<T>(const fame6d_mass*, int, int, int, cpx<T>*, long long, Skip,      \
+                                 hipStream_t);                                                     \
+  template void launch_pass_c<T>(const fame6d_mass*, int, int, const PassCArgs<T>&, hipStream_t);  \
+  template void mass_apply<T>(fame6d_mass*, const cpx<T>*, long long, cpx<T>*, long long, int,     \
+                              hipStream_t);                                                        \
+  template DevBuf& work_buf<T>(fame6d_mass*);
+F6_INST(double)
+F6_INST(float)
+#undef F6_INST
+
+}  // namespace fame6d

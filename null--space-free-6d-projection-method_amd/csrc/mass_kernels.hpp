@@ -1,0 +1,375 @@
+// Circulant 6D mass operator  M_s x = fftn(eps .* ifftn(x))  on the (2N)^6 torus.
+//
+// Reference: applyMij_fft (exp1/applyMij_fft.m:9-23); the rank-1 lattice form
+// applyMij_rank1_exact (exp1/applyMij_rank1_exact.m:4-5) computes the same matrix
+// (kernel indices are reduced mod nGrid, build_exact_rank1_scalar_kernel_fft.m:26-34).
+//
+// Layout: a grid vector is the column-major (2N)^6 array, linear index
+//   i = k1 + n*k2 + n^2*k3 + S*(k4 + n*k5 + n^2*k6),  n = 2N, S = n^3,
+// i.e. "inner" position a = i mod S (dims 1-3) and "outer" position b = i / S
+// (dims 4-6).  Complex values are interleaved.
+//
+// The apply is three HBM sweeps:
+//   pass A: per outer position b (a contiguous S-block): pre-op, inverse DFT over
+//           dims 1,2,3 (register pencil along dim 1, LDS exchange for dims 2,3);
+//   pass B: per tile of A consecutive inner positions: inverse DFT over dims 4,5,6,
+//           multiply by eps/NF, forward DFT over dims 6,5,4 (register pencils,
+//           LDS exchange), data stays on chip between the two transforms;
+//   pass C: per S-block: forward DFT over dims 3,2,1 and post-op.
+// The pre/post ops fuse the Method-C modewise work around the scalar applies
+// (apply_Mhat_C_scalar_compact.m:8-21): one read of x feeds the three scalar
+// applies M_s z, M_s(W1 z), M_s(W2 z), and pass C accumulates
+// invC*(s0 + W1 s1 + W2 s2) in registers.
+#pragma once
+
+#include "common.hpp"
+#include "dft.hpp"
+
+namespace fame6d {
+
+// Per-group device state of a conjugate-gradient solve (all scalars FP64).
+struct CgState {
+  double rho;       // r'*r of the current residual
+  double alpha;     // rho / (p'*q)
+  double beta;      // rho_new / rho_old
+  double pq;        // p'*q
+  double normr;     // ||r|| (recurrence)
+  double tolb;      // tol * ||b||
+  double n2b;       // ||b||
+  double rho_prev;  // rho before the last update
+  int iter;         // completed iterations
+  int done;         // 1: stop issuing work for this group
+  int flag;         // MATLAB pcg flag convention (0 ok, 1 maxit, 3 stagnation, 4 breakdown)
+  int maxit;
+  int check_every;  // stop after every iteration for a true-residual check
+  int pad[3];
+};
+static_assert(sizeof(CgState) == 96, "CgState layout");
+
+// Per-group early-exit mask: group g is skipped when p[g*stride] != 0.
+// Pointing it at CgState::done (int index 17, stride 24) skips finished solves.
+struct Skip {
+  const int* p;
+  int stride;
+};
+__host__ __device__ inline Skip no_skip() { return Skip{nullptr, 0}; }
+__host__ inline Skip skip_done(const CgState* st) {
+  return Skip{st ? reinterpret_cast<const int*>(st) + 17 : nullptr, 24};
+}
+__device__ __forceinline__ bool skipped(Skip s, int g) { return s.p != nullptr && s.p[g * s.stride] != 0; }
+// Group -> q-system slot (nullptr = identity).
+__device__ __forceinline__ long long qslot(const int* qmap, int g) { return qmap ? qmap[g] : g; }
+
+enum PreOp : int {
+  PRE_COPY = 0,     // t = x                                  (1 output)
+  PRE_MHAT = 1,     // z = invC*x; t = [z, W1 z, W2 z]        (3 outputs)
+  PRE_MHAT_CG = 2,  // p = r + beta p (p = r first); then as PRE_MHAT on p
+};
+enum PostOp : int {
+  POST_COPY = 0,  // y = s
+  POST_MHAT = 1,  // y = invC*(s0 + W1 s1 + W2 s2)  [W real]
+};
+
+template <class T>
+struct PassAArgs {
+  const cpx<T>* x;  // input of group g at x + g*ldx (the residual r for PRE_MHAT_CG)
+  long long ldx;
+  cpx<T>* t;  // output o of group g at t + (g*nout + o)*ldt
+  long long ldt;
+  const T* W1;  // modewise diagonals of group g at W1 + g*ldq
+  const T* W2;
+  const T* invC;
+  long long ldq;
+  cpx<T>* p;  // PRE_MHAT_CG search direction of group g at p + g*ldx
+  const CgState* st;  // per group (PRE_MHAT_CG)
+  Skip skip;
+  const int* qmap;
+};
+
+template <class T>
+struct PassBArgs {
+  cpx<T>* t;  // vector r of group g at t + (g*NR + r)*ldt
+  long long ldt;
+  const T* eps;  // eps / NF
+  Skip skip;
+};
+
+template <class T>
+struct PassCArgs {
+  const cpx<T>* t;  // as PassAArgs::t
+  long long ldt;
+  cpx<T>* y;  // output of group g at y + g*ldy
+  long long ldy;
+  const T* W1;
+  const T* W2;
+  const T* invC;
+  long long ldq;
+  const cpx<T>* dotp;  // if set: partial sums of Re(dotp^H y) per workgroup
+  long long lddot;
+  double* partials;  // partials[g*gridDim.x + blockIdx.x]
+  Skip skip;
+  const int* qmap;
+};
+
+template <int n>
+struct Grid {
+  static constexpr int P2 = n * n;
+  static constexpr int S = n * n * n;
+};
+
+// Workgroup sum of one double per thread in a fixed order; result valid in
+// thread 0.  `red` must hold THREADS doubles.  Values are staged through LDS so
+// that partial waves never shuffle from inactive lanes; wave 0 (always full when
+// THREADS >= 64) then folds them.  Callers synchronise before reusing `red`.
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  double s = 0.0;
+  if constexpr (THREADS >= 64) {
+    if (threadIdx.x < 64) {
+      for (int i = threadIdx.x; i < THREADS; i += 64) s += red[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    }
+  } else {
+    if (threadIdx.x == 0)
+      for (int i = 0; i < THREADS; ++i) s += red[i];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------- pass A
+template <class T, int n, int BPW, int OP>
+__global__ __launch_bounds__(BPW* n* n) void k_pass_a(PassAArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int NOUT = (OP == PRE_COPY) ? 1 : 3;
+  __shared__ cpx<T> lds[BPW * S];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int blk = tid / P2, pp = tid - blk * P2;
+  const long long off = ((long long)blockIdx.x * BPW + blk) * S;
+  cpx<T>* L = lds + blk * S;
+
+  // dim-1 pencil of this thread: (k2, k3) = (pp % n, pp / n)
+  const int base1 = (pp % n) * n + (pp / n) * P2;
+  const cpx<T>* xg = a.x + g * a.ldx + off + base1;
+  cpx<T> pn[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) pn[k] = xg[k];
+
+  if constexpr (OP == PRE_MHAT_CG) {
+    cpx<T>* pg = a.p + g * a.ldx + off + base1;
+    if (a.st[g].iter > 0) {
+      const T beta = static_cast<T>(a.st[g].beta);
+#pragma unroll
+      for (int k = 0; k < n; ++k) pn[k] = pn[k] + beta * pg[k];
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k) pg[k] = pn[k];
+  }
+
+  const long long qoff = qslot(a.qmap, g) * a.ldq + off + base1;
+#pragma unroll 1
+  for (int o = 0; o < NOUT; ++o) {
+    cpx<T> u[n];
+    if constexpr (OP == PRE_COPY) {
+#pragma unroll
+      for (int k = 0; k < n; ++k) u[k] = pn[k];
+    } else {
+      const T* wo = (o == 1) ? a.W1 : a.W2;
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        T w = a.invC[qoff + k];
+        if (o > 0) w *= wo[qoff + k];
+        u[k] = w * pn[k];
+      }
+    }
+    dft<T, n, +1>(u);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[base1 + k] = u[k];
+    __syncthreads();
+    {  // dim 2: (k1, k3) = (pp % n, pp / n)
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+      dft<T, n, +1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+    }
+    __syncthreads();
+    {  // dim 3: (k1, k2) = pp; store straight to HBM (coalesced across pp)
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[pp + P2 * k];
+      dft<T, n, +1>(w);
+      cpx<T>* tg = a.t + (g * NOUT + o) * a.ldt + off + pp;
+#pragma unroll
+      for (int k = 0; k < n; ++k) tg[P2 * k] = w[k];
+    }
+    if (o + 1 < NOUT) __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- pass B
+// MODE 0: inverse DFT dims 4-6, * eps/NF, forward DFT dims 6-4 (the mass apply).
+// MODE 1: inverse DFT dims 4-6 only (completes an unnormalised 6D ifft after pass A).
+template <class T, int n, int A, int NR, int MODE>
+__global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  __shared__ cpx<T> L[A * S];  // l = al + A*(k4 + n*k5 + n^2*k6)
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int al = tid % A, qq = tid / A;  // qq in [0, n^2)
+  const long long a0 = (long long)blockIdx.x * A + al;
+
+  // eps for the dim-6 pencil (k4 + n*k5 = qq)
+  T e[n];
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) e[k] = a.eps[a0 + (long long)S * (qq + P2 * k)];
+  }
+
+#pragma unroll 1
+  for (int r = 0; r < NR; ++r) {
+    cpx<T>* v = a.t + (g * NR + r) * a.ldt + a0;
+    {  // dim 4 inverse: (k5, k6) = qq
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = v[(long long)S * (k + n * qq)];
+      dft<T, n, +1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[al + A * (k + n * qq)] = w[k];
+    }
+    __syncthreads();
+    const int b5 = al + A * ((qq % n) + P2 * (qq / n));  // (k4, k6) = qq
+    {
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
+      dft<T, n, +1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
+    }
+    __syncthreads();
+    {  // dim 6: inverse, eps, forward
+      const int b6 = al + A * qq;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[b6 + A * P2 * k];
+      dft<T, n, +1>(w);
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) v[(long long)S * (qq + P2 * k)] = w[k];
+        __syncthreads();
+        continue;
+      } else {
+#pragma unroll
+        for (int k = 0; k < n; ++k) w[k] = e[k] * w[k];
+        dft<T, n, -1>(w);
+#pragma unroll
+        for (int k = 0; k < n; ++k) L[b6 + A * P2 * k] = w[k];
+      }
+    }
+    __syncthreads();
+    {
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
+    }
+    __syncthreads();
+    {  // dim 4 forward, store
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[al + A * (k + n * qq)];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) v[(long long)S * (k + n * qq)] = w[k];
+    }
+    // the next r first writes exactly the LDS slots this thread just read
+  }
+}
+
+// ---------------------------------------------------------------- pass C
+template <class T, int n, int BPW, int OP, int NOUT>
+__global__ __launch_bounds__(BPW* n* n) void k_pass_c(PassCArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int THREADS = BPW * P2;
+  __shared__ cpx<T> lds[BPW * S];
+  __shared__ double red[THREADS];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int blk = tid / P2, pp = tid - blk * P2;
+  const long long off = ((long long)blockIdx.x * BPW + blk) * S;
+  cpx<T>* L = lds + blk * S;
+  const int base1 = (pp % n) * n + (pp / n) * P2;
+  const long long qoff = qslot(a.qmap, g) * a.ldq + off + base1;
+
+  cpx<T> acc[n];
+#pragma unroll 1
+  for (int o = 0; o < NOUT; ++o) {
+    {  // dim 3 forward, loaded straight from HBM
+      const cpx<T>* tg = a.t + (g * NOUT + o) * a.ldt + off + pp;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = tg[P2 * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[pp + P2 * k] = w[k];
+    }
+    __syncthreads();
+    {
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+    }
+    __syncthreads();
+    {
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+      dft<T, n, -1>(w);
+      if (o == 0) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) acc[k] = w[k];
+      } else {
+        const T* wo = (o == 1) ? a.W1 : a.W2;
+#pragma unroll
+        for (int k = 0; k < n; ++k) acc[k] = acc[k] + wo[qoff + k] * w[k];
+      }
+    }
+    if (o + 1 < NOUT) __syncthreads();
+  }
+  if constexpr (OP == POST_MHAT) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) acc[k] = a.invC[qoff + k] * acc[k];
+  }
+  cpx<T>* yg = a.y + g * a.ldy + off + base1;
+#pragma unroll
+  for (int k = 0; k < n; ++k) yg[k] = acc[k];
+
+  if (a.partials != nullptr) {
+    const cpx<T>* dg = a.dotp + g * a.lddot + off + base1;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      const cpx<T> d = dg[k];
+      s += static_cast<double>(d.re * acc[k].re + d.im * acc[k].im);
+    }
+    s = block_sum<THREADS>(s, red);
+    if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+}  // namespace fame6d

@@ -1,0 +1,571 @@
+// Null-space-free q-system setup, the fused Mhat operator, the device-side
+// PCG on Mhat and the Method-C explicit reduced inverse K_r^{-1}
+// (FP64, and FP32 inner solves with FP64 iterative refinement).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+#include "krylov_kernels.hpp"
+
+namespace fame6d {
+
+// ------------------------------------------------------------------ q-system (host)
+// Restates assemble_K_and_nullspace_free_basis.m:18-105 and
+// build_experiment1_q_system_C_compact.m:9-31 for real P (6x3), T = 2*pi.
+fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P) {
+  if (G <= 0) throw Error(FAME6D_EARG, "need at least one Bloch point");
+  const int N = m->N, n = m->n;
+  const long long NF = m->NF;
+  auto* S = new fame6d_qsys();
+  S->m = m;
+  S->G = G;
+  S->NF = NF;
+  std::memcpy(S->P, P, sizeof(double) * 18);
+  S->q.assign(q, q + 3 * G);
+  S->klift.assign(6 * G, 0.0);
+  S->min_abs_det.assign(G, 0.0);
+  S->min_qq.assign(G, 0.0);
+  S->n_singular.assign(G, 0);
+
+  // P^T P (3x3) and its inverse (k_lift = P (P^T P)^{-1} q)
+  double A[9] = {0};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      for (int l = 0; l < 6; ++l) A[r * 3 + c] += P[l * 3 + r] * P[l * 3 + c];
+  const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                     A[2] * (A[3] * A[7] - A[4] * A[6]);
+  if (!(std::fabs(det) > 0)) throw Error(FAME6D_EARG, "P^T P is singular");
+  double Ai[9];
+  Ai[0] = (A[4] * A[8] - A[5] * A[7]) / det;
+  Ai[1] = (A[2] * A[7] - A[1] * A[8]) / det;
+  Ai[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+  Ai[3] = (A[5] * A[6] - A[3] * A[8]) / det;
+  Ai[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+  Ai[5] = (A[2] * A[3] - A[0] * A[5]) / det;
+  Ai[6] = (A[3] * A[7] - A[4] * A[6]) / det;
+  Ai[7] = (A[1] * A[6] - A[0] * A[7]) / det;
+  Ai[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+
+  const size_t GN = (size_t)G * NF;
+  std::vector<double> V(6 * GN), W(3 * GN), Qm(3 * GN);
+  for (int g = 0; g < G; ++g) {
+    const double* qq = q + 3 * g;
+    if (!(std::isfinite(qq[0]) && std::isfinite(qq[1]) && std::isfinite(qq[2])))
+      throw Error(FAME6D_ENONFINITE, "Bloch vector contains NaN/Inf");
+    double y[3];
+    for (int r = 0; r < 3; ++r) y[r] = Ai[r * 3 + 0] * qq[0] + Ai[r * 3 + 1] * qq[1] + Ai[r * 3 + 2] * qq[2];
+    double* kl = &S->klift[6 * g];
+    for (int l = 0; l < 6; ++l) kl[l] = P[l * 3 + 0] * y[0] + P[l * 3 + 1] * y[1] + P[l * 3 + 2] * y[2];
+
+    double* V11 = &V[0 * GN + g * NF];
+    double* V12 = &V[1 * GN + g * NF];
+    double* V21 = &V[2 * GN + g * NF];
+    double* V22 = &V[3 * GN + g * NF];
+    double* V31 = &V[4 * GN + g * NF];
+    double* V32 = &V[5 * GN + g * NF];
+    double* W1 = &W[0 * GN + g * NF];
+    double* W2 = &W[1 * GN + g * NF];
+    double* iC = &W[2 * GN + g * NF];
+    double* q1 = &Qm[0 * GN + g * NF];
+    double* q2 = &Qm[1 * GN + g * NF];
+    double* q3 = &Qm[2 * GN + g * NF];
+    double minqq = INFINITY, mindet = INFINITY;
+    int nsing = 0;
+    for (long long i = 0; i < NF; ++i) {
+      long long t = i;
+      double qv[3] = {0, 0, 0};
+      for (int l = 0; l < 6; ++l) {
+        const int k = (int)(t % n);
+        t /= n;
+        const double xi = (k < N) ? k : k - n;  // build_fourier_grid_6d.m:12
+        const double sx = xi + kl[l];
+        for (int c = 0; c < 3; ++c) qv[c] += P[l * 3 + c] * sx;
+      }
+      q1[i] = qv[0];
+      q2[i] = qv[1];
+      q3[i] = qv[2];
+      const double Qq = qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2];
+      minqq = std::min(minqq, Qq);
+      const double sq = std::sqrt(Qq);
+      const double Qs = qv[0] + qv[1] + qv[2];
+      const double den = 3 * Qq - Qs * Qs;
+      double u1[3], u2[3];
+      if (den > 1e-13) {
+        const double s1 = 1.0 / std::sqrt(Qq * den), s2 = 1.0 / std::sqrt(den);
+        for (int c = 0; c < 3; ++c) u1[c] = (Qq - qv[c] * Qs) * s1;
+        u2[0] = (qv[2] - qv[1]) * s2;
+        u2[1] = (qv[0] - qv[2]) * s2;
+        u2[2] = (qv[1] - qv[0]) * s2;
+      } else {
+        // singular branch (assemble_K_and_nullspace_free_basis.m:75-91)
+        ++nsing;
+        double h[3] = {qv[0] / sq, qv[1] / sq, qv[2] / sq};
+        int imin = 0;
+        for (int c = 1; c < 3; ++c)
+          if (std::fabs(h[c]) < std::fabs(h[imin])) imin = c;
+        double a[3] = {0, 0, 0};
+        a[imin] = 1;
+        const double ha = h[imin];
+        double z1[3] = {a[0] - h[0] * ha, a[1] - h[1] * ha, a[2] - h[2] * ha};
+        double nz = std::sqrt(z1[0] * z1[0] + z1[1] * z1[1] + z1[2] * z1[2]);
+        for (int c = 0; c < 3; ++c) z1[c] /= nz;
+        double z2[3] = {h[1] * z1[2] - h[2] * z1[1], h[2] * z1[0] - h[0] * z1[2],
+                        h[0] * z1[1] - h[1] * z1[0]};
+        nz = std::sqrt(z2[0] * z2[0] + z2[1] * z2[1] + z2[2] * z2[2]);
+        for (int c = 0; c < 3; ++c) {
+          u1[c] = z1[c];
+          u2[c] = z2[c] / nz;
+        }
+      }
+      V11[i] = u1[0] * sq;
+      V21[i] = u1[1] * sq;
+      V31[i] = u1[2] * sq;
+      V12[i] = u2[0] * sq;
+      V22[i] = u2[1] * sq;
+      V32[i] = u2[2] * sq;
+      const double d1 = V11[i] * V22[i] - V12[i] * V21[i];
+      mindet = std::min(mindet, std::fabs(d1));
+      W1[i] = (V22[i] * V31[i] - V21[i] * V32[i]) / d1;
+      W2[i] = (V11[i] * V32[i] - V12[i] * V31[i]) / d1;
+      const double korth = 1.0 + W1[i] * W1[i] + W2[i] * W2[i];
+      iC[i] = 1.0 / std::sqrt(korth);
+    }
+    if (minqq <= 1e-13) {
+      delete S;
+      throw Error(FAME6D_EGAMMA,
+                  "A zero or nearly zero projected Fourier mode was detected; the null-space-free "
+                  "reduction requires Gamma-point regularization (min |q|^2 = " +
+                      std::to_string(minqq) + ")");
+    }
+    S->min_qq[g] = minqq;
+    S->min_abs_det[g] = mindet;
+    S->n_singular[g] = nsing;
+  }
+  std::vector<float> Wf(W.begin(), W.end());
+  S->Vd.ensure(sizeof(double) * V.size());
+  S->Wd.ensure(sizeof(double) * W.size());
+  S->Wf.ensure(sizeof(float) * Wf.size());
+  S->Qd.ensure(sizeof(double) * Qm.size());
+  F6_HIP(hipMemcpyAsync(S->Vd.ptr, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(S->Wd.ptr, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(S->Wf.ptr, Wf.data(), sizeof(float) * Wf.size(), hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(S->Qd.ptr, Qm.data(), sizeof(double) * Qm.size(), hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipStreamSynchronize(m->stream));
+  return S;
+}
+
+// ------------------------------------------------------------------ helpers
+template <class T>
+static const T* Wt(const fame6d_qsys* S, int i) {
+  if constexpr (std::is_same<T, double>::value) return S->W64(i);
+  else return S->W32(i);
+}
+
+template <class T>
+void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
+                hipStream_t s) {
+  fame6d_mass* m = S->m;
+  const long long NF = m->NF;
+  DevBuf& w = work_buf<T>(m);
+  w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
+  cpx<T>* t = w.as<cpx<T>>();
+  PassAArgs<T> a{};
+  a.x = x;
+  a.ldx = NF;
+  a.t = t;
+  a.ldt = NF;
+  a.W1 = Wt<T>(S, 0);
+  a.W2 = Wt<T>(S, 1);
+  a.invC = Wt<T>(S, 2);
+  a.ldq = NF;
+  a.skip = skip;
+  a.qmap = qmap;
+  launch_pass_a<T>(m, PRE_MHAT, nact, a, s);
+  launch_pass_b<T>(m, 3, 0, nact, t, NF, skip, s);
+  PassCArgs<T> c{};
+  c.t = t;
+  c.ldt = NF;
+  c.y = y;
+  c.ldy = NF;
+  c.W1 = a.W1;
+  c.W2 = a.W2;
+  c.invC = a.invC;
+  c.ldq = NF;
+  c.skip = skip;
+  c.qmap = qmap;
+  launch_pass_c<T>(m, POST_MHAT, nact, c, s);
+}
+
+template <class T>
+void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, const cpx<T>* b,
+                long long ldb, long long n, double* out_re_host, double* out_im_host) {
+  fame6d_mass* mm = const_cast<fame6d_mass*>(m);
+  const int nb = red_blocks(n);
+  mm->partials.ensure(sizeof(double) * (size_t)nb * G * 2);
+  mm->scal.ensure(sizeof(double) * (size_t)G * 2 + 64);
+  double* pre = mm->partials.as<double>();
+  double* pim = pre + (size_t)nb * G;
+  double* out = mm->scal.as<double>();
+  k_dot_partials<T><<<dim3(nb, G), kRedThreads, 0, m->stream>>>(a, lda, b, ldb, n, pre, pim);
+  F6_CHECK_LAUNCH();
+  k_reduce_final<<<G, kRedThreads, 0, m->stream>>>(pre, nb, out);
+  k_reduce_final<<<G, kRedThreads, 0, m->stream>>>(pim, nb, out + G);
+  F6_CHECK_LAUNCH();
+  std::vector<double> h(2 * G);
+  F6_HIP(hipMemcpyAsync(h.data(), out, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, m->stream));
+  F6_HIP(hipStreamSynchronize(m->stream));
+  for (int g = 0; g < G; ++g) {
+    if (out_re_host) out_re_host[g] = h[g];
+    if (out_im_host) out_im_host[g] = h[G + g];
+  }
+}
+
+// ------------------------------------------------------------------ PCG on Mhat
+// MATLAB pcg (no preconditioner, x0 = 0) for nact independent right-hand sides,
+// each kept in lock-step but with its own scalars; finished solves are skipped
+// on the device.  Semantics restated: convergence when the recurrence residual
+// meets tol*||b||, confirmed by the true residual b - Mhat x; a failed check
+// continues from the true residual and checks every later iteration ("moresteps",
+// at most 5); flag 1 at maxit, 4 on breakdown (p'q <= 0 or beta 0/Inf).
+// Not restated: the stagnation test ||p||*|alpha| < eps*||x|| and the
+// minimum-residual iterate returned on failure.
+template <class T>
+static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
+                     cpx<T>* r, cpx<T>* p, cpx<T>* q, double tol, int maxit,
+                     std::vector<PcgOut>& out, long long* applies) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF;
+  const long long total = NF * nact;
+  const int nbC = pass_c_blocks(m, std::is_same<T, float>::value);
+  const int nbU = red_blocks(NF);
+  const int nbD = red_blocks(NF);
+  S->st.ensure(sizeof(CgState) * nact);
+  S->part.ensure(sizeof(double) * (size_t)std::max(nbC, nbD) * nact * 2);
+  S->part2.ensure(sizeof(double) * (size_t)nbU * nact);
+  S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
+  CgState* st = S->st.as<CgState>();
+  double* part = S->part.as<double>();
+  double* part2 = S->part2.as<double>();
+  int* mask = S->ints.as<int>();
+  DevBuf& w = work_buf<T>(m);
+  w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
+  cpx<T>* t = w.as<cpx<T>>();
+
+  k_fill_zero<T><<<grid_for(total), 256, 0, s>>>(total, x);
+  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<T>) * total, hipMemcpyDeviceToDevice, s));
+  k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, NF, b, NF, NF, part, nullptr);
+  k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part, nbD, tol, maxit);
+  F6_CHECK_LAUNCH();
+
+  out.assign(nact, PcgOut{});
+  std::vector<int> finished(nact, 0), moresteps(nact, 0);
+  std::vector<CgState> hs(nact);
+  const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
+
+  PassAArgs<T> pa{};
+  pa.x = r;
+  pa.ldx = NF;
+  pa.t = t;
+  pa.ldt = NF;
+  pa.W1 = Wt<T>(S, 0);
+  pa.W2 = Wt<T>(S, 1);
+  pa.invC = Wt<T>(S, 2);
+  pa.ldq = NF;
+  pa.p = p;
+  pa.st = st;
+  pa.skip = skip_done(st);
+  pa.qmap = qmap_dev;
+  PassCArgs<T> pc{};
+  pc.t = t;
+  pc.ldt = NF;
+  pc.y = q;
+  pc.ldy = NF;
+  pc.W1 = pa.W1;
+  pc.W2 = pa.W2;
+  pc.invC = pa.invC;
+  pc.ldq = NF;
+  pc.dotp = p;
+  pc.lddot = NF;
+  pc.partials = part;
+  pc.skip = skip_done(st);
+  pc.qmap = qmap_dev;
+
+  auto read_state = [&]() {
+    F6_HIP(hipMemcpyAsync(hs.data(), st, sizeof(CgState) * nact, hipMemcpyDeviceToHost, s));
+    F6_HIP(hipStreamSynchronize(s));
+  };
+  read_state();
+  for (int g = 0; g < nact; ++g) {
+    if (hs[g].done) {
+      finished[g] = 1;
+      out[g].flag = hs[g].flag;
+      out[g].iter = 0;
+      out[g].relres = hs[g].n2b > 0 ? hs[g].normr / hs[g].n2b : 0.0;
+    }
+  }
+  int& hint_ref = std::is_same<T, float>::value ? S->chunk_hint32 : S->chunk_hint;
+  int hint = hint_ref > 0 ? hint_ref : 8;
+  int issued = 0;
+  while (true) {
+    bool any_open = false, any_check = false;
+    for (int g = 0; g < nact; ++g) {
+      if (!finished[g]) any_open = true;
+      if (!finished[g] && hs[g].check_every) any_check = true;
+    }
+    if (!any_open) break;
+    int chunk = any_check ? 1 : std::max(1, std::min(hint - issued, 64));
+    if (issued >= hint) chunk = any_check ? 1 : 2;
+    for (int c = 0; c < chunk; ++c) {
+      launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
+      launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s);
+      launch_pass_c<T>(m, POST_MHAT, nact, pc, s);
+      k_cg_alpha<<<nact, kRedThreads, 0, s>>>(st, part, nbC);
+      k_cg_update<T><<<dim3(nbU, nact), kRedThreads, 0, s>>>(x, p, r, q, NF, NF, st, part2);
+      k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU);
+      F6_CHECK_LAUNCH();
+    }
+    issued += chunk;
+    read_state();
+    std::vector<int> need(nact, 0);
+    bool any_need = false;
+    for (int g = 0; g < nact; ++g) {
+      if (finished[g] || !hs[g].done) continue;
+      if (hs[g].flag == 0) {
+        need[g] = 1;
+        any_need = true;
+      } else {
+        finished[g] = 1;
+        out[g].flag = hs[g].flag;
+        out[g].iter = hs[g].iter;
+        out[g].relres = hs[g].normr / hs[g].n2b;
+      }
+    }
+    if (any_need) {
+      // true residual r = b - Mhat x for the groups that stopped on the recurrence
+      std::vector<int> skiph(nact);
+      for (int g = 0; g < nact; ++g) skiph[g] = need[g] ? 0 : 1;
+      F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+      Skip sk{mask, 1};
+      mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
+      k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
+      k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part, nullptr);
+      F6_CHECK_LAUNCH();
+      std::vector<double> rr(nact);
+      {
+        double* outd = part2;  // reuse: nact doubles
+        k_reduce_final<<<nact, kRedThreads, 0, s>>>(part, nbD, outd);
+        F6_HIP(hipMemcpyAsync(rr.data(), outd, sizeof(double) * nact, hipMemcpyDeviceToHost, s));
+        F6_HIP(hipStreamSynchronize(s));
+      }
+      std::vector<int> resume(nact, 0);
+      bool any_resume = false;
+      for (int g = 0; g < nact; ++g) {
+        if (!need[g]) continue;
+        ++out[g].checks;
+        if (applies) *applies += 3;
+        const double nact_r = std::sqrt(rr[g]);
+        if (nact_r <= hs[g].tolb) {
+          finished[g] = 1;
+          out[g].flag = 0;
+          out[g].iter = hs[g].iter;
+          out[g].relres = nact_r / hs[g].n2b;
+        } else {
+          moresteps[g] += 1;
+          if (moresteps[g] >= maxmsteps || hs[g].iter >= maxit) {
+            finished[g] = 1;
+            out[g].flag = (moresteps[g] >= maxmsteps) ? 3 : 1;
+            out[g].iter = hs[g].iter;
+            out[g].relres = nact_r / hs[g].n2b;
+          } else {
+            resume[g] = 1;
+            any_resume = true;
+          }
+        }
+      }
+      if (any_resume) {
+        F6_HIP(hipMemcpyAsync(mask + nact, resume.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbD, mask + nact);
+        F6_CHECK_LAUNCH();
+        read_state();
+      }
+    }
+  }
+  int min_seen = 1 << 30;
+  for (int g = 0; g < nact; ++g) min_seen = std::min(min_seen, out[g].iter);
+  hint_ref = std::max(2, min_seen - 1);
+}
+
+// ------------------------------------------------------------------ K_r^{-1}
+struct KrinvWs64 {
+  cpx<double>*r, *mr, *s, *bhat, *u, *cr, *cp, *cq, *res, *y;
+};
+
+void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
+                  const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,
+                  std::vector<PcgOut>* outs) {
+  if (nact <= 0) return;
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF;
+  const size_t gn = (size_t)NF * nact;
+  const auto t0 = std::chrono::steady_clock::now();
+  // device q-slot map
+  S->sc.ensure(sizeof(int) * 256);
+  int* qmap = S->sc.as<int>();
+  std::vector<int> qm(qmap_host, qmap_host + nact);
+  for (int v : qm)
+    if (v < 0 || v >= S->G) throw Error(FAME6D_EARG, "bad q-system slot");
+  if (nact > 64) throw Error(FAME6D_EARG, "at most 64 Bloch points per batched K_r^{-1}");
+  F6_HIP(hipMemcpyAsync(qmap, qm.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+
+  // FP64 workspace: r, mr, s (2NF each) + bhat, u, cr, cp, cq, res, y (NF each)
+  S->kr64.ensure(sizeof(cpx<double>) * gn * 13);
+  cpx<double>* base = S->kr64.as<cpx<double>>();
+  KrinvWs64 w{base, base + 2 * gn, base + 4 * gn, base + 6 * gn, base + 7 * gn,
+              base + 8 * gn, base + 9 * gn, base + 10 * gn, base + 11 * gn, base + 12 * gn};
+  const Skip none = no_skip();
+  const int gridm = grid_for(NF);
+
+  // r = V1^{-*} q; mr = M_s r (both families)
+  k_v1h_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(qin, 2 * NF, NF, S->V(0), S->V(1), S->V(2),
+                                                         S->V(3), w.r, 2 * NF, none, qmap);
+  F6_CHECK_LAUNCH();
+  mass_apply<double>(m, w.r, NF, w.mr, NF, 2 * nact, s);
+  // bhat = -invC (W1 mr1 + W2 mr2)
+  k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
+                                                    w.bhat, NF, none, qmap);
+  F6_CHECK_LAUNCH();
+  S->stats.mass_applies_fp64 += 2 * nact;
+
+  std::vector<PcgOut> res;
+  if (precision == FAME6D_FP64) {
+    pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, tol, maxit, res, nullptr);
+    for (int g = 0; g < nact; ++g) {
+      S->stats.total_iterations += res[g].iter;
+      S->stats.total_fp64_iterations += res[g].iter;
+      S->stats.mass_applies_fp64 += 3LL * (res[g].iter + res[g].checks);
+    }
+  } else if (precision == FAME6D_MIXED) {
+    // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.
+    S->kr32.ensure(sizeof(cpx<float>) * gn * 5);
+    cpx<float>* b32 = S->kr32.as<cpx<float>>();
+    cpx<float>* x32 = b32 + gn;
+    cpx<float>* r32 = b32 + 2 * gn;
+    cpx<float>* p32 = b32 + 3 * gn;
+    cpx<float>* q32 = b32 + 4 * gn;
+    k_fill_zero<double><<<grid_for(gn), 256, 0, s>>>((long long)gn, w.u);
+    F6_HIP(hipMemcpyAsync(w.res, w.bhat, sizeof(cpx<double>) * gn, hipMemcpyDeviceToDevice, s));
+    std::vector<double> nb(nact), nr(nact);
+    dot_groups<double>(m, nact, w.bhat, NF, w.bhat, NF, NF, nb.data(), nullptr);
+    for (int g = 0; g < nact; ++g) {
+      nb[g] = std::sqrt(nb[g]);
+      nr[g] = nb[g];
+      if (!std::isfinite(nb[g])) throw Error(FAME6D_ENONFINITE, "Mhat right-hand side contains NaN/Inf.");
+    }
+    res.assign(nact, PcgOut{});
+    std::vector<int> active(nact);
+    for (int g = 0; g < nact; ++g) active[g] = (nb[g] > 0) ? 1 : 0;
+    const double tol_in = S->mixed_inner_tol;
+    int cycles = 0;
+    while (true) {
+      bool any = false;
+      for (int g = 0; g < nact; ++g) any = any || active[g];
+      if (!any) break;
+      if (cycles >= S->mixed_max_cycles)
+        throw Error(FAME6D_ENOCONV, "Mhat iterative refinement did not converge");
+      ++cycles;
+      // pack active groups: b32 = res / ||res||
+      std::vector<int> act, slots;
+      for (int g = 0; g < nact; ++g)
+        if (active[g]) {
+          act.push_back(g);
+          slots.push_back(qm[g]);
+        }
+      const int na = (int)act.size();
+      for (int i = 0; i < na; ++i) {
+        const int g = act[i];
+        k_convert<float, double><<<gridm, 256, 0, s>>>(NF, w.res + (size_t)g * NF, 1.0 / nr[g],
+                                                      b32 + (size_t)i * NF);
+      }
+      F6_CHECK_LAUNCH();
+      int* qmap_a = qmap + 128;
+      F6_HIP(hipMemcpyAsync(qmap_a, slots.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+      std::vector<PcgOut> inner;
+      pcg_mhat<float>(S, na, qmap_a, b32, x32, r32, p32, q32, tol_in, maxit, inner, nullptr);
+      for (int i = 0; i < na; ++i) {
+        const int g = act[i];
+        if (inner[i].flag != 0 && inner[i].flag != 1 && inner[i].flag != 3)
+          throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(inner[i].flag) + ")");
+        S->stats.total_iterations += inner[i].iter;
+        S->stats.mass_applies_fp32 += 3LL * (inner[i].iter + inner[i].checks);
+        res[g].iter += inner[i].iter;
+        k_accumulate<float><<<gridm, 256, 0, s>>>(NF, x32 + (size_t)i * NF, nr[g], w.u + (size_t)g * NF);
+      }
+      F6_CHECK_LAUNCH();
+      // FP64 residual of every group still refining
+      std::vector<int> skiph(nact);
+      for (int g = 0; g < nact; ++g) skiph[g] = active[g] ? 0 : 1;
+      int* mask = qmap + 192;
+      F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+      Skip sk{mask, 1};
+      mhat_apply<double>(S, nact, qmap, w.u, w.y, sk, s);
+      k_residual<double><<<dim3(gridm, nact), 256, 0, s>>>(w.bhat, w.y, w.res, NF, NF, sk);
+      F6_CHECK_LAUNCH();
+      S->stats.mass_applies_fp64 += 3LL * na;
+      std::vector<double> rr(nact);
+      dot_groups<double>(m, nact, w.res, NF, w.res, NF, NF, rr.data(), nullptr);
+      for (int g = 0; g < nact; ++g) {
+        if (!active[g]) continue;
+        nr[g] = std::sqrt(rr[g]);
+        if (!std::isfinite(nr[g])) throw Error(FAME6D_ENONFINITE, "Mhat refinement residual is NaN/Inf.");
+        if (nr[g] <= tol * nb[g]) {
+          active[g] = 0;
+          res[g].flag = 0;
+          res[g].relres = nr[g] / nb[g];
+        }
+      }
+    }
+    S->stats.refinement_cycles += cycles;
+  } else {
+    throw Error(FAME6D_EARG, "K_r^{-1} precision must be FAME6D_FP64 or FAME6D_MIXED");
+  }
+  for (int g = 0; g < nact; ++g) {
+    const PcgOut& o = res[g];
+    if (!std::isfinite(o.relres) || o.flag < 0 || (o.flag != 0 && o.relres > 10 * tol))
+      throw Error(FAME6D_ENOCONV, "Mhat CG failed: flag=" + std::to_string(o.flag) +
+                                      ", relres=" + std::to_string(o.relres) +
+                                      ", iter=" + std::to_string(o.iter) + ".");
+    S->stats.max_iterations = std::max(S->stats.max_iterations, o.iter);
+    S->stats.max_relres = std::max(S->stats.max_relres, o.relres);
+    S->stats.last_iterations = o.iter;
+    S->stats.last_flag = o.flag;
+    S->stats.last_relres = o.relres;
+  }
+  // z = invC u; s = M_s [W1 z; W2 z]; y = V1^{-1}(mr + s)
+  k_zw<double><<<dim3(gridm, nact), 256, 0, s>>>(w.u, NF, NF, S->W64(0), S->W64(1), S->W64(2), w.s,
+                                                  2 * NF, none, qmap);
+  F6_CHECK_LAUNCH();
+  mass_apply<double>(m, w.s, NF, w.s, NF, 2 * nact, s);
+  k_v1_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, w.s, 2 * NF, NF, S->V(0), S->V(1),
+                                                        S->V(2), S->V(3), yout, 2 * NF, none, qmap);
+  F6_CHECK_LAUNCH();
+  S->stats.mass_applies_fp64 += 2 * nact;
+  S->stats.calls += nact;
+  if (outs) *outs = res;
+  F6_HIP(hipStreamSynchronize(s));
+  S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
+                                 hipStream_t);
+template void mhat_apply<float>(fame6d_qsys*, int, const int*, const cpx<float>*, cpx<float>*, Skip,
+                                hipStream_t);
+template void dot_groups<double>(const fame6d_mass*, int, const cpx<double>*, long long,
+                                 const cpx<double>*, long long, long long, double*, double*);
+template void dot_groups<float>(const fame6d_mass*, int, const cpx<float>*, long long,
+                                const cpx<float>*, long long, long long, double*, double*);
+
+}  // namespace fame6d

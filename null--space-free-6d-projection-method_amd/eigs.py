@@ -1,0 +1,199 @@
+"""Method-C eigensolver on the MI355X: q-system, Mhat, K_r^{-1}, Lanczos, recovery.
+
+Mirrors build_experiment1_q_system_C_compact (exp1/build_experiment1_q_system_C_compact.m:7-38),
+make_explicit_krinv_compact (exp1/make_explicit_krinv_compact.m:1-98),
+run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:1-36),
+recover_original_eigenvectors_exp2 (exp2/recover_original_eigenvectors_exp2.m:1-57) and
+compute_fourier_diagnostics_exp2 (exp2/compute_fourier_diagnostics_exp2.m:1-27).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FP32, FP64, MIXED, call, ptr
+from .mass import P_DEFAULT, MassBackend, precision_code, start_vector
+
+
+class QSystem:
+    """Null-space-free system of one Bloch point on a mass backend."""
+
+    def __init__(self, mass: MassBackend, q, P=P_DEFAULT):
+        self.mass = mass
+        self.q = np.asarray(q, dtype=np.float64).reshape(3)
+        self.P = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(6, 3))
+        self.NF = mass.NF
+        self.nr = 2 * mass.NF
+        self.ndof = 3 * mass.NF
+        self._h = C.c_void_p()
+        call("fame6d_qsys_create", mass.handle, self.q.ctypes.data_as(C.c_void_p),
+             self.P.ctypes.data_as(C.c_void_p), C.byref(self._h))
+
+    def close(self):
+        if self._h:
+            call("fame6d_qsys_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def info(self):
+        i = _lib.QsysInfo()
+        call("fame6d_qsys_info", self._h, C.byref(i))
+        return dict(NF=i.NF, minAbsDetV1=i.min_abs_det_v1, minQq=i.min_qq, nSingular=i.n_singular,
+                    kLift=np.array(list(i.k_lift)))
+
+    def diagonals(self):
+        """Host copies of V11, V12, V21, V22, V31, V32, W1, W2, invC."""
+        names = ["V11", "V12", "V21", "V22", "V31", "V32", "W1", "W2", "invC"]
+        arrs = [np.empty(self.NF) for _ in names]
+        call("fame6d_qsys_diagonals", self._h, *[a.ctypes.data_as(C.c_void_p) for a in arrs])
+        return dict(zip(names, arrs))
+
+    def set_mixed(self, inner_tol=3e-5, max_cycles=30):
+        call("fame6d_qsys_set_mixed", self._h, float(inner_tol), int(max_cycles))
+
+    def _dev(self, x, rows, dtype=torch.complex128):
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x))
+        x = x.to(device=self.mass.device, dtype=dtype).contiguous()
+        if x.numel() != rows:
+            raise _lib.Fame6dError(1, "expected %d entries (BadSize)" % rows)
+        return x
+
+    def Mhat(self, x, precision="double"):
+        """apply_Mhat_C_scalar_compact (exp1/apply_Mhat_C_scalar_compact.m:8-21)."""
+        pc = precision_code(precision)
+        x = self._dev(x, self.NF, torch.complex64 if pc == FP32 else torch.complex128)
+        y = torch.empty_like(x)
+        call("fame6d_mhat_apply", self._h, pc, ptr(x), ptr(y))
+        return y
+
+    def apply_V(self, y):
+        """apply_V_blockdiag (exp1/apply_V_blockdiag.m:9-16)."""
+        y = self._dev(y, self.nr)
+        out = torch.empty(self.ndof, dtype=torch.complex128, device=self.mass.device)
+        call("fame6d_apply_v", self._h, ptr(y), ptr(out))
+        return out
+
+    def Kop(self, x):
+        """Kop (exp1/assemble_K_and_nullspace_free_basis.m:110-119)."""
+        x = self._dev(x, self.ndof)
+        out = torch.empty_like(x)
+        call("fame6d_apply_k", self._h, ptr(x), ptr(out))
+        return out
+
+    def krinv(self, q, tol=1e-10, maxit=1200, precision="double"):
+        """One K_r^{-1} action; returns (y, stats)."""
+        q = self._dev(q, self.nr)
+        y = torch.empty_like(q)
+        st = _lib.KrinvStats()
+        call("fame6d_krinv_apply", self._h, precision_code(precision), ptr(q), ptr(y), float(tol),
+             int(maxit), C.byref(st))
+        return y, st.as_dict()
+
+
+def build_experiment1_q_system_C_compact(common, qPhys, cfg=None):
+    """Reference-shaped constructor: common['mass'] is a MassBackend."""
+    mass = common["mass"] if isinstance(common, dict) else common
+    P = common.get("P", P_DEFAULT) if isinstance(common, dict) else P_DEFAULT
+    return QSystem(mass, qPhys, P)
+
+
+def make_explicit_krinv_compact(sys: QSystem, tolCG, maxitCG, precision="double"):
+    """Returns (KrInvOp, getStats) as exp1/make_explicit_krinv_compact.m:1-10."""
+    acc = dict(inverseCalls=0, totalMhatIterations=0, maxMhatIterations=0, maxMhatRelativeResidual=0.0,
+               iterations=[], relativeResiduals=[], flags=[])
+
+    def op(q):
+        qq = q if (isinstance(q, torch.Tensor) and q.dim() == 1) or np.ndim(q) == 1 else None
+        if qq is None:
+            return torch.stack([op(q[j]) for j in range(q.shape[0])])
+        y, st = sys.krinv(q, tolCG, maxitCG, precision)
+        acc["inverseCalls"] += 1
+        acc["totalMhatIterations"] += st["total_iterations"]
+        acc["maxMhatIterations"] = max(acc["maxMhatIterations"], st["last_iterations"])
+        acc["maxMhatRelativeResidual"] = max(acc["maxMhatRelativeResidual"], st["last_relres"])
+        acc["iterations"].append(st["last_iterations"])
+        acc["relativeResiduals"].append(st["last_relres"])
+        acc["flags"].append(st["last_flag"])
+        return y
+
+    def get_stats():
+        s = dict(acc)
+        s["averageMhatIterations"] = (s["totalMhatIterations"] / s["inverseCalls"]) if s["inverseCalls"] else float("nan")
+        return s
+
+    return op, get_stats
+
+
+def run_method_C(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10, maxit_cg=1200,
+                 seed=20260905, v0=None, precision="double", want_vectors=False):
+    """run_method_C_fig1bc + Lanczos_hybrid_identity_gpuop with a device-resident basis.
+    Returns dict(lambda (ascending), Y (reduced Ritz vectors, (k, 2NF) tensor or None), stats)."""
+    if v0 is None:
+        v0 = start_vector(sys.nr, seed)
+    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    if v0.size != sys.nr:
+        raise _lib.Fame6dError(1, "start vector must have 2NF entries")
+    lam = np.empty(k)
+    Y = torch.empty((k, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
+    st = _lib.LanczosStats()
+    call("fame6d_eigs_method_c", sys.handle, precision_code(precision), int(k), int(p), float(tol),
+         int(max_restart), float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p),
+         lam.ctypes.data_as(C.c_void_p), ptr(Y), C.byref(st))
+    return dict(lambda_=lam, Y=Y, stats=st.as_dict())
+
+
+def run_method_C_batch(mass: MassBackend, qs, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10,
+                       maxit_cg=1200, seed=20260905, v0=None, precision="double", P=P_DEFAULT,
+                       want_vectors=False):
+    """All Bloch points of a path solved together; one batched K_r^{-1} per Lanczos step."""
+    qs = np.ascontiguousarray(np.asarray(qs, dtype=np.float64).reshape(-1, 3))
+    nq = qs.shape[0]
+    nr = 2 * mass.NF
+    if v0 is None:
+        v0 = np.stack([start_vector(nr, seed + i) for i in range(nq)])
+    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(nq, nr))
+    Pc = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(6, 3))
+    lam = np.empty((nq, k))
+    Y = torch.empty((nq, k, nr), dtype=torch.complex128, device=mass.device) if want_vectors else None
+    st = (_lib.LanczosStats * nq)()
+    call("fame6d_eigs_method_c_batch", mass.handle, nq, qs.ctypes.data_as(C.c_void_p),
+         Pc.ctypes.data_as(C.c_void_p), precision_code(precision), int(k), int(p), float(tol),
+         int(max_restart), float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p),
+         lam.ctypes.data_as(C.c_void_p), ptr(Y), st)
+    return dict(lambda_=lam, Y=Y, stats=[s.as_dict() for s in st])
+
+
+def recover_original_eigenvectors(Y, sys: QSystem, tol=1e-12, maxit=3000):
+    """X = M^{-1} V Y, M-normalised; returns (X, MX, info) with (k, 3NF) tensors."""
+    Y = Y.to(sys.mass.device, torch.complex128).contiguous()
+    k = Y.shape[0]
+    X = torch.empty((k, sys.ndof), dtype=torch.complex128, device=sys.mass.device)
+    MX = torch.empty_like(X)
+    st = _lib.RecoverStats()
+    call("fame6d_recover_eigenvectors", sys.handle, ptr(Y), k, float(tol), int(maxit), ptr(X), ptr(MX),
+         C.byref(st))
+    info = dict(iterations=list(st.iterations)[:k], relativeResiduals=list(st.relres)[:k],
+                massNorm2=list(st.mass_norm2)[:k])
+    return X, MX, info
+
+
+def compute_fourier_diagnostics(X, MX, lam, sys: QSystem):
+    k = X.shape[0]
+    lam = np.ascontiguousarray(np.asarray(lam, dtype=np.float64).reshape(-1))
+    res, div, ray = np.empty(k), np.empty(k), np.empty(k)
+    call("fame6d_fourier_diagnostics", sys.handle, ptr(X.contiguous()), ptr(MX.contiguous()),
+         lam.ctypes.data_as(C.c_void_p), k, res.ctypes.data_as(C.c_void_p),
+         div.ctypes.data_as(C.c_void_p), ray.ctypes.data_as(C.c_void_p))
+    return dict(resGEVP=res, divRelative2=div, rayleigh=ray, maxResGEVP=res.max(), maxDivRelative2=div.max())

@@ -1,0 +1,604 @@
+"""CPU oracle: a numpy FP64 restatement of the reference MATLAB hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker.  The product
+(the HIP library and its host package) never calls it.
+
+Every function cites the reference file:line it restates (paths relative to the
+reference repository root).  Conventions (vectorisation, frequency order, FFT
+normalisation, stacking) are those of the reference:
+
+- a grid vector is the (2N)^6 array vectorised column-major (first index fastest);
+- frequency index k maps to k for k < N and k - 2N otherwise;
+- fftn is unnormalised and ifftn carries 1/NF, M_s x = fftn(eps .* ifftn(x));
+- multi-component vectors are stacked component-first.
+
+Parity pins (see tests/test_oracle.py): the reference-held eigenvalues of
+exp3/make_config_experiment4.m:40, the rank-1 lattice identity of
+exp1/smoke_test_rank1_lowmem.m:18-23, dense brute force at N = 1, closed forms
+(constant eps, the inverse mass with 1/eps), and the toy-operator Lanczos test of
+exp1/smoke_test_rank1_lowmem.m:44-54.
+"""
+import math
+
+import numpy as np
+
+# Projection and period of every reference configuration (exp1/make_config_exp1bc.m:17-26).
+P_REF = np.array([[1.0, 0.0, 0.0],
+                  [0.0, 1.0, 0.0],
+                  [0.0, 0.0, 1.0],
+                  [math.sqrt(2.0), 0.0, 0.0],
+                  [0.0, math.sqrt(3.0), 0.0],
+                  [0.0, 0.0, math.sqrt(5.0)]])
+
+EPS = np.finfo(float).eps
+
+
+# ----------------------------------------------------------------------------- grid
+def fourier_freq(N):
+    """freq1 = [0:N-1, -N:-1] (exp1/build_fourier_grid_6d.m:12)."""
+    return np.concatenate([np.arange(N), np.arange(-N, 0)]).astype(float)
+
+
+def grid_shape(N):
+    return (2 * N,) * 6
+
+
+def vec(a):
+    """Column-major vectorisation of a 6D grid array."""
+    return np.asarray(a).reshape(-1, order="F")
+
+
+def unvec(x, N):
+    return np.asarray(x).reshape(grid_shape(N), order="F")
+
+
+def coord_grids(N):
+    """x_k = 2*pi*j/nGrid on the 6D grid (exp1/build_experiment1_common.m:10-11)."""
+    n = 2 * N
+    c = np.arange(n) / n * 2 * np.pi
+    return np.meshgrid(*([c] * 6), indexing="ij")
+
+
+def eps_reference(N, epsC=4.0, alpha=1.2):
+    """eps = epsC*exp((alpha/6)*sum cos x_k) (exp1/build_smooth_scalar_material_compact.m:3-7)."""
+    xs = coord_grids(N)
+    phi = (alpha / 6.0) * sum(np.cos(x) for x in xs)
+    return vec(epsC * np.exp(phi))
+
+
+def eps_cosine(N, amp, phases):
+    """eps = 1 + (amp/6)*sum cos(2*pi*y_k + phi_k), y_k = j/nGrid: the cosine family of the
+    benchmark configurations with the amplitude split over the six terms so that eps > 0."""
+    n = 2 * N
+    y = np.arange(n) / n
+    ys = np.meshgrid(*([y] * 6), indexing="ij")
+    s = sum(np.cos(2 * np.pi * ys[k] + phases[k]) for k in range(6))
+    return vec(1.0 + (amp / 6.0) * s)
+
+
+# ----------------------------------------------------------------------------- mass operator
+def mass_apply(eps, x, N):
+    """M_s x = fftn(eps .* ifftn(x)) per column (exp1/applyMij_fft.m:9-23)."""
+    x = np.asarray(x, dtype=complex)
+    E = unvec(eps, N)
+    if x.ndim == 1:
+        return vec(np.fft.fftn(E * np.fft.ifftn(unvec(x, N))))
+    return np.stack([mass_apply(eps, x[:, j], N) for j in range(x.shape[1])], axis=1)
+
+
+def mass_inverse_apply(eps, x, N):
+    """M_s^{-1} x = fftn(ifftn(x) ./ eps): closed form of the inverse circulant operator."""
+    E = unvec(eps, N)
+    return vec(np.fft.fftn(np.fft.ifftn(unvec(x, N)) / E))
+
+
+def rank1_plan(N, dim=6):
+    """Exact radix rank-1 plan (exp1/build_radix_exact_rank1_plan_scalar.m:18-63)."""
+    n = 2 * N
+    B = 2 * n - 1
+    NF = n ** dim
+    z = B ** np.arange(dim)
+    M = n * B ** (dim - 1)
+    s = n - 1
+    if M >= 2 ** 32 - 1:
+        raise ValueError("Rank-1 length exceeds uint32 indexing range")
+    idx = np.arange(NF)
+    tmp = idx.copy()
+    code = np.zeros(NF, dtype=np.int64)
+    for j in range(dim):
+        kj = tmp % n
+        tmp //= n
+        code += kj * z[j]
+    shift = (s * int(z.sum())) % M
+    target = (code + shift) % M
+    return dict(N=N, n=n, B=B, NF=NF, M=M, z=z, input=code, target=target)
+
+
+def rank1_kernel_fft(eps, plan):
+    """kernelFFT = fft(h), h(code) = epsHat[residue(code)]/NF
+    (exp1/build_exact_rank1_scalar_kernel_fft.m:9-37)."""
+    N, n, B, M = plan["N"], plan["n"], plan["B"], plan["M"]
+    Ehat = vec(np.fft.fftn(unvec(eps, N)))
+    code = np.arange(M)
+    tmp = code.copy()
+    res = np.zeros(M, dtype=np.int64)
+    stride = 1
+    s = n - 1
+    for _ in range(6):
+        digit = tmp % B
+        tmp //= B
+        res += ((digit - s) % n) * stride
+        stride *= n
+    h = Ehat[res] / plan["NF"]
+    return np.fft.fft(h)
+
+
+def rank1_apply(x, kfft, plan):
+    """applyMij_rank1_exact: scatter -> fft -> .*kernelFFT -> ifft -> gather
+    (exp1/applyMij_rank1_exact.m:4-5, rank1_exact_fft_scatter.m:10-12, rank1_exact_ifft_gather.m:9-10)."""
+    w = np.zeros(plan["M"], dtype=complex)
+    w[plan["input"]] = x
+    w = np.fft.ifft(kfft * np.fft.fft(w))
+    return w[plan["target"]]
+
+
+# ----------------------------------------------------------------------------- q-system
+def klift(q, P=P_REF):
+    """k_lift = P*((P.'*P)\\q) (exp1/build_experiment1_q_system_C_compact.m:9)."""
+    return P @ np.linalg.solve(P.T @ P, np.asarray(q, float))
+
+
+def projected_q_modes(N, q, P=P_REF):
+    """qModes(:,c) = sum_l P(l,c)*(Xi_l + k_lift_l) (exp3/build_projected_q_modes.m:14-20);
+    Xi = freq on the grid with T = 2*pi (exp1/build_fourier_grid_6d.m:14-19)."""
+    f = fourier_freq(N)
+    Xi = [vec(g) for g in np.meshgrid(*([f] * 6), indexing="ij")]
+    kl = klift(q, P)
+    Q = np.zeros((len(Xi[0]), 3))
+    for l in range(6):
+        for c in range(3):
+            Q[:, c] += P[l, c] * (Xi[l] + kl[l])
+    return Q
+
+
+def qsystem(N, q, P=P_REF):
+    """Diagonals of the null-space-free basis and the compact Method-C system
+    (exp1/assemble_K_and_nullspace_free_basis.m:18-105,
+     exp1/build_experiment1_q_system_C_compact.m:9-31)."""
+    Q = projected_q_modes(N, q, P)
+    q1, q2, q3 = Q[:, 0], Q[:, 1], Q[:, 2]
+    Qq = q1 ** 2 + q2 ** 2 + q3 ** 2
+    if Qq.min() <= 1e-13:
+        raise ValueError("Gamma point: zero projected Fourier mode")
+    sq = np.sqrt(Qq)
+    Qs = q1 + q2 + q3
+    den = 3 * Qq - Qs ** 2
+    reg = den > 1e-13
+    u = np.zeros((3, 2, len(Qq)))
+    s1 = 1.0 / np.sqrt(Qq[reg] * den[reg])
+    s2 = 1.0 / np.sqrt(den[reg])
+    for c, qc in enumerate((q1, q2, q3)):
+        u[c, 0, reg] = (Qq[reg] - qc[reg] * Qs[reg]) * s1
+    u[0, 1, reg] = (q3[reg] - q2[reg]) * s2
+    u[1, 1, reg] = (q1[reg] - q3[reg]) * s2
+    u[2, 1, reg] = (q2[reg] - q1[reg]) * s2
+    for m in np.nonzero(~reg)[0]:  # singular branch (:75-91)
+        qv = np.array([q1[m], q2[m], q3[m]])
+        h = qv / np.linalg.norm(qv)
+        a = np.zeros(3)
+        a[np.argmin(np.abs(h))] = 1.0
+        z1 = a - h * (h @ a)
+        z1 /= np.linalg.norm(z1)
+        z2 = np.cross(h, z1)
+        z2 /= np.linalg.norm(z2)
+        u[:, 0, m] = z1
+        u[:, 1, m] = z2
+    V = u * sq
+    V11, V21, V31 = V[0, 0], V[1, 0], V[2, 0]
+    V12, V22, V32 = V[0, 1], V[1, 1], V[2, 1]
+    det = V11 * V22 - V12 * V21
+    W1 = (V22 * V31 - V21 * V32) / det
+    W2 = (V11 * V32 - V12 * V31) / det
+    invC = 1.0 / np.sqrt(1.0 + W1 ** 2 + W2 ** 2)
+    return dict(N=N, NF=len(Qq), q1=q1, q2=q2, q3=q3, Qq=Qq, V11=V11, V12=V12, V21=V21, V22=V22,
+                V31=V31, V32=V32, det=det, W1=W1, W2=W2, invC=invC, klift=klift(q, P),
+                nsing=int((~reg).sum()))
+
+
+def apply_V(sys, y):
+    """V y (exp1/apply_V_blockdiag.m:12-16)."""
+    n = sys["NF"]
+    y1, y2 = y[:n], y[n:]
+    return np.concatenate([sys["V11"] * y1 + sys["V12"] * y2,
+                           sys["V21"] * y1 + sys["V22"] * y2,
+                           sys["V31"] * y1 + sys["V32"] * y2])
+
+
+def apply_VH(sys, x):
+    """V^* x (exp1/apply_VH_blockdiag.m:13-15)."""
+    n = sys["NF"]
+    x1, x2, x3 = x[:n], x[n:2 * n], x[2 * n:]
+    return np.concatenate([sys["V11"] * x1 + sys["V21"] * x2 + sys["V31"] * x3,
+                           sys["V12"] * x1 + sys["V22"] * x2 + sys["V32"] * x3])
+
+
+def apply_K(sys, x):
+    """Kop (exp1/assemble_K_and_nullspace_free_basis.m:110-119)."""
+    n = sys["NF"]
+    a, b, c = sys["q1"], sys["q2"], sys["q3"]
+    x1, x2, x3 = x[:n], x[n:2 * n], x[2 * n:]
+    return np.concatenate([(b * b + c * c) * x1 - a * b * x2 - a * c * x3,
+                           -a * b * x1 + (a * a + c * c) * x2 - b * c * x3,
+                           -a * c * x1 - b * c * x2 + (a * a + b * b) * x3])
+
+
+def mass3_apply(eps, x, N):
+    """mass.Mop: M = I3 (x) M_s on component-stacked vectors."""
+    n = (2 * N) ** 6
+    return np.concatenate([mass_apply(eps, x[c * n:(c + 1) * n], N) for c in range(3)])
+
+
+def kr_apply(sys, eps, y, N):
+    """K_r y = V^* M^{-1} V y with the closed-form inverse circulant (1/eps)."""
+    n = sys["NF"]
+    v = apply_V(sys, y)
+    return apply_VH(sys, np.concatenate(
+        [mass_inverse_apply(eps, v[c * n:(c + 1) * n], N) for c in range(3)]))
+
+
+def solve_v1h(sys, q):
+    """y = V1^{-*} q (exp1/solveV1H_blockdiag.m:7-8)."""
+    n = sys["NF"]
+    x1, x2 = q[:n], q[n:]
+    d = np.conj(sys["det"])
+    return np.concatenate([(np.conj(sys["V22"]) * x1 - np.conj(sys["V21"]) * x2) / d,
+                           (-np.conj(sys["V12"]) * x1 + np.conj(sys["V11"]) * x2) / d])
+
+
+def solve_v1(sys, x):
+    """y = V1^{-1} x (exp1/solveV1_blockdiag.m:7-8)."""
+    n = sys["NF"]
+    x1, x2 = x[:n], x[n:]
+    d = sys["det"]
+    return np.concatenate([(sys["V22"] * x1 - sys["V12"] * x2) / d,
+                           (-sys["V21"] * x1 + sys["V11"] * x2) / d])
+
+
+def mhat_apply(sys, eps, x, N):
+    """Mhat x = invC.*(M z + conj(W1).*M(W1.*z) + conj(W2).*M(W2.*z)), z = invC.*x
+    (exp1/apply_Mhat_C_scalar_compact.m:8-21)."""
+    z = sys["invC"] * x
+    s = mass_apply(eps, z, N)
+    s = s + np.conj(sys["W1"]) * mass_apply(eps, sys["W1"] * z, N)
+    s = s + np.conj(sys["W2"]) * mass_apply(eps, sys["W2"] * z, N)
+    return sys["invC"] * s
+
+
+def pcg(A, b, tol, maxit):
+    """MATLAB pcg without preconditioner, x0 = 0 (call sites
+    exp1/make_explicit_krinv_compact.m:43, exp2/recover_original_eigenvectors_exp2.m:21).
+    Returns (x, flag, relres, iter).  Restates the recurrence, the true-residual
+    confirmation and the 'moresteps' continuation; the stagnation test and the
+    minimum-residual iterate on failure are not restated."""
+    n = b.shape[0]
+    n2b = np.linalg.norm(b)
+    x = np.zeros_like(b, dtype=complex)
+    if n2b == 0:
+        return x, 0, 0.0, 0
+    tolb = tol * n2b
+    r = b.astype(complex).copy()
+    normr = n2b
+    if normr <= tolb:
+        return x, 0, normr / n2b, 0
+    rho = 1.0
+    p = None
+    moresteps = 0
+    maxmsteps = min(n // 50, 5, n - maxit)
+    flag, it = 1, maxit
+    for ii in range(1, maxit + 1):
+        rho1 = rho
+        rho = np.vdot(r, r).real
+        if rho == 0 or np.isinf(rho):
+            flag, it = 4, ii - 1
+            break
+        if ii == 1:
+            p = r.copy()
+        else:
+            beta = rho / rho1
+            if beta == 0 or np.isinf(beta):
+                flag, it = 4, ii - 1
+                break
+            p = r + beta * p
+        qv = A(p)
+        pq = np.vdot(p, qv).real
+        if pq <= 0 or np.isinf(pq):
+            flag, it = 4, ii - 1
+            break
+        alpha = rho / pq
+        x = x + alpha * p
+        r = r - alpha * qv
+        normr = np.linalg.norm(r)
+        if normr <= tolb or moresteps:
+            r = b - A(x)
+            normr = np.linalg.norm(r)
+            if normr <= tolb:
+                flag, it = 0, ii
+                break
+            moresteps += 1
+            if moresteps >= maxmsteps:
+                flag, it = 3, ii
+                break
+    return x, flag, normr / n2b, it
+
+
+def krinv_apply(sys, eps, q, N, tol=1e-10, maxit=1200):
+    """Method-C K_r^{-1} q (exp1/make_explicit_krinv_compact.m:24-64)."""
+    n = sys["NF"]
+    r = solve_v1h(sys, q)
+    mr1 = mass_apply(eps, r[:n], N)
+    mr2 = mass_apply(eps, r[n:], N)
+    bhat = sys["invC"] * (-(np.conj(sys["W1"]) * mr1 + np.conj(sys["W2"]) * mr2))
+    u, flag, relres, it = pcg(lambda v: mhat_apply(sys, eps, v, N), bhat, tol, maxit)
+    if not np.all(np.isfinite(u)) or not np.isfinite(relres) or (flag != 0 and relres > 10 * tol):
+        raise RuntimeError("Mhat CG failed: flag=%d relres=%.3e iter=%d" % (flag, relres, it))
+    z = sys["invC"] * u
+    mr1 = mr1 + mass_apply(eps, sys["W1"] * z, N)
+    mr2 = mr2 + mass_apply(eps, sys["W2"] * z, N)
+    return solve_v1(sys, np.concatenate([mr1, mr2])), dict(flag=flag, relres=relres, iter=it)
+
+
+# ----------------------------------------------------------------------------- Lanczos
+def lanczos_hybrid(A, n, k, tol, max_restart, p, v0):
+    """Thick-restart Lanczos for the largest eigenvalues of a Hermitian operator
+    (exp1/Lanczos_hybrid_identity_gpuop.m:11-130), with the reference's column-by-column
+    Gram-Schmidt (:132-136).  Returns (theta[k] descending, ritz vectors n x k, info)."""
+    v = np.asarray(v0, dtype=complex).copy()
+    nv = np.linalg.norm(v)
+    if not (np.isfinite(nv) and nv > 0):
+        raise ValueError("Invalid Lanczos start vector.")
+    v /= nv
+    Q = np.zeros((n, p), dtype=complex)
+    k0 = k
+    Alpha, Beta = [], []
+    beta = 0.0
+    d = np.zeros(0)
+    c = np.zeros(0)
+    just_restarted = False
+    sizeV = 1
+    kk = 0
+    info = dict(converged=False, num_converged=0, hit_max_restart=False, restarts=0, steps=0)
+
+    def gs(Qm, r, jj):
+        for ii in range(jj):
+            r = r - np.vdot(Qm[:, ii], r) * Qm[:, ii]
+        return r
+
+    Us = ind = None
+    for mm in range(1, max_restart + 1):
+        for jj in range(sizeV, p + 1):
+            Q[:, jj - 1] = v
+            Aq = A(Q[:, jj - 1])
+            if not np.all(np.isfinite(Aq)):
+                raise RuntimeError("Lanczos inverse action returned invalid data.")
+            alpha = np.real(np.vdot(Aq, Q[:, jj - 1]))
+            if jj == 1:
+                r = Aq - alpha * Q[:, jj - 1]
+            elif just_restarted:
+                r = gs(Q, Aq, jj)
+                just_restarted = False
+            else:
+                r = Aq - alpha * Q[:, jj - 1] - beta * Q[:, jj - 2]
+                corr = np.vdot(Q[:, jj - 2], r)
+                r = r - corr * Q[:, jj - 2]
+            corr = np.vdot(Q[:, jj - 1], r)
+            r = r - corr * Q[:, jj - 1]
+            r = gs(Q, r, jj)
+            r = gs(Q, r, jj)
+            beta2 = np.real(np.vdot(r, r))
+            if beta2 <= 0 or math.sqrt(beta2) < 1e-14:
+                beta = math.sqrt(max(beta2, 0.0))
+                break
+            beta = math.sqrt(beta2)
+            v = r / beta
+            Alpha.append(alpha)
+            Beta.append(beta)
+            kk += 1
+        na, nd = len(Alpha), len(d)
+        m = nd + na
+        if m == 0:
+            raise RuntimeError("Lanczos produced no projected data.")
+        H = np.zeros((m, m))
+        H[:nd, :nd] = np.diag(d)
+        H[nd:, nd:] = np.diag(Alpha) + (np.diag(Beta[:-1], 1) + np.diag(Beta[:-1], -1) if na > 1 else 0)
+        if nd:
+            H[:k, k] = c
+            H[k, :k] = c
+        Alpha, Beta = [], []
+        w, U = np.linalg.eigh(H)
+        res = np.abs(beta * U[-1, :])
+        ind = np.argsort(-w, kind="stable")
+        d = w[ind]
+        res = res[ind]
+        Us = U
+        nw = min(k0, len(res))
+        thr = tol * np.maximum(EPS ** (2.0 / 3.0), np.abs(d[:nw]))
+        nconv = int(np.sum(res[:nw] < thr))
+        info.update(num_converged=nconv, restarts=mm, steps=kk, max_ritz_residual=float(res[:nw].max()))
+        if nconv >= k0:
+            info["converged"] = True
+            break
+        if mm == max_restart:
+            info["hit_max_restart"] = True
+            break
+        k = k0 + min(nconv, (p - k0) // 2)
+        if k == 1 and p > 3:
+            k = p // 2
+        Uk = U[:, ind[:k]]
+        Q[:, :k] = Q[:, :m] @ Uk
+        d = d[:k]
+        c = beta * Uk[-1, :]
+        just_restarted = True
+        sizeV = k + 1
+    if ind is None or len(ind) < k0:
+        raise RuntimeError("Lanczos ended without k inverse Ritz values.")
+    theta = d[:k0].real
+    Y = Q[:, :Us.shape[0]] @ Us[:, ind[:k0]]
+    return theta, Y, info
+
+
+def eigs_method_c(N, eps, q, v0, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10,
+                  maxit_cg=1200, P=P_REF):
+    """run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:4-21): lambda = sort(1./theta)."""
+    sys = qsystem(N, q, P)
+    nr = 2 * sys["NF"]
+    stats = dict(calls=0, iters=0)
+
+    def A(x):
+        y, st = krinv_apply(sys, eps, x, N, tol_cg, maxit_cg)
+        stats["calls"] += 1
+        stats["iters"] += st["iter"]
+        return y
+
+    theta, Y, info = lanczos_hybrid(A, nr, k, tol, max_restart, p, v0)
+    if not info["converged"]:
+        raise RuntimeError("Method C outer Lanczos did not converge")
+    if np.any(theta <= 0):
+        raise RuntimeError("Method C invalid inverse Ritz values")
+    order = np.argsort(1.0 / theta, kind="stable")
+    info.update(stats)
+    return (1.0 / theta)[order], Y[:, order], info, sys
+
+
+def recover_eigenvectors(sys, eps, Y, N, tol=1e-12, maxit=3000):
+    """x_j = M^{-1} V y_j by PCG, M-normalised (exp2/recover_original_eigenvectors_exp2.m:18-51)."""
+    X, MX, its = [], [], []
+    for j in range(Y.shape[1]):
+        rhs = apply_V(sys, Y[:, j])
+        x, flag, rr, it = pcg(lambda v: mass3_apply(eps, v, N), rhs, tol, maxit)
+        if flag != 0 and rr > 10 * tol:
+            raise RuntimeError("Recovery CG failed")
+        mx = mass3_apply(eps, x, N)
+        nj2 = np.real(np.vdot(x, mx))
+        if not (np.isfinite(nj2) and nj2 > 0):
+            raise RuntimeError("BadMassNorm")
+        X.append(x / math.sqrt(nj2))
+        MX.append(mx / math.sqrt(nj2))
+        its.append(it)
+    return np.stack(X, 1), np.stack(MX, 1), its
+
+
+def fourier_diagnostics(sys, X, MX, lam):
+    """GEVP residual, divergence and Rayleigh quotient (exp2/compute_fourier_diagnostics_exp2.m:10-19)."""
+    n = sys["NF"]
+    res, div, ray = [], [], []
+    for j in range(X.shape[1]):
+        x, mx = X[:, j], MX[:, j]
+        kx = apply_K(sys, x)
+        ray.append(np.real(np.vdot(x, kx) / np.vdot(x, mx)))
+        den = np.linalg.norm(kx) + abs(lam[j]) * np.linalg.norm(mx)
+        res.append(np.linalg.norm(kx - lam[j] * mx) / max(den, EPS))
+        g = sys["q1"] * mx[:n] + sys["q2"] * mx[n:2 * n] + sys["q3"] * mx[2 * n:]
+        div.append(np.linalg.norm(g) / max(np.linalg.norm(mx), EPS))
+    return np.array(res), np.array(div), np.array(ray)
+
+
+# ----------------------------------------------------------------------------- Taylor
+def multi_index_list(p):
+    """All alpha with |alpha| <= p, a1 outermost (exp2/build_multi_index_list_3d_exp3.m:3-14)."""
+    out = []
+    for a1 in range(p + 1):
+        for a2 in range(p - a1 + 1):
+            for a3 in range(p - a1 - a2 + 1):
+                out.append((a1, a2, a3))
+    return np.array(out, dtype=int)
+
+
+def taylor_coeffs(p):
+    """coeffAlpha = i^|alpha| * exp(-sum gammaln(alpha+1)) (exp3/prepare_taylor_fourier_basis_exp4.m:27-31)."""
+    al = multi_index_list(p)
+    return np.array([(1j) ** int(a.sum()) * math.exp(-sum(math.lgamma(v + 1) for v in a)) for a in al])
+
+
+def taylor_moments(qmodes, coeff, p, centers):
+    """moments(alpha, c) = coeffAlpha .* (Qalpha.' * (exp(i qModes*r_c) .* coeff))
+    (exp3/reconstruct_component_taylor_batch_exp4.m:14-23, prepare_taylor_fourier_basis_exp4.m:12-24)."""
+    al = multi_index_list(p)
+    Qa = np.stack([qmodes[:, 0] ** a[0] * qmodes[:, 1] ** a[1] * qmodes[:, 2] ** a[2] for a in al], 1)
+    phase = np.exp(1j * (qmodes @ np.asarray(centers).T))
+    return taylor_coeffs(p)[:, None] * (Qa.T @ (phase * coeff[:, None]))
+
+
+def taylor_evaluate(points, center_id, centers, moments, p):
+    """u(pt) = sum_alpha (r - r_c)^alpha moments(alpha, c(pt))
+    (exp3/reconstruct_component_taylor_batch_exp4.m:33-38, build_taylor_geometry_exp4.m:7-23)."""
+    al = multi_index_list(p)
+    R = points - np.asarray(centers)[center_id]
+    u = np.zeros(len(points), dtype=complex)
+    for ia, a in enumerate(al):
+        u += R[:, 0] ** a[0] * R[:, 1] ** a[1] * R[:, 2] ** a[2] * moments[ia, center_id]
+    return u
+
+
+def direct_reconstruct(qmodes, coeff, points, block=256):
+    """u = exp(i R q^T) c in point blocks (exp2/reconstruct_field_direct_blocked_exp3.m:10-15)."""
+    u = np.zeros(len(points), dtype=complex)
+    for s in range(0, len(points), block):
+        u[s:s + block] = np.exp(1j * (points[s:s + block] @ qmodes.T)) @ coeff
+    return u
+
+
+def yee_points(n, h, comp):
+    """Plus-grid Yee points of component comp (0,1,2): h*(I + 0.5 e_comp), I in -(n+1)..n+1
+    (exp2/build_yee_edge_points_plus.m:11-17)."""
+    idx = np.arange(-(n + 1), n + 2, dtype=float)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    pts = np.stack([I.reshape(-1, order="F"), J.reshape(-1, order="F"), K.reshape(-1, order="F")], 1)
+    pts[:, comp] += 0.5
+    return h * pts
+
+
+def block_partition(m, nblocks):
+    """nblocks^3 nearly equal contiguous blocks of an m^3 grid; first mod(m,nblocks) ranges one
+    longer; block order i fastest (exp2/build_yee_block_partition_exp3.m:1-31).  Returns the
+    block id of every grid point (column-major)."""
+    base, rem = divmod(m, nblocks)
+    lab = np.zeros(m, dtype=int)
+    s = 0
+    for b in range(nblocks):
+        ln = base + (1 if b < rem else 0)
+        lab[s:s + ln] = b
+        s += ln
+    I, J, K = np.meshgrid(lab, lab, lab, indexing="ij")
+    bid = I + nblocks * J + nblocks * nblocks * K
+    return bid.reshape(-1, order="F")
+
+
+def block_centers(points, bid, nb3):
+    """Center = midpoint of each block's bounding box (exp3/build_taylor_geometry_exp4.m:10-11)."""
+    C = np.zeros((nb3, 3))
+    for b in range(nb3):
+        R = points[bid == b]
+        C[b] = (R.min(0) + R.max(0)) / 2
+    return C
+
+
+# ----------------------------------------------------------------------------- adjoint
+def mass_adjoint_eps(eps, x, gy, N):
+    """d/d eps of Re<gy, M_s x> = NF * Re(conj(ifftn gy) .* ifftn x) (derivative of exp1/applyMij_fft.m:11)."""
+    n = (2 * N) ** 6
+    u = np.fft.ifftn(unvec(x, N))
+    v = np.fft.ifftn(unvec(gy, N))
+    return vec(n * np.real(np.conj(v) * u))
+
+
+def eig_grad_eps(X, lam, N):
+    """Hellmann-Feynman: d lambda / d eps_j = -lambda * NF * sum_c |ifftn(x_c)_j|^2 for M-normalised x."""
+    n = (2 * N) ** 6
+    out = []
+    for j in range(X.shape[1]):
+        g = np.zeros(grid_shape(N))
+        for c in range(3):
+            g += np.abs(np.fft.ifftn(unvec(X[c * n:(c + 1) * n, j], N))) ** 2
+        out.append(vec(-lam[j] * n * g))
+    return np.stack(out, 1)

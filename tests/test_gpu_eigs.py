@@ -1,0 +1,124 @@
+"""GPU parity of the q-system, Mhat, K_r^{-1}, Lanczos, recovery and batching."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN_N2 = (1.201276300237e-02, 4.496454358674e-02)  # exp3/make_config_experiment4.m:40
+Q_EXP4 = (0.13, 0.21, 0.17)
+Q_EXP1 = (0.05, 0.04, 0.03)
+
+
+def rand_c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def n2(pkg, oracle):
+    torch.cuda.set_device(0)
+    N = 2
+    eps = oracle.eps_reference(N)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, Q_EXP4)
+    yield dict(N=N, eps=eps, m=m, s=s, osys=oracle.qsystem(N, Q_EXP4))
+    s.close()
+    m.close()
+
+
+def test_qsystem_diagonals(n2):
+    print("case: qsys diagonals N=2", flush=True)
+    d = n2["s"].diagonals()
+    o = n2["osys"]
+    for k in ("V11", "V12", "V21", "V22", "V31", "V32", "W1", "W2", "invC"):
+        assert np.allclose(d[k], o[k], rtol=1e-13, atol=1e-13), k
+    assert np.allclose(n2["s"].info["kLift"], o["klift"], atol=1e-16)
+
+
+def test_mhat(n2, oracle):
+    print("case: Mhat N=2", flush=True)
+    rng = np.random.default_rng(1)
+    x = rand_c(rng, n2["m"].NF)
+    ref = oracle.mhat_apply(n2["osys"], n2["eps"], x, n2["N"])
+    y = n2["s"].Mhat(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
+    y32 = n2["s"].Mhat(torch.as_tensor(x, device="cuda").to(torch.complex64), "single").cpu().numpy()
+    assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 5e-6
+
+
+def test_krinv_fp64_and_mixed(n2, oracle):
+    print("case: krinv N=2", flush=True)
+    rng = np.random.default_rng(2)
+    q = rand_c(rng, 2 * n2["m"].NF)
+    ref, st_ref = oracle.krinv_apply(n2["osys"], n2["eps"], q, n2["N"], 1e-10, 3000)
+    y, st = n2["s"].krinv(torch.as_tensor(q, device="cuda"), 1e-10, 3000)
+    y = y.cpu().numpy()
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-9
+    assert st["last_flag"] == 0 and abs(st["last_iterations"] - st_ref["iter"]) <= 2
+    ym, stm = n2["s"].krinv(torch.as_tensor(q, device="cuda"), 1e-10, 3000, "mixed")
+    ym = ym.cpu().numpy()
+    assert np.linalg.norm(ym - ref) / np.linalg.norm(ref) < 1e-9
+    assert stm["refinement_cycles"] >= 1
+
+
+def test_eigs_golden_n2(pkg, n2):
+    print("case: eigs golden N=2 fp64", flush=True)
+    v0 = pkg.start_vector(n2["s"].nr, 20260831)
+    out = pkg.run_method_C(n2["s"], k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
+                           v0=v0, want_vectors=True)
+    lam = out["lambda_"]
+    assert out["stats"]["converged"]
+    assert abs(lam[0] - GOLDEN_N2[0]) / GOLDEN_N2[0] < 1e-10
+    assert abs(lam[9] - GOLDEN_N2[1]) / GOLDEN_N2[1] < 1e-10
+    n2["lam"] = lam
+    n2["Y"] = out["Y"]
+
+
+def test_eigs_vs_oracle_and_mixed(pkg, oracle, n2):
+    print("case: eigs oracle + mixed N=2", flush=True)
+    v0 = pkg.start_vector(n2["s"].nr, 5)
+    lam_o, Y_o, info, _ = oracle.eigs_method_c(n2["N"], n2["eps"], Q_EXP4, v0, k=10, p=40, tol=1e-10,
+                                               max_restart=30, tol_cg=1e-10, maxit_cg=3000)
+    out = pkg.run_method_C(n2["s"], k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000, v0=v0)
+    assert np.max(np.abs(out["lambda_"] - lam_o) / lam_o) < 1e-10
+    outm = pkg.run_method_C(n2["s"], k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
+                            v0=v0, precision="mixed")
+    assert np.max(np.abs(outm["lambda_"] - lam_o) / lam_o) < 1e-8
+
+
+def test_recovery_and_diagnostics(pkg, oracle, n2):
+    print("case: recovery N=2", flush=True)
+    if "Y" not in n2:
+        pytest.skip("needs test_eigs_golden_n2")
+    Y = n2["Y"][:2]
+    X, MX, info = pkg.recover_original_eigenvectors(Y, n2["s"])
+    Xo, MXo, _ = oracle.recover_eigenvectors(n2["osys"], n2["eps"], Y.cpu().numpy().T, n2["N"])
+    # eigenvectors agree up to a phase
+    for j in range(2):
+        a = X[j].cpu().numpy()
+        b = Xo[:, j]
+        ph = np.vdot(b, a) / abs(np.vdot(b, a))
+        assert np.linalg.norm(a - ph * b) / np.linalg.norm(b) < 1e-9
+    d = pkg.compute_fourier_diagnostics(X, MX, n2["lam"][:2], n2["s"])
+    res_o, div_o, ray_o = oracle.fourier_diagnostics(n2["osys"], Xo, MXo, n2["lam"][:2])
+    assert np.allclose(d["rayleigh"], ray_o, rtol=1e-9)
+    assert np.allclose(d["resGEVP"], res_o, rtol=1e-3, atol=1e-12)
+
+
+def test_batched_bloch_points(pkg, oracle, n2):
+    print("case: batched 3 Bloch points N=2", flush=True)
+    qs = np.array([[0.05, 0.04, 0.03], [0.13, 0.21, 0.17], [0.2, 0.1, 0.05]])
+    nr = n2["s"].nr
+    v0 = np.stack([pkg.start_vector(nr, 10 + i) for i in range(3)])
+    outb = pkg.run_method_C_batch(n2["m"], qs, k=6, p=30, tol=1e-10, max_restart=40, v0=v0)
+    for i in range(3):
+        s = pkg.QSystem(n2["m"], qs[i])
+        out = pkg.run_method_C(s, k=6, p=30, tol=1e-10, max_restart=40, v0=v0[i])
+        assert np.max(np.abs(outb["lambda_"][i] - out["lambda_"]) / out["lambda_"]) < 1e-11
+        s.close()
+
+
+def test_gamma_point_rejected(pkg, n2):
+    with pytest.raises(pkg.Fame6dError) as e:
+        pkg.QSystem(n2["m"], (0.0, 0.0, 0.0))
+    assert e.value.name == "EGAMMA"
