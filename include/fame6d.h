@@ -70,6 +70,10 @@ typedef struct {
  * eps_host: NF real permittivity samples, column-major. stream: hipStream_t or NULL. */
 int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass** out);
 int fame6d_mass_destroy(fame6d_mass* m);
+/* Replace the permittivity samples of an existing backend (same N); used by gradient
+ * checks and optimisation loops over eps.  No reference counterpart (the reference
+ * rebuilds the backend, exp1/build_experiment1_common.m:12-18). */
+int fame6d_mass_set_eps(fame6d_mass* m, const double* eps_host);
 int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
 
 /* Project-written tile autotuner: times every compiled tile variant of the three
@@ -209,6 +213,12 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
  * (device), u: M complex128 (device). */
 int fame6d_taylor_evaluate(long long M, const double* points, const int* center_id, int p, int C,
                            const double* centers_host, const void* moments, void* u, void* stream);
+/* The same evaluation on the (2n+3)^3 plus-grid Yee points of component comp (0,1,2),
+ * generated on the device (exp2/build_yee_edge_points_plus.m:11-17) and split into
+ * nblocks^3 centers (exp2/build_yee_block_partition_exp3.m:1-31); u: (2n+3)^3 complex128
+ * in column-major point order; C = nblocks^3. */
+int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, int C,
+                               const double* centers_host, const void* moments, void* u, void* stream);
 /* reconstruct_field_direct_blocked_exp3 (exp2/reconstruct_field_direct_blocked_exp3.m:10-15):
  * u = exp(i R q^T) coeff. */
 int fame6d_direct_reconstruct(long long NF, const double* qmodes, const void* coeff, long long M,

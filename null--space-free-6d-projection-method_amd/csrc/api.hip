@@ -104,6 +104,36 @@ extern "C" {
 const char* fame6d_last_error(void) { return g_last_error.c_str(); }
 int fame6d_version(void) { return 10000; }
 
+static void upload_eps(fame6d_mass* m, const double* eps_host) {
+  const long long NF = m->NF;
+  double mn = INFINITY, mx = -INFINITY, sum = 0;
+  for (long long i = 0; i < NF; ++i) {
+    const double e = eps_host[i];
+    if (!std::isfinite(e)) throw Error(FAME6D_ENONFINITE, "permittivity contains NaN/Inf");
+    mn = std::min(mn, e);
+    mx = std::max(mx, e);
+    sum += e;
+  }
+  if (!(mn > 0))
+    throw Error(FAME6D_EARG, "permittivity must be positive: M_s = F diag(eps) F^{-1} is then Hermitian "
+                             "positive definite, which the PCG solves require");
+  m->eps_min = mn;
+  m->eps_max = mx;
+  m->eps_mean = sum / (double)NF;
+  m->eps_host.assign(eps_host, eps_host + NF);
+  std::vector<double> e64(NF);
+  std::vector<float> e32(NF);
+  for (long long i = 0; i < NF; ++i) {
+    e64[i] = eps_host[i] / (double)NF;
+    e32[i] = (float)e64[i];
+  }
+  m->eps64.ensure(sizeof(double) * NF);
+  m->eps32.ensure(sizeof(float) * NF);
+  F6_HIP(hipMemcpyAsync(m->eps64.ptr, e64.data(), sizeof(double) * NF, hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(m->eps32.ptr, e32.data(), sizeof(float) * NF, hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipStreamSynchronize(m->stream));
+}
+
 int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass** out) {
   F6_API_BEGIN
   F6_REQUIRE(out != nullptr && eps_host != nullptr, FAME6D_EARG, "null argument");
@@ -118,32 +148,7 @@ int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass*
     m->NF = NF;
     m->stream = static_cast<hipStream_t>(stream);
     F6_HIP(hipGetDevice(&m->device));
-    m->eps_host.assign(eps_host, eps_host + NF);
-    double mn = INFINITY, mx = -INFINITY, sum = 0;
-    for (long long i = 0; i < NF; ++i) {
-      const double e = eps_host[i];
-      if (!std::isfinite(e)) throw Error(FAME6D_ENONFINITE, "permittivity contains NaN/Inf");
-      mn = std::min(mn, e);
-      mx = std::max(mx, e);
-      sum += e;
-    }
-    if (!(mn > 0))
-      throw Error(FAME6D_EARG, "permittivity must be positive: M_s = F diag(eps) F^{-1} is then Hermitian "
-                               "positive definite, which the PCG solves require");
-    m->eps_min = mn;
-    m->eps_max = mx;
-    m->eps_mean = sum / (double)NF;
-    std::vector<double> e64(NF);
-    std::vector<float> e32(NF);
-    for (long long i = 0; i < NF; ++i) {
-      e64[i] = eps_host[i] / (double)NF;
-      e32[i] = (float)e64[i];
-    }
-    m->eps64.ensure(sizeof(double) * NF);
-    m->eps32.ensure(sizeof(float) * NF);
-    F6_HIP(hipMemcpyAsync(m->eps64.ptr, e64.data(), sizeof(double) * NF, hipMemcpyHostToDevice, m->stream));
-    F6_HIP(hipMemcpyAsync(m->eps32.ptr, e32.data(), sizeof(float) * NF, hipMemcpyHostToDevice, m->stream));
-    F6_HIP(hipStreamSynchronize(m->stream));
+    upload_eps(m, eps_host);
   } catch (...) {
     m->eps64.release();
     m->eps32.release();
@@ -151,6 +156,13 @@ int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass*
     throw;
   }
   *out = m;
+  F6_API_END
+}
+
+int fame6d_mass_set_eps(fame6d_mass* m, const double* eps_host) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && eps_host, FAME6D_EARG, "null argument");
+  upload_eps(m, eps_host);
   F6_API_END
 }
 

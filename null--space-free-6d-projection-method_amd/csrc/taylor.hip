@@ -14,72 +14,151 @@
 namespace fame6d {
 
 constexpr int kTaylorMaxP = 16;
-constexpr int kTaylorChunk = 32;  // terms per workgroup in the moment kernel
 
 __host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 
-// Moments: workgroup (c, chunk) accumulates terms [t0, t0+32) for center c.
-__global__ __launch_bounds__(256) void k_taylor_moments(long long NF, const double* qx, const double* qy,
-                                                        const double* qz, const cpx<double>* coeff, int p,
-                                                        const double* centers, const int* alist,
-                                                        const cpx<double>* calpha, int L,
-                                                        cpx<double>* mom) {
-  __shared__ double red[256];
-  __shared__ int al[kTaylorChunk * 3];
-  const int c = blockIdx.x;
-  const int t0 = blockIdx.y * kTaylorChunk;
-  const int tc = min(kTaylorChunk, L - t0);
-  for (int i = threadIdx.x; i < tc * 3; i += blockDim.x) al[i] = alist[t0 * 3 + i];
-  __syncthreads();
-  const double rx = centers[3 * c], ry = centers[3 * c + 1], rz = centers[3 * c + 2];
-  double ar[kTaylorChunk], ai[kTaylorChunk];
+// ---------------------------------------------------------------------------- moments
+// moments = coeff_alpha .* (Q_alpha^T B), B(j, c) = exp(i q_j . r_c) coeff_j: a real
+// (L x NF) by complex (NF x C) product, done as one real GEMM with 2C columns
+// (re/im interleaved per center) on the FP64 matrix cores.
+//
+// v_mfma_f64_16x16x4f64 (wave64): lane l supplies A[m = l%16][k = l/16] and
+// B[k = l/16][n = l%16]; the 4 accumulator doubles of lane l are D[4*(l/16) + r][l%16].
+// Workgroup = 4 waves, one 16-column tile (8 centers) and one slice of the mode
+// range; the operands are generated in LDS per chunk of kTChunk modes: powers
+// q_x^s, q_y^s, q_z^s (A = products of three of them) and B = exp(i q.r_c) coeff.
+// Wave w owns row tiles w, w+4, ... of the (L rounded up to 16) alpha rows.
+constexpr int kTChunk = 64;
+constexpr int kTMaxTilesPerWave = 16;  // L <= 1024 (p <= 16)
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_taylor_moments_mfma(long long NF, const double* qx, const double* qy,
+                                                             const double* qz, const cpx<double>* coeff, int p,
+                                                             int L, int C, const double* centers,
+                                                             const unsigned char* alist, long long span,
+                                                             double* part) {
+  const int Lpad = (L + 15) & ~15;
+  const int T = Lpad / 16;
+  const int Cpad2 = ((C + 7) / 8) * 16;  // real columns
+  __shared__ double pw[3][(kTaylorMaxP + 1) * kTChunk];
+  __shared__ double bt[kTChunk * 16];
+  __shared__ unsigned char al[3 * 1024];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ct = blockIdx.x;  // column tile (8 centers)
+  const long long j_begin = (long long)blockIdx.y * span;
+  const long long j_end = min(NF, j_begin + span);
+  for (int i = tid; i < 3 * Lpad; i += 256) al[i] = (i < 3 * L) ? alist[i] : 0;
+
+  double4_t acc[kTMaxTilesPerWave];
 #pragma unroll
-  for (int k = 0; k < kTaylorChunk; ++k) ar[k] = ai[k] = 0.0;
-  for (long long j = blockIdx.z * (long long)blockDim.x + threadIdx.x; j < NF;
-       j += (long long)gridDim.z * blockDim.x) {
-    const double a = qx[j], b = qy[j], d = qz[j];
-    double sn, cs;
-    sincos(a * rx + b * ry + d * rz, &sn, &cs);
-    const cpx<double> z = mk(cs, sn) * coeff[j];
-    double px[kTaylorMaxP + 1], py[kTaylorMaxP + 1], pz[kTaylorMaxP + 1];
-    px[0] = py[0] = pz[0] = 1.0;
-    for (int s = 1; s <= p; ++s) {
-      px[s] = px[s - 1] * a;
-      py[s] = py[s - 1] * b;
-      pz[s] = pz[s - 1] * d;
+  for (int t = 0; t < kTMaxTilesPerWave; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  for (long long j0 = j_begin; j0 < j_end; j0 += kTChunk) {
+    __syncthreads();  // previous chunk fully consumed (and the alpha table staged)
+    for (int jj = tid; jj < kTChunk; jj += 256) {
+      const long long j = j0 + jj;
+      const bool ok = j < j_end;
+      const double a = ok ? qx[j] : 0.0, b = ok ? qy[j] : 0.0, d = ok ? qz[j] : 0.0;
+      double x = ok ? 1.0 : 0.0, y = x, z = x;
+      for (int s = 0; s <= p; ++s) {
+        pw[0][s * kTChunk + jj] = x;
+        pw[1][s * kTChunk + jj] = y;
+        pw[2][s * kTChunk + jj] = z;
+        x *= a;
+        y *= b;
+        z *= d;
+      }
     }
+    for (int e = tid; e < kTChunk * 8; e += 256) {
+      const int jj = e >> 3, cl = e & 7;
+      const long long j = j0 + jj;
+      const int c = ct * 8 + cl;
+      double re = 0.0, im = 0.0;
+      if (j < j_end && c < C) {
+        double sn, cs;
+        sincos(qx[j] * centers[3 * c] + qy[j] * centers[3 * c + 1] + qz[j] * centers[3 * c + 2], &sn, &cs);
+        const cpx<double> v = mk(cs, sn) * coeff[j];
+        re = v.re;
+        im = v.im;
+      }
+      bt[jj * 16 + 2 * cl] = re;
+      bt[jj * 16 + 2 * cl + 1] = im;
+    }
+    __syncthreads();
+    const int kl = lane >> 4, nl = lane & 15;
+#pragma unroll 2
+    for (int k4 = 0; k4 < kTChunk; k4 += 4) {
+      const int kk = k4 + kl;
+      const double bv = bt[kk * 16 + nl];
 #pragma unroll
-    for (int k = 0; k < kTaylorChunk; ++k) {
-      if (k < tc) {
-        const double w = px[al[3 * k]] * py[al[3 * k + 1]] * pz[al[3 * k + 2]];
-        ar[k] += w * z.re;
-        ai[k] += w * z.im;
+      for (int t = 0; t < kTMaxTilesPerWave; ++t) {
+        const int tile = wave + 4 * t;
+        if (tile < T) {
+          const int alpha = tile * 16 + nl;
+          const double av = pw[0][al[3 * alpha] * kTChunk + kk] * pw[1][al[3 * alpha + 1] * kTChunk + kk] *
+                            pw[2][al[3 * alpha + 2] * kTChunk + kk];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(alpha < L ? av : 0.0, bv, acc[t], 0, 0, 0);
+        }
       }
     }
   }
-  for (int k = 0; k < tc; ++k) {
-    const double sr = block_sum<256>(ar[k], red);
-    __syncthreads();
-    const double si = block_sum<256>(ai[k], red);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      // partial over grid.z slices accumulated by atomic-free second stage
-      mom[((long long)blockIdx.z * gridDim.x + c) * L + t0 + k] = mk(sr, si);
+  // partial sums of this mode slice: part[(slice*Lpad + alpha)*Cpad2 + col]
+#pragma unroll
+  for (int t = 0; t < kTMaxTilesPerWave; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile < T) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int alpha = tile * 16 + 4 * (lane >> 4) + r;
+        const int col = ct * 16 + (lane & 15);
+        part[((long long)blockIdx.y * Lpad + alpha) * Cpad2 + col] = acc[t][r];
+      }
     }
   }
 }
 
-// Final moments: sum the z-slices and multiply by i^|alpha|/alpha!.
-__global__ void k_taylor_finish(int L, int C, int nz, const cpx<double>* part, const cpx<double>* calpha,
+// Final moments: sum the mode slices, form re + i*im and multiply by i^|alpha|/alpha!.
+__global__ void k_taylor_finish(int L, int C, int nslices, const double* part, const cpx<double>* calpha,
                                 cpx<double>* mom) {
+  const int Lpad = (L + 15) & ~15;
+  const int Cpad2 = ((C + 7) / 8) * 16;
   const long long tot = (long long)L * C;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
        i += (long long)gridDim.x * blockDim.x) {
     const int c = (int)(i / L), a = (int)(i % L);
-    cpx<double> s = mk(0.0, 0.0);
-    for (int zz = 0; zz < nz; ++zz) s = s + part[((long long)zz * C + c) * L + a];
-    mom[(long long)c * L + a] = calpha[a] * s;  // column-major L x C
+    double re = 0.0, im = 0.0;
+    for (int z = 0; z < nslices; ++z) {
+      const double* row = part + ((long long)z * Lpad + a) * Cpad2;
+      re += row[2 * c];
+      im += row[2 * c + 1];
+    }
+    mom[(long long)c * L + a] = calpha[a] * mk(re, im);  // column-major L x C
   }
+}
+
+// Nested Horner of sum_alpha dx^a1 dy^a2 dz^a3 m[alpha] (alpha ordered a1, a2, a3).
+__device__ __forceinline__ cpx<double> taylor_horner(const cpx<double>* mc, int p, double dx, double dy,
+                                                     double dz) {
+  cpx<double> acc1 = mk(0.0, 0.0);
+  double x1 = 1.0;
+  int idx = 0;
+  for (int a1 = 0; a1 <= p; ++a1) {
+    cpx<double> acc2 = mk(0.0, 0.0);
+    double y2 = 1.0;
+    for (int a2 = 0; a2 <= p - a1; ++a2) {
+      cpx<double> acc3 = mk(0.0, 0.0);
+      double z3 = 1.0;
+      for (int a3 = 0; a3 <= p - a1 - a2; ++a3) {
+        acc3 = acc3 + z3 * mc[idx++];
+        z3 *= dz;
+      }
+      acc2 = acc2 + y2 * acc3;
+      y2 *= dy;
+    }
+    acc1 = acc1 + x1 * acc2;
+    x1 *= dx;
+  }
+  return acc1;
 }
 
 // Evaluation: one thread per point, nested Horner over (a1, a2, a3).
@@ -93,29 +172,39 @@ __global__ void k_taylor_eval(long long M, const double* px, const double* py, c
       u[i] = mk<double>(NAN, NAN);
       continue;
     }
-    const double dx = px[i] - centers[3 * c], dy = py[i] - centers[3 * c + 1], dz = pz[i] - centers[3 * c + 2];
-    const cpx<double>* mc = mom + (long long)c * L;
-    // alpha index of (a1, a2, a3): terms are ordered a1 outer, a2, a3 inner
-    cpx<double> acc1 = mk(0.0, 0.0);
-    double x1 = 1.0;
-    int idx = 0;
-    for (int a1 = 0; a1 <= p; ++a1) {
-      cpx<double> acc2 = mk(0.0, 0.0);
-      double y2 = 1.0;
-      for (int a2 = 0; a2 <= p - a1; ++a2) {
-        cpx<double> acc3 = mk(0.0, 0.0);
-        double z3 = 1.0;
-        for (int a3 = 0; a3 <= p - a1 - a2; ++a3) {
-          acc3 = acc3 + z3 * mc[idx++];
-          z3 *= dz;
-        }
-        acc2 = acc2 + y2 * acc3;
-        y2 *= dy;
-      }
-      acc1 = acc1 + x1 * acc2;
-      x1 *= dx;
+    u[i] = taylor_horner(mom + (long long)c * L, p, px[i] - centers[3 * c], py[i] - centers[3 * c + 1],
+                         pz[i] - centers[3 * c + 2]);
+  }
+}
+
+// Block of a plus-grid index: nb nearly equal ranges, the first mod(m, nb) one longer
+// (build_yee_block_partition_exp3.m:21-31).
+__device__ __forceinline__ int block_of(int x, int base, int rem) {
+  const int cut = rem * (base + 1);
+  return x < cut ? x / (base + 1) : rem + (x - cut) / base;
+}
+
+// Taylor evaluation on the (2n+3)^3 plus-grid points of Yee component comp, generated on the
+// fly: point (I,J,K) = h*((I,J,K) - (n+1) + 0.5 e_comp) (build_yee_edge_points_plus.m:11-17),
+// column-major order, center = block (bI + nb*bJ + nb^2*bK).
+__global__ void k_taylor_eval_yee(int n, double h, int comp, int nb, int p, int L, int C,
+                                  const double* centers, const cpx<double>* mom, cpx<double>* u) {
+  const int m = 2 * n + 3;
+  const long long M = (long long)m * m * m;
+  const int base = m / nb, rem = m % nb;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < M;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int I = (int)(i % m), J = (int)((i / m) % m), K = (int)(i / ((long long)m * m));
+    const int c = block_of(I, base, rem) + nb * block_of(J, base, rem) + nb * nb * block_of(K, base, rem);
+    if (c >= C) {
+      u[i] = mk<double>(NAN, NAN);
+      continue;
     }
-    u[i] = acc1;
+    const double x = h * ((I - (n + 1)) + (comp == 0 ? 0.5 : 0.0));
+    const double y = h * ((J - (n + 1)) + (comp == 1 ? 0.5 : 0.0));
+    const double z = h * ((K - (n + 1)) + (comp == 2 ? 0.5 : 0.0));
+    u[i] = taylor_horner(mom + (long long)c * L, p, x - centers[3 * c], y - centers[3 * c + 1],
+                         z - centers[3 * c + 2]);
   }
 }
 
@@ -173,22 +262,31 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
     std::vector<cpx<double>> ca;
     alpha_tables(p, al, ca);
     const int L = (int)ca.size();
-    const int nz = (int)std::min<long long>(64, std::max<long long>(1, NF / 4096));
+    const int Lpad = (L + 15) & ~15;
+    const int ctiles = (C + 7) / 8;
+    const int Cpad2 = ctiles * 16;
+    // mode slices: enough workgroups to fill the chip, each slice a multiple of the chunk
+    int nslices = (int)std::max<long long>(1, std::min<long long>(NF / kTChunk, (1024 + ctiles - 1) / ctiles));
+    long long span = (NF + nslices - 1) / nslices;
+    span = ((span + kTChunk - 1) / kTChunk) * kTChunk;
+    nslices = (int)((NF + span - 1) / span);
+    std::vector<unsigned char> alb(al.begin(), al.end());
     DevBuf tabs, part;
-    tabs.ensure(sizeof(int) * al.size() + sizeof(cpx<double>) * ca.size() + sizeof(double) * 3 * C + 64);
+    const size_t off_ca = ((alb.size() + 255) / 256) * 256;
+    const size_t off_cen = off_ca + sizeof(cpx<double>) * ca.size();
+    tabs.ensure(off_cen + sizeof(double) * 3 * C);
     char* b = tabs.as<char>();
-    int* al_d = (int*)b;
-    cpx<double>* ca_d = (cpx<double>*)(b + ((sizeof(int) * al.size() + 15) / 16) * 16);
-    double* cen_d = (double*)(ca_d + ca.size());
-    F6_HIP(hipMemcpyAsync(al_d, al.data(), sizeof(int) * al.size(), hipMemcpyHostToDevice, s));
-    F6_HIP(hipMemcpyAsync(ca_d, ca.data(), sizeof(cpx<double>) * ca.size(), hipMemcpyHostToDevice, s));
-    F6_HIP(hipMemcpyAsync(cen_d, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
-    part.ensure(sizeof(cpx<double>) * (size_t)nz * C * L);
-    dim3 grid(C, (L + kTaylorChunk - 1) / kTaylorChunk, nz);
-    k_taylor_moments<<<grid, 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF, (const cpx<double>*)coeff,
-                                          p, cen_d, al_d, ca_d, L, part.as<cpx<double>>());
+    F6_HIP(hipMemcpyAsync(b, alb.data(), alb.size(), hipMemcpyHostToDevice, s));
+    F6_HIP(hipMemcpyAsync(b + off_ca, ca.data(), sizeof(cpx<double>) * ca.size(), hipMemcpyHostToDevice, s));
+    F6_HIP(hipMemcpyAsync(b + off_cen, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
+    part.ensure(sizeof(double) * (size_t)nslices * Lpad * Cpad2);
+    dim3 grid(ctiles, nslices);
+    k_taylor_moments_mfma<<<grid, 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF,
+                                               (const cpx<double>*)coeff, p, L, C, (const double*)(b + off_cen),
+                                               (const unsigned char*)b, span, part.as<double>());
     F6_CHECK_LAUNCH();
-    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, nz, part.as<cpx<double>>(), ca_d,
+    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, nslices, part.as<double>(),
+                                                               (const cpx<double>*)(b + off_ca),
                                                                (cpx<double>*)moments);
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
@@ -217,6 +315,31 @@ int fame6d_taylor_evaluate(long long M, const double* points, const int* center_
                                                  (cpx<double>*)u);
       F6_CHECK_LAUNCH();
     }
+    F6_HIP(hipStreamSynchronize(s));
+    cen.release();
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  }
+  return FAME6D_OK;
+}
+
+int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, int C,
+                               const double* centers_host, const void* moments, void* u, void* stream) {
+  try {
+    if (!centers_host || !moments || !u) throw Error(FAME6D_EARG, "null argument");
+    if (n < 0 || comp < 0 || comp > 2 || nblocks < 1 || nblocks > 2 * n + 3 || p < 0 || p > kTaylorMaxP ||
+        C != nblocks * nblocks * nblocks)
+      throw Error(FAME6D_EARG, "bad Yee/Taylor sizes");
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf cen;
+    cen.ensure(sizeof(double) * 3 * C);
+    F6_HIP(hipMemcpyAsync(cen.ptr, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
+    const long long m = 2LL * n + 3;
+    k_taylor_eval_yee<<<grid_for(m * m * m), 256, 0, s>>>(n, h, comp, nblocks, p, taylor_terms(p), C,
+                                                           cen.as<double>(), (const cpx<double>*)moments,
+                                                           (cpx<double>*)u);
+    F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
     cen.release();
   } catch (const Error& e) {

@@ -117,6 +117,14 @@ class MassBackend:
         except Exception:
             pass
 
+    def set_eps(self, eps):
+        """Replaces the permittivity samples (same grid)."""
+        eps = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(-1))
+        if eps.size != self.NF:
+            raise ValueError("eps must have NF samples")
+        call("fame6d_mass_set_eps", self._h, eps.ctypes.data_as(C.c_void_p))
+        self._eps = eps
+
     @property
     def handle(self):
         return self._h

@@ -1,0 +1,138 @@
+/*
+ * fame6d_mex.c - MATLAB MEX gateway onto the C ABI of libfame6d (include/fame6d.h).
+ *
+ * MATLAB's gpuArray/mxGPUArray API targets NVIDIA devices only, so this gateway
+ * takes host interleaved-complex arrays (R2018a API, compile with -R2018a) and keeps
+ * device state behind uint64 handles.  Build where MATLAB is installed:
+ *
+ *   mex -R2018a fame6d_mex.c -I<repo>/include -L<pkg dir> -lfame6d -L/opt/rocm/lib -lamdhip64
+ *
+ * Usage (mirrors the reference handles of build_mass_backend_large_scalar,
+ * exp1/build_mass_backend_large_scalar.m:27-36, and run_method_C_fig1bc,
+ * exp1/run_method_C_fig1bc.m:4-21):
+ *   h  = fame6d_mex('mass_create', N, eps)            % eps: NF real (column-major grid)
+ *   y  = fame6d_mex('mscalar', h, x)                    % MscalarOp, x: NF x r complex
+ *   y  = fame6d_mex('mop', h, U)                        % Mop, U: 3NF x r complex
+ *   s  = fame6d_mex('qsys_create', h, q, P)             % q: 3x1, P: 6x3
+ *   y  = fame6d_mex('krinv', s, x, tol, maxit)          % KrInvOp (Method C), x: 2NF x 1
+ *   [lambda, Y] = fame6d_mex('eigs', s, k, p, tol, maxRestart, tolCG, maxitCG, v0, precision)
+ *   fame6d_mex('qsys_destroy', s); fame6d_mex('mass_destroy', h)
+ */
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "fame6d.h"
+#include "mex.h"
+
+static void fail(int code) {
+  if (code != FAME6D_OK) mexErrMsgIdAndTxt("fame6d:error", "fame6d status %d: %s", code, fame6d_last_error());
+}
+
+static void* handle_in(const mxArray* a) {
+  if (!mxIsUint64(a) || mxGetNumberOfElements(a) != 1) mexErrMsgIdAndTxt("fame6d:handle", "expected a uint64 handle");
+  return (void*)(uintptr_t)(*mxGetUint64s(a));
+}
+
+static mxArray* handle_out(void* p) {
+  mxArray* a = mxCreateNumericMatrix(1, 1, mxUINT64_CLASS, mxREAL);
+  *mxGetUint64s(a) = (uint64_t)(uintptr_t)p;
+  return a;
+}
+
+/* device copy of a host complex matrix (interleaved complex128) */
+static void* to_device(const mxArray* a, size_t* n_out) {
+  size_t n = mxGetNumberOfElements(a);
+  void* d = NULL;
+  if (hipMalloc(&d, n * 16) != hipSuccess) mexErrMsgIdAndTxt("fame6d:nomem", "hipMalloc failed");
+  if (mxIsComplex(a)) {
+    hipMemcpy(d, mxGetComplexDoubles(a), n * 16, hipMemcpyHostToDevice);
+  } else {
+    /* promote real input to complex */
+    double* tmp = (double*)mxCalloc(2 * n, sizeof(double));
+    const double* re = mxGetDoubles(a);
+    for (size_t i = 0; i < n; ++i) tmp[2 * i] = re[i];
+    hipMemcpy(d, tmp, n * 16, hipMemcpyHostToDevice);
+    mxFree(tmp);
+  }
+  *n_out = n;
+  return d;
+}
+
+static mxArray* from_device(const void* d, size_t rows, size_t cols) {
+  mxArray* a = mxCreateDoubleMatrix(rows, cols, mxCOMPLEX);
+  hipMemcpy(mxGetComplexDoubles(a), d, rows * cols * 16, hipMemcpyDeviceToHost);
+  return a;
+}
+
+void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  char cmd[64];
+  if (nrhs < 1 || mxGetString(prhs[0], cmd, sizeof(cmd)) != 0)
+    mexErrMsgIdAndTxt("fame6d:usage", "first argument must be a command string");
+
+  if (!strcmp(cmd, "mass_create")) {
+    int N = (int)mxGetScalar(prhs[1]);
+    fame6d_mass* m = NULL;
+    fail(fame6d_mass_create(N, mxGetDoubles(prhs[2]), NULL, &m));
+    plhs[0] = handle_out(m);
+  } else if (!strcmp(cmd, "mass_destroy")) {
+    fail(fame6d_mass_destroy((fame6d_mass*)handle_in(prhs[1])));
+  } else if (!strcmp(cmd, "mscalar") || !strcmp(cmd, "mop")) {
+    fame6d_mass* m = (fame6d_mass*)handle_in(prhs[1]);
+    size_t rows = mxGetM(prhs[2]), cols = mxGetN(prhs[2]), n;
+    void* x = to_device(prhs[2], &n);
+    void* y = NULL;
+    hipMalloc(&y, n * 16);
+    if (!strcmp(cmd, "mscalar")) fail(fame6d_mass_apply_scalar(m, FAME6D_FP64, x, y, (int)cols));
+    else fail(fame6d_mass_apply_3comp(m, FAME6D_FP64, x, y, (int)cols));
+    hipDeviceSynchronize();
+    plhs[0] = from_device(y, rows, cols);
+    hipFree(x);
+    hipFree(y);
+  } else if (!strcmp(cmd, "qsys_create")) {
+    fame6d_mass* m = (fame6d_mass*)handle_in(prhs[1]);
+    const double* Pcm = mxGetDoubles(prhs[3]); /* MATLAB 6x3 column-major */
+    double P[18];
+    for (int l = 0; l < 6; ++l)
+      for (int c = 0; c < 3; ++c) P[l * 3 + c] = Pcm[l + 6 * c];
+    fame6d_qsys* s = NULL;
+    fail(fame6d_qsys_create(m, mxGetDoubles(prhs[2]), P, &s));
+    plhs[0] = handle_out(s);
+  } else if (!strcmp(cmd, "qsys_destroy")) {
+    fail(fame6d_qsys_destroy((fame6d_qsys*)handle_in(prhs[1])));
+  } else if (!strcmp(cmd, "krinv")) {
+    fame6d_qsys* s = (fame6d_qsys*)handle_in(prhs[1]);
+    size_t n;
+    void* x = to_device(prhs[2], &n);
+    void* y = NULL;
+    hipMalloc(&y, n * 16);
+    fame6d_krinv_stats_t st;
+    fail(fame6d_krinv_apply(s, FAME6D_FP64, x, y, mxGetScalar(prhs[3]), (int)mxGetScalar(prhs[4]), &st));
+    plhs[0] = from_device(y, n, 1);
+    hipFree(x);
+    hipFree(y);
+  } else if (!strcmp(cmd, "eigs")) {
+    fame6d_qsys* s = (fame6d_qsys*)handle_in(prhs[1]);
+    int k = (int)mxGetScalar(prhs[2]), p = (int)mxGetScalar(prhs[3]);
+    double tol = mxGetScalar(prhs[4]);
+    int maxRestart = (int)mxGetScalar(prhs[5]);
+    double tolCG = mxGetScalar(prhs[6]);
+    int maxitCG = (int)mxGetScalar(prhs[7]);
+    const mxArray* v0 = prhs[8];
+    int prec = (nrhs > 9) ? (int)mxGetScalar(prhs[9]) : FAME6D_FP64;
+    size_t n2 = mxGetNumberOfElements(v0);
+    if (!mxIsComplex(v0)) mexErrMsgIdAndTxt("fame6d:usage", "v0 must be complex");
+    plhs[0] = mxCreateDoubleMatrix(k, 1, mxREAL);
+    void* Y = NULL;
+    if (nlhs > 1) hipMalloc(&Y, n2 * k * 16);
+    fame6d_lanczos_stats_t st;
+    fail(fame6d_eigs_method_c(s, prec, k, p, tol, maxRestart, tolCG, maxitCG, mxGetComplexDoubles(v0),
+                              mxGetDoubles(plhs[0]), Y, &st));
+    if (nlhs > 1) {
+      plhs[1] = from_device(Y, n2, k);
+      hipFree(Y);
+    }
+  } else {
+    mexErrMsgIdAndTxt("fame6d:usage", "unknown command %s", cmd);
+  }
+}
