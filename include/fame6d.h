@@ -80,7 +80,9 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
  * passes at this N and precision and keeps the fastest (no reference counterpart;
  * the reference has one implicit cuFFT plan). */
 int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
-/* Set tile variants explicitly (variant indices 0/1 for passes A/C and B). */
+/* Set tile variants explicitly: variant_ac in 0..3 (bit 0: blocks per workgroup of
+ * passes A/C, bit 1 set: dim-1 register-pencil I/O instead of the coalesced dim-3-first
+ * I/O), variant_b in 0..1 (inner positions per workgroup of pass B). */
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
 
 /* mass.MscalarOp: applyMij_rank1_exact_lowmem (exp1/applyMij_rank1_exact_lowmem.m:8-22),
@@ -181,6 +183,18 @@ int fame6d_eigs_method_c_batch(fame6d_mass* m, int nq, const double* q_host, con
                                int precision, int k, int p, double tol, int max_restart,
                                double tol_cg, int maxit_cg, const void* v0_host,
                                double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
+
+/* make_nested_krinv_instrumented applyOne (exp1/make_nested_krinv_instrumented.m:33-132), Method B:
+ * x = K_r^{-1} b by an outer PCG on K_r (tol_kr, maxit_kr) preconditioned with
+ * eps_c * r ./ sigma_r (:47), each K_r z = V^*(M^{-1}(V z)) with an inner PCG on M
+ * (tol_m, maxit_m).  b, x: 2NF complex128 (FP64). */
+int fame6d_krinv_nested_apply(fame6d_qsys* s, const void* b, void* x, double tol_kr, double tol_m,
+                              int maxit_kr, int maxit_m, double eps_c, fame6d_krinv_stats_t* stats);
+/* run_method_B_fig1bc: the Lanczos of fame6d_eigs_method_c driving the Method-B inverse
+ * (exp1/make_nested_krinv_instrumented.m; cfg.nested in exp1/make_config_exp1bc.m:83-87). */
+int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_restart, double tol_kr,
+                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
+                         double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
 
 typedef struct {
   int iterations[64];

@@ -94,6 +94,8 @@ SIGNATURES = {
     "fame6d_krinv_apply": [P, I, P, P, D, I, C.POINTER(KrinvStats)],
     "fame6d_eigs_method_c": [P, I, I, I, D, I, D, I, P, P, P, C.POINTER(LanczosStats)],
     "fame6d_eigs_method_c_batch": [P, I, P, P, I, I, I, D, I, D, I, P, P, P, C.POINTER(LanczosStats)],
+    "fame6d_krinv_nested_apply": [P, P, P, D, D, I, I, D, C.POINTER(KrinvStats)],
+    "fame6d_eigs_method_b": [P, I, I, D, I, D, D, I, I, D, P, P, P, C.POINTER(LanczosStats)],
     "fame6d_recover_eigenvectors": [P, P, I, D, I, P, P, C.POINTER(RecoverStats)],
     "fame6d_fourier_diagnostics": [P, P, P, P, I, P, P, P],
     "fame6d_taylor_moments": [LL, P, P, I, I, P, P, P],

@@ -15,6 +15,9 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
                    double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
                    fame6d_lanczos_stats_t* stats);
+void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, double tolKr, double tolM,
+                  int maxitKr, int maxitM, double epsC, fame6d_krinv_stats_t* st);
+void set_nested(bool on, double tolM, int maxitM, double epsC);
 void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double tol, int maxit,
                           cpx<double>* X, cpx<double>* MX, fame6d_recover_stats_t* stats);
 void fourier_diagnostics(fame6d_qsys* S, const cpx<double>* X, const cpx<double>* MX,
@@ -201,9 +204,10 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   F6_API_BEGIN
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
-  F6_REQUIRE(va >= 0 && va < 2 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant must be 0 or 1");
+  F6_REQUIRE(va >= 0 && va < 4 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
   TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
-  t.a = va;
+  t.a = va & 1;
+  t.io = (va >> 1) ? 0 : 1;
   t.b = vb;
   F6_API_END
 }
@@ -227,8 +231,10 @@ int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps) {
   TileChoice& tc = f32 ? m->tile32 : m->tile64;
   float best = 1e30f;
   TileChoice bestc = tc;
+  for (int vio = 0; vio < 2; ++vio)
   for (int va = 0; va < 2; ++va) {
     for (int vb = 0; vb < 2; ++vb) {
+      tc.io = vio;
       tc.a = va;
       tc.b = vb;
       auto run = [&]() {
@@ -414,6 +420,43 @@ int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
   eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
                 lambda_host, (cpx<double>*)Y, stats);
+  F6_API_END
+}
+
+int fame6d_krinv_nested_apply(fame6d_qsys* s, const void* b, void* x, double tol_kr, double tol_m,
+                               int maxit_kr, int maxit_m, double eps_c, fame6d_krinv_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && b && x, FAME6D_EARG, "null argument");
+  F6_REQUIRE(tol_kr > 0 && tol_m > 0 && eps_c > 0, FAME6D_EARG, "tolerances and eps_c must be positive");
+  const fame6d_krinv_stats_t st0 = s->stats;
+  krinv_nested(s, (const cpx<double>*)b, (cpx<double>*)x, tol_kr, tol_m, maxit_kr, maxit_m, eps_c, nullptr);
+  if (stats) {
+    *stats = s->stats;
+    stats->calls -= st0.calls;
+    stats->total_iterations -= st0.total_iterations;
+    stats->total_fp64_iterations -= st0.total_fp64_iterations;
+    stats->mass_applies_fp64 -= st0.mass_applies_fp64;
+    stats->seconds -= st0.seconds;
+  }
+  F6_API_END
+}
+
+int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_restart, double tol_kr,
+                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
+                         double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
+  F6_REQUIRE(tol_m > 0 && eps_c > 0, FAME6D_EARG, "tol_m and eps_c must be positive");
+  set_nested(true, tol_m, maxit_m, eps_c);
+  try {
+    eigs_method_c(s, FAME6D_FP64, k, p, tol, max_restart, tol_kr, maxit_kr, (const cpx<double>*)v0_host,
+                  lambda_host, (cpx<double>*)Y, stats);
+  } catch (...) {
+    set_nested(false, 0, 0, 0);
+    throw;
+  }
+  set_nested(false, 0, 0, 0);
   F6_API_END
 }
 

@@ -59,6 +59,7 @@ struct DevBuf {
 struct TileChoice {
   int a = 0;  // pass A / pass C variant
   int b = 0;  // pass B variant
+  int io = 1;  // passes A/C: 1 = coalesced dim-3-first I/O, 0 = dim-1 register pencils
 };
 
 }  // namespace fame6d

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <numeric>
 
 #include "internal.hpp"
@@ -237,6 +238,24 @@ static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, 
 }
 
 // Eigensolve of all G Bloch points of S.  v0: G x 2NF host start vectors.
+// Nested-method settings when the eigensolve runs Method B (tolM, maxitM, epsC).
+struct NestedCfg {
+  bool on = false;
+  double tolM = 1e-12, epsC = 1.0;
+  int maxitM = 1600;
+};
+static thread_local NestedCfg g_nested;
+
+void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, double tolKr, double tolM,
+                  int maxitKr, int maxitM, double epsC, fame6d_krinv_stats_t* st);
+
+void set_nested(bool on, double tolM, int maxitM, double epsC) {
+  g_nested.on = on;
+  g_nested.tolM = tolM;
+  g_nested.maxitM = maxitM;
+  g_nested.epsC = epsC;
+}
+
 void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
                    double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
                    fame6d_lanczos_stats_t* stats) {
@@ -305,7 +324,13 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
       slots[i] = Lg.slot;
     }
-    krinv_groups(S, precision, na, slots.data(), qin, qout, tol_cg, maxit_cg, nullptr);
+    if (g_nested.on) {
+      for (int i = 0; i < na; ++i)
+        krinv_nested(S, qin + (size_t)n2 * i, qout + (size_t)n2 * i, tol_cg, g_nested.tolM, maxit_cg,
+                     g_nested.maxitM, g_nested.epsC, nullptr);
+    } else {
+      krinv_groups(S, precision, na, slots.data(), qin, qout, tol_cg, maxit_cg, nullptr);
+    }
     ++steps;
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
@@ -432,44 +457,59 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
                                       std::to_string(L[g].nconv) + "/" + std::to_string(k0) + ".");
 }
 
-// ------------------------------------------------------------------ recovery
-// pcg on M = I3 (x) M_s for one 3NF right-hand side (host-stepped MATLAB pcg).
-static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cpx<double>* r,
-                        cpx<double>* p, cpx<double>* q, double tol, int maxit) {
+// ------------------------------------------------------------------ host-stepped PCG
+// MATLAB pcg (x0 = 0) with an optional preconditioner z = Minv(r), host-stepped: one
+// synchronisation per iteration.  Used where the matvec itself is an inner solve
+// (Method B) and for the eigenvector recovery.
+using Matvec = std::function<void(const cpx<double>*, cpx<double>*)>;
+
+static PcgOut pcg_host(fame6d_mass* m, long long n, const Matvec& A, const Matvec* Minv,
+                       const cpx<double>* b, cpx<double>* x, cpx<double>* r, cpx<double>* p,
+                       cpx<double>* q, cpx<double>* z, double tol, int maxit) {
   hipStream_t s = m->stream;
-  const long long NF = m->NF, n3 = 3 * NF;
-  const int gr = grid_for(n3);
+  const int gr = grid_for(n);
   auto dot = [&](const cpx<double>* a, const cpx<double>* c) {
     double re;
-    dot_groups<double>(m, 1, a, n3, c, n3, n3, &re, nullptr);
+    dot_groups<double>(m, 1, a, n, c, n, n, &re, nullptr);
     return re;
   };
   PcgOut o;
-  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, x);
-  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n, x);
+  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<double>) * n, hipMemcpyDeviceToDevice, s));
   const double n2b = std::sqrt(dot(b, b));
-  if (!std::isfinite(n2b)) throw Error(FAME6D_ENONFINITE, "recovery right-hand side contains NaN/Inf");
+  if (!std::isfinite(n2b)) throw Error(FAME6D_ENONFINITE, "CG right-hand side contains NaN/Inf");
   if (n2b == 0) return o;
   const double tolb = tol * n2b;
   double rho = 1, normr = n2b;
   int moresteps = 0;
-  const int maxmsteps = (int)std::min<long long>({NF * 3 / 50, 5, std::max<long long>(n3 - maxit, 0)});
+  const int maxmsteps = (int)std::min<long long>({n / 50, 5, std::max<long long>(n - maxit, 0)});
   o.flag = 1;
+  o.iter = maxit;
   for (int ii = 1; ii <= maxit; ++ii) {
+    const cpx<double>* zz = r;
+    if (Minv) {
+      (*Minv)(r, z);
+      zz = z;
+    }
     const double rho1 = rho;
-    rho = dot(r, r);
+    rho = dot(r, zz);
     if (rho == 0 || std::isinf(rho)) {
       o.flag = 4;
       o.iter = ii - 1;
       break;
     }
     if (ii == 1) {
-      F6_HIP(hipMemcpyAsync(p, r, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+      F6_HIP(hipMemcpyAsync(p, zz, sizeof(cpx<double>) * n, hipMemcpyDeviceToDevice, s));
     } else {
       const double beta = rho / rho1;
-      k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), r, mk(beta, 0.0), p, p);
+      if (beta == 0 || std::isinf(beta)) {
+        o.flag = 4;
+        o.iter = ii - 1;
+        break;
+      }
+      k_axpby<double><<<gr, 256, 0, s>>>(n, mk(1.0, 0.0), zz, mk(beta, 0.0), p, p);
     }
-    mass_apply<double>(m, p, NF, q, NF, 3, s);
+    A(p, q);
     const double pq = dot(p, q);
     if (!(pq > 0) || std::isinf(pq)) {
       o.flag = 4;
@@ -477,14 +517,15 @@ static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cp
       break;
     }
     const double alpha = rho / pq;
-    k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), x, mk(alpha, 0.0), p, x);
-    k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), r, mk(-alpha, 0.0), q, r);
+    k_axpby<double><<<gr, 256, 0, s>>>(n, mk(1.0, 0.0), x, mk(alpha, 0.0), p, x);
+    k_axpby<double><<<gr, 256, 0, s>>>(n, mk(1.0, 0.0), r, mk(-alpha, 0.0), q, r);
     normr = std::sqrt(dot(r, r));
     o.iter = ii;
     if (normr <= tolb || moresteps) {
-      mass_apply<double>(m, x, NF, q, NF, 3, s);
-      k_axpby<double><<<gr, 256, 0, s>>>(n3, mk(1.0, 0.0), b, mk(-1.0, 0.0), q, r);
+      A(x, q);
+      k_axpby<double><<<gr, 256, 0, s>>>(n, mk(1.0, 0.0), b, mk(-1.0, 0.0), q, r);
       const double act = std::sqrt(dot(r, r));
+      ++o.checks;
       normr = act;
       if (act <= tolb) {
         o.flag = 0;
@@ -499,6 +540,104 @@ static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cp
   }
   o.relres = normr / n2b;
   return o;
+}
+
+static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cpx<double>* r,
+                        cpx<double>* p, cpx<double>* q, double tol, int maxit) {
+  const long long NF = m->NF;
+  Matvec A = [&](const cpx<double>* in, cpx<double>* out) { mass_apply<double>(m, in, NF, out, NF, 3, m->stream); };
+  return pcg_host(m, 3 * NF, A, nullptr, b, x, r, p, q, nullptr, tol, maxit);
+}
+
+// ------------------------------------------------------------------ Method B
+// make_nested_krinv_instrumented (exp1/make_nested_krinv_instrumented.m:33-132): outer PCG
+// on K_r x = b with preconditioner epsC * r ./ sigma_r (sigma_r = [|q|^2; |q|^2], :47),
+// every K_r z = V^* (M^{-1} (V z)) with an inner PCG on M = I3 (x) M_s (tolM, maxitM).
+__global__ void k_sigma_precond(const double* q1, const double* q2, const double* q3, long long n,
+                                double scale, const cpx<double>* r, cpx<double>* z) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double qq = q1[i] * q1[i] + q2[i] * q2[i] + q3[i] * q3[i];
+    z[i] = (scale / qq) * r[i];
+    z[n + i] = (scale / qq) * r[n + i];
+  }
+}
+
+// y(2NF) = V^* x(3NF)   (apply_VH_blockdiag.m:13-15; V real)
+__global__ void k_apply_vh(const cpx<double>* x, long long n, const double* V11, const double* V12,
+                           const double* V21, const double* V22, const double* V31, const double* V32,
+                           cpx<double>* y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cpx<double> x1 = x[i], x2 = x[n + i], x3 = x[2 * n + i];
+    y[i] = V11[i] * x1 + V21[i] * x2 + V31[i] * x3;
+    y[n + i] = V12[i] * x1 + V22[i] * x2 + V32[i] * x3;
+  }
+}
+
+void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, double tolKr, double tolM,
+                  int maxitKr, int maxitM, double epsC, fame6d_krinv_stats_t* st) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF, n2 = 2 * NF, n3 = 3 * NF;
+  if (S->G != 1) throw Error(FAME6D_EARG, "Method B expects a single-point q-system");
+  DevBuf ws;
+  ws.ensure(sizeof(cpx<double>) * (size_t)(5 * n2 + 6 * n3));
+  cpx<double>* r = ws.as<cpx<double>>();
+  cpx<double>* p = r + n2;
+  cpx<double>* q = p + n2;
+  cpx<double>* z = q + n2;
+  cpx<double>* xo = z + n2;
+  cpx<double>* vz = xo + n2;
+  cpx<double>* mi = vz + n3;
+  cpx<double>* ir = mi + n3;
+  cpx<double>* ip = ir + n3;
+  cpx<double>* iq = ip + n3;
+  cpx<double>* tmp = iq + n3;
+  const int gr = grid_for(NF);
+  long long inner_its = 0, inner_calls = 0;
+  int inner_max = 0;
+  double inner_relres = 0;
+  Matvec Kr = [&](const cpx<double>* zin, cpx<double>* yo) {
+    k_apply_v<<<dim3(gr, 1), 256, 0, s>>>(zin, n2, NF, S->V(0), S->V(1), S->V(2), S->V(3), S->V(4), S->V(5),
+                                          vz, n3, no_skip(), nullptr);
+    F6_CHECK_LAUNCH();
+    PcgOut o = pcg_mass3(m, vz, mi, ir, ip, iq, tolM, maxitM);
+    if (!std::isfinite(o.relres) || (o.flag != 0 && o.relres > 10 * tolM))
+      throw Error(FAME6D_ENOCONV, "Nested M solve failed: flag=" + std::to_string(o.flag));
+    inner_its += o.iter;
+    inner_calls += 1;
+    inner_max = std::max(inner_max, o.iter);
+    inner_relres = std::max(inner_relres, o.relres);
+    k_apply_vh<<<gr, 256, 0, s>>>(mi, NF, S->V(0), S->V(1), S->V(2), S->V(3), S->V(4), S->V(5), yo);
+    F6_CHECK_LAUNCH();
+  };
+  Matvec Pc = [&](const cpx<double>* rin, cpx<double>* zo) {
+    k_sigma_precond<<<gr, 256, 0, s>>>(S->Qm(0), S->Qm(1), S->Qm(2), NF, epsC, rin, zo);
+    F6_CHECK_LAUNCH();
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  PcgOut o = pcg_host(m, n2, Kr, &Pc, bin, xout, r, p, q, z, tolKr, maxitKr);
+  (void)xo;
+  (void)tmp;
+  if (!std::isfinite(o.relres) || (o.flag != 0 && o.relres > 10 * tolKr))
+    throw Error(FAME6D_ENOCONV, "Nested K_r solve failed: flag=" + std::to_string(o.flag) +
+                                    ", relres=" + std::to_string(o.relres));
+  F6_HIP(hipStreamSynchronize(s));
+  ws.release();
+  S->stats.calls += 1;
+  S->stats.total_iterations += o.iter;
+  S->stats.total_fp64_iterations += inner_its;
+  S->stats.max_iterations = std::max(S->stats.max_iterations, o.iter);
+  S->stats.last_iterations = o.iter;
+  S->stats.last_flag = o.flag;
+  S->stats.last_relres = o.relres;
+  S->stats.max_relres = std::max(S->stats.max_relres, o.relres);
+  S->stats.mass_applies_fp64 += 3 * (inner_its + inner_calls);
+  S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st) *st = S->stats;
+  (void)inner_max;
+  (void)inner_relres;
 }
 
 void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double tol, int maxit,

@@ -61,11 +61,20 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
     auto go = [&](auto bc) {
       constexpr int BPW = decltype(bc)::value;
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
-      switch (op) {
-        case PRE_COPY: k_pass_a<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
-        case PRE_MHAT: k_pass_a<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
-        case PRE_MHAT_CG: k_pass_a<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
-        default: throw Error(FAME6D_EARG, "bad pass-A op");
+      if (tile_of<T>(m).io == 1) {
+        switch (op) {
+          case PRE_COPY: k_pass_a_lin<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
+          case PRE_MHAT: k_pass_a_lin<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
+          case PRE_MHAT_CG: k_pass_a_lin<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
+          default: throw Error(FAME6D_EARG, "bad pass-A op");
+        }
+      } else {
+        switch (op) {
+          case PRE_COPY: k_pass_a<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
+          case PRE_MHAT: k_pass_a<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
+          case PRE_MHAT_CG: k_pass_a<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
+          default: throw Error(FAME6D_EARG, "bad pass-A op");
+        }
       }
     };
     if (v == 0) go(std::integral_constant<int, Tiles<n, T>::bpw[0]>{});
@@ -107,7 +116,10 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
     auto go = [&](auto bc) {
       constexpr int BPW = decltype(bc)::value;
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
-      if (op == POST_COPY) k_pass_c<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
+      const bool lin = tile_of<T>(m).io == 1;
+      if (op == POST_COPY && lin) k_pass_c_lin<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
+      else if (op == POST_MHAT && lin) k_pass_c_lin<T, n, BPW, POST_MHAT, 3><<<grid, block, 0, s>>>(a);
+      else if (op == POST_COPY) k_pass_c<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
       else if (op == POST_MHAT) k_pass_c<T, n, BPW, POST_MHAT, 3><<<grid, block, 0, s>>>(a);
       else throw Error(FAME6D_EARG, "bad pass-C op");
     };

@@ -213,6 +213,87 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a(PassAArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- pass A, coalesced I/O
+// Same transform as k_pass_a with every HBM access coalesced: the register pencil of
+// the first DFT is along dim 3 (stride n^2, so consecutive threads read consecutive
+// modes), dims 2 and 1 go through LDS, and the result leaves LDS in linear order.
+// THREADS = BPW*n^2 and each thread moves exactly n values per block set.
+template <class T, int n, int BPW, int OP>
+__global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int NOUT = (OP == PRE_COPY) ? 1 : 3;
+  constexpr int THREADS = BPW * P2;
+  __shared__ cpx<T> lds[BPW * S];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int blk = tid / P2, pp = tid - blk * P2;
+  const long long wg0 = (long long)blockIdx.x * BPW * S;  // first mode of this workgroup
+  const long long off = wg0 + (long long)blk * S;
+  cpx<T>* L = lds + blk * S;
+
+  // dim-3 pencil of this thread: (k1, k2) = pp, elements pp + n^2*k
+  const cpx<T>* xg = a.x + g * a.ldx + off + pp;
+  cpx<T> pn[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) pn[k] = xg[P2 * k];
+  if constexpr (OP == PRE_MHAT_CG) {
+    cpx<T>* pg = a.p + g * a.ldx + off + pp;
+    if (a.st[g].iter > 0) {
+      const T beta = static_cast<T>(a.st[g].beta);
+#pragma unroll
+      for (int k = 0; k < n; ++k) pn[k] = pn[k] + beta * pg[P2 * k];
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k) pg[P2 * k] = pn[k];
+  }
+  const long long qoff = qslot(a.qmap, g) * a.ldq + off + pp;
+#pragma unroll 1
+  for (int o = 0; o < NOUT; ++o) {
+    cpx<T> u[n];
+    if constexpr (OP == PRE_COPY) {
+#pragma unroll
+      for (int k = 0; k < n; ++k) u[k] = pn[k];
+    } else {
+      const T* wo = (o == 1) ? a.W1 : a.W2;
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        T w = a.invC[qoff + P2 * k];
+        if (o > 0) w *= wo[qoff + P2 * k];
+        u[k] = w * pn[k];
+      }
+    }
+    dft<T, n, +1>(u);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[pp + P2 * k] = u[k];
+    __syncthreads();
+    {  // dim 2: (k1, k3) = (pp % n, pp / n)
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+      dft<T, n, +1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+    }
+    __syncthreads();
+    {  // dim 1: (k2, k3) = (pp % n, pp / n)
+      const int base1 = (pp % n) * n + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+      dft<T, n, +1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base1 + k] = w[k];
+    }
+    __syncthreads();
+    cpx<T>* tg = a.t + (g * NOUT + o) * a.ldt + wg0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) tg[tid + THREADS * i] = lds[tid + THREADS * i];
+    if (o + 1 < NOUT) __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- pass B
 // MODE 0: inverse DFT dims 4-6, * eps/NF, forward DFT dims 6-4 (the mass apply).
 // MODE 1: inverse DFT dims 4-6 only (completes an unnormalised 6D ifft after pass A).
@@ -366,6 +447,88 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_c(PassCArgs<T> a) {
     for (int k = 0; k < n; ++k) {
       const cpx<T> d = dg[k];
       s += static_cast<double>(d.re * acc[k].re + d.im * acc[k].im);
+    }
+    s = block_sum<THREADS>(s, red);
+    if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------- pass C, coalesced I/O
+// Forward DFT over dims 3 (register pencil straight from HBM), 2, 1 (LDS); the
+// post-op reads the transformed block from LDS in linear order, so the weights, the
+// output and the p'q dot operand are all accessed coalesced.
+template <class T, int n, int BPW, int OP, int NOUT>
+__global__ __launch_bounds__(BPW* n* n) void k_pass_c_lin(PassCArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int THREADS = BPW * P2;
+  __shared__ cpx<T> lds[BPW * S];
+  __shared__ double red[THREADS];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int blk = tid / P2, pp = tid - blk * P2;
+  const long long wg0 = (long long)blockIdx.x * BPW * S;
+  const long long off = wg0 + (long long)blk * S;
+  cpx<T>* L = lds + blk * S;
+  const long long qoff = qslot(a.qmap, g) * a.ldq + wg0 + tid;
+
+  cpx<T> acc[n];
+#pragma unroll 1
+  for (int o = 0; o < NOUT; ++o) {
+    {
+      const cpx<T>* tg = a.t + (g * NOUT + o) * a.ldt + off + pp;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = tg[P2 * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[pp + P2 * k] = w[k];
+    }
+    __syncthreads();
+    {
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+    }
+    __syncthreads();
+    {
+      const int base1 = (pp % n) * n + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base1 + k] = w[k];
+    }
+    __syncthreads();
+    if (o == 0) {
+#pragma unroll
+      for (int i = 0; i < n; ++i) acc[i] = lds[tid + THREADS * i];
+    } else {
+      const T* wo = (o == 1) ? a.W1 : a.W2;
+#pragma unroll
+      for (int i = 0; i < n; ++i) acc[i] = acc[i] + wo[qoff + THREADS * i] * lds[tid + THREADS * i];
+    }
+    if (o + 1 < NOUT) __syncthreads();
+  }
+  if constexpr (OP == POST_MHAT) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) acc[i] = a.invC[qoff + THREADS * i] * acc[i];
+  }
+  cpx<T>* yg = a.y + g * a.ldy + wg0 + tid;
+#pragma unroll
+  for (int i = 0; i < n; ++i) yg[THREADS * i] = acc[i];
+  if (a.partials != nullptr) {
+    const cpx<T>* dg = a.dotp + g * a.lddot + wg0 + tid;
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      const cpx<T> d = dg[THREADS * i];
+      s += static_cast<double>(d.re * acc[i].re + d.im * acc[i].im);
     }
     s = block_sum<THREADS>(s, red);
     if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;

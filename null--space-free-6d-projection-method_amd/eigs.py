@@ -154,6 +154,34 @@ def run_method_C(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-
     return dict(lambda_=lam, Y=Y, stats=st.as_dict())
 
 
+def krinv_nested(sys: QSystem, b, tol_kr=1e-10, tol_m=1e-12, maxit_kr=600, maxit_m=1600, eps_c=None):
+    """Method-B K_r^{-1} b (make_nested_krinv_instrumented); eps_c defaults to the mean permittivity."""
+    eps_c = sys.mass.info["epsMean"] if eps_c is None else eps_c
+    b = sys._dev(b, sys.nr)
+    x = torch.empty_like(b)
+    st = _lib.KrinvStats()
+    call("fame6d_krinv_nested_apply", sys.handle, ptr(b), ptr(x), float(tol_kr), float(tol_m), int(maxit_kr),
+         int(maxit_m), float(eps_c), C.byref(st))
+    return x, st.as_dict()
+
+
+def run_method_B(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_kr=1e-10, tol_m=1e-12, maxit_kr=600,
+                 maxit_m=1600, eps_c=None, seed=20260905, v0=None, want_vectors=False):
+    """Method B (nested PCG) eigensolve with the same Lanczos as run_method_C
+    (cfg.nested of exp1/make_config_exp1bc.m:83-87)."""
+    eps_c = sys.mass.info["epsMean"] if eps_c is None else eps_c
+    if v0 is None:
+        v0 = start_vector(sys.nr, seed)
+    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    lam = np.empty(k)
+    Y = torch.empty((k, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
+    st = _lib.LanczosStats()
+    call("fame6d_eigs_method_b", sys.handle, int(k), int(p), float(tol), int(max_restart), float(tol_kr),
+         float(tol_m), int(maxit_kr), int(maxit_m), float(eps_c), v0.ctypes.data_as(C.c_void_p),
+         lam.ctypes.data_as(C.c_void_p), ptr(Y), C.byref(st))
+    return dict(lambda_=lam, Y=Y, stats=st.as_dict())
+
+
 def run_method_C_batch(mass: MassBackend, qs, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10,
                        maxit_cg=1200, seed=20260905, v0=None, precision="double", P=P_DEFAULT,
                        want_vectors=False):

@@ -275,8 +275,8 @@ def mhat_apply(sys, eps, x, N):
     return sys["invC"] * s
 
 
-def pcg(A, b, tol, maxit):
-    """MATLAB pcg without preconditioner, x0 = 0 (call sites
+def pcg(A, b, tol, maxit, Minv=None):
+    """MATLAB pcg with optional preconditioner z = Minv(r), x0 = 0 (call sites
     exp1/make_explicit_krinv_compact.m:43, exp2/recover_original_eigenvectors_exp2.m:21).
     Returns (x, flag, relres, iter).  Restates the recurrence, the true-residual
     confirmation and the 'moresteps' continuation; the stagnation test and the
@@ -297,19 +297,20 @@ def pcg(A, b, tol, maxit):
     maxmsteps = min(n // 50, 5, n - maxit)
     flag, it = 1, maxit
     for ii in range(1, maxit + 1):
+        z = r if Minv is None else Minv(r)
         rho1 = rho
-        rho = np.vdot(r, r).real
+        rho = np.vdot(r, z).real
         if rho == 0 or np.isinf(rho):
             flag, it = 4, ii - 1
             break
         if ii == 1:
-            p = r.copy()
+            p = z.copy()
         else:
             beta = rho / rho1
             if beta == 0 or np.isinf(beta):
                 flag, it = 4, ii - 1
                 break
-            p = r + beta * p
+            p = z + beta * p
         qv = A(p)
         pq = np.vdot(p, qv).real
         if pq <= 0 or np.isinf(pq):
@@ -346,6 +347,23 @@ def krinv_apply(sys, eps, q, N, tol=1e-10, maxit=1200):
     mr1 = mr1 + mass_apply(eps, sys["W1"] * z, N)
     mr2 = mr2 + mass_apply(eps, sys["W2"] * z, N)
     return solve_v1(sys, np.concatenate([mr1, mr2])), dict(flag=flag, relres=relres, iter=it)
+
+
+def krinv_nested(sys, eps, b, N, tol_kr=1e-10, tol_m=1e-12, maxit_kr=600, maxit_m=1600, epsC=4.0):
+    """Method B: PCG on K_r x = b with preconditioner epsC*r./sigma_r, every K_r z =
+    V^*(M^{-1}(V z)) by an inner PCG on M (exp1/make_nested_krinv_instrumented.m:33-132)."""
+    sig = np.concatenate([sys["Qq"], sys["Qq"]])
+
+    def Kr(z):
+        mi, flag, rr, it = pcg(lambda v: mass3_apply(eps, v, N), apply_V(sys, z), tol_m, maxit_m)
+        if flag != 0 and rr > 10 * tol_m:
+            raise RuntimeError("Nested M solve failed")
+        return apply_VH(sys, mi)
+
+    x, flag, relres, it = pcg(Kr, b, tol_kr, maxit_kr, Minv=lambda r: epsC * r / sig)
+    if flag != 0 and relres > 10 * tol_kr:
+        raise RuntimeError("Nested K_r solve failed")
+    return x, dict(flag=flag, relres=relres, iter=it)
 
 
 # ----------------------------------------------------------------------------- Lanczos

@@ -122,3 +122,17 @@ def test_gamma_point_rejected(pkg, n2):
     with pytest.raises(pkg.Fame6dError) as e:
         pkg.QSystem(n2["m"], (0.0, 0.0, 0.0))
     assert e.value.name == "EGAMMA"
+
+
+def test_method_b_nested(pkg, oracle, n2):
+    print("case: Method B nested N=2", flush=True)
+    rng = np.random.default_rng(12)
+    b = rand_c(rng, 2 * n2["m"].NF)
+    ref, _ = oracle.krinv_nested(n2["osys"], n2["eps"], b, n2["N"], 1e-10, 1e-12, 600, 1600, 4.0)
+    x, st = pkg.krinv_nested(n2["s"], torch.as_tensor(b, device="cuda"), 1e-10, 1e-12, 600, 1600, 4.0)
+    assert np.linalg.norm(x.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-8
+    # cross-method agreement of the eigenvalues (exp1/check_fig1bc_results.m:23-25 warns above 1e-7)
+    v0 = pkg.start_vector(n2["s"].nr, 3)
+    lb = pkg.run_method_B(n2["s"], k=4, p=24, tol=1e-10, max_restart=40, eps_c=4.0, v0=v0)["lambda_"]
+    lc = pkg.run_method_C(n2["s"], k=4, p=24, tol=1e-10, max_restart=40, v0=v0)["lambda_"]
+    assert np.max(np.abs(lb - lc) / lc) < 1e-7

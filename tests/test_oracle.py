@@ -244,3 +244,16 @@ def test_hellmann_feynman_fd_n1(oracle):
         h = 1e-6
         fd = (lam_and_vec(eps + h * d)[0][j] - lam_and_vec(eps - h * d)[0][j]) / (2 * h)
         assert abs(fd - g @ d) < 1e-6 * abs(fd) + 1e-12
+
+
+def test_method_b_matches_method_c(oracle):
+    # both realise K_r^{-1}: the nested solve and the explicit Method-C inverse agree
+    N = 1
+    eps = oracle.eps_reference(N)
+    s = oracle.qsystem(N, Q_EXP4)
+    rng = np.random.default_rng(31)
+    b = rng.standard_normal(2 * s["NF"]) + 1j * rng.standard_normal(2 * s["NF"])
+    xb, stb = oracle.krinv_nested(s, eps, b, N, 1e-12, 1e-13, 600, 1600, 4.0)
+    xc, stc = oracle.krinv_apply(s, eps, b, N, 1e-12, 3000)
+    assert np.linalg.norm(xb - xc) / np.linalg.norm(xc) < 1e-9
+    assert np.linalg.norm(oracle.kr_apply(s, eps, xb, N) - b) / np.linalg.norm(b) < 1e-10
