@@ -80,9 +80,10 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
  * passes at this N and precision and keeps the fastest (no reference counterpart;
  * the reference has one implicit cuFFT plan). */
 int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
-/* Set tile variants explicitly: variant_ac in 0..3 (bit 0: blocks per workgroup of
- * passes A/C, bit 1 set: dim-1 register-pencil I/O instead of the coalesced dim-3-first
- * I/O), variant_b in 0..1 (inner positions per workgroup of pass B). */
+/* Set tile variants explicitly: variant_ac in 0..7 (bit 0: blocks per workgroup of the
+ * plain passes A/C, bit 1 set: their dim-1 register-pencil I/O instead of the coalesced
+ * dim-3-first I/O, bit 2: blocks per workgroup of the three-group Mhat passes A/C),
+ * variant_b in 0..1 (inner positions per workgroup of pass B). */
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
 
 /* mass.MscalarOp: applyMij_rank1_exact_lowmem (exp1/applyMij_rank1_exact_lowmem.m:8-22),
@@ -118,10 +119,12 @@ int fame6d_qsys_info(const fame6d_qsys* s, fame6d_qsys_info_t* info);
 int fame6d_qsys_diagonals(const fame6d_qsys* s, double* V11, double* V12, double* V21, double* V22,
                           double* V31, double* V32, double* W1, double* W2, double* invC);
 
-/* Parameters of the FAME6D_MIXED Mhat solve: relative tolerance of each FP32 inner PCG
- * and the maximum number of FP64 refinement cycles (no reference counterpart; the
- * reference solves in FP64 only, cfg.gpu.precision = 'double'). */
-int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles);
+/* Parameters of the FAME6D_MIXED Mhat solve: relative tolerance of each FP32 inner PCG,
+ * the maximum number of FP64 refinement cycles, and whether the FP32 inner PCG is
+ * preconditioned by Mhat evaluated with 1/eps (exact for constant eps; the FP64
+ * refinement keeps the result independent of this choice).  No reference counterpart:
+ * the reference solves in FP64 only (cfg.gpu.precision = 'double'). */
+int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles, int precondition);
 
 /* apply_Mhat_C_scalar_compact (exp1/apply_Mhat_C_scalar_compact.m:8-21): x, y NF. */
 int fame6d_mhat_apply(fame6d_qsys* s, int precision, const void* x, void* y);

@@ -80,6 +80,92 @@ __global__ void k_apply_k_api(const double* q1, const double* q2, const double* 
   }
 }
 
+// Times a callable `reps` times on the mass stream (after one warm-up call).
+template <class F>
+static float time_ms(fame6d_mass* m, hipEvent_t e0, hipEvent_t e1, int reps, F&& f) {
+  f();
+  F6_HIP(hipStreamSynchronize(m->stream));
+  F6_HIP(hipEventRecord(e0, m->stream));
+  for (int r = 0; r < reps; ++r) f();
+  F6_HIP(hipEventRecord(e1, m->stream));
+  F6_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  F6_HIP(hipEventElapsedTime(&ms, e0, e1));
+  return ms;
+}
+
+template <class T>
+static void autotune_t(fame6d_mass* m, int reps) {
+  const long long NF = m->NF;
+  TileChoice& tc = std::is_same<T, float>::value ? m->tile32 : m->tile64;
+  DevBuf vec, wts;
+  vec.ensure(sizeof(cpx<T>) * NF * 4);
+  wts.ensure(sizeof(T) * NF * 3);
+  cpx<T>* x = vec.as<cpx<T>>();
+  k_fill_zero<T><<<grid_for(NF * 4), 256, 0, m->stream>>>(NF * 4, x);
+  k_fill_zero_real<T><<<grid_for(NF * 3), 256, 0, m->stream>>>(NF * 3, wts.as<T>());
+  F6_CHECK_LAUNCH();
+  hipEvent_t e0, e1;
+  F6_HIP(hipEventCreate(&e0));
+  F6_HIP(hipEventCreate(&e1));
+  // 1) Mhat (three scalar applies): pass-B tile and the thread-split pass A/C tile
+  DevBuf& w = work_buf<T>(m);
+  w.ensure(sizeof(cpx<T>) * NF * 3);
+  cpx<T>* t = w.as<cpx<T>>();
+  PassAArgs<T> pa{};
+  pa.x = x;
+  pa.ldx = NF;
+  pa.t = t;
+  pa.ldt = NF;
+  pa.W1 = wts.as<T>();
+  pa.W2 = wts.as<T>() + NF;
+  pa.invC = wts.as<T>() + 2 * NF;
+  pa.ldq = NF;
+  PassCArgs<T> pc{};
+  pc.t = t;
+  pc.ldt = NF;
+  pc.y = x + NF;
+  pc.ldy = NF;
+  pc.W1 = pa.W1;
+  pc.W2 = pa.W2;
+  pc.invC = pa.invC;
+  pc.ldq = NF;
+  float best = 1e30f;
+  TileChoice bc = tc;
+  for (int vb = 0; vb < 2; ++vb)
+    for (int v3 = 0; v3 < 2; ++v3) {
+      tc.b = vb;
+      tc.a3 = v3;
+      const float ms = time_ms(m, e0, e1, reps, [&]() {
+        launch_pass_a<T>(m, PRE_MHAT, 1, pa, m->stream);
+        launch_pass_b<T>(m, 3, 0, 1, t, NF, no_skip(), m->stream);
+        launch_pass_c<T>(m, POST_MHAT, 1, pc, m->stream);
+      });
+      if (ms < best) {
+        best = ms;
+        bc = tc;
+      }
+    }
+  tc = bc;
+  // 2) plain applies (two right-hand sides, as around each K_r^{-1}): pass A/C tile and I/O form
+  best = 1e30f;
+  for (int vio = 0; vio < 2; ++vio)
+    for (int va = 0; va < 2; ++va) {
+      tc.io = vio;
+      tc.a = va;
+      const float ms = time_ms(m, e0, e1, reps, [&]() { mass_apply<T>(m, x, NF, x + 2 * NF, NF, 2, m->stream); });
+      if (ms < best) {
+        best = ms;
+        bc = tc;
+      }
+    }
+  tc = bc;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  vec.release();
+  wts.release();
+}
+
 }  // namespace fame6d
 
 using namespace fame6d;
@@ -124,16 +210,22 @@ static void upload_eps(fame6d_mass* m, const double* eps_host) {
   m->eps_max = mx;
   m->eps_mean = sum / (double)NF;
   m->eps_host.assign(eps_host, eps_host + NF);
-  std::vector<double> e64(NF);
-  std::vector<float> e32(NF);
+  std::vector<double> e64(NF), i64(NF);
+  std::vector<float> e32(NF), i32(NF);
   for (long long i = 0; i < NF; ++i) {
     e64[i] = eps_host[i] / (double)NF;
     e32[i] = (float)e64[i];
+    i64[i] = 1.0 / (eps_host[i] * (double)NF);
+    i32[i] = (float)i64[i];
   }
   m->eps64.ensure(sizeof(double) * NF);
   m->eps32.ensure(sizeof(float) * NF);
+  m->ieps64.ensure(sizeof(double) * NF);
+  m->ieps32.ensure(sizeof(float) * NF);
   F6_HIP(hipMemcpyAsync(m->eps64.ptr, e64.data(), sizeof(double) * NF, hipMemcpyHostToDevice, m->stream));
   F6_HIP(hipMemcpyAsync(m->eps32.ptr, e32.data(), sizeof(float) * NF, hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(m->ieps64.ptr, i64.data(), sizeof(double) * NF, hipMemcpyHostToDevice, m->stream));
+  F6_HIP(hipMemcpyAsync(m->ieps32.ptr, i32.data(), sizeof(float) * NF, hipMemcpyHostToDevice, m->stream));
   F6_HIP(hipStreamSynchronize(m->stream));
 }
 
@@ -175,6 +267,8 @@ int fame6d_mass_destroy(fame6d_mass* m) {
   (void)hipStreamSynchronize(m->stream);
   m->eps64.release();
   m->eps32.release();
+  m->ieps64.release();
+  m->ieps32.release();
   m->work64.release();
   m->work32.release();
   m->io64.release();
@@ -204,10 +298,11 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   F6_API_BEGIN
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
-  F6_REQUIRE(va >= 0 && va < 4 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
+  F6_REQUIRE(va >= 0 && va < 8 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
   TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
   t.a = va & 1;
-  t.io = (va >> 1) ? 0 : 1;
+  t.io = ((va >> 1) & 1) ? 0 : 1;
+  t.a3 = (va >> 2) & 1;
   t.b = vb;
   F6_API_END
 }
@@ -217,54 +312,8 @@ int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps) {
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
   F6_REQUIRE(precision == FAME6D_FP64 || precision == FAME6D_FP32, FAME6D_EARG, "precision");
   if (reps < 1) reps = 1;
-  const bool f32 = precision == FAME6D_FP32;
-  const long long NF = m->NF;
-  const size_t esz = f32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-  DevBuf buf;
-  buf.ensure(esz * NF * 2);
-  if (f32) k_fill_zero<float><<<grid_for(NF * 2), 256, 0, m->stream>>>(NF * 2, buf.as<cpx<float>>());
-  else k_fill_zero<double><<<grid_for(NF * 2), 256, 0, m->stream>>>(NF * 2, buf.as<cpx<double>>());
-  F6_CHECK_LAUNCH();
-  hipEvent_t e0, e1;
-  F6_HIP(hipEventCreate(&e0));
-  F6_HIP(hipEventCreate(&e1));
-  TileChoice& tc = f32 ? m->tile32 : m->tile64;
-  float best = 1e30f;
-  TileChoice bestc = tc;
-  for (int vio = 0; vio < 2; ++vio)
-  for (int va = 0; va < 2; ++va) {
-    for (int vb = 0; vb < 2; ++vb) {
-      tc.io = vio;
-      tc.a = va;
-      tc.b = vb;
-      auto run = [&]() {
-        if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 3, m->stream);
-        else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
-      };
-      // one warm-up (and the workspace allocation) outside the timed region
-      if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 1, m->stream);
-      else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
-      F6_HIP(hipStreamSynchronize(m->stream));
-      F6_HIP(hipEventRecord(e0, m->stream));
-      for (int r = 0; r < reps; ++r) {
-        if (f32) mass_apply<float>(m, buf.as<cpx<float>>(), NF, buf.as<cpx<float>>() + NF, NF, 1, m->stream);
-        else mass_apply<double>(m, buf.as<cpx<double>>(), NF, buf.as<cpx<double>>() + NF, NF, 1, m->stream);
-      }
-      F6_HIP(hipEventRecord(e1, m->stream));
-      F6_HIP(hipEventSynchronize(e1));
-      (void)run;
-      float ms = 0;
-      F6_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bestc = tc;
-      }
-    }
-  }
-  tc = bestc;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  buf.release();
+  if (precision == FAME6D_FP32) autotune_t<float>(m, reps);
+  else autotune_t<double>(m, reps);
   F6_API_END
 }
 
@@ -404,11 +453,12 @@ int fame6d_krinv_apply(fame6d_qsys* s, int precision, const void* q, void* y, do
   F6_API_END
 }
 
-int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles) {
+int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles, int precondition) {
   F6_API_BEGIN
   F6_REQUIRE(s && inner_tol > 0 && inner_tol < 1 && max_cycles > 0, FAME6D_EARG, "bad mixed parameters");
   s->mixed_inner_tol = inner_tol;
   s->mixed_max_cycles = max_cycles;
+  s->mixed_precond = precondition != 0;
   F6_API_END
 }
 

@@ -60,6 +60,7 @@ struct TileChoice {
   int a = 0;  // pass A / pass C variant
   int b = 0;  // pass B variant
   int io = 1;  // passes A/C: 1 = coalesced dim-3-first I/O, 0 = dim-1 register pencils
+  int a3 = 0;  // Mhat passes A/C (three thread groups) variant
 };
 
 }  // namespace fame6d
@@ -72,6 +73,7 @@ struct fame6d_mass {
   int device = 0;
   double eps_min = 0, eps_max = 0, eps_mean = 0;
   fame6d::DevBuf eps64, eps32;  // eps / NF
+  fame6d::DevBuf ieps64, ieps32;  // (1/eps) / NF: the inverse mass M_s^{-1} = F diag(1/eps) F^{-1}
   fame6d::DevBuf work64, work32;   // pass intermediates
   fame6d::DevBuf io64;             // staging for component wrappers
   fame6d::DevBuf partials;         // reduction partials
@@ -88,7 +90,7 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
 // mode 0: fused inverse / eps / forward over dims 4-6; mode 1: inverse only.
 template <class T>
 void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
-                   Skip skip, hipStream_t s);
+                   Skip skip, hipStream_t s, bool inverse_eps = false);
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s);
 // Number of workgroups pass C uses per group (size of its partials per group).
@@ -128,6 +130,7 @@ struct fame6d_qsys {
   // Method-C workspaces (grown to the number of active groups)
   fame6d::DevBuf kr64, kr32, st, part, part2, ints, sc;
   double mixed_inner_tol = 3e-5;
+  bool mixed_precond = true;  // FP32 inner PCG preconditioned by Mhat evaluated with 1/eps
   int mixed_max_cycles = 30;
   int chunk_hint = 0, chunk_hint32 = 0;
   fame6d_krinv_stats_t stats{};
@@ -144,7 +147,8 @@ fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P)
 // y = Mhat x for nact groups in q-system slots qmap (device ints, nullptr = identity).
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
-                hipStream_t s);
+                hipStream_t s, bool inverse_eps = false, const cpx<T>* dotp = nullptr,
+                double* partials = nullptr);
 // Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).
 void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                   const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,

@@ -182,6 +182,65 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_init(CgState* st, con
   }
 }
 
+// Preconditioned start: n2b from ||r||^2 partials, rho = r'z from the r'z partials.
+static __global__ __launch_bounds__(kRedThreads) void k_pcg_init(CgState* st, const double* prr, int nbr,
+                                                                 const double* prz, int nbz, double tol,
+                                                                 int maxit) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  const double s = sum_partials(prr + (long long)g * nbr, nbr, red);
+  __syncthreads();
+  const double rz = sum_partials(prz + (long long)g * nbz, nbz, red);
+  if (threadIdx.x == 0) {
+    CgState c{};
+    c.n2b = sqrt(s);
+    c.tolb = tol * c.n2b;
+    c.rho = rz;
+    c.normr = c.n2b;
+    c.maxit = maxit;
+    c.done = (!(c.n2b > 0.0) || c.normr <= c.tolb || maxit <= 0) ? 1 : 0;
+    if (!isfinite(c.n2b) || !isfinite(rz)) c.flag = -1;
+    if (maxit <= 0 && c.done && c.n2b > 0.0 && c.normr > c.tolb) c.flag = 1;
+    st[g] = c;
+  }
+}
+
+// Preconditioned step end: normr from ||r||^2 partials, rho_new = r'z from r'z partials.
+static __global__ __launch_bounds__(kRedThreads) void k_pcg_beta(CgState* st, const double* prr, int nbr,
+                                                                 const double* prz, int nbz) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (st[g].done) return;
+  const double rr = sum_partials(prr + (long long)g * nbr, nbr, red);
+  __syncthreads();
+  const double rho_new = sum_partials(prz + (long long)g * nbz, nbz, red);
+  if (threadIdx.x == 0) {
+    CgState c = st[g];
+    c.iter += 1;
+    c.normr = sqrt(rr);
+    if (!isfinite(rr) || !isfinite(rho_new)) {
+      c.flag = -1;
+      c.done = 1;
+    } else if (c.normr <= c.tolb || c.check_every) {
+      c.flag = 0;
+      c.done = 1;
+    } else if (c.iter >= c.maxit) {
+      c.flag = 1;
+      c.done = 1;
+    } else {
+      const double beta = rho_new / c.rho;
+      if (beta == 0.0 || isinf(beta) || rho_new == 0.0) {
+        c.flag = 4;
+        c.done = 1;
+      }
+      c.beta = beta;
+    }
+    c.rho_prev = c.rho;
+    c.rho = rho_new;
+    st[g] = c;
+  }
+}
+
 static __global__ __launch_bounds__(kRedThreads) void k_cg_alpha(CgState* st, const double* partials,
                                                           int nb) {
   __shared__ double red[kRedThreads];
@@ -394,6 +453,89 @@ static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long lon
     cpx<double> acc = mk(0.0, 0.0);
     for (int l = 0; l < m; ++l) acc = acc + Us[c * m + l] * tile[l * kRestartRows + rr];
     if (i < n) Q[(long long)c * ld + i] = acc;
+  }
+}
+
+// ------------------------------------------------------------------ fused CGS2 passes
+// One sweep over the basis Q (n rows x jj columns, column-major, ld = n) in tiles of 64
+// rows staged in LDS.  MODE 0: h = Q^H w.  MODE 1: r = w - Q hin, then h = Q^H r.
+// MODE 2: r = w - Q hin, then ||r||^2.  Per-workgroup partials (column k of workgroup b
+// at pre[k*gridDim.x + b]) are folded by k_reduce_final; thread t owns column t/4 of the
+// dot products across all of its workgroup's tiles, so no per-tile reduction is needed.
+constexpr int kCgsRows = 64;
+constexpr int kCgsMaxCols = 64;
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cgs_sweep(const cpx<double>* Q, long long n, int jj,
+                                                   const cpx<double>* w, const double* hin_re,
+                                                   const double* hin_im, cpx<double>* r, double* pre,
+                                                   double* pim) {
+  __shared__ cpx<double> Qs[kCgsMaxCols * kCgsRows];
+  __shared__ cpx<double> rs[kCgsRows];
+  __shared__ cpx<double> hs[kCgsMaxCols];
+  __shared__ cpx<double> part[256];
+  const int tid = threadIdx.x;
+  if (MODE != 0) {
+    for (int k = tid; k < jj; k += 256) hs[k] = mk(hin_re[k], hin_im[k]);
+  }
+  const int kc = tid >> 2, quarter = tid & 3;  // dot ownership: column kc, rows quarter*16 .. +16
+  double ar = 0.0, ai = 0.0;
+  const long long ntiles = (n + kCgsRows - 1) / kCgsRows;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long r0 = tile * kCgsRows;
+    __syncthreads();  // previous tile consumed (and hs staged)
+    for (int e = tid; e < jj * kCgsRows; e += 256) {
+      const int k = e / kCgsRows, row = e - k * kCgsRows;
+      const long long i = r0 + row;
+      Qs[e] = (i < n) ? Q[(long long)k * n + i] : mk(0.0, 0.0);
+    }
+    if (tid < kCgsRows) {
+      const long long i = r0 + tid;
+      rs[tid] = (i < n) ? w[i] : mk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (MODE != 0) {
+      // r(row) = w(row) - sum_k Q(row,k) h(k): four threads per row, each a quarter of k
+      const int row = tid >> 2, q4 = tid & 3;
+      cpx<double> sacc = mk(0.0, 0.0);
+      for (int k = q4; k < jj; k += 4) sacc = sacc + Qs[k * kCgsRows + row] * hs[k];
+      part[tid] = sacc;
+      __syncthreads();
+      if (tid < kCgsRows) {
+        const cpx<double> tot = part[4 * tid] + part[4 * tid + 1] + part[4 * tid + 2] + part[4 * tid + 3];
+        const cpx<double> rv = rs[tid] - tot;
+        rs[tid] = rv;
+        const long long i = r0 + tid;
+        if (i < n) r[i] = rv;
+      }
+      __syncthreads();
+    }
+    if (MODE == 2) {
+      if (tid < kCgsRows) {
+        const cpx<double> v = rs[tid];
+        ar += v.re * v.re + v.im * v.im;
+      }
+    } else if (kc < jj) {
+      for (int row = quarter * 16; row < quarter * 16 + 16; ++row) {
+        const cpx<double> q = Qs[kc * kCgsRows + row], v = rs[row];
+        ar += q.re * v.re + q.im * v.im;
+        ai += q.re * v.im - q.im * v.re;
+      }
+    }
+  }
+  if (MODE == 2) {
+    __shared__ double red[256];
+    const double sres = block_sum<256>(ar, red);
+    if (tid == 0) pre[blockIdx.x] = sres;
+  } else {
+    // fold the four quarters of each column (lanes 4k..4k+3 are in one wave)
+    ar += __shfl_down(ar, 2, 64);
+    ai += __shfl_down(ai, 2, 64);
+    ar += __shfl_down(ar, 1, 64);
+    ai += __shfl_down(ai, 1, 64);
+    if (quarter == 0 && kc < jj) {
+      pre[(long long)kc * gridDim.x + blockIdx.x] = ar;
+      pim[(long long)kc * gridDim.x + blockIdx.x] = ai;
+    }
   }
 }
 

@@ -6,10 +6,13 @@
 // per step, the projected matrix H = [diag(d) c'; c tridiag], the Ritz residual
 // |beta * U(end,:)|, the convergence threshold tol*max(eps^(2/3), |theta|) and
 // the restart size k0 + min(nconv, floor((p-k0)/2)).  The basis stays on the
-// device; each re-orthogonalisation pass is classical Gram-Schmidt (one
-// Q^H r product and one Q h update) instead of the reference's column loop,
-// which is the same projection up to rounding.  Several Bloch points step
-// together and share one batched K_r^{-1} per step.
+// device.  The orthogonalisation of each new vector w = K_r^{-1} q_j is two full
+// classical Gram-Schmidt projections r = (I - QQ^H)(I - QQ^H) w done in three fused
+// sweeps over the basis; the reference's three-term recurrence, its two extra
+// single-vector projections and its two column-by-column passes compute the same
+// projection in exact arithmetic (alpha = Re(q_j^H w), beta = ||r||), so results
+// agree up to rounding.  Several Bloch points step together and share one batched
+// K_r^{-1} per step.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -125,19 +128,22 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
   F6_CHECK_LAUNCH();
 }
 
-// r -= Q(:,0:j) (Q(:,0:j)^H r)  -- one classical Gram-Schmidt pass over j columns
-static void cgs_pass(long long n, const Basis& B, int j, hipStream_t s) {
-  const int nb = red_blocks(n);
+// One Lanczos orthogonalisation step in three sweeps over the basis.  h1 is kept in
+// sc[kScH..] (its entry jj-1 is alpha = q_j^H w), h2 in sc[kScH2..], ||r||^2 in sc[1].
+constexpr int kScH2 = 8 + 256, kScH2im = 8 + 384;
+constexpr int kCgsBlocks = 1024;
+static void cgs2_step(long long n, const Basis& B, int jj, const cpx<double>* w, hipStream_t s) {
+  const int nb = kCgsBlocks;
   double* pre = B.part;
-  double* pim = B.part + (size_t)nb * 128;
-  constexpr int KB = 8;
-  for (int k0 = 0; k0 < j; k0 += KB) {
-    const int kc = std::min(KB, j - k0);
-    k_gemv_c_partials<KB><<<nb, kRedThreads, 0, s>>>(B.Q, n, k0, kc, B.r, n, pre, pim);
-  }
-  k_reduce_final<<<j, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH);
-  k_reduce_final<<<j, kRedThreads, 0, s>>>(pim, nb, B.sc + kScHim);
-  k_gemv_n_sub<<<grid_for(n), 256, 0, s>>>(B.Q, n, j, B.sc + kScH, B.sc + kScHim, B.r, n);
+  double* pim = B.part + (size_t)nb * kCgsMaxCols;
+  k_cgs_sweep<0><<<nb, 256, 0, s>>>(B.Q, n, jj, w, nullptr, nullptr, nullptr, pre, pim);
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH);
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, B.sc + kScHim);
+  k_cgs_sweep<1><<<nb, 256, 0, s>>>(B.Q, n, jj, w, B.sc + kScH, B.sc + kScHim, B.r, pre, pim);
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH2);
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, B.sc + kScH2im);
+  k_cgs_sweep<2><<<nb, 256, 0, s>>>(B.Q, n, jj, B.r, B.sc + kScH2, B.sc + kScH2im, B.r, pre, nullptr);
+  k_reduce_final<<<1, kRedThreads, 0, s>>>(pre, nb, B.sc + 1);
   F6_CHECK_LAUNCH();
 }
 
@@ -275,7 +281,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   DevBuf basis, scal, part, io, ubuf;
   basis.ensure(sizeof(cpx<double>) * (size_t)n2 * (p + 2) * G);
   scal.ensure(sizeof(double) * 512 * G);
-  part.ensure(sizeof(double) * (size_t)nb * 256 * G);
+  part.ensure(sizeof(double) * (size_t)std::max(nb * 256, kCgsBlocks * kCgsMaxCols * 2) * G);
   io.ensure(sizeof(cpx<double>) * (size_t)n2 * 2 * G);
   std::vector<LzState> L(G);
   for (int g = 0; g < G; ++g) {
@@ -285,7 +291,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     L[g].B.v = b0 + (size_t)n2 * p;
     L[g].B.r = b0 + (size_t)n2 * (p + 1);
     L[g].B.sc = scal.as<double>() + 512 * g;
-    L[g].B.part = part.as<double>() + (size_t)nb * 256 * g;
+    L[g].B.part = part.as<double>() + (size_t)std::max(nb * 256, kCgsBlocks * kCgsMaxCols * 2) * g;
     L[g].k = k0;
   }
   cpx<double>* qin = io.as<cpx<double>>();
@@ -335,34 +341,18 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       const Basis& B = Lg.B;
-      const int j = Lg.jj - 1;
+      const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
       const cpx<double>* w = qout + (size_t)n2 * i;
-      cpx<double>* Qj = B.Q + (size_t)n2 * j;
-      // alpha = real(Aq' * Q(:,jj))
-      dot_dev(n2, Qj, w, B.part, B.sc + 0, nullptr, s);
-      F6_HIP(hipMemcpyAsync(B.r, w, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
-      if (Lg.jj == 1) {
-        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 0, nullptr, -1.0);
-      } else if (Lg.justRestarted) {
-        cgs_pass(n2, B, Lg.jj, s);
-        Lg.justRestarted = false;
-      } else {
-        const cpx<double>* Qm1 = B.Q + (size_t)n2 * (j - 1);
-        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 0, nullptr, -1.0);
-        k_axpby<double><<<gr, 256, 0, s>>>(n2, mk(1.0, 0.0), B.r, mk(-Lg.beta, 0.0), Qm1, B.r);
-        dot_dev(n2, Qm1, B.r, B.part, B.sc + 2, B.sc + 3, s);
-        k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qm1, B.sc + 2, B.sc + 3, -1.0);
-      }
-      dot_dev(n2, Qj, B.r, B.part, B.sc + 2, B.sc + 3, s);
-      k_axpy_dev<<<gr, 256, 0, s>>>(n2, B.r, Qj, B.sc + 2, B.sc + 3, -1.0);
-      cgs_pass(n2, B, Lg.jj, s);
-      cgs_pass(n2, B, Lg.jj, s);
-      dot_dev(n2, B.r, B.r, B.part, B.sc + 1, nullptr, s);
-      F6_CHECK_LAUNCH();
+      // CGS2: h1 = Q^H w (alpha = Re h1[j]); r = w - Q h1 and h2 = Q^H r; r -= Q h2 with ||r||^2
+      cgs2_step(n2, B, jj, w, s);
+      Lg.justRestarted = false;
     }
     std::vector<double> ab(2 * na);
-    for (int i = 0; i < na; ++i)
-      F6_HIP(hipMemcpyAsync(&ab[2 * i], L[act[i]].B.sc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    for (int i = 0; i < na; ++i) {
+      const LzState& Lg = L[act[i]];
+      F6_HIP(hipMemcpyAsync(&ab[2 * i], Lg.B.sc + kScH + (Lg.jj - 1), sizeof(double), hipMemcpyDeviceToHost, s));
+      F6_HIP(hipMemcpyAsync(&ab[2 * i + 1], Lg.B.sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
     F6_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];

@@ -12,24 +12,26 @@ namespace fame6d {
 // LDS = A*n^3 complex values).  Every entry divides S = n^3.
 template <int n, class T>
 struct Tiles;
-#define F6_TILES(N_, T_, BPW0, BPW1, A0, A1)                 \
+// Mhat passes (three thread groups): BPW3 blocks per workgroup, threads = 3*BPW3*n^2.
+#define F6_TILES(N_, T_, BPW0, BPW1, A0, A1, B30, B31)        \
   template <>                                                 \
   struct Tiles<N_, T_> {                                      \
     static constexpr int bpw[2] = {BPW0, BPW1};               \
     static constexpr int a[2] = {A0, A1};                     \
+    static constexpr int bpw3[2] = {B30, B31};                \
   };
-F6_TILES(2, double, 8, 8, 8, 8)
-F6_TILES(2, float, 8, 8, 8, 8)
-F6_TILES(4, double, 16, 8, 16, 32)
-F6_TILES(4, float, 16, 8, 16, 32)
-F6_TILES(6, double, 8, 6, 8, 24)
-F6_TILES(6, float, 8, 6, 8, 24)
-F6_TILES(8, double, 4, 8, 8, 4)
-F6_TILES(8, float, 4, 8, 8, 16)
-F6_TILES(10, double, 4, 2, 4, 8)
-F6_TILES(10, float, 4, 2, 8, 4)
-F6_TILES(12, double, 2, 2, 4, 2)
-F6_TILES(12, float, 2, 2, 4, 6)
+F6_TILES(2, double, 8, 8, 8, 8, 8, 8)
+F6_TILES(2, float, 8, 8, 8, 8, 8, 8)
+F6_TILES(4, double, 16, 8, 16, 32, 8, 4)
+F6_TILES(4, float, 16, 8, 16, 32, 8, 4)
+F6_TILES(6, double, 8, 6, 8, 24, 4, 2)
+F6_TILES(6, float, 8, 6, 8, 24, 4, 2)
+F6_TILES(8, double, 4, 8, 8, 4, 2, 1)
+F6_TILES(8, float, 4, 8, 8, 16, 2, 4)
+F6_TILES(10, double, 4, 2, 4, 8, 1, 2)
+F6_TILES(10, float, 4, 2, 8, 4, 1, 2)
+F6_TILES(12, double, 2, 2, 4, 2, 1, 1)
+F6_TILES(12, float, 2, 2, 4, 6, 1, 2)
 #undef F6_TILES
 
 template <class F>
@@ -55,6 +57,23 @@ template <class T>
 void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
   const int v = tile_of<T>(m).a;
+  if (op == PRE_MHAT || op == PRE_MHAT_CG) {
+    const int v3 = tile_of<T>(m).a3;
+    with_n(m->n, [&](auto nc) {
+      constexpr int n = decltype(nc)::value;
+      constexpr int S = n * n * n;
+      auto go = [&](auto bc) {
+        constexpr int BPW = decltype(bc)::value;
+        dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
+        if (op == PRE_MHAT) k_pass_a3<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a);
+        else k_pass_a3<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a);
+      };
+      if (v3 == 0) go(std::integral_constant<int, Tiles<n, T>::bpw3[0]>{});
+      else go(std::integral_constant<int, Tiles<n, T>::bpw3[1]>{});
+    });
+    F6_CHECK_LAUNCH();
+    return;
+  }
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
@@ -85,16 +104,18 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
 
 template <class T>
 void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
-                   Skip skip, hipStream_t s) {
+                   Skip skip, hipStream_t s, bool inverse_eps) {
   if (ngroups <= 0) return;
-  PassBArgs<T> a{t, ldt, (const T*)(std::is_same<T, double>::value ? m->eps64.ptr : m->eps32.ptr), skip};
+  const void* e = std::is_same<T, double>::value ? (inverse_eps ? m->ieps64.ptr : m->eps64.ptr)
+                                                 : (inverse_eps ? m->ieps32.ptr : m->eps32.ptr);
+  PassBArgs<T> a{t, ldt, (const T*)e, skip};
   const int v = tile_of<T>(m).b;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
     auto go = [&](auto ac) {
       constexpr int A = decltype(ac)::value;
-      dim3 grid(S / A, ngroups), block(A * n * n);
+      dim3 grid(S / A, ngroups * nr), block(A * n * n);
       if (mode == 0 && nr == 1) k_pass_b<T, n, A, 1, 0><<<grid, block, 0, s>>>(a);
       else if (mode == 0 && nr == 3) k_pass_b<T, n, A, 3, 0><<<grid, block, 0, s>>>(a);
       else if (mode == 1 && nr == 1) k_pass_b<T, n, A, 1, 1><<<grid, block, 0, s>>>(a);
@@ -109,6 +130,22 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
+  if (op == POST_MHAT) {
+    const int v3 = tile_of<T>(m).a3;
+    with_n(m->n, [&](auto nc) {
+      constexpr int n = decltype(nc)::value;
+      constexpr int S = n * n * n;
+      auto go = [&](auto bc) {
+        constexpr int BPW = decltype(bc)::value;
+        dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
+        k_pass_c3<T, n, BPW><<<grid, block, 0, s>>>(a);
+      };
+      if (v3 == 0) go(std::integral_constant<int, Tiles<n, T>::bpw3[0]>{});
+      else go(std::integral_constant<int, Tiles<n, T>::bpw3[1]>{});
+    });
+    F6_CHECK_LAUNCH();
+    return;
+  }
   const int v = tile_of<T>(m).a;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
@@ -130,12 +167,13 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
 }
 
 int pass_c_blocks(const fame6d_mass* m, bool fp32) {
+  // workgroups per group of the Mhat pass C (the only pass C that writes partials)
   int out = 0;
-  const int v = fp32 ? m->tile32.a : m->tile64.a;
+  const int v = fp32 ? m->tile32.a3 : m->tile64.a3;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
-    const int bpw = fp32 ? Tiles<n, float>::bpw[v] : Tiles<n, double>::bpw[v];
+    const int bpw = fp32 ? Tiles<n, float>::bpw3[v] : Tiles<n, double>::bpw3[v];
     out = S / bpw;
   });
   return out;
@@ -190,7 +228,7 @@ void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long 
 #define F6_INST(T)                                                                                 \
   template void launch_pass_a<T>(const fame6d_mass*, int, int, const PassAArgs<T>&, hipStream_t);  \
   template void launch_pass_b<T>(const fame6d_mass*, int, int, int, cpx<T>*, long long, Skip,      \
-                                 hipStream_t);                                                     \
+                                 hipStream_t, bool);                                               \
   template void launch_pass_c<T>(const fame6d_mass*, int, int, const PassCArgs<T>&, hipStream_t);  \
   template void mass_apply<T>(fame6d_mass*, const cpx<T>*, long long, cpx<T>*, long long, int,     \
                               hipStream_t);                                                        \

@@ -297,83 +297,72 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
 // ---------------------------------------------------------------- pass B
 // MODE 0: inverse DFT dims 4-6, * eps/NF, forward DFT dims 6-4 (the mass apply).
 // MODE 1: inverse DFT dims 4-6 only (completes an unnormalised 6D ifft after pass A).
+// grid.y enumerates vectors; vector v belongs to group v / NR (for the skip mask),
+// so the NR vectors of one Mhat apply run in separate workgroups.
 template <class T, int n, int A, int NR, int MODE>
 __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
   constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
   __shared__ cpx<T> L[A * S];  // l = al + A*(k4 + n*k5 + n^2*k6)
-  const int g = blockIdx.y;
-  if (skipped(a.skip, g)) return;
+  const int vidx = blockIdx.y;
+  if (skipped(a.skip, vidx / NR)) return;
   const int tid = threadIdx.x;
   const int al = tid % A, qq = tid / A;  // qq in [0, n^2)
   const long long a0 = (long long)blockIdx.x * A + al;
-
-  // eps for the dim-6 pencil (k4 + n*k5 = qq)
-  T e[n];
-  if constexpr (MODE == 0) {
+  cpx<T>* v = a.t + vidx * a.ldt + a0;
+  {  // dim 4 inverse: (k5, k6) = qq
+    cpx<T> w[n];
 #pragma unroll
-    for (int k = 0; k < n; ++k) e[k] = a.eps[a0 + (long long)S * (qq + P2 * k)];
+    for (int k = 0; k < n; ++k) w[k] = v[(long long)S * (k + n * qq)];
+    dft<T, n, +1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[al + A * (k + n * qq)] = w[k];
   }
-
-#pragma unroll 1
-  for (int r = 0; r < NR; ++r) {
-    cpx<T>* v = a.t + (g * NR + r) * a.ldt + a0;
-    {  // dim 4 inverse: (k5, k6) = qq
-      cpx<T> w[n];
+  __syncthreads();
+  const int b5 = al + A * ((qq % n) + P2 * (qq / n));  // (k4, k6) = qq
+  {
+    cpx<T> w[n];
 #pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = v[(long long)S * (k + n * qq)];
-      dft<T, n, +1>(w);
+    for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
+    dft<T, n, +1>(w);
 #pragma unroll
-      for (int k = 0; k < n; ++k) L[al + A * (k + n * qq)] = w[k];
-    }
-    __syncthreads();
-    const int b5 = al + A * ((qq % n) + P2 * (qq / n));  // (k4, k6) = qq
-    {
-      cpx<T> w[n];
+    for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
+  }
+  __syncthreads();
+  {  // dim 6: inverse, eps, forward
+    const int b6 = al + A * qq;
+    cpx<T> w[n];
 #pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
-      dft<T, n, +1>(w);
+    for (int k = 0; k < n; ++k) w[k] = L[b6 + A * P2 * k];
+    dft<T, n, +1>(w);
+    if constexpr (MODE == 1) {
 #pragma unroll
-      for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
-    }
-    __syncthreads();
-    {  // dim 6: inverse, eps, forward
-      const int b6 = al + A * qq;
-      cpx<T> w[n];
+      for (int k = 0; k < n; ++k) v[(long long)S * (qq + P2 * k)] = w[k];
+      return;
+    } else {
 #pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = L[b6 + A * P2 * k];
-      dft<T, n, +1>(w);
-      if constexpr (MODE == 1) {
-#pragma unroll
-        for (int k = 0; k < n; ++k) v[(long long)S * (qq + P2 * k)] = w[k];
-        __syncthreads();
-        continue;
-      } else {
-#pragma unroll
-        for (int k = 0; k < n; ++k) w[k] = e[k] * w[k];
-        dft<T, n, -1>(w);
-#pragma unroll
-        for (int k = 0; k < n; ++k) L[b6 + A * P2 * k] = w[k];
-      }
-    }
-    __syncthreads();
-    {
-      cpx<T> w[n];
-#pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
+      for (int k = 0; k < n; ++k) w[k] = a.eps[a0 + (long long)S * (qq + P2 * k)] * w[k];
       dft<T, n, -1>(w);
 #pragma unroll
-      for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
+      for (int k = 0; k < n; ++k) L[b6 + A * P2 * k] = w[k];
     }
-    __syncthreads();
-    {  // dim 4 forward, store
-      cpx<T> w[n];
+  }
+  __syncthreads();
+  {
+    cpx<T> w[n];
 #pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = L[al + A * (k + n * qq)];
-      dft<T, n, -1>(w);
+    for (int k = 0; k < n; ++k) w[k] = L[b5 + A * n * k];
+    dft<T, n, -1>(w);
 #pragma unroll
-      for (int k = 0; k < n; ++k) v[(long long)S * (k + n * qq)] = w[k];
-    }
-    // the next r first writes exactly the LDS slots this thread just read
+    for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
+  }
+  __syncthreads();
+  {  // dim 4 forward, store
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = L[al + A * (k + n * qq)];
+    dft<T, n, -1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) v[(long long)S * (k + n * qq)] = w[k];
   }
 }
 
@@ -530,6 +519,148 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_c_lin(PassCArgs<T> a) {
       const cpx<T> d = dg[THREADS * i];
       s += static_cast<double>(d.re * acc[i].re + d.im * acc[i].im);
     }
+    s = block_sum<THREADS>(s, red);
+    if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------- Mhat passes, split
+// Pass A and pass C of Mhat with the three scalar applies (z, W1 z, W2 z) carried by
+// three thread groups of one workgroup (THREADS = 3*BPW*n^2): each group owns an LDS
+// region and one output, so a 16 MB grid vector yields three times the waves in
+// flight of a one-output workgroup.  All HBM accesses are coalesced (dim-3 register
+// pencils, linear LDS-staged stores).
+template <class T, int n, int BPW, int OP>
+__global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int TG = BPW * P2;  // threads per group
+  __shared__ cpx<T> lds[3 * BPW * S];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int o = tid / TG, tl = tid - o * TG;
+  const int blk = tl / P2, pp = tl - blk * P2;
+  const long long wg0 = (long long)blockIdx.x * BPW * S;
+  const long long off = wg0 + (long long)blk * S;
+  cpx<T>* R = lds + o * (BPW * S);
+  cpx<T>* L = R + blk * S;
+
+  const cpx<T>* xg = a.x + g * a.ldx + off + pp;
+  cpx<T> pn[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) pn[k] = xg[P2 * k];
+  if constexpr (OP == PRE_MHAT_CG) {
+    cpx<T>* pg = a.p + g * a.ldx + off + pp;
+    if (a.st[g].iter > 0) {
+      const T beta = static_cast<T>(a.st[g].beta);
+#pragma unroll
+      for (int k = 0; k < n; ++k) pn[k] = pn[k] + beta * pg[P2 * k];
+    }
+    __syncthreads();  // every group has read the old p before group 0 overwrites it
+    if (o == 0) {
+#pragma unroll
+      for (int k = 0; k < n; ++k) pg[P2 * k] = pn[k];
+    }
+  }
+  const long long qoff = qslot(a.qmap, g) * a.ldq + off + pp;
+  const T* wo = (o == 1) ? a.W1 : a.W2;
+  cpx<T> u[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    T w = a.invC[qoff + P2 * k];
+    if (o > 0) w *= wo[qoff + P2 * k];
+    u[k] = w * pn[k];
+  }
+  dft<T, n, +1>(u);
+#pragma unroll
+  for (int k = 0; k < n; ++k) L[pp + P2 * k] = u[k];
+  __syncthreads();
+  {
+    const int base2 = (pp % n) + (pp / n) * P2;
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+    dft<T, n, +1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+  }
+  __syncthreads();
+  {
+    const int base1 = (pp % n) * n + (pp / n) * P2;
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+    dft<T, n, +1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[base1 + k] = w[k];
+  }
+  __syncthreads();
+  cpx<T>* tg = a.t + (g * 3 + o) * a.ldt + wg0;
+#pragma unroll
+  for (int i = 0; i < n; ++i) tg[tl + TG * i] = R[tl + TG * i];
+}
+
+template <class T, int n, int BPW>
+__global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int TG = BPW * P2;
+  constexpr int THREADS = 3 * TG;
+  constexpr int E = BPW * S;  // modes per workgroup
+  __shared__ cpx<T> lds[3 * E];
+  __shared__ double red[THREADS];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int o = tid / TG, tl = tid - o * TG;
+  const int blk = tl / P2, pp = tl - blk * P2;
+  const long long wg0 = (long long)blockIdx.x * E;
+  const long long off = wg0 + (long long)blk * S;
+  cpx<T>* L = lds + o * E + blk * S;
+  {
+    const cpx<T>* tg = a.t + (g * 3 + o) * a.ldt + off + pp;
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = tg[P2 * k];
+    dft<T, n, -1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[pp + P2 * k] = w[k];
+  }
+  __syncthreads();
+  {
+    const int base2 = (pp % n) + (pp / n) * P2;
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+    dft<T, n, -1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+  }
+  __syncthreads();
+  {
+    const int base1 = (pp % n) * n + (pp / n) * P2;
+    cpx<T> w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+    dft<T, n, -1>(w);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[base1 + k] = w[k];
+  }
+  __syncthreads();
+  // combine y = invC*(s0 + W1 s1 + W2 s2) in linear mode order
+  const long long qbase = qslot(a.qmap, g) * a.ldq + wg0;
+  cpx<T>* yg = a.y + g * a.ldy + wg0;
+  const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
+  double s = 0.0;
+  for (int e = tid; e < E; e += THREADS) {
+    const long long q = qbase + e;
+    const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+    yg[e] = v;
+    if (dg) {
+      const cpx<T> d = dg[e];
+      s += static_cast<double>(d.re * v.re + d.im * v.im);
+    }
+  }
+  if (a.partials != nullptr) {
     s = block_sum<THREADS>(s, red);
     if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;
   }

@@ -165,7 +165,7 @@ static const T* Wt(const fame6d_qsys* S, int i) {
 
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
-                hipStream_t s) {
+                hipStream_t s, bool inverse_eps, const cpx<T>* dotp, double* partials) {
   fame6d_mass* m = S->m;
   const long long NF = m->NF;
   DevBuf& w = work_buf<T>(m);
@@ -183,7 +183,7 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   a.skip = skip;
   a.qmap = qmap;
   launch_pass_a<T>(m, PRE_MHAT, nact, a, s);
-  launch_pass_b<T>(m, 3, 0, nact, t, NF, skip, s);
+  launch_pass_b<T>(m, 3, 0, nact, t, NF, skip, s, inverse_eps);
   PassCArgs<T> c{};
   c.t = t;
   c.ldt = NF;
@@ -193,6 +193,9 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   c.W2 = a.W2;
   c.invC = a.invC;
   c.ldq = NF;
+  c.dotp = dotp;
+  c.lddot = NF;
+  c.partials = partials;
   c.skip = skip;
   c.qmap = qmap;
   launch_pass_c<T>(m, POST_MHAT, nact, c, s);
@@ -223,18 +226,20 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 }
 
 // ------------------------------------------------------------------ PCG on Mhat
-// MATLAB pcg (no preconditioner, x0 = 0) for nact independent right-hand sides,
-// each kept in lock-step but with its own scalars; finished solves are skipped
-// on the device.  Semantics restated: convergence when the recurrence residual
-// meets tol*||b||, confirmed by the true residual b - Mhat x; a failed check
-// continues from the true residual and checks every later iteration ("moresteps",
-// at most 5); flag 1 at maxit, 4 on breakdown (p'q <= 0 or beta 0/Inf).
-// Not restated: the stagnation test ||p||*|alpha| < eps*||x|| and the
-// minimum-residual iterate returned on failure.
+// MATLAB pcg (x0 = 0) for nact independent right-hand sides, each kept in lock-step
+// but with its own scalars; finished solves are skipped on the device.  Semantics
+// restated: convergence when the recurrence residual meets tol*||b||, confirmed by
+// the true residual b - Mhat x; a failed check continues from the true residual and
+// checks every later iteration ("moresteps", at most 5); flag 1 at maxit, 4 on
+// breakdown (p'q <= 0 or beta 0/Inf).  Not restated: the stagnation test
+// ||p||*|alpha| < eps*||x|| and the minimum-residual iterate returned on failure.
+// With `precond` the iteration is preconditioned by P^{-1} = Mhat evaluated with
+// 1/eps (exact inverse for constant eps); the reference's solves are unpreconditioned,
+// and this build uses `precond` only for the FP32 inner solves of the mixed form.
 template <class T>
 static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
-                     cpx<T>* r, cpx<T>* p, cpx<T>* q, double tol, int maxit,
-                     std::vector<PcgOut>& out, long long* applies) {
+                     cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
+                     std::vector<PcgOut>& out) {
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const long long NF = m->NF;
@@ -244,7 +249,7 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const int nbD = red_blocks(NF);
   S->st.ensure(sizeof(CgState) * nact);
   S->part.ensure(sizeof(double) * (size_t)std::max(nbC, nbD) * nact * 2);
-  S->part2.ensure(sizeof(double) * (size_t)nbU * nact);
+  S->part2.ensure(sizeof(double) * (size_t)std::max(nbU, nbD) * nact + 64);
   S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
   CgState* st = S->st.as<CgState>();
   double* part = S->part.as<double>();
@@ -253,11 +258,17 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   DevBuf& w = work_buf<T>(m);
   w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
   cpx<T>* t = w.as<cpx<T>>();
+  const cpx<T>* zr = precond ? z : r;  // the vector the search direction is built from
 
   k_fill_zero<T><<<grid_for(total), 256, 0, s>>>(total, x);
   F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<T>) * total, hipMemcpyDeviceToDevice, s));
-  k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, NF, b, NF, NF, part, nullptr);
-  k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part, nbD, tol, maxit);
+  k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, NF, b, NF, NF, part2, nullptr);
+  if (precond) {
+    mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, r, part);
+    k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbC, tol, maxit);
+  } else {
+    k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, tol, maxit);
+  }
   F6_CHECK_LAUNCH();
 
   out.assign(nact, PcgOut{});
@@ -266,7 +277,7 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
 
   PassAArgs<T> pa{};
-  pa.x = r;
+  pa.x = zr;
   pa.ldx = NF;
   pa.t = t;
   pa.ldt = NF;
@@ -324,7 +335,12 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       launch_pass_c<T>(m, POST_MHAT, nact, pc, s);
       k_cg_alpha<<<nact, kRedThreads, 0, s>>>(st, part, nbC);
       k_cg_update<T><<<dim3(nbU, nact), kRedThreads, 0, s>>>(x, p, r, q, NF, NF, st, part2);
-      k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU);
+      if (precond) {
+        mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, r, part);
+        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU, part, nbC);
+      } else {
+        k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU);
+      }
       F6_CHECK_LAUNCH();
     }
     issued += chunk;
@@ -343,54 +359,61 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
         out[g].relres = hs[g].normr / hs[g].n2b;
       }
     }
-    if (any_need) {
-      // true residual r = b - Mhat x for the groups that stopped on the recurrence
-      std::vector<int> skiph(nact);
-      for (int g = 0; g < nact; ++g) skiph[g] = need[g] ? 0 : 1;
-      F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
-      Skip sk{mask, 1};
-      mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
-      k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
-      k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part, nullptr);
-      F6_CHECK_LAUNCH();
-      std::vector<double> rr(nact);
-      {
-        double* outd = part2;  // reuse: nact doubles
-        k_reduce_final<<<nact, kRedThreads, 0, s>>>(part, nbD, outd);
-        F6_HIP(hipMemcpyAsync(rr.data(), outd, sizeof(double) * nact, hipMemcpyDeviceToHost, s));
-        F6_HIP(hipStreamSynchronize(s));
-      }
-      std::vector<int> resume(nact, 0);
-      bool any_resume = false;
-      for (int g = 0; g < nact; ++g) {
-        if (!need[g]) continue;
-        ++out[g].checks;
-        if (applies) *applies += 3;
-        const double nact_r = std::sqrt(rr[g]);
-        if (nact_r <= hs[g].tolb) {
+    if (!any_need) continue;
+    // true residual r = b - Mhat x for the groups that stopped on the recurrence
+    std::vector<int> skiph(nact);
+    for (int g = 0; g < nact; ++g) skiph[g] = need[g] ? 0 : 1;
+    F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+    Skip sk{mask, 1};
+    mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
+    k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
+    k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr);
+    F6_CHECK_LAUNCH();
+    std::vector<double> rr(nact);
+    {
+      double* outd = part2 + (size_t)nbD * nact;
+      k_reduce_final<<<nact, kRedThreads, 0, s>>>(part2, nbD, outd);
+      F6_HIP(hipMemcpyAsync(rr.data(), outd, sizeof(double) * nact, hipMemcpyDeviceToHost, s));
+      F6_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<int> resume(nact, 0);
+    bool any_resume = false;
+    for (int g = 0; g < nact; ++g) {
+      if (!need[g]) continue;
+      ++out[g].checks;
+      const double nact_r = std::sqrt(rr[g]);
+      if (nact_r <= hs[g].tolb) {
+        finished[g] = 1;
+        out[g].flag = 0;
+        out[g].iter = hs[g].iter;
+        out[g].relres = nact_r / hs[g].n2b;
+      } else {
+        moresteps[g] += 1;
+        if (moresteps[g] >= maxmsteps || hs[g].iter >= maxit) {
           finished[g] = 1;
-          out[g].flag = 0;
+          out[g].flag = (moresteps[g] >= maxmsteps) ? 3 : 1;
           out[g].iter = hs[g].iter;
           out[g].relres = nact_r / hs[g].n2b;
         } else {
-          moresteps[g] += 1;
-          if (moresteps[g] >= maxmsteps || hs[g].iter >= maxit) {
-            finished[g] = 1;
-            out[g].flag = (moresteps[g] >= maxmsteps) ? 3 : 1;
-            out[g].iter = hs[g].iter;
-            out[g].relres = nact_r / hs[g].n2b;
-          } else {
-            resume[g] = 1;
-            any_resume = true;
-          }
+          resume[g] = 1;
+          any_resume = true;
         }
       }
-      if (any_resume) {
-        F6_HIP(hipMemcpyAsync(mask + nact, resume.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
-        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbD, mask + nact);
-        F6_CHECK_LAUNCH();
-        read_state();
+    }
+    if (any_resume) {
+      F6_HIP(hipMemcpyAsync(mask + nact, resume.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+      if (precond) {
+        // z = P^{-1} r_true for the resumed groups; rho = r'z
+        std::vector<int> sk2(nact);
+        for (int g = 0; g < nact; ++g) sk2[g] = resume[g] ? 0 : 1;
+        F6_HIP(hipMemcpyAsync(mask, sk2.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+        mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{mask, 1}, s, true, r, part);
+        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbC, mask + nact);
+      } else {
+        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, mask + nact);
       }
+      F6_CHECK_LAUNCH();
+      read_state();
     }
   }
   int min_seen = 1 << 30;
@@ -442,7 +465,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
 
   std::vector<PcgOut> res;
   if (precision == FAME6D_FP64) {
-    pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, tol, maxit, res, nullptr);
+    pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, res);
     for (int g = 0; g < nact; ++g) {
       S->stats.total_iterations += res[g].iter;
       S->stats.total_fp64_iterations += res[g].iter;
@@ -450,12 +473,13 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     }
   } else if (precision == FAME6D_MIXED) {
     // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.
-    S->kr32.ensure(sizeof(cpx<float>) * gn * 5);
+    S->kr32.ensure(sizeof(cpx<float>) * gn * 6);
     cpx<float>* b32 = S->kr32.as<cpx<float>>();
     cpx<float>* x32 = b32 + gn;
     cpx<float>* r32 = b32 + 2 * gn;
     cpx<float>* p32 = b32 + 3 * gn;
     cpx<float>* q32 = b32 + 4 * gn;
+    cpx<float>* z32 = b32 + 5 * gn;
     k_fill_zero<double><<<grid_for(gn), 256, 0, s>>>((long long)gn, w.u);
     F6_HIP(hipMemcpyAsync(w.res, w.bhat, sizeof(cpx<double>) * gn, hipMemcpyDeviceToDevice, s));
     std::vector<double> nb(nact), nr(nact);
@@ -494,13 +518,14 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       int* qmap_a = qmap + 128;
       F6_HIP(hipMemcpyAsync(qmap_a, slots.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
       std::vector<PcgOut> inner;
-      pcg_mhat<float>(S, na, qmap_a, b32, x32, r32, p32, q32, tol_in, maxit, inner, nullptr);
+      pcg_mhat<float>(S, na, qmap_a, b32, x32, r32, p32, q32, z32, S->mixed_precond, tol_in, maxit, inner);
       for (int i = 0; i < na; ++i) {
         const int g = act[i];
         if (inner[i].flag != 0 && inner[i].flag != 1 && inner[i].flag != 3)
           throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(inner[i].flag) + ")");
         S->stats.total_iterations += inner[i].iter;
-        S->stats.mass_applies_fp32 += 3LL * (inner[i].iter + inner[i].checks);
+        S->stats.mass_applies_fp32 +=
+            3LL * ((S->mixed_precond ? 2 : 1) * inner[i].iter + inner[i].checks + (S->mixed_precond ? 1 : 0));
         res[g].iter += inner[i].iter;
         k_accumulate<float><<<gridm, 256, 0, s>>>(NF, x32 + (size_t)i * NF, nr[g], w.u + (size_t)g * NF);
       }
@@ -560,9 +585,9 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
 }
 
 template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
-                                 hipStream_t);
+                                 hipStream_t, bool, const cpx<double>*, double*);
 template void mhat_apply<float>(fame6d_qsys*, int, const int*, const cpx<float>*, cpx<float>*, Skip,
-                                hipStream_t);
+                                hipStream_t, bool, const cpx<float>*, double*);
 template void dot_groups<double>(const fame6d_mass*, int, const cpx<double>*, long long,
                                  const cpx<double>*, long long, long long, double*, double*);
 template void dot_groups<float>(const fame6d_mass*, int, const cpx<float>*, long long,

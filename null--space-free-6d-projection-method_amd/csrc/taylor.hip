@@ -23,7 +23,8 @@ __host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * 
 // (re/im interleaved per center) on the FP64 matrix cores.
 //
 // v_mfma_f64_16x16x4f64 (wave64): lane l supplies A[m = l%16][k = l/16] and
-// B[k = l/16][n = l%16]; the 4 accumulator doubles of lane l are D[4*(l/16) + r][l%16].
+// B[k = l/16][n = l%16]; accumulator double r of lane l is D[(l/16) + 4*r][l%16]
+// (row-interleaved; measured on gfx950 with a one-wave probe).
 // Workgroup = 4 waves, one 16-column tile (8 centers) and one slice of the mode
 // range; the operands are generated in LDS per chunk of kTChunk modes: powers
 // q_x^s, q_y^s, q_z^s (A = products of three of them) and B = exp(i q.r_c) coeff.
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_taylor_moments_mfma(long long NF, const
     if (tile < T) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int alpha = tile * 16 + 4 * (lane >> 4) + r;
+        const int alpha = tile * 16 + (lane >> 4) + 4 * r;
         const int col = ct * 16 + (lane & 15);
         part[((long long)blockIdx.y * Lpad + alpha) * Cpad2 + col] = acc[t][r];
       }

@@ -59,8 +59,8 @@ class QSystem:
         call("fame6d_qsys_diagonals", self._h, *[a.ctypes.data_as(C.c_void_p) for a in arrs])
         return dict(zip(names, arrs))
 
-    def set_mixed(self, inner_tol=3e-5, max_cycles=30):
-        call("fame6d_qsys_set_mixed", self._h, float(inner_tol), int(max_cycles))
+    def set_mixed(self, inner_tol=3e-5, max_cycles=30, precondition=True):
+        call("fame6d_qsys_set_mixed", self._h, float(inner_tol), int(max_cycles), int(bool(precondition)))
 
     def _dev(self, x, rows, dtype=torch.complex128):
         if not isinstance(x, torch.Tensor):

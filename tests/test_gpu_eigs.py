@@ -35,15 +35,20 @@ def test_qsystem_diagonals(n2):
     assert np.allclose(n2["s"].info["kLift"], o["klift"], atol=1e-16)
 
 
-def test_mhat(n2, oracle):
-    print("case: Mhat N=2", flush=True)
+@pytest.mark.parametrize("v3", [0, 1])
+def test_mhat(n2, oracle, v3):
+    print("case: Mhat N=2 variant %d" % v3, flush=True)
     rng = np.random.default_rng(1)
     x = rand_c(rng, n2["m"].NF)
     ref = oracle.mhat_apply(n2["osys"], n2["eps"], x, n2["N"])
+    n2["m"].set_tiles("double", 4 * v3, 0)
+    n2["m"].set_tiles("single", 4 * v3, 1)
     y = n2["s"].Mhat(torch.as_tensor(x, device="cuda")).cpu().numpy()
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
     y32 = n2["s"].Mhat(torch.as_tensor(x, device="cuda").to(torch.complex64), "single").cpu().numpy()
     assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 5e-6
+    n2["m"].set_tiles("double", 0, 0)
+    n2["m"].set_tiles("single", 0, 0)
 
 
 def test_krinv_fp64_and_mixed(n2, oracle):
@@ -55,10 +60,13 @@ def test_krinv_fp64_and_mixed(n2, oracle):
     y = y.cpu().numpy()
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-9
     assert st["last_flag"] == 0 and abs(st["last_iterations"] - st_ref["iter"]) <= 2
-    ym, stm = n2["s"].krinv(torch.as_tensor(q, device="cuda"), 1e-10, 3000, "mixed")
-    ym = ym.cpu().numpy()
-    assert np.linalg.norm(ym - ref) / np.linalg.norm(ref) < 1e-9
-    assert stm["refinement_cycles"] >= 1
+    for precond in (True, False):
+        n2["s"].set_mixed(3e-5, 30, precond)
+        ym, stm = n2["s"].krinv(torch.as_tensor(q, device="cuda"), 1e-10, 3000, "mixed")
+        ym = ym.cpu().numpy()
+        assert np.linalg.norm(ym - ref) / np.linalg.norm(ref) < 1e-9, precond
+        assert stm["refinement_cycles"] >= 1
+    n2["s"].set_mixed(3e-5, 30, True)
 
 
 def test_eigs_golden_n2(pkg, n2):
