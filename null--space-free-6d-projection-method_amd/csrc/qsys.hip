@@ -249,7 +249,8 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const int nbD = red_blocks(NF);
   S->st.ensure(sizeof(CgState) * nact);
   S->part.ensure(sizeof(double) * (size_t)std::max(nbC, nbD) * nact * 2);
-  S->part2.ensure(sizeof(double) * (size_t)std::max(nbU, nbD) * nact + 64);
+  // per-group partials, then one reduced value per group behind them
+  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbU, nbD) * nact + (size_t)nact + 8));
   S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
   CgState* st = S->st.as<CgState>();
   double* part = S->part.as<double>();

@@ -144,3 +144,18 @@ def test_method_b_nested(pkg, oracle, n2):
     lb = pkg.run_method_B(n2["s"], k=4, p=24, tol=1e-10, max_restart=40, eps_c=4.0, v0=v0)["lambda_"]
     lc = pkg.run_method_C(n2["s"], k=4, p=24, tol=1e-10, max_restart=40, v0=v0)["lambda_"]
     assert np.max(np.abs(lb - lc) / lc) < 1e-7
+
+
+def test_batched_twelve_point_path(pkg, n2):
+    """12 Bloch points on t*(0.5,0.4,0.3), t = k/12 (q = 0 excluded): one batched solve equals
+    the per-point solves.  Also exercises per-group buffers sized by the point count."""
+    print("case: batched 12-point path N=2", flush=True)
+    qs = np.array([[0.5 * t, 0.4 * t, 0.3 * t] for t in np.arange(1, 13) / 12.0])
+    nr = n2["s"].nr
+    v0 = np.stack([pkg.start_vector(nr, 40 + i) for i in range(12)])
+    outb = pkg.run_method_C_batch(n2["m"], qs, k=4, p=20, tol=1e-10, max_restart=60, v0=v0)
+    for i in (0, 5, 11):
+        s = pkg.QSystem(n2["m"], qs[i])
+        out = pkg.run_method_C(s, k=4, p=20, tol=1e-10, max_restart=60, v0=v0[i])
+        assert np.max(np.abs(outb["lambda_"][i] - out["lambda_"]) / out["lambda_"]) < 1e-11
+        s.close()
