@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter collection: a 64 MB device copy (calibration), "
+                         "then plain and Mhat applies in FP64 and FP32; no solves")
     ap.add_argument("--extras", action="store_true",
                     help="also time the batched 12-point Bloch path at N=4 and the Taylor reconstruction")
     return ap.parse_args()
@@ -231,6 +234,26 @@ def mixed_comparison(torch, pkg, N, amp, reps):
                 outer_steps=o64["stats"]["outer_steps"])
 
 
+def probe(torch, pkg, N, reps=5):
+    """Fixed kernel sequence for counter runs: a copy of known size, then the applies."""
+    eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
+    mass = pkg.MassBackend(N, eps)
+    mass.autotune(reps=3)
+    sysq = pkg.QSystem(mass, Q)
+    src = torch.ones(64 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    for dt, name in ((torch.complex128, "double"), (torch.complex64, "single")):
+        x = torch.randn(mass.NF, dtype=dt, device="cuda")
+        for _ in range(reps):
+            mass.MscalarOp(x)
+        for _ in range(reps):
+            sysq.Mhat(x, name)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "N": N, "copy_bytes": 64 * 2 ** 20, "tiles": mass.info["tiles"]}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -246,6 +269,9 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     N = a.N
+    if a.probe:
+        probe(torch, pkg, N)
+        return
     NF = (2 * N) ** 6
     eps, phases = pkg.cosine_material(N, AMP, EPS_SEED)
     mass = pkg.MassBackend(N, eps)

@@ -456,86 +456,26 @@ static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long lon
   }
 }
 
-// ------------------------------------------------------------------ fused CGS2 passes
-// One sweep over the basis Q (n rows x jj columns, column-major, ld = n) in tiles of 64
-// rows staged in LDS.  MODE 0: h = Q^H w.  MODE 1: r = w - Q hin, then h = Q^H r.
-// MODE 2: r = w - Q hin, then ||r||^2.  Per-workgroup partials (column k of workgroup b
-// at pre[k*gridDim.x + b]) are folded by k_reduce_final; thread t owns column t/4 of the
-// dot products across all of its workgroup's tiles, so no per-tile reduction is needed.
-constexpr int kCgsRows = 64;
-constexpr int kCgsMaxCols = 64;
-template <int MODE>
-__global__ __launch_bounds__(256) void k_cgs_sweep(const cpx<double>* Q, long long n, int jj,
-                                                   const cpx<double>* w, const double* hin_re,
-                                                   const double* hin_im, cpx<double>* r, double* pre,
-                                                   double* pim) {
-  __shared__ cpx<double> Qs[kCgsMaxCols * kCgsRows];
-  __shared__ cpx<double> rs[kCgsRows];
-  __shared__ cpx<double> hs[kCgsMaxCols];
-  __shared__ cpx<double> part[256];
-  const int tid = threadIdx.x;
-  if (MODE != 0) {
-    for (int k = tid; k < jj; k += 256) hs[k] = mk(hin_re[k], hin_im[k]);
+// out = in - Q(:, 0:j) h (h complex, on device); optional ||out||^2 partials per workgroup.
+static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
+                                                           const double* hre, const double* him,
+                                                           const cpx<double>* in, cpx<double>* out,
+                                                           long long n, double* norm_partials) {
+  __shared__ cpx<double> hs[128];
+  __shared__ double red[256];
+  for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);
+  __syncthreads();
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    cpx<double> acc = in[i];
+    for (int k = 0; k < j; ++k) acc = acc - Q[(long long)k * ld + i] * hs[k];
+    out[i] = acc;
+    s += acc.re * acc.re + acc.im * acc.im;
   }
-  const int kc = tid >> 2, quarter = tid & 3;  // dot ownership: column kc, rows quarter*16 .. +16
-  double ar = 0.0, ai = 0.0;
-  const long long ntiles = (n + kCgsRows - 1) / kCgsRows;
-  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long long r0 = tile * kCgsRows;
-    __syncthreads();  // previous tile consumed (and hs staged)
-    for (int e = tid; e < jj * kCgsRows; e += 256) {
-      const int k = e / kCgsRows, row = e - k * kCgsRows;
-      const long long i = r0 + row;
-      Qs[e] = (i < n) ? Q[(long long)k * n + i] : mk(0.0, 0.0);
-    }
-    if (tid < kCgsRows) {
-      const long long i = r0 + tid;
-      rs[tid] = (i < n) ? w[i] : mk(0.0, 0.0);
-    }
-    __syncthreads();
-    if (MODE != 0) {
-      // r(row) = w(row) - sum_k Q(row,k) h(k): four threads per row, each a quarter of k
-      const int row = tid >> 2, q4 = tid & 3;
-      cpx<double> sacc = mk(0.0, 0.0);
-      for (int k = q4; k < jj; k += 4) sacc = sacc + Qs[k * kCgsRows + row] * hs[k];
-      part[tid] = sacc;
-      __syncthreads();
-      if (tid < kCgsRows) {
-        const cpx<double> tot = part[4 * tid] + part[4 * tid + 1] + part[4 * tid + 2] + part[4 * tid + 3];
-        const cpx<double> rv = rs[tid] - tot;
-        rs[tid] = rv;
-        const long long i = r0 + tid;
-        if (i < n) r[i] = rv;
-      }
-      __syncthreads();
-    }
-    if (MODE == 2) {
-      if (tid < kCgsRows) {
-        const cpx<double> v = rs[tid];
-        ar += v.re * v.re + v.im * v.im;
-      }
-    } else if (kc < jj) {
-      for (int row = quarter * 16; row < quarter * 16 + 16; ++row) {
-        const cpx<double> q = Qs[kc * kCgsRows + row], v = rs[row];
-        ar += q.re * v.re + q.im * v.im;
-        ai += q.re * v.im - q.im * v.re;
-      }
-    }
-  }
-  if (MODE == 2) {
-    __shared__ double red[256];
-    const double sres = block_sum<256>(ar, red);
-    if (tid == 0) pre[blockIdx.x] = sres;
-  } else {
-    // fold the four quarters of each column (lanes 4k..4k+3 are in one wave)
-    ar += __shfl_down(ar, 2, 64);
-    ai += __shfl_down(ai, 2, 64);
-    ar += __shfl_down(ar, 1, 64);
-    ai += __shfl_down(ai, 1, 64);
-    if (quarter == 0 && kc < jj) {
-      pre[(long long)kc * gridDim.x + blockIdx.x] = ar;
-      pim[(long long)kc * gridDim.x + blockIdx.x] = ai;
-    }
+  if (norm_partials != nullptr) {
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) norm_partials[blockIdx.x] = s;
   }
 }
 

@@ -7,8 +7,8 @@
 // |beta * U(end,:)|, the convergence threshold tol*max(eps^(2/3), |theta|) and
 // the restart size k0 + min(nconv, floor((p-k0)/2)).  The basis stays on the
 // device.  The orthogonalisation of each new vector w = K_r^{-1} q_j is two full
-// classical Gram-Schmidt projections r = (I - QQ^H)(I - QQ^H) w done in three fused
-// sweeps over the basis; the reference's three-term recurrence, its two extra
+// classical Gram-Schmidt projections r = (I - QQ^H)(I - QQ^H) w (four streaming
+// sweeps over the basis); the reference's three-term recurrence, its two extra
 // single-vector projections and its two column-by-column passes compute the same
 // projection in exact arithmetic (alpha = Re(q_j^H w), beta = ||r||), so results
 // agree up to rounding.  Several Bloch points step together and share one batched
@@ -128,22 +128,32 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
   F6_CHECK_LAUNCH();
 }
 
-// One Lanczos orthogonalisation step in three sweeps over the basis.  h1 is kept in
-// sc[kScH..] (its entry jj-1 is alpha = q_j^H w), h2 in sc[kScH2..], ||r||^2 in sc[1].
+// One Lanczos orthogonalisation step: h1 = Q^H w (its entry jj-1 is alpha = q_j^H w),
+// r = w - Q h1, h2 = Q^H r, r -= Q h2 with ||r||^2 fused into the last update.
+// h1 in sc[kScH..], h2 in sc[kScH2..], ||r||^2 in sc[1].
 constexpr int kScH2 = 8 + 256, kScH2im = 8 + 384;
-constexpr int kCgsBlocks = 1024;
-static void cgs2_step(long long n, const Basis& B, int jj, const cpx<double>* w, hipStream_t s) {
-  const int nb = kCgsBlocks;
+constexpr int kGemvKB = 8;
+static void gemv_c(long long n, const Basis& B, int jj, const cpx<double>* v, double* dre, double* dim,
+                   hipStream_t s) {
+  const int nb = red_blocks(n);
   double* pre = B.part;
-  double* pim = B.part + (size_t)nb * kCgsMaxCols;
-  k_cgs_sweep<0><<<nb, 256, 0, s>>>(B.Q, n, jj, w, nullptr, nullptr, nullptr, pre, pim);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, B.sc + kScHim);
-  k_cgs_sweep<1><<<nb, 256, 0, s>>>(B.Q, n, jj, w, B.sc + kScH, B.sc + kScHim, B.r, pre, pim);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, B.sc + kScH2);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, B.sc + kScH2im);
-  k_cgs_sweep<2><<<nb, 256, 0, s>>>(B.Q, n, jj, B.r, B.sc + kScH2, B.sc + kScH2im, B.r, pre, nullptr);
-  k_reduce_final<<<1, kRedThreads, 0, s>>>(pre, nb, B.sc + 1);
+  double* pim = B.part + (size_t)nb * 128;
+  for (int k0 = 0; k0 < jj; k0 += kGemvKB) {
+    const int kc = std::min(kGemvKB, jj - k0);
+    k_gemv_c_partials<kGemvKB><<<nb, kRedThreads, 0, s>>>(B.Q, n, k0, kc, v, n, pre, pim);
+  }
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, dre);
+  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, dim);
+  F6_CHECK_LAUNCH();
+}
+
+static void cgs2_step(long long n, const Basis& B, int jj, const cpx<double>* w, hipStream_t s) {
+  const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
+  gemv_c(n, B, jj, w, B.sc + kScH, B.sc + kScHim, s);
+  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n, nullptr);
+  gemv_c(n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s);
+  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n, B.part);
+  k_reduce_final<<<1, kRedThreads, 0, s>>>(B.part, gn, B.sc + 1);
   F6_CHECK_LAUNCH();
 }
 
@@ -281,7 +291,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   DevBuf basis, scal, part, io, ubuf;
   basis.ensure(sizeof(cpx<double>) * (size_t)n2 * (p + 2) * G);
   scal.ensure(sizeof(double) * 512 * G);
-  part.ensure(sizeof(double) * (size_t)std::max(nb * 256, kCgsBlocks * kCgsMaxCols * 2) * G);
+  part.ensure(sizeof(double) * (size_t)std::max(nb * 256, 2048) * G);
   io.ensure(sizeof(cpx<double>) * (size_t)n2 * 2 * G);
   std::vector<LzState> L(G);
   for (int g = 0; g < G; ++g) {
@@ -291,7 +301,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     L[g].B.v = b0 + (size_t)n2 * p;
     L[g].B.r = b0 + (size_t)n2 * (p + 1);
     L[g].B.sc = scal.as<double>() + 512 * g;
-    L[g].B.part = part.as<double>() + (size_t)std::max(nb * 256, kCgsBlocks * kCgsMaxCols * 2) * g;
+    L[g].B.part = part.as<double>() + (size_t)std::max(nb * 256, 2048) * g;
     L[g].k = k0;
   }
   cpx<double>* qin = io.as<cpx<double>>();

@@ -111,6 +111,18 @@ struct PassCArgs {
   const int* qmap;
 };
 
+// XCD-aware block order: the dispatcher deals consecutive linear block ids to the 8 XCDs
+// round-robin, each XCD with its own L2.  This bijection of [0, total) gives the blocks
+// an XCD runs consecutive logical ids, so neighbouring tiles that share cache lines
+// meet in the same L2.
+__device__ __forceinline__ int xcd_swizzle(int lin, int total) {
+  constexpr int X = 8;
+  const int per = (total + X - 1) / X;  // ids per XCD (the first `tall` XCDs get per)
+  const int tall = (total % X == 0) ? X : total % X;
+  const int xcd = lin % X, local = lin / X;
+  return (xcd < tall) ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
 template <int n>
 struct Grid {
   static constexpr int P2 = n * n;
@@ -303,11 +315,13 @@ template <class T, int n, int A, int NR, int MODE>
 __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
   constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
   __shared__ cpx<T> L[A * S];  // l = al + A*(k4 + n*k5 + n^2*k6)
-  const int vidx = blockIdx.y;
+  const int nt = gridDim.x;
+  const int logical = xcd_swizzle(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
+  const int tile = logical % nt, vidx = logical / nt;
   if (skipped(a.skip, vidx / NR)) return;
   const int tid = threadIdx.x;
   const int al = tid % A, qq = tid / A;  // qq in [0, n^2)
-  const long long a0 = (long long)blockIdx.x * A + al;
+  const long long a0 = (long long)tile * A + al;
   cpx<T>* v = a.t + vidx * a.ldt + a0;
   {  // dim 4 inverse: (k5, k6) = qq
     cpx<T> w[n];
