@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--probe", action="store_true",
-                    help="short run for hardware-counter collection: a 64 MB device copy (calibration), "
-                         "then plain and Mhat applies in FP64 and FP32; no solves")
+                    help="short run for hardware-counter collection: a 64 MiB device copy (calibration), "
+                         "then one FP64 solve of the benchmark configuration")
     ap.add_argument("--extras", action="store_true",
                     help="also time the batched 12-point Bloch path at N=4 and the Taylor reconstruction")
     return ap.parse_args()
@@ -202,6 +202,45 @@ def extras(torch, pkg):
                 batched_vs_sequential_max_rel_diff=maxdiff, taylor=rows)
 
 
+# kernel family (library timer slot) -> kernel-name prefix in rocprofv3 output
+FAMILY_KERNEL = {
+    "mhat_pass_a": "k_pass_a3<double, %d",
+    "mhat_pass_b": "k_pass_b<double, %d",
+    "mhat_pass_c": "k_pass_c3<double, %d",
+    "cg_update": "k_cg_update<double>",
+    "lanczos_gemv_c": "k_gemv_c_partials",
+    "lanczos_gemv_n": "k_gemv_n_out",
+}
+FAMILY_DESC = {
+    "mhat_pass_a": "Mhat pass A in PCG (p = r + beta p; z, W1 z, W2 z; inverse DFT dims 1-3): "
+                   "NF*(6*16 + 3*8) bytes per launch",
+    "mhat_pass_b": "Mhat pass B (inverse DFT dims 4-6, *eps/NF, forward DFT dims 6-4, p'q partials; 3 vectors): "
+                   "NF*(6*16 + 8) bytes per launch",
+    "mhat_pass_c": "Mhat pass C (forward DFT dims 3-1, y = invC(s0 + W1 s1 + W2 s2)): NF*(4*16 + 3*8) bytes per launch",
+    "cg_update": "CG update x += alpha p, r -= alpha q, ||r||^2 partials: NF*6*16 bytes per launch",
+    "lanczos_gemv_c": "Lanczos CGS2 sweep h = Q(:,1:j)^H w: (j+1)*2NF*16 bytes per sweep",
+    "lanczos_gemv_n": "Lanczos CGS2 sweep r = w - Q(:,1:j) h (+||r||^2): (j+2)*2NF*16 bytes per sweep",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "method_c_N5_counters.md")
+
+
+def pmc_traffic(kernel_prefix):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by its 64 MiB copy calibration), or None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith("traffic-json:"):
+            tab = json.loads(line.split(":", 1)[1])
+            for name, row in tab["kernels"].items():
+                if name.startswith(kernel_prefix):
+                    return dict(bytes=row["fetch_kib"] * 1024 * tab["fetch_scale"] + row["write_kib"] * 1024 * tab["write_scale"],
+                                kernel=name, source=os.path.relpath(TRAFFIC_FILE, ROOT))
+    return None
+
+
 def solve_timed(torch, pkg, sysq, v0, precision, reps=1):
     best = None
     out = None
@@ -234,24 +273,23 @@ def mixed_comparison(torch, pkg, N, amp, reps):
                 outer_steps=o64["stats"]["outer_steps"])
 
 
-def probe(torch, pkg, N, reps=5):
-    """Fixed kernel sequence for counter runs: a copy of known size, then the applies."""
+def probe(torch, pkg, N):
+    """Fixed kernel sequence for counter runs: a 64 MiB device copy (calibration), then one
+    FP64 solve of the benchmark configuration (the kernels and launch shapes of a step)."""
     eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
     mass = pkg.MassBackend(N, eps)
     mass.autotune(reps=3)
     sysq = pkg.QSystem(mass, Q)
+    v0 = pkg.start_vector(sysq.nr, 20260905)
     src = torch.ones(64 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
-    for dt, name in ((torch.complex128, "double"), (torch.complex64, "single")):
-        x = torch.randn(mass.NF, dtype=dt, device="cuda")
-        for _ in range(reps):
-            mass.MscalarOp(x)
-        for _ in range(reps):
-            sysq.Mhat(x, name)
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "done", "N": N, "copy_bytes": 64 * 2 ** 20, "tiles": mass.info["tiles"]}), flush=True)
+    out = pkg.run_method_C(sysq, v0=v0, precision="double", **LANCZOS)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "N": N, "copy_bytes": 64 * 2 ** 20, "tiles": mass.info["tiles"],
+                      "outer_steps": out["stats"]["outer_steps"]}), flush=True)
 
 
 def main():
@@ -290,6 +328,8 @@ def main():
     # ---- FP64 steps (the configuration the metric is quoted on)
     for _ in range(a.warmup):
         pkg.run_method_C(sysq, v0=v0, precision="double", **LANCZOS)
+    # kernel families timed live with HIP events on the library stream (every 16th launch)
+    mass.profile(16)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     outs = []
@@ -298,6 +338,14 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     per_step = elapsed / max(a.steps, 1)
+    ktimes = mass.kernel_times()
+    mass.profile(0)
+    # the dominant family: largest estimated share of the step (sampled time x stride)
+    dom = max(ktimes, key=lambda k: ktimes[k]["ms"])
+    kd = ktimes[dom]
+    achieved = kd["GBs"]
+    traffic = pmc_traffic(FAMILY_KERNEL[dom] % (2 * N) if "%d" in FAMILY_KERNEL[dom] else FAMILY_KERNEL[dom])
+    alg_per_launch = kd["bytes"] / kd["launches"]
     lam64 = outs[-1]["lambda_"]
     st64 = outs[-1]["stats"]
     # mixed form on the same configuration (reported, untimed by the step contract)
@@ -344,9 +392,14 @@ def main():
         "mixed_vs_fp64": comparisons,
         "m_apply": {"fp64_us": t64 * 1e6, "fp64_alg_GBs": gbs64, "fp32_us": t32 * 1e6, "fp32_alg_GBs": gbs32,
                     "alg_bytes_fp64": bytes64, "alg_bytes_fp32": bytes32, "tiles": tiles},
-        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "scalar M_s apply FP64 (passes A+B+C), 40*NF algorithmic bytes per apply"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "kernel": FAMILY_DESC[dom], "avg_launch_us": kd["avg_us"],
+                     "algorithmic_bytes_per_launch": alg_per_launch,
+                     "traffic_source": traffic, "timing": "HIP events on the library stream around every "
+                     "4th launch of each kernel family during the timed steps"},
+        "kernel_families": ktimes,
         "cpu_baseline": cpu,
         "extras": ext,
     }

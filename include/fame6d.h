@@ -86,6 +86,22 @@ int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
  * variant_b in 0..1 (inner positions per workgroup of pass B). */
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
 
+/* Kernel-family timing (no reference counterpart; the reference times whole operator
+ * calls with tic/toc, Lanczos_hybrid_identity_gpuop.m:37-39).  While enabled, every
+ * `enable`-th launch of a timed family on the handle's stream is bracketed by HIP
+ * events (enable = 0 switches timing off); the durations and the algorithmic HBM bytes
+ * of the timed launches accumulate per slot until read.  Slots: 0 Mhat pass A in PCG,
+ * 1 Mhat pass B, 2 Mhat pass C, 3 CG update, 4 Lanczos Q^H w sweeps, 5 Lanczos w - Q h
+ * sweeps.  Every call of fame6d_mass_profile resets the sums. */
+#define FAME6D_NUM_TIMER_SLOTS 6
+typedef struct {
+  long long launches;
+  double milliseconds;  /* summed kernel durations */
+  double bytes;         /* summed algorithmic HBM bytes */
+} fame6d_kernel_time_t;
+int fame6d_mass_profile(fame6d_mass* m, int enable);
+int fame6d_mass_kernel_times(fame6d_mass* m, fame6d_kernel_time_t* out /* FAME6D_NUM_TIMER_SLOTS */);
+
 /* mass.MscalarOp: applyMij_rank1_exact_lowmem (exp1/applyMij_rank1_exact_lowmem.m:8-22),
  * applyMij_fft (exp1/applyMij_fft.m:9-23).  x, y: NF x nrhs.  y may alias x. */
 int fame6d_mass_apply_scalar(fame6d_mass* m, int precision, const void* x, void* y, int nrhs);

@@ -64,6 +64,13 @@ class RecoverStats(C.Structure):
     _fields_ = [("iterations", C.c_int * 64), ("relres", C.c_double * 64), ("mass_norm2", C.c_double * 64)]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("launches", C.c_longlong), ("milliseconds", C.c_double), ("bytes", C.c_double)]
+
+
+NUM_TIMER_SLOTS = 6
+TIMER_SLOTS = ("mhat_pass_a", "mhat_pass_b", "mhat_pass_c", "cg_update", "lanczos_gemv_c", "lanczos_gemv_n")
+
 P = C.c_void_p
 I = C.c_int
 D = C.c_double
@@ -79,6 +86,8 @@ SIGNATURES = {
     "fame6d_mass_info": [P, C.POINTER(MassInfo)],
     "fame6d_mass_autotune": [P, I, I],
     "fame6d_mass_set_tiles": [P, I, I, I],
+    "fame6d_mass_profile": [P, I],
+    "fame6d_mass_kernel_times": [P, C.POINTER(KernelTime)],
     "fame6d_mass_apply_scalar": [P, I, P, P, I],
     "fame6d_mass_apply_3comp": [P, I, P, P, I],
     "fame6d_mass_apply_cols12": [P, I, P, P, I],

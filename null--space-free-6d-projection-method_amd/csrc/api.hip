@@ -243,10 +243,12 @@ int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass*
     m->NF = NF;
     m->stream = static_cast<hipStream_t>(stream);
     F6_HIP(hipGetDevice(&m->device));
+    m->pin.init(1 << 16);
     upload_eps(m, eps_host);
   } catch (...) {
     m->eps64.release();
     m->eps32.release();
+    m->pin.release();
     delete m;
     throw;
   }
@@ -274,6 +276,8 @@ int fame6d_mass_destroy(fame6d_mass* m) {
   m->io64.release();
   m->partials.release();
   m->scal.release();
+  m->timer.destroy();
+  m->pin.release();
   delete m;
   F6_API_END
 }
@@ -304,6 +308,27 @@ int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   t.io = ((va >> 1) & 1) ? 0 : 1;
   t.a3 = (va >> 2) & 1;
   t.b = vb;
+  F6_API_END
+}
+
+int fame6d_mass_profile(fame6d_mass* m, int enable) {
+  F6_API_BEGIN
+  F6_REQUIRE(m, FAME6D_EARG, "null handle");
+  F6_REQUIRE(enable >= 0, FAME6D_EARG, "profile stride must be >= 0");
+  m->timer.enable(enable);
+  F6_API_END
+}
+
+int fame6d_mass_kernel_times(fame6d_mass* m, fame6d_kernel_time_t* out) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && out, FAME6D_EARG, "null argument");
+  F6_HIP(hipStreamSynchronize(m->stream));
+  m->timer.harvest();
+  for (int i = 0; i < FAME6D_NUM_TIMER_SLOTS; ++i) {
+    out[i].launches = m->timer.count[i];
+    out[i].milliseconds = m->timer.ms[i];
+    out[i].bytes = m->timer.bytes[i];
+  }
   F6_API_END
 }
 

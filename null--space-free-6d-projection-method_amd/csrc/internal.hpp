@@ -1,6 +1,7 @@
 // Library-internal declarations shared by the translation units of libfame6d.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -55,6 +56,70 @@ struct DevBuf {
   }
 };
 
+// Page-locked host staging for the small control transfers (CG state, dot products,
+// Lanczos scalars, group maps): a device-to-host read then costs one DMA and one stream
+// synchronisation, and host-to-device copies are asynchronous.  Bump allocation; the
+// space is recycled after each synchronisation of the owning stream.
+struct HostPin {
+  char* ptr = nullptr;
+  size_t bytes = 0, used = 0;
+  void init(size_t n) {
+    F6_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr), n, hipHostMallocDefault));
+    bytes = n;
+    used = 0;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = used = 0;
+  }
+  // n bytes of staging; synchronises `s` first when the buffer is exhausted
+  void* take(size_t n, hipStream_t s) {
+    n = (n + 63) & ~size_t(63);
+    if (n > bytes) throw Error(FAME6D_EARG, "host staging request of " + std::to_string(n) + " bytes");
+    if (used + n > bytes) {
+      F6_HIP(hipStreamSynchronize(s));
+      used = 0;
+    }
+    void* p = ptr + used;
+    used += n;
+    return p;
+  }
+};
+
+// Optional per-kernel-family timing (enabled through fame6d_mass_profile).  A timed
+// launch goes through hipExtLaunchKernel with a start/stop event pair, so the events
+// hold that kernel's own start and end times; durations are harvested when the pool
+// fills or when the sums are read.  `bytes` accumulates the algorithmic HBM bytes of
+// each timed launch as stated by its launcher.
+struct KernelTimer {
+  static constexpr int kSlots = FAME6D_NUM_TIMER_SLOTS;
+  static constexpr int kPool = 16384;
+  bool on = false;
+  int stride = 1;                      // time every stride-th launch of each family
+  bool armed = false;                  // the next launch_k of the current family is timed
+  bool fired = false;                  // ... and has been issued
+  long long seen[kSlots] = {0};
+  std::vector<hipEvent_t> ev;          // 2*kPool events
+  std::vector<int> slot;               // slot of pending pair i
+  std::vector<double> bytes_pending;   // bytes of pending pair i (< 0: discarded)
+  std::vector<int> tag;                // caller tag of pending pair i (-1: none)
+  int pending = 0;
+  long long count[kSlots] = {0};
+  double ms[kSlots] = {0}, bytes[kSlots] = {0};
+  void enable(int stride_or_off);
+  void begin(int sl);
+  void end(double alg_bytes, int tag = -1);
+  hipEvent_t ev0() const { return ev[2 * pending]; }
+  hipEvent_t ev1() const { return ev[2 * pending + 1]; }
+  // discards pending pairs from index `first` on whose tag is >= `threshold` (launches
+  // that the device skipped, e.g. CG iterations after convergence)
+  void discard_tags(int first, int threshold);
+  void harvest();  // waits for the pending events and accumulates
+  void reset();
+  void destroy();
+};
+
 // Tile variant indices of the three passes (chosen by the autotuner or defaults).
 struct TileChoice {
   int a = 0;  // pass A / pass C variant
@@ -80,6 +145,8 @@ struct fame6d_mass {
   fame6d::DevBuf scal;             // small device scalars
   fame6d::TileChoice tile64, tile32;
   std::vector<double> eps_host;    // unscaled samples (for adjoint / info)
+  fame6d::HostPin pin;             // page-locked staging of small transfers
+  mutable fame6d::KernelTimer timer;
 };
 
 namespace fame6d {
@@ -90,11 +157,13 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
 // mode 0: fused inverse / eps / forward over dims 4-6; mode 1: inverse only.
 template <class T>
 void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
-                   Skip skip, hipStream_t s, bool inverse_eps = false);
+                   Skip skip, hipStream_t s, bool inverse_eps = false, double* partials = nullptr);
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s);
 // Number of workgroups pass C uses per group (size of its partials per group).
 int pass_c_blocks(const fame6d_mass* m, bool fp32);
+// Number of pass-B tiles per vector (its partials per vector; NR vectors per group).
+int pass_b_blocks(const fame6d_mass* m, bool fp32);
 int num_tile_variants(int n, bool fp32, bool pass_b);
 void tile_variant_desc(int n, bool fp32, bool pass_b, int v, int* param, int* threads);
 
@@ -105,6 +174,41 @@ void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long 
 
 template <class T>
 DevBuf& work_buf(fame6d_mass* m);
+
+// Small transfers through the page-locked staging of m.  d2h_sync synchronises the
+// stream and recycles the staging; h2d_async returns at once.
+void d2h_sync(fame6d_mass* m, void* host, const void* dev, size_t bytes);
+void h2d_async(fame6d_mass* m, void* dev, const void* host, size_t bytes);
+
+// Kernel-family timer scopes (no-ops unless profiling is enabled on m).
+enum TimerSlot : int {
+  TS_MHAT_PASS_A = 0,  // Mhat pass A inside PCG (p = r + beta p fused)
+  TS_MHAT_PASS_B = 1,  // Mhat pass B (three vectors)
+  TS_MHAT_PASS_C = 2,  // Mhat pass C
+  TS_CG_UPDATE = 3,    // x += alpha p, r -= alpha q, ||r||^2 partials
+  TS_GEMV_C = 4,       // Lanczos Q^H w sweeps
+  TS_GEMV_N = 5,       // Lanczos w - Q h sweeps
+};
+inline void timer_begin(const fame6d_mass* m, int slot) {
+  if (m->timer.on) m->timer.begin(slot);
+}
+inline void timer_end(const fame6d_mass* m, double bytes, int tag = -1) {
+  if (m->timer.on) m->timer.end(bytes, tag);
+}
+
+// kernel<<<grid, block, 0, s>>>(args...), with the arguments converted to the kernel's
+// parameter types; a launch inside an armed timer scope carries the timer's events.
+template <typename... KArgs, typename... Args>
+inline void launch_k(const fame6d_mass* m, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                     Args&&... args) {
+  if (m->timer.armed && !m->timer.fired) {
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, m->timer.ev0(), m->timer.ev1(), 0,
+                          static_cast<KArgs>(args)...);
+    m->timer.fired = true;
+  } else {
+    kernel<<<grid, block, 0, s>>>(static_cast<KArgs>(args)...);
+  }
+}
 
 // Per-solve results of a (batched) PCG on Mhat.
 struct PcgOut {
@@ -133,6 +237,7 @@ struct fame6d_qsys {
   bool mixed_precond = true;  // FP32 inner PCG preconditioned by Mhat evaluated with 1/eps
   int mixed_max_cycles = 30;
   int chunk_hint = 0, chunk_hint32 = 0;
+  std::vector<int> qmap_uploaded;  // q-slot map last copied to sc (skips repeated uploads)
   fame6d_krinv_stats_t stats{};
   const double* V(int i) const { return Vd.as<double>() + (size_t)i * G * NF; }
   const double* W64(int i) const { return Wd.as<double>() + (size_t)i * G * NF; }
@@ -145,10 +250,11 @@ namespace fame6d {
 // Builds the q-system of G Bloch points on the host (FP64) and uploads it.
 fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P);
 // y = Mhat x for nact groups in q-system slots qmap (device ints, nullptr = identity).
+// energy_partials (optional): pass-B partials whose per-group sum is x' Mhat x
+// (3 * pass_b_blocks per group).
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
-                hipStream_t s, bool inverse_eps = false, const cpx<T>* dotp = nullptr,
-                double* partials = nullptr);
+                hipStream_t s, bool inverse_eps = false, double* energy_partials = nullptr);
 // Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).
 void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                   const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,

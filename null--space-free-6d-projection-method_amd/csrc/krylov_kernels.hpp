@@ -118,9 +118,10 @@ template <class T>
 __global__ __launch_bounds__(kRedThreads) void k_dot_partials(const cpx<T>* a, long long lda,
                                                               const cpx<T>* b, long long ldb,
                                                               long long n, double* pre,
-                                                              double* pim) {
+                                                              double* pim, Skip skip = no_skip()) {
   __shared__ double red[kRedThreads];
   const int g = blockIdx.y;
+  if (skipped(skip, g)) return;
   const cpx<T>* ag = a + g * lda;
   const cpx<T>* bg = b + g * ldb;
   double sr = 0.0, si = 0.0;
@@ -162,7 +163,7 @@ __device__ __forceinline__ double sum_partials(const double* partials, int nb, d
 }
 
 static __global__ __launch_bounds__(kRedThreads) void k_cg_init(CgState* st, const double* partials,
-                                                         int nb, double tol, int maxit) {
+                                                         int nb, double tol, int maxit, int maxmsteps) {
   __shared__ double red[kRedThreads];
   const int g = blockIdx.x;
   const double s = sum_partials(partials + (long long)g * nb, nb, red);
@@ -174,6 +175,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_init(CgState* st, con
     c.normr = c.n2b;
     c.iter = 0;
     c.maxit = maxit;
+    c.maxmsteps = maxmsteps;
     c.flag = 0;
     c.done = (!(c.n2b > 0.0) || c.normr <= c.tolb || maxit <= 0) ? 1 : 0;
     if (!isfinite(c.n2b)) c.flag = -1;
@@ -185,7 +187,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_init(CgState* st, con
 // Preconditioned start: n2b from ||r||^2 partials, rho = r'z from the r'z partials.
 static __global__ __launch_bounds__(kRedThreads) void k_pcg_init(CgState* st, const double* prr, int nbr,
                                                                  const double* prz, int nbz, double tol,
-                                                                 int maxit) {
+                                                                 int maxit, int maxmsteps) {
   __shared__ double red[kRedThreads];
   const int g = blockIdx.x;
   const double s = sum_partials(prr + (long long)g * nbr, nbr, red);
@@ -198,6 +200,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_pcg_init(CgState* st, co
     c.rho = rz;
     c.normr = c.n2b;
     c.maxit = maxit;
+    c.maxmsteps = maxmsteps;
     c.done = (!(c.n2b > 0.0) || c.normr <= c.tolb || maxit <= 0) ? 1 : 0;
     if (!isfinite(c.n2b) || !isfinite(rz)) c.flag = -1;
     if (maxit <= 0 && c.done && c.n2b > 0.0 && c.normr > c.tolb) c.flag = 1;
@@ -318,22 +321,82 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, con
   }
 }
 
-// After a failed true-residual check the iteration continues from r = b - A x
-// (MATLAB pcg "moresteps"): beta uses the true residual, and every later
-// iteration is checked against the true residual as well.
-static __global__ __launch_bounds__(kRedThreads) void k_cg_resume(CgState* st, const double* partials,
-                                                           int nb, const int* resume) {
+// True-residual check of MATLAB pcg, on the device.  k_cg_need_check marks the groups
+// that stopped on the recurrence with flag 0 (need[g] = 0 means "check g"; every other
+// group gets 1, and groups that stopped with a failure flag are finished here).  The host
+// then applies A to x for the marked groups and reduces ||b - A x||^2 into partials, and
+// k_cg_true_check decides: converged (finished, flag 0), failed (flag 3 after maxmsteps
+// failed checks, flag 1 at maxit) or resume ("moresteps": continue from the true
+// residual, checking every later iteration).  resume[g] = 0 marks the groups that resume
+// (for the preconditioned form, which needs z = P^{-1} r first); 1 for all others.
+static __global__ void k_cg_need_check(CgState* st, int G, int* need) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+    CgState& c = st[g];
+    int skip = 1;
+    if (!c.finished && c.done) {
+      if (c.flag == 0) skip = 0;
+      else c.finished = 1;
+    }
+    need[g] = skip;
+  }
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_cg_true_check(CgState* st, const double* partials,
+                                                                      int nb, const int* need, int* resume,
+                                                                      int precond) {
   __shared__ double red[kRedThreads];
   const int g = blockIdx.x;
-  if (!resume[g]) return;
+  if (need[g]) {
+    if (threadIdx.x == 0) resume[g] = 1;
+    return;
+  }
+  const double rr = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    CgState c = st[g];
+    const double nr = sqrt(rr);
+    c.checks += 1;
+    c.normr_true = nr;
+    int res = 1;
+    if (!isfinite(rr)) {
+      c.flag = -1;
+      c.finished = 1;
+    } else if (nr <= c.tolb) {
+      c.flag = 0;
+      c.finished = 1;
+    } else {
+      c.moresteps += 1;
+      if (c.moresteps >= c.maxmsteps || c.iter >= c.maxit) {
+        c.flag = (c.moresteps >= c.maxmsteps) ? 3 : 1;
+        c.finished = 1;
+      } else {
+        // continue from r = b - A x: beta uses the true residual
+        c.flag = 0;
+        c.done = 0;
+        c.check_every = 1;
+        c.normr = nr;
+        if (precond) {
+          res = 0;  // rho = r'z after z = P^{-1} r (k_cg_resume)
+        } else {
+          c.beta = rr / c.rho_prev;
+          c.rho = rr;
+        }
+      }
+    }
+    st[g] = c;
+    resume[g] = res;
+  }
+}
+
+// Preconditioned resume: rho = r'z from the partials, beta = rho / rho_prev.
+static __global__ __launch_bounds__(kRedThreads) void k_cg_resume(CgState* st, const double* partials,
+                                                                  int nb, const int* resume) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (resume[g]) return;
   const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
   if (threadIdx.x == 0) {
     st[g].beta = rho_new / st[g].rho_prev;
     st[g].rho = rho_new;
-    st[g].normr = sqrt(rho_new);
-    st[g].done = (st[g].iter >= st[g].maxit) ? 1 : 0;
-    st[g].flag = st[g].done ? 1 : 0;
-    st[g].check_every = 1;
   }
 }
 
@@ -383,19 +446,23 @@ __global__ void k_fill_zero_real(long long n, T* x) {
 }
 
 // ------------------------------------------------------------------ Lanczos basis
-// Partials of h_k = Q(:,k)' r for k in [k0, k0+KB) (basis columns contiguous, ld = n).
+// Partials of h_k = Q(:,k)' r for all k < jj in one launch.  Block (rb, c) of the
+// logical grid covers rows of row block rb for columns [c*KB, c*KB + KB); the logical
+// order (column chunk fastest) is mapped XCD-aware, so the chunks of one row block run
+// on one XCD and re-read r from its L2.  pre/pim[k*nrb + rb].
 template <int KB>
-__global__ __launch_bounds__(kRedThreads) void k_gemv_c_partials(const cpx<double>* Q, long long ld,
-                                                                 int k0, int kcount,
-                                                                 const cpx<double>* r,
-                                                                 long long n, double* pre,
-                                                                 double* pim) {
+__global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
+                                                            const cpx<double>* r, long long n, int nrb,
+                                                            double* pre, double* pim) {
   __shared__ double red[kRedThreads];
+  const int nch = (jj + KB - 1) / KB;
+  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int rb = logical / nch, ch = logical - rb * nch;
+  const int k0 = ch * KB, kcount = min(KB, jj - k0);
   double ar[KB], ai[KB];
 #pragma unroll
   for (int k = 0; k < KB; ++k) ar[k] = ai[k] = 0.0;
-  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kRedThreads) {
+  for (long long i = rb * (long long)kRedThreads + threadIdx.x; i < n; i += (long long)nrb * kRedThreads) {
     const cpx<double> v = r[i];
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
@@ -412,8 +479,8 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_partials(const cpx<doubl
     const double t = block_sum<kRedThreads>(ai[k], red);
     __syncthreads();
     if (threadIdx.x == 0) {
-      pre[(long long)(k0 + k) * gridDim.x + blockIdx.x] = s;
-      pim[(long long)(k0 + k) * gridDim.x + blockIdx.x] = t;
+      pre[(long long)(k0 + k) * nrb + rb] = s;
+      pim[(long long)(k0 + k) * nrb + rb] = t;
     }
   }
 }

@@ -133,26 +133,34 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
 // h1 in sc[kScH..], h2 in sc[kScH2..], ||r||^2 in sc[1].
 constexpr int kScH2 = 8 + 256, kScH2im = 8 + 384;
 constexpr int kGemvKB = 8;
-static void gemv_c(long long n, const Basis& B, int jj, const cpx<double>* v, double* dre, double* dim,
-                   hipStream_t s) {
+static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* v, double* dre,
+                   double* dim, hipStream_t s) {
   const int nb = red_blocks(n);
+  const int nch = (jj + kGemvKB - 1) / kGemvKB;
   double* pre = B.part;
   double* pim = B.part + (size_t)nb * 128;
-  for (int k0 = 0; k0 < jj; k0 += kGemvKB) {
-    const int kc = std::min(kGemvKB, jj - k0);
-    k_gemv_c_partials<kGemvKB><<<nb, kRedThreads, 0, s>>>(B.Q, n, k0, kc, v, n, pre, pim);
-  }
+  timer_begin(m, TS_GEMV_C);
+  launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim);
+  timer_end(m, (jj + 1) * 16.0 * n);
   k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, dre);
   k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, dim);
   F6_CHECK_LAUNCH();
 }
 
-static void cgs2_step(long long n, const Basis& B, int jj, const cpx<double>* w, hipStream_t s) {
+static void cgs2_step(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* w,
+                      hipStream_t s) {
   const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
-  gemv_c(n, B, jj, w, B.sc + kScH, B.sc + kScHim, s);
-  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n, nullptr);
-  gemv_c(n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s);
-  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n, B.part);
+  const double vb = 16.0 * n;  // bytes of one basis vector
+  gemv_c(m, n, B, jj, w, B.sc + kScH, B.sc + kScHim, s);
+  timer_begin(m, TS_GEMV_N);
+  launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n,
+           nullptr);
+  timer_end(m, (jj + 2) * vb);
+  gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s);
+  timer_begin(m, TS_GEMV_N);
+  launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n,
+           B.part);
+  timer_end(m, (jj + 2) * vb);
   k_reduce_final<<<1, kRedThreads, 0, s>>>(B.part, gn, B.sc + 1);
   F6_CHECK_LAUNCH();
 }
@@ -354,16 +362,25 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
       const cpx<double>* w = qout + (size_t)n2 * i;
       // CGS2: h1 = Q^H w (alpha = Re h1[j]); r = w - Q h1 and h2 = Q^H r; r -= Q h2 with ||r||^2
-      cgs2_step(n2, B, jj, w, s);
+      cgs2_step(m, n2, B, jj, w, s);
       Lg.justRestarted = false;
     }
+    // alpha = Re h1[jj-1] and beta^2 of every active run: one staged read, one sync
     std::vector<double> ab(2 * na);
-    for (int i = 0; i < na; ++i) {
-      const LzState& Lg = L[act[i]];
-      F6_HIP(hipMemcpyAsync(&ab[2 * i], Lg.B.sc + kScH + (Lg.jj - 1), sizeof(double), hipMemcpyDeviceToHost, s));
-      F6_HIP(hipMemcpyAsync(&ab[2 * i + 1], Lg.B.sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    {
+      const size_t per = sizeof(double) * (kScH + p);
+      char* stage = static_cast<char*>(m->pin.take(per * na, s));
+      for (int i = 0; i < na; ++i)
+        F6_HIP(hipMemcpyAsync(stage + per * i, L[act[i]].B.sc, sizeof(double) * (kScH + L[act[i]].jj),
+                              hipMemcpyDeviceToHost, s));
+      F6_HIP(hipStreamSynchronize(s));
+      m->pin.used = 0;
+      for (int i = 0; i < na; ++i) {
+        const double* v = reinterpret_cast<const double*>(stage + per * i);
+        ab[2 * i] = v[kScH + (L[act[i]].jj - 1)];
+        ab[2 * i + 1] = v[1];
+      }
     }
-    F6_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       const double alpha = ab[2 * i], beta2 = ab[2 * i + 1];

@@ -3,6 +3,8 @@
 // autotuner chooses between.
 #include "internal.hpp"
 
+#include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 namespace fame6d {
@@ -65,8 +67,8 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
         dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
-        if (op == PRE_MHAT) k_pass_a3<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a);
-        else k_pass_a3<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a);
+        if (op == PRE_MHAT) launch_k(m, k_pass_a3<T, n, BPW, PRE_MHAT>, grid, block, s, a);
+        else launch_k(m, k_pass_a3<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a);
       };
       if (v3 == 0) go(std::integral_constant<int, Tiles<n, T>::bpw3[0]>{});
       else go(std::integral_constant<int, Tiles<n, T>::bpw3[1]>{});
@@ -82,16 +84,16 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
       if (tile_of<T>(m).io == 1) {
         switch (op) {
-          case PRE_COPY: k_pass_a_lin<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
-          case PRE_MHAT: k_pass_a_lin<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
-          case PRE_MHAT_CG: k_pass_a_lin<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
+          case PRE_COPY: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_COPY>, grid, block, s, a); break;
+          case PRE_MHAT: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_MHAT>, grid, block, s, a); break;
+          case PRE_MHAT_CG: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a); break;
           default: throw Error(FAME6D_EARG, "bad pass-A op");
         }
       } else {
         switch (op) {
-          case PRE_COPY: k_pass_a<T, n, BPW, PRE_COPY><<<grid, block, 0, s>>>(a); break;
-          case PRE_MHAT: k_pass_a<T, n, BPW, PRE_MHAT><<<grid, block, 0, s>>>(a); break;
-          case PRE_MHAT_CG: k_pass_a<T, n, BPW, PRE_MHAT_CG><<<grid, block, 0, s>>>(a); break;
+          case PRE_COPY: launch_k(m, k_pass_a<T, n, BPW, PRE_COPY>, grid, block, s, a); break;
+          case PRE_MHAT: launch_k(m, k_pass_a<T, n, BPW, PRE_MHAT>, grid, block, s, a); break;
+          case PRE_MHAT_CG: launch_k(m, k_pass_a<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a); break;
           default: throw Error(FAME6D_EARG, "bad pass-A op");
         }
       }
@@ -104,11 +106,11 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
 
 template <class T>
 void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* t, long long ldt,
-                   Skip skip, hipStream_t s, bool inverse_eps) {
+                   Skip skip, hipStream_t s, bool inverse_eps, double* partials) {
   if (ngroups <= 0) return;
   const void* e = std::is_same<T, double>::value ? (inverse_eps ? m->ieps64.ptr : m->eps64.ptr)
                                                  : (inverse_eps ? m->ieps32.ptr : m->eps32.ptr);
-  PassBArgs<T> a{t, ldt, (const T*)e, skip};
+  PassBArgs<T> a{t, ldt, (const T*)e, skip, partials};
   const int v = tile_of<T>(m).b;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
@@ -116,9 +118,9 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
     auto go = [&](auto ac) {
       constexpr int A = decltype(ac)::value;
       dim3 grid(S / A, ngroups * nr), block(A * n * n);
-      if (mode == 0 && nr == 1) k_pass_b<T, n, A, 1, 0><<<grid, block, 0, s>>>(a);
-      else if (mode == 0 && nr == 3) k_pass_b<T, n, A, 3, 0><<<grid, block, 0, s>>>(a);
-      else if (mode == 1 && nr == 1) k_pass_b<T, n, A, 1, 1><<<grid, block, 0, s>>>(a);
+      if (mode == 0 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 0>, grid, block, s, a);
+      else if (mode == 0 && nr == 3) launch_k(m, k_pass_b<T, n, A, 3, 0>, grid, block, s, a);
+      else if (mode == 1 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 1>, grid, block, s, a);
       else throw Error(FAME6D_EARG, "bad pass-B configuration");
     };
     if (v == 0) go(std::integral_constant<int, Tiles<n, T>::a[0]>{});
@@ -138,7 +140,7 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
         dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
-        k_pass_c3<T, n, BPW><<<grid, block, 0, s>>>(a);
+        launch_k(m, k_pass_c3<T, n, BPW>, grid, block, s, a);
       };
       if (v3 == 0) go(std::integral_constant<int, Tiles<n, T>::bpw3[0]>{});
       else go(std::integral_constant<int, Tiles<n, T>::bpw3[1]>{});
@@ -154,10 +156,10 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
       constexpr int BPW = decltype(bc)::value;
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
       const bool lin = tile_of<T>(m).io == 1;
-      if (op == POST_COPY && lin) k_pass_c_lin<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
-      else if (op == POST_MHAT && lin) k_pass_c_lin<T, n, BPW, POST_MHAT, 3><<<grid, block, 0, s>>>(a);
-      else if (op == POST_COPY) k_pass_c<T, n, BPW, POST_COPY, 1><<<grid, block, 0, s>>>(a);
-      else if (op == POST_MHAT) k_pass_c<T, n, BPW, POST_MHAT, 3><<<grid, block, 0, s>>>(a);
+      if (op == POST_COPY && lin) launch_k(m, k_pass_c_lin<T, n, BPW, POST_COPY, 1>, grid, block, s, a);
+      else if (op == POST_MHAT && lin) launch_k(m, k_pass_c_lin<T, n, BPW, POST_MHAT, 3>, grid, block, s, a);
+      else if (op == POST_COPY) launch_k(m, k_pass_c<T, n, BPW, POST_COPY, 1>, grid, block, s, a);
+      else if (op == POST_MHAT) launch_k(m, k_pass_c<T, n, BPW, POST_MHAT, 3>, grid, block, s, a);
       else throw Error(FAME6D_EARG, "bad pass-C op");
     };
     if (v == 0) go(std::integral_constant<int, Tiles<n, T>::bpw[0]>{});
@@ -175,6 +177,19 @@ int pass_c_blocks(const fame6d_mass* m, bool fp32) {
     constexpr int S = n * n * n;
     const int bpw = fp32 ? Tiles<n, float>::bpw3[v] : Tiles<n, double>::bpw3[v];
     out = S / bpw;
+  });
+  return out;
+}
+
+int pass_b_blocks(const fame6d_mass* m, bool fp32) {
+  // tiles of pass B per vector (size of its partials per vector)
+  int out = 0;
+  const int v = fp32 ? m->tile32.b : m->tile64.b;
+  with_n(m->n, [&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    constexpr int S = n * n * n;
+    const int A = fp32 ? Tiles<n, float>::a[v] : Tiles<n, double>::a[v];
+    out = S / A;
   });
   return out;
 }
@@ -225,10 +240,90 @@ void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long 
   launch_pass_c<T>(m, POST_COPY, nrhs, c, s);
 }
 
+// ------------------------------------------------------------------ host plumbing
+void d2h_sync(fame6d_mass* m, void* host, const void* dev, size_t bytes) {
+  void* p = m->pin.take(bytes, m->stream);
+  F6_HIP(hipMemcpyAsync(p, dev, bytes, hipMemcpyDeviceToHost, m->stream));
+  F6_HIP(hipStreamSynchronize(m->stream));
+  std::memcpy(host, p, bytes);
+  m->pin.used = 0;
+}
+
+void h2d_async(fame6d_mass* m, void* dev, const void* host, size_t bytes) {
+  void* p = m->pin.take(bytes, m->stream);
+  std::memcpy(p, host, bytes);
+  F6_HIP(hipMemcpyAsync(dev, p, bytes, hipMemcpyHostToDevice, m->stream));
+}
+
+void KernelTimer::enable(int e) {
+  if (e > 0 && ev.empty()) {
+    ev.resize(2 * kPool);
+    for (auto& x : ev) F6_HIP(hipEventCreate(&x));
+    slot.resize(kPool);
+    bytes_pending.resize(kPool);
+    tag.resize(kPool);
+  }
+  harvest();
+  reset();
+  on = e > 0;
+  stride = e > 0 ? e : 1;
+  armed = fired = false;
+}
+
+void KernelTimer::begin(int sl) {
+  armed = (seen[sl]++ % stride) == 0;
+  fired = false;
+  if (!armed) return;
+  if (pending >= kPool) harvest();
+  slot[pending] = sl;
+}
+
+void KernelTimer::end(double b, int tg) {
+  if (armed && fired) {
+    bytes_pending[pending] = b;
+    tag[pending] = tg;
+    ++pending;
+  }
+  armed = fired = false;
+}
+
+void KernelTimer::discard_tags(int first, int threshold) {
+  for (int i = std::max(first, 0); i < pending; ++i)
+    if (tag[i] >= threshold) bytes_pending[i] = -1.0;
+}
+
+void KernelTimer::harvest() {
+  if (pending == 0) return;
+  F6_HIP(hipEventSynchronize(ev[2 * pending - 1]));
+  for (int i = 0; i < pending; ++i) {
+    if (bytes_pending[i] < 0) continue;
+    float t = 0.f;
+    F6_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+    count[slot[i]] += 1;
+    ms[slot[i]] += t;
+    bytes[slot[i]] += bytes_pending[i];
+  }
+  pending = 0;
+}
+
+void KernelTimer::reset() {
+  for (int i = 0; i < kSlots; ++i) {
+    count[i] = seen[i] = 0;
+    ms[i] = bytes[i] = 0;
+  }
+}
+
+void KernelTimer::destroy() {
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  ev.clear();
+  pending = 0;
+  on = false;
+}
+
 #define F6_INST(T)                                                                                 \
   template void launch_pass_a<T>(const fame6d_mass*, int, int, const PassAArgs<T>&, hipStream_t);  \
   template void launch_pass_b<T>(const fame6d_mass*, int, int, int, cpx<T>*, long long, Skip,      \
-                                 hipStream_t, bool);                                               \
+                                 hipStream_t, bool, double*);                                      \
   template void launch_pass_c<T>(const fame6d_mass*, int, int, const PassCArgs<T>&, hipStream_t);  \
   template void mass_apply<T>(fame6d_mass*, const cpx<T>*, long long, cpx<T>*, long long, int,     \
                               hipStream_t);                                                        \

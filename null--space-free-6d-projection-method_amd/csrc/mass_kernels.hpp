@@ -22,39 +22,49 @@
 // invC*(s0 + W1 s1 + W2 s2) in registers.
 #pragma once
 
+#include <cstddef>
+
 #include "common.hpp"
 #include "dft.hpp"
 
 namespace fame6d {
 
-// Per-group device state of a conjugate-gradient solve (all scalars FP64).
+// Per-group device state of a conjugate-gradient solve (all scalars FP64).  The
+// device runs MATLAB pcg's whole control flow (recurrence test, true-residual check,
+// "moresteps" resume); the host only reads `finished` once per launched chunk.
 struct CgState {
-  double rho;       // r'*r of the current residual
-  double alpha;     // rho / (p'*q)
-  double beta;      // rho_new / rho_old
-  double pq;        // p'*q
-  double normr;     // ||r|| (recurrence)
-  double tolb;      // tol * ||b||
-  double n2b;       // ||b||
-  double rho_prev;  // rho before the last update
-  int iter;         // completed iterations
-  int done;         // 1: stop issuing work for this group
-  int flag;         // MATLAB pcg flag convention (0 ok, 1 maxit, 3 stagnation, 4 breakdown)
+  double rho;         // r'*r (r'*z preconditioned) of the current residual
+  double alpha;       // rho / (p'*q)
+  double beta;        // rho_new / rho_old
+  double pq;          // p'*q
+  double normr;       // ||r|| (recurrence)
+  double tolb;        // tol * ||b||
+  double n2b;         // ||b||
+  double rho_prev;    // rho before the last update
+  double normr_true;  // ||b - A x|| of the last true-residual check
+  int iter;           // completed iterations
+  int done;           // 1: the iteration kernels skip this group
+  int flag;           // MATLAB pcg flag convention (0 ok, 1 maxit, 3 stagnation, 4 breakdown, -1 NaN)
   int maxit;
-  int check_every;  // stop after every iteration for a true-residual check
-  int pad[3];
+  int check_every;    // every later iteration ends with a true-residual check
+  int finished;       // 1: solve complete (confirmed by the true residual, or failed)
+  int moresteps;      // failed true-residual checks so far
+  int checks;         // true-residual checks (one A x each)
+  int maxmsteps;      // failed checks allowed before flag 3
+  int pad[5];
 };
-static_assert(sizeof(CgState) == 96, "CgState layout");
+static_assert(sizeof(CgState) == 128, "CgState layout");
 
 // Per-group early-exit mask: group g is skipped when p[g*stride] != 0.
-// Pointing it at CgState::done (int index 17, stride 24) skips finished solves.
 struct Skip {
   const int* p;
   int stride;
 };
 __host__ __device__ inline Skip no_skip() { return Skip{nullptr, 0}; }
 __host__ inline Skip skip_done(const CgState* st) {
-  return Skip{st ? reinterpret_cast<const int*>(st) + 17 : nullptr, 24};
+  if (!st) return no_skip();
+  return Skip{reinterpret_cast<const int*>(reinterpret_cast<const char*>(st) + offsetof(CgState, done)),
+              (int)(sizeof(CgState) / sizeof(int))};
 }
 __device__ __forceinline__ bool skipped(Skip s, int g) { return s.p != nullptr && s.p[g * s.stride] != 0; }
 // Group -> q-system slot (nullptr = identity).
@@ -92,6 +102,10 @@ struct PassBArgs {
   long long ldt;
   const T* eps;  // eps / NF
   Skip skip;
+  // MODE 0, optional: partials[v*gridDim.x + tile] = sum eps/NF * |G u_v|^2 over the tile,
+  // G u_v the unnormalised 6D inverse DFT of vector v.  Summed over the NR vectors of a
+  // group this is u'(F D G)u = p' Mhat p for the Mhat pre-op vectors u_r = W_r invC p.
+  double* partials;
 };
 
 template <class T>
@@ -309,15 +323,17 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
 // ---------------------------------------------------------------- pass B
 // MODE 0: inverse DFT dims 4-6, * eps/NF, forward DFT dims 6-4 (the mass apply).
 // MODE 1: inverse DFT dims 4-6 only (completes an unnormalised 6D ifft after pass A).
-// grid.y enumerates vectors; vector v belongs to group v / NR (for the skip mask),
-// so the NR vectors of one Mhat apply run in separate workgroups.
+// grid = (tiles, vectors); vector v belongs to group v / NR (for the skip mask).  The
+// block order is vector-fastest within a tile, so the NR (and batched-group) vectors of
+// one tile run back to back on one XCD and share the eps tile in its L2.
 template <class T, int n, int A, int NR, int MODE>
 __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
   constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int THREADS = A * P2;
   __shared__ cpx<T> L[A * S];  // l = al + A*(k4 + n*k5 + n^2*k6)
-  const int nt = gridDim.x;
-  const int logical = xcd_swizzle(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
-  const int tile = logical % nt, vidx = logical / nt;
+  const int nt = gridDim.x, nv = gridDim.y;
+  const int logical = xcd_swizzle(blockIdx.x + nt * blockIdx.y, nt * nv);
+  const int tile = logical / nv, vidx = logical - tile * nv;
   if (skipped(a.skip, vidx / NR)) return;
   const int tid = threadIdx.x;
   const int al = tid % A, qq = tid / A;  // qq in [0, n^2)
@@ -342,6 +358,7 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
     for (int k = 0; k < n; ++k) L[b5 + A * n * k] = w[k];
   }
   __syncthreads();
+  double en = 0.0;  // sum eps/NF |G6 z|^2 = Re z^H F6 (eps/NF) G6 z of this thread's dim-6 pencil z
   {  // dim 6: inverse, eps, forward
     const int b6 = al + A * qq;
     cpx<T> w[n];
@@ -356,8 +373,18 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < n; ++k) w[k] = a.eps[a0 + (long long)S * (qq + P2 * k)] * w[k];
       dft<T, n, -1>(w);
+      if (a.partials != nullptr) {
+        // z (still in LDS) against F6 eps G6 z: no extra registers over the plain apply
 #pragma unroll
-      for (int k = 0; k < n; ++k) L[b6 + A * P2 * k] = w[k];
+        for (int k = 0; k < n; ++k) {
+          const cpx<T> z = L[b6 + A * P2 * k];
+          en += static_cast<double>(z.re) * w[k].re + static_cast<double>(z.im) * w[k].im;
+          L[b6 + A * P2 * k] = w[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < n; ++k) L[b6 + A * P2 * k] = w[k];
+      }
     }
   }
   __syncthreads();
@@ -377,6 +404,12 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
     dft<T, n, -1>(w);
 #pragma unroll
     for (int k = 0; k < n; ++k) v[(long long)S * (k + n * qq)] = w[k];
+  }
+  if (a.partials != nullptr) {
+    static_assert(sizeof(cpx<T>) * A * S >= sizeof(double) * THREADS, "LDS reuse for the reduction");
+    __syncthreads();  // every thread has read L
+    en = block_sum<THREADS>(en, reinterpret_cast<double*>(L));
+    if (tid == 0) a.partials[(long long)vidx * nt + tile] = en;
   }
 }
 

@@ -165,7 +165,7 @@ static const T* Wt(const fame6d_qsys* S, int i) {
 
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
-                hipStream_t s, bool inverse_eps, const cpx<T>* dotp, double* partials) {
+                hipStream_t s, bool inverse_eps, double* energy_partials) {
   fame6d_mass* m = S->m;
   const long long NF = m->NF;
   DevBuf& w = work_buf<T>(m);
@@ -183,7 +183,7 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   a.skip = skip;
   a.qmap = qmap;
   launch_pass_a<T>(m, PRE_MHAT, nact, a, s);
-  launch_pass_b<T>(m, 3, 0, nact, t, NF, skip, s, inverse_eps);
+  launch_pass_b<T>(m, 3, 0, nact, t, NF, skip, s, inverse_eps, energy_partials);
   PassCArgs<T> c{};
   c.t = t;
   c.ldt = NF;
@@ -193,9 +193,6 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   c.W2 = a.W2;
   c.invC = a.invC;
   c.ldq = NF;
-  c.dotp = dotp;
-  c.lddot = NF;
-  c.partials = partials;
   c.skip = skip;
   c.qmap = qmap;
   launch_pass_c<T>(m, POST_MHAT, nact, c, s);
@@ -217,8 +214,7 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
   k_reduce_final<<<G, kRedThreads, 0, m->stream>>>(pim, nb, out + G);
   F6_CHECK_LAUNCH();
   std::vector<double> h(2 * G);
-  F6_HIP(hipMemcpyAsync(h.data(), out, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, m->stream));
-  F6_HIP(hipStreamSynchronize(m->stream));
+  d2h_sync(mm, h.data(), out, sizeof(double) * 2 * G);
   for (int g = 0; g < G; ++g) {
     if (out_re_host) out_re_host[g] = h[g];
     if (out_im_host) out_im_host[g] = h[G + g];
@@ -233,6 +229,9 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 // checks every later iteration ("moresteps", at most 5); flag 1 at maxit, 4 on
 // breakdown (p'q <= 0 or beta 0/Inf).  Not restated: the stagnation test
 // ||p||*|alpha| < eps*||x|| and the minimum-residual iterate returned on failure.
+// The whole control flow runs on the device (k_cg_* kernels); the host launches a
+// chunk of iterations followed by the check kernels and reads the states once per
+// chunk.  p'q (and r'z) come from pass B as sum eps/NF |G u|^2 (PassBArgs::partials).
 // With `precond` the iteration is preconditioned by P^{-1} = Mhat evaluated with
 // 1/eps (exact inverse for constant eps); the reference's solves are unpreconditioned,
 // and this build uses `precond` only for the FP32 inner solves of the mixed form.
@@ -244,38 +243,35 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   hipStream_t s = m->stream;
   const long long NF = m->NF;
   const long long total = NF * nact;
-  const int nbC = pass_c_blocks(m, std::is_same<T, float>::value);
+  constexpr bool f32 = std::is_same<T, float>::value;
+  const int nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
   const int nbU = red_blocks(NF);
   const int nbD = red_blocks(NF);
   S->st.ensure(sizeof(CgState) * nact);
-  S->part.ensure(sizeof(double) * (size_t)std::max(nbC, nbD) * nact * 2);
-  // per-group partials, then one reduced value per group behind them
-  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbU, nbD) * nact + (size_t)nact + 8));
+  S->part.ensure(sizeof(double) * (size_t)nbE * nact);
+  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbU, nbD) * nact + 8));
   S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
   CgState* st = S->st.as<CgState>();
   double* part = S->part.as<double>();
   double* part2 = S->part2.as<double>();
-  int* mask = S->ints.as<int>();
+  int* need = S->ints.as<int>();
+  int* resume = need + nact;
   DevBuf& w = work_buf<T>(m);
   w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
   cpx<T>* t = w.as<cpx<T>>();
   const cpx<T>* zr = precond ? z : r;  // the vector the search direction is built from
+  const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
 
   k_fill_zero<T><<<grid_for(total), 256, 0, s>>>(total, x);
   F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<T>) * total, hipMemcpyDeviceToDevice, s));
   k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, NF, b, NF, NF, part2, nullptr);
   if (precond) {
-    mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, r, part);
-    k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbC, tol, maxit);
+    mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, part);
+    k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbE, tol, maxit, maxmsteps);
   } else {
-    k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, tol, maxit);
+    k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, tol, maxit, maxmsteps);
   }
   F6_CHECK_LAUNCH();
-
-  out.assign(nact, PcgOut{});
-  std::vector<int> finished(nact, 0), moresteps(nact, 0);
-  std::vector<CgState> hs(nact);
-  const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
 
   PassAArgs<T> pa{};
   pa.x = zr;
@@ -299,127 +295,82 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   pc.W2 = pa.W2;
   pc.invC = pa.invC;
   pc.ldq = NF;
-  pc.dotp = p;
-  pc.lddot = NF;
-  pc.partials = part;
   pc.skip = skip_done(st);
   pc.qmap = qmap_dev;
 
-  auto read_state = [&]() {
-    F6_HIP(hipMemcpyAsync(hs.data(), st, sizeof(CgState) * nact, hipMemcpyDeviceToHost, s));
-    F6_HIP(hipStreamSynchronize(s));
-  };
-  read_state();
-  for (int g = 0; g < nact; ++g) {
-    if (hs[g].done) {
-      finished[g] = 1;
-      out[g].flag = hs[g].flag;
-      out[g].iter = 0;
-      out[g].relres = hs[g].n2b > 0 ? hs[g].normr / hs[g].n2b : 0.0;
-    }
-  }
-  int& hint_ref = std::is_same<T, float>::value ? S->chunk_hint32 : S->chunk_hint;
-  int hint = hint_ref > 0 ? hint_ref : 8;
-  int issued = 0;
+  // algorithmic HBM bytes per launch (cs: complex element, rs: real element)
+  const double cs = sizeof(cpx<T>), rs = sizeof(T), gb = (double)NF * nact;
+  const double bytesA = gb * (6 * cs + 3 * rs);  // zr, p, invC, W1, W2 in; p, 3 vectors out
+  const double bytesB = gb * (6 * cs + rs);      // 3 vectors + eps in; 3 vectors out
+  const double bytesC = gb * (4 * cs + 3 * rs);  // 3 vectors + invC, W1, W2 in; q out
+  const double bytesU = gb * 6 * cs;             // x, p, r, q in; x, r out
+  const int timer_first = m->timer.pending;
+
+  int& hint_ref = f32 ? S->chunk_hint32 : S->chunk_hint;
+  int chunk = hint_ref > 0 ? hint_ref : 8;
+  std::vector<CgState> hs(nact);
+  int issued = 0, rounds = 0;
   while (true) {
-    bool any_open = false, any_check = false;
-    for (int g = 0; g < nact; ++g) {
-      if (!finished[g]) any_open = true;
-      if (!finished[g] && hs[g].check_every) any_check = true;
-    }
-    if (!any_open) break;
-    int chunk = any_check ? 1 : std::max(1, std::min(hint - issued, 64));
-    if (issued >= hint) chunk = any_check ? 1 : 2;
-    for (int c = 0; c < chunk; ++c) {
+    if (++rounds > maxit + 8) throw Error(FAME6D_ENOCONV, "Mhat CG control loop did not terminate");
+    for (int c = 0; c < chunk; ++c, ++issued) {
+      timer_begin(m, TS_MHAT_PASS_A);
       launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
-      launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s);
+      timer_end(m, bytesA, issued);
+      timer_begin(m, TS_MHAT_PASS_B);
+      launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s, false, part);
+      timer_end(m, bytesB, issued);
+      timer_begin(m, TS_MHAT_PASS_C);
       launch_pass_c<T>(m, POST_MHAT, nact, pc, s);
-      k_cg_alpha<<<nact, kRedThreads, 0, s>>>(st, part, nbC);
-      k_cg_update<T><<<dim3(nbU, nact), kRedThreads, 0, s>>>(x, p, r, q, NF, NF, st, part2);
+      timer_end(m, bytesC, issued);
+      k_cg_alpha<<<nact, kRedThreads, 0, s>>>(st, part, nbE);
+      timer_begin(m, TS_CG_UPDATE);
+      launch_k(m, k_cg_update<T>, dim3(nbU, nact), dim3(kRedThreads), s, x, p, r, q, NF, NF, st, part2);
+      timer_end(m, bytesU, issued);
       if (precond) {
-        mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, r, part);
-        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU, part, nbC);
+        mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
+        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU, part, nbE);
       } else {
         k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU);
       }
       F6_CHECK_LAUNCH();
     }
-    issued += chunk;
-    read_state();
-    std::vector<int> need(nact, 0);
-    bool any_need = false;
-    for (int g = 0; g < nact; ++g) {
-      if (finished[g] || !hs[g].done) continue;
-      if (hs[g].flag == 0) {
-        need[g] = 1;
-        any_need = true;
-      } else {
-        finished[g] = 1;
-        out[g].flag = hs[g].flag;
-        out[g].iter = hs[g].iter;
-        out[g].relres = hs[g].normr / hs[g].n2b;
-      }
-    }
-    if (!any_need) continue;
-    // true residual r = b - Mhat x for the groups that stopped on the recurrence
-    std::vector<int> skiph(nact);
-    for (int g = 0; g < nact; ++g) skiph[g] = need[g] ? 0 : 1;
-    F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
-    Skip sk{mask, 1};
+    // true residual r = b - Mhat x of the groups that stopped on the recurrence
+    k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
+    const Skip sk{need, 1};
     mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
     k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
-    k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr);
+    k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr, sk);
+    k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, need, resume, precond ? 1 : 0);
+    if (precond) {
+      // z = P^{-1} r_true and rho = r'z for the groups that resume
+      mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{resume, 1}, s, true, part);
+      k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbE, resume);
+    }
     F6_CHECK_LAUNCH();
-    std::vector<double> rr(nact);
-    {
-      double* outd = part2 + (size_t)nbD * nact;
-      k_reduce_final<<<nact, kRedThreads, 0, s>>>(part2, nbD, outd);
-      F6_HIP(hipMemcpyAsync(rr.data(), outd, sizeof(double) * nact, hipMemcpyDeviceToHost, s));
-      F6_HIP(hipStreamSynchronize(s));
-    }
-    std::vector<int> resume(nact, 0);
-    bool any_resume = false;
+    d2h_sync(m, hs.data(), st, sizeof(CgState) * nact);
+    bool all = true, any_check = false;
     for (int g = 0; g < nact; ++g) {
-      if (!need[g]) continue;
-      ++out[g].checks;
-      const double nact_r = std::sqrt(rr[g]);
-      if (nact_r <= hs[g].tolb) {
-        finished[g] = 1;
-        out[g].flag = 0;
-        out[g].iter = hs[g].iter;
-        out[g].relres = nact_r / hs[g].n2b;
-      } else {
-        moresteps[g] += 1;
-        if (moresteps[g] >= maxmsteps || hs[g].iter >= maxit) {
-          finished[g] = 1;
-          out[g].flag = (moresteps[g] >= maxmsteps) ? 3 : 1;
-          out[g].iter = hs[g].iter;
-          out[g].relres = nact_r / hs[g].n2b;
-        } else {
-          resume[g] = 1;
-          any_resume = true;
-        }
-      }
+      if (hs[g].finished) continue;
+      all = false;
+      if (hs[g].check_every) any_check = true;
     }
-    if (any_resume) {
-      F6_HIP(hipMemcpyAsync(mask + nact, resume.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
-      if (precond) {
-        // z = P^{-1} r_true for the resumed groups; rho = r'z
-        std::vector<int> sk2(nact);
-        for (int g = 0; g < nact; ++g) sk2[g] = resume[g] ? 0 : 1;
-        F6_HIP(hipMemcpyAsync(mask, sk2.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
-        mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{mask, 1}, s, true, r, part);
-        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbC, mask + nact);
-      } else {
-        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, mask + nact);
-      }
-      F6_CHECK_LAUNCH();
-      read_state();
-    }
+    if (all) break;
+    chunk = any_check ? 1 : 2;
   }
-  int min_seen = 1 << 30;
-  for (int g = 0; g < nact; ++g) min_seen = std::min(min_seen, out[g].iter);
-  hint_ref = std::max(2, min_seen - 1);
+  out.assign(nact, PcgOut{});
+  int max_iter = 0;
+  for (int g = 0; g < nact; ++g) {
+    const CgState& c = hs[g];
+    out[g].flag = c.flag;
+    out[g].iter = c.iter;
+    out[g].checks = c.checks;
+    const double nr = c.checks > 0 ? c.normr_true : c.normr;
+    out[g].relres = c.n2b > 0 ? nr / c.n2b : (std::isnan(c.n2b) ? c.n2b : 0.0);
+    max_iter = std::max(max_iter, c.iter);
+  }
+  // iterations issued after the last group finished ran as skipped launches
+  m->timer.discard_tags(timer_first, max_iter);
+  hint_ref = std::max(2, max_iter + 1);
 }
 
 // ------------------------------------------------------------------ K_r^{-1}
@@ -443,7 +394,10 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   for (int v : qm)
     if (v < 0 || v >= S->G) throw Error(FAME6D_EARG, "bad q-system slot");
   if (nact > 64) throw Error(FAME6D_EARG, "at most 64 Bloch points per batched K_r^{-1}");
-  F6_HIP(hipMemcpyAsync(qmap, qm.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+  if (S->qmap_uploaded != qm) {
+    h2d_async(m, qmap, qm.data(), sizeof(int) * nact);
+    S->qmap_uploaded = qm;
+  }
 
   // FP64 workspace: r, mr, s (2NF each) + bhat, u, cr, cp, cq, res, y (NF each)
   S->kr64.ensure(sizeof(cpx<double>) * gn * 13);
@@ -517,7 +471,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       }
       F6_CHECK_LAUNCH();
       int* qmap_a = qmap + 128;
-      F6_HIP(hipMemcpyAsync(qmap_a, slots.data(), sizeof(int) * na, hipMemcpyHostToDevice, s));
+      h2d_async(m, qmap_a, slots.data(), sizeof(int) * na);
       std::vector<PcgOut> inner;
       pcg_mhat<float>(S, na, qmap_a, b32, x32, r32, p32, q32, z32, S->mixed_precond, tol_in, maxit, inner);
       for (int i = 0; i < na; ++i) {
@@ -535,7 +489,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       std::vector<int> skiph(nact);
       for (int g = 0; g < nact; ++g) skiph[g] = active[g] ? 0 : 1;
       int* mask = qmap + 192;
-      F6_HIP(hipMemcpyAsync(mask, skiph.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+      h2d_async(m, mask, skiph.data(), sizeof(int) * nact);
       Skip sk{mask, 1};
       mhat_apply<double>(S, nact, qmap, w.u, w.y, sk, s);
       k_residual<double><<<dim3(gridm, nact), 256, 0, s>>>(w.bhat, w.y, w.res, NF, NF, sk);
@@ -581,14 +535,13 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   S->stats.mass_applies_fp64 += 2 * nact;
   S->stats.calls += nact;
   if (outs) *outs = res;
-  F6_HIP(hipStreamSynchronize(s));
   S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
-                                 hipStream_t, bool, const cpx<double>*, double*);
+                                 hipStream_t, bool, double*);
 template void mhat_apply<float>(fame6d_qsys*, int, const int*, const cpx<float>*, cpx<float>*, Skip,
-                                hipStream_t, bool, const cpx<float>*, double*);
+                                hipStream_t, bool, double*);
 template void dot_groups<double>(const fame6d_mass*, int, const cpx<double>*, long long,
                                  const cpx<double>*, long long, long long, double*, double*);
 template void dot_groups<float>(const fame6d_mass*, int, const cpx<float>*, long long,

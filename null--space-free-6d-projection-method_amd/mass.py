@@ -150,6 +150,24 @@ class MassBackend:
     def set_tiles(self, precision, va, vb):
         call("fame6d_mass_set_tiles", self._h, precision_code(precision), va, vb)
 
+    def profile(self, stride=1):
+        """Time every `stride`-th launch of the timed kernel families with HIP events on
+        the library stream (stride 0 switches timing off); resets the sums."""
+        call("fame6d_mass_profile", self._h, int(stride))
+
+    def kernel_times(self):
+        """{family: {launches, ms, bytes, avg_us, GBs}} accumulated since profile()."""
+        from ._lib import KernelTime, NUM_TIMER_SLOTS, TIMER_SLOTS
+        arr = (KernelTime * NUM_TIMER_SLOTS)()
+        call("fame6d_mass_kernel_times", self._h, arr)
+        out = {}
+        for name, t in zip(TIMER_SLOTS, arr):
+            d = dict(launches=int(t.launches), ms=float(t.milliseconds), bytes=float(t.bytes))
+            d["avg_us"] = 1e3 * d["ms"] / d["launches"] if d["launches"] else None
+            d["GBs"] = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else None
+            out[name] = d
+        return out
+
     # -- applies ---------------------------------------------------------------------
     def _prep(self, x, rows):
         if not isinstance(x, torch.Tensor):

@@ -159,3 +159,26 @@ def test_batched_twelve_point_path(pkg, n2):
         out = pkg.run_method_C(s, k=4, p=20, tol=1e-10, max_restart=60, v0=v0[i])
         assert np.max(np.abs(outb["lambda_"][i] - out["lambda_"]) / out["lambda_"]) < 1e-11
         s.close()
+
+
+def test_kernel_family_timer(n2, pkg):
+    """The in-library HIP-event timer sees every timed family of a solve, with the stated
+    algorithmic bytes per launch (Mhat passes at N=2: NF*(6*16+3*8), NF*(6*16+8), NF*(4*16+3*8))."""
+    print("case: kernel timer N=2", flush=True)
+    m, s = n2["m"], n2["s"]
+    NF = m.NF
+    m.profile(1)
+    pkg.run_method_C(s, k=2, p=8, tol=1e-10, max_restart=50, tol_cg=1e-10, maxit_cg=3000,
+                     v0=pkg.start_vector(s.nr, 7), precision="double")
+    kt = m.kernel_times()
+    m.profile(0)
+    per = {"mhat_pass_a": NF * (6 * 16 + 3 * 8), "mhat_pass_b": NF * (6 * 16 + 8),
+           "mhat_pass_c": NF * (4 * 16 + 3 * 8), "cg_update": NF * 6 * 16}
+    for fam, b in per.items():
+        d = kt[fam]
+        assert d["launches"] > 0 and d["ms"] > 0, fam
+        assert abs(d["bytes"] / d["launches"] - b) < 1e-6 * b, fam
+    for fam in ("lanczos_gemv_c", "lanczos_gemv_n"):
+        assert kt[fam]["launches"] > 0 and kt[fam]["bytes"] > 0, fam
+    # skipped launches after convergence are discarded, so the three passes agree
+    assert kt["mhat_pass_a"]["launches"] == kt["mhat_pass_b"]["launches"] == kt["mhat_pass_c"]["launches"]
