@@ -207,7 +207,6 @@ FAMILY_KERNEL = {
     "mhat_pass_a": "k_pass_a3<double, %d",
     "mhat_pass_b": "k_pass_b<double, %d",
     "mhat_pass_c": "k_pass_c3<double, %d",
-    "cg_update": "k_cg_update<double>",
     "lanczos_gemv_c": "k_gemv_c_partials",
     "lanczos_gemv_n": "k_gemv_n_out",
 }
@@ -216,8 +215,8 @@ FAMILY_DESC = {
                    "NF*(6*16 + 3*8) bytes per launch",
     "mhat_pass_b": "Mhat pass B (inverse DFT dims 4-6, *eps/NF, forward DFT dims 6-4, p'q partials; 3 vectors): "
                    "NF*(6*16 + 8) bytes per launch",
-    "mhat_pass_c": "Mhat pass C (forward DFT dims 3-1, y = invC(s0 + W1 s1 + W2 s2)): NF*(4*16 + 3*8) bytes per launch",
-    "cg_update": "CG update x += alpha p, r -= alpha q, ||r||^2 partials: NF*6*16 bytes per launch",
+    "mhat_pass_c": "Mhat pass C in PCG (forward DFT dims 3-1, q = invC(s0 + W1 s1 + W2 s2) in registers, "
+                   "alpha, x += alpha p, r -= alpha q, ||r||^2 partials): NF*(8*16 + 3*8) bytes per launch",
     "lanczos_gemv_c": "Lanczos CGS2 sweep h = Q(:,1:j)^H w: (j+1)*2NF*16 bytes per sweep",
     "lanczos_gemv_n": "Lanczos CGS2 sweep r = w - Q(:,1:j) h (+||r||^2): (j+2)*2NF*16 bytes per sweep",
 }
@@ -397,8 +396,8 @@ def main():
                      "traffic": traffic["bytes"] if traffic else None,
                      "kernel": FAMILY_DESC[dom], "avg_launch_us": kd["avg_us"],
                      "algorithmic_bytes_per_launch": alg_per_launch,
-                     "traffic_source": traffic, "timing": "HIP events on the library stream around every "
-                     "4th launch of each kernel family during the timed steps"},
+                     "traffic_source": traffic, "timing": "start/stop HIP events of hipExtLaunchKernel on the "
+                     "library stream, every 16th launch of each kernel family during the timed steps"},
         "kernel_families": ktimes,
         "cpu_baseline": cpu,
         "extras": ext,

@@ -91,9 +91,9 @@ int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int var
  * `enable`-th launch of a timed family on the handle's stream is bracketed by HIP
  * events (enable = 0 switches timing off); the durations and the algorithmic HBM bytes
  * of the timed launches accumulate per slot until read.  Slots: 0 Mhat pass A in PCG,
- * 1 Mhat pass B, 2 Mhat pass C, 3 CG update, 4 Lanczos Q^H w sweeps, 5 Lanczos w - Q h
- * sweeps.  Every call of fame6d_mass_profile resets the sums. */
-#define FAME6D_NUM_TIMER_SLOTS 6
+ * 1 Mhat pass B, 2 Mhat pass C with the fused CG update, 3 Lanczos Q^H w sweeps,
+ * 4 Lanczos w - Q h sweeps.  Every call of fame6d_mass_profile resets the sums. */
+#define FAME6D_NUM_TIMER_SLOTS 5
 typedef struct {
   long long launches;
   double milliseconds;  /* summed kernel durations */

@@ -68,8 +68,8 @@ class KernelTime(C.Structure):
     _fields_ = [("launches", C.c_longlong), ("milliseconds", C.c_double), ("bytes", C.c_double)]
 
 
-NUM_TIMER_SLOTS = 6
-TIMER_SLOTS = ("mhat_pass_a", "mhat_pass_b", "mhat_pass_c", "cg_update", "lanczos_gemv_c", "lanczos_gemv_n")
+NUM_TIMER_SLOTS = 5
+TIMER_SLOTS = ("mhat_pass_a", "mhat_pass_b", "mhat_pass_c", "lanczos_gemv_c", "lanczos_gemv_n")
 
 P = C.c_void_p
 I = C.c_int

@@ -183,11 +183,10 @@ void h2d_async(fame6d_mass* m, void* dev, const void* host, size_t bytes);
 // Kernel-family timer scopes (no-ops unless profiling is enabled on m).
 enum TimerSlot : int {
   TS_MHAT_PASS_A = 0,  // Mhat pass A inside PCG (p = r + beta p fused)
-  TS_MHAT_PASS_B = 1,  // Mhat pass B (three vectors)
-  TS_MHAT_PASS_C = 2,  // Mhat pass C
-  TS_CG_UPDATE = 3,    // x += alpha p, r -= alpha q, ||r||^2 partials
-  TS_GEMV_C = 4,       // Lanczos Q^H w sweeps
-  TS_GEMV_N = 5,       // Lanczos w - Q h sweeps
+  TS_MHAT_PASS_B = 1,  // Mhat pass B (three vectors, p'q partials)
+  TS_MHAT_PASS_C = 2,  // Mhat pass C with alpha, x += alpha p, r -= alpha q, ||r||^2 partials
+  TS_GEMV_C = 3,       // Lanczos Q^H w sweeps
+  TS_GEMV_N = 4,       // Lanczos w - Q h sweeps
 };
 inline void timer_begin(const fame6d_mass* m, int slot) {
   if (m->timer.on) m->timer.begin(slot);

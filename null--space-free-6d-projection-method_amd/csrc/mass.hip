@@ -132,7 +132,7 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
-  if (op == POST_MHAT) {
+  if (op == POST_MHAT || op == POST_MHAT_CG) {
     const int v3 = tile_of<T>(m).a3;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
@@ -140,7 +140,8 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
         dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
-        launch_k(m, k_pass_c3<T, n, BPW>, grid, block, s, a);
+        if (op == POST_MHAT_CG) launch_k(m, k_pass_c3<T, n, BPW, true>, grid, block, s, a);
+        else launch_k(m, k_pass_c3<T, n, BPW, false>, grid, block, s, a);
       };
       if (v3 == 0) go(std::integral_constant<int, Tiles<n, T>::bpw3[0]>{});
       else go(std::integral_constant<int, Tiles<n, T>::bpw3[1]>{});

@@ -76,8 +76,9 @@ enum PreOp : int {
   PRE_MHAT_CG = 2,  // p = r + beta p (p = r first); then as PRE_MHAT on p
 };
 enum PostOp : int {
-  POST_COPY = 0,  // y = s
-  POST_MHAT = 1,  // y = invC*(s0 + W1 s1 + W2 s2)  [W real]
+  POST_COPY = 0,     // y = s
+  POST_MHAT = 1,     // y = invC*(s0 + W1 s1 + W2 s2)  [W real]
+  POST_MHAT_CG = 2,  // q = as POST_MHAT, not stored: x += alpha p, r -= alpha q (CG update)
 };
 
 template <class T>
@@ -123,6 +124,15 @@ struct PassCArgs {
   double* partials;  // partials[g*gridDim.x + blockIdx.x]
   Skip skip;
   const int* qmap;
+  // POST_MHAT_CG: alpha = rho / p'q with p'q the sum of npq pass-B energy partials per
+  // group; x, p, r of group g at base + g*ldx; partials receive ||r_new||^2.
+  CgState* cg;
+  const double* pq_partials;
+  int npq;
+  cpx<T>* x;
+  const cpx<T>* p;
+  cpx<T>* r;
+  long long ldx;
 };
 
 // XCD-aware block order: the dispatcher deals consecutive linear block ids to the 8 XCDs
@@ -647,7 +657,11 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
   for (int i = 0; i < n; ++i) tg[tl + TG * i] = R[tl + TG * i];
 }
 
-template <class T, int n, int BPW>
+// CG = true fuses the conjugate-gradient step of MATLAB pcg into the epilogue: every
+// workgroup reduces the pass-B partials of p'q in the same fixed order, so all agree on
+// alpha = rho/p'q (and on a breakdown), workgroup 0 records it, and each workgroup updates
+// x and r on its own modes; q itself never reaches HBM.
+template <class T, int n, int BPW, bool CG = false>
 __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
   constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
   constexpr int TG = BPW * P2;
@@ -695,16 +709,51 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
   __syncthreads();
   // combine y = invC*(s0 + W1 s1 + W2 s2) in linear mode order
   const long long qbase = qslot(a.qmap, g) * a.ldq + wg0;
-  cpx<T>* yg = a.y + g * a.ldy + wg0;
-  const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
   double s = 0.0;
-  for (int e = tid; e < E; e += THREADS) {
-    const long long q = qbase + e;
-    const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
-    yg[e] = v;
-    if (dg) {
-      const cpx<T> d = dg[e];
-      s += static_cast<double>(d.re * v.re + d.im * v.im);
+  if constexpr (CG) {
+    double pq = 0.0;
+    {
+      const double* pp = a.pq_partials + (long long)g * a.npq;
+      for (int i = tid; i < a.npq; i += THREADS) pq += pp[i];
+      pq = block_sum<THREADS>(pq, red);
+      if (tid == 0) red[0] = pq;
+      __syncthreads();
+      pq = red[0];
+      __syncthreads();  // red is reused below
+    }
+    const double alpha = a.cg[g].rho / pq;
+    const bool ok = (pq > 0.0) && !isinf(pq) && !isinf(alpha) && !isnan(alpha);
+    if (blockIdx.x == 0 && tid == 0) {
+      a.cg[g].pq = pq;
+      if (ok) {
+        a.cg[g].alpha = alpha;
+      } else {
+        a.cg[g].flag = 4;
+        a.cg[g].done = 1;
+      }
+    }
+    if (!ok) return;
+    const T al = static_cast<T>(alpha);
+    const long long xo = g * a.ldx + wg0;
+    for (int e = tid; e < E; e += THREADS) {
+      const long long q = qbase + e;
+      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      a.x[xo + e] = a.x[xo + e] + al * a.p[xo + e];
+      const cpx<T> rn = a.r[xo + e] - al * v;
+      a.r[xo + e] = rn;
+      s += static_cast<double>(rn.re) * rn.re + static_cast<double>(rn.im) * rn.im;
+    }
+  } else {
+    cpx<T>* yg = a.y + g * a.ldy + wg0;
+    const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
+    for (int e = tid; e < E; e += THREADS) {
+      const long long q = qbase + e;
+      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      yg[e] = v;
+      if (dg) {
+        const cpx<T> d = dg[e];
+        s += static_cast<double>(d.re * v.re + d.im * v.im);
+      }
     }
   }
   if (a.partials != nullptr) {

@@ -245,11 +245,11 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const long long total = NF * nact;
   constexpr bool f32 = std::is_same<T, float>::value;
   const int nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
-  const int nbU = red_blocks(NF);
+  const int nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
   const int nbD = red_blocks(NF);
   S->st.ensure(sizeof(CgState) * nact);
   S->part.ensure(sizeof(double) * (size_t)nbE * nact);
-  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbU, nbD) * nact + 8));
+  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
   S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
   CgState* st = S->st.as<CgState>();
   double* part = S->part.as<double>();
@@ -297,13 +297,20 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   pc.ldq = NF;
   pc.skip = skip_done(st);
   pc.qmap = qmap_dev;
+  pc.partials = part2;  // ||r_new||^2
+  pc.cg = st;
+  pc.pq_partials = part;
+  pc.npq = nbE;
+  pc.x = x;
+  pc.p = p;
+  pc.r = r;
+  pc.ldx = NF;
 
   // algorithmic HBM bytes per launch (cs: complex element, rs: real element)
   const double cs = sizeof(cpx<T>), rs = sizeof(T), gb = (double)NF * nact;
   const double bytesA = gb * (6 * cs + 3 * rs);  // zr, p, invC, W1, W2 in; p, 3 vectors out
   const double bytesB = gb * (6 * cs + rs);      // 3 vectors + eps in; 3 vectors out
-  const double bytesC = gb * (4 * cs + 3 * rs);  // 3 vectors + invC, W1, W2 in; q out
-  const double bytesU = gb * 6 * cs;             // x, p, r, q in; x, r out
+  const double bytesC = gb * (8 * cs + 3 * rs);  // 3 vectors, invC, W1, W2, x, p, r in; x, r out
   const int timer_first = m->timer.pending;
 
   int& hint_ref = f32 ? S->chunk_hint32 : S->chunk_hint;
@@ -320,17 +327,13 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s, false, part);
       timer_end(m, bytesB, issued);
       timer_begin(m, TS_MHAT_PASS_C);
-      launch_pass_c<T>(m, POST_MHAT, nact, pc, s);
+      launch_pass_c<T>(m, POST_MHAT_CG, nact, pc, s);  // alpha, x += alpha p, r -= alpha q
       timer_end(m, bytesC, issued);
-      k_cg_alpha<<<nact, kRedThreads, 0, s>>>(st, part, nbE);
-      timer_begin(m, TS_CG_UPDATE);
-      launch_k(m, k_cg_update<T>, dim3(nbU, nact), dim3(kRedThreads), s, x, p, r, q, NF, NF, st, part2);
-      timer_end(m, bytesU, issued);
       if (precond) {
         mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
-        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU, part, nbE);
+        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, part, nbE);
       } else {
-        k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbU);
+        k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC);
       }
       F6_CHECK_LAUNCH();
     }
