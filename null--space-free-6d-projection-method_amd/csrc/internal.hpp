@@ -185,8 +185,8 @@ enum TimerSlot : int {
   TS_MHAT_PASS_A = 0,  // Mhat pass A inside PCG (p = r + beta p fused)
   TS_MHAT_PASS_B = 1,  // Mhat pass B (three vectors, p'q partials)
   TS_MHAT_PASS_C = 2,  // Mhat pass C with alpha, x += alpha p, r -= alpha q, ||r||^2 partials
-  TS_GEMV_C = 3,       // Lanczos Q^H w sweeps
-  TS_GEMV_N = 4,       // Lanczos w - Q h sweeps
+  TS_GEMV_C = 3,       // Lanczos h = Q^H w sweeps
+  TS_GEMV_N = 4,       // Lanczos r = w - Q h sweeps (+||r||^2)
 };
 inline void timer_begin(const fame6d_mass* m, int slot) {
   if (m->timer.on) m->timer.begin(slot);
@@ -198,15 +198,20 @@ inline void timer_end(const fame6d_mass* m, double bytes, int tag = -1) {
 // kernel<<<grid, block, 0, s>>>(args...), with the arguments converted to the kernel's
 // parameter types; a launch inside an armed timer scope carries the timer's events.
 template <typename... KArgs, typename... Args>
-inline void launch_k(const fame6d_mass* m, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
-                     Args&&... args) {
+inline void launch_ks(const fame6d_mass* m, void (*kernel)(KArgs...), dim3 grid, dim3 block, unsigned shmem,
+                      hipStream_t s, Args&&... args) {
   if (m->timer.armed && !m->timer.fired) {
-    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, m->timer.ev0(), m->timer.ev1(), 0,
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, m->timer.ev0(), m->timer.ev1(), 0,
                           static_cast<KArgs>(args)...);
     m->timer.fired = true;
   } else {
-    kernel<<<grid, block, 0, s>>>(static_cast<KArgs>(args)...);
+    kernel<<<grid, block, shmem, s>>>(static_cast<KArgs>(args)...);
   }
+}
+template <typename... KArgs, typename... Args>
+inline void launch_k(const fame6d_mass* m, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                     Args&&... args) {
+  launch_ks(m, kernel, grid, block, 0u, s, static_cast<Args&&>(args)...);
 }
 
 // Per-solve results of a (batched) PCG on Mhat.
