@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter collection: a 64 MiB device copy (calibration), "
                          "then one FP64 solve of the benchmark configuration")
+    ap.add_argument("--tiles", default=None,
+                    help="FP64 tile variants 'ac,b' (fame6d_mass_set_tiles codes) instead of the autotuner; "
+                         "counter runs use the choice the timed run printed (its config.tile_variants)")
     ap.add_argument("--extras", action="store_true",
                     help="also time the batched 12-point Bloch path at N=4 and the Taylor reconstruction")
     return ap.parse_args()
@@ -202,13 +205,13 @@ def extras(torch, pkg):
                 batched_vs_sequential_max_rel_diff=maxdiff, taylor=rows)
 
 
-# kernel family (library timer slot) -> kernel-name prefix in rocprofv3 output
+# kernel family (library timer slot) -> kernel-name pattern in rocprofv3 output (%d: nGrid)
 FAMILY_KERNEL = {
-    "mhat_pass_a": "k_pass_a3<double, %d",
-    "mhat_pass_b": "k_pass_b<double, %d",
-    "mhat_pass_c": "k_pass_c3<double, %d",
-    "lanczos_gemv_c": "k_gemv_c_partials",
-    "lanczos_gemv_n": "k_gemv_n_out",
+    "mhat_pass_a": r"k_pass_a3<double, %d, \d+, 2>",
+    "mhat_pass_b": r"k_pass_b<double, %d, \d+, 3, 0>",
+    "mhat_pass_c": r"k_pass_c3<double, %d, \d+, true>",
+    "lanczos_gemv_c": r"k_gemv_c_all<\d+>",
+    "lanczos_gemv_n": r"k_gemv_n_out",
 }
 FAMILY_DESC = {
     "mhat_pass_a": "Mhat pass A in PCG (p = r + beta p; z, W1 z, W2 z; inverse DFT dims 1-3): "
@@ -223,9 +226,11 @@ FAMILY_DESC = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "method_c_N5_counters.md")
 
 
-def pmc_traffic(kernel_prefix):
+def pmc_traffic(pattern):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by its 64 MiB copy calibration), or None."""
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by its 64 MiB copy calibration; the
+    variant with the most dispatches when several tile variants match), or None."""
+    import re
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
@@ -233,10 +238,12 @@ def pmc_traffic(kernel_prefix):
     for line in text.splitlines():
         if line.startswith("traffic-json:"):
             tab = json.loads(line.split(":", 1)[1])
-            for name, row in tab["kernels"].items():
-                if name.startswith(kernel_prefix):
-                    return dict(bytes=row["fetch_kib"] * 1024 * tab["fetch_scale"] + row["write_kib"] * 1024 * tab["write_scale"],
-                                kernel=name, source=os.path.relpath(TRAFFIC_FILE, ROOT))
+            rows = [(name, row) for name, row in tab["kernels"].items() if re.fullmatch(pattern, name)]
+            if not rows:
+                return None
+            name, row = max(rows, key=lambda kv: kv[1]["dispatches"])
+            return dict(bytes=row["fetch_kib"] * 1024 * tab["fetch_scale"] + row["write_kib"] * 1024 * tab["write_scale"],
+                        kernel=name, source=os.path.relpath(TRAFFIC_FILE, ROOT))
     return None
 
 
@@ -272,12 +279,21 @@ def mixed_comparison(torch, pkg, N, amp, reps):
                 outer_steps=o64["stats"]["outer_steps"])
 
 
-def probe(torch, pkg, N):
+def tune(mass, tiles):
+    if tiles:
+        va, vb = (int(v) for v in tiles.split(","))
+        mass.set_tiles("double", va, vb)
+    else:
+        mass.autotune(reps=5)
+    return mass.info["tiles"]
+
+
+def probe(torch, pkg, N, tiles):
     """Fixed kernel sequence for counter runs: a 64 MiB device copy (calibration), then one
     FP64 solve of the benchmark configuration (the kernels and launch shapes of a step)."""
     eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
     mass = pkg.MassBackend(N, eps)
-    mass.autotune(reps=3)
+    tune(mass, tiles)
     sysq = pkg.QSystem(mass, Q)
     v0 = pkg.start_vector(sysq.nr, 20260905)
     src = torch.ones(64 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
@@ -307,12 +323,13 @@ def main():
     torch.cuda.set_device(0)
     N = a.N
     if a.probe:
-        probe(torch, pkg, N)
+        probe(torch, pkg, N, a.tiles)
         return
     NF = (2 * N) ** 6
     eps, phases = pkg.cosine_material(N, AMP, EPS_SEED)
     mass = pkg.MassBackend(N, eps)
-    tiles = mass.autotune(reps=5)
+    tiles = tune(mass, a.tiles)
+    variants = mass.info["variants"]
     sysq = pkg.QSystem(mass, Q)
     v0 = pkg.start_vector(sysq.nr, 20260905)
 
@@ -344,6 +361,8 @@ def main():
     kd = ktimes[dom]
     achieved = kd["GBs"]
     traffic = pmc_traffic(FAMILY_KERNEL[dom] % (2 * N) if "%d" in FAMILY_KERNEL[dom] else FAMILY_KERNEL[dom])
+    if traffic is not None and N != 5:
+        traffic = None  # the committed counter summary is for N = 5
     alg_per_launch = kd["bytes"] / kd["launches"]
     lam64 = outs[-1]["lambda_"]
     st64 = outs[-1]["stats"]
@@ -382,6 +401,8 @@ def main():
                         "q=(0.05,0.04,0.03), Method C Lanczos k=10 p=40 tol=1e-10, PCG(Mhat) tol=1e-10, FP64"
                         % (N, NF, 2 * NF, AMP),
             "N": N, "NF": NF, "eps_phases": [float(p) for p in phases], "lanczos": LANCZOS,
+            "tile_variants": {"double": "%d,%d" % tuple(variants["double"]),
+                              "single": "%d,%d" % tuple(variants["single"])},
             "parallelism": "single GPU",
         },
         "lambda_fp64": lam64.tolist(),

@@ -62,6 +62,8 @@ typedef struct {
   long long NF;
   double eps_min, eps_max, eps_mean;
   int tile_a64, tile_b64, tile_a32, tile_b32; /* autotuned tile parameters */
+  int variant_ac64, variant_b64;              /* fame6d_mass_set_tiles codes of the choice */
+  int variant_ac32, variant_b32;
 } fame6d_mass_info_t;
 
 /* build_mass_backend_large_scalar (exp1/build_mass_backend_large_scalar.m:1-49) with

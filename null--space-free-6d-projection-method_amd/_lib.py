@@ -29,7 +29,9 @@ class Fame6dError(RuntimeError):
 class MassInfo(C.Structure):
     _fields_ = [("N", C.c_int), ("nGrid", C.c_int), ("NF", C.c_longlong),
                 ("eps_min", C.c_double), ("eps_max", C.c_double), ("eps_mean", C.c_double),
-                ("tile_a64", C.c_int), ("tile_b64", C.c_int), ("tile_a32", C.c_int), ("tile_b32", C.c_int)]
+                ("tile_a64", C.c_int), ("tile_b64", C.c_int), ("tile_a32", C.c_int), ("tile_b32", C.c_int),
+                ("variant_ac64", C.c_int), ("variant_b64", C.c_int), ("variant_ac32", C.c_int),
+                ("variant_b32", C.c_int)]
 
 
 class QsysInfo(C.Structure):

@@ -139,7 +139,8 @@ class MassBackend:
         call("fame6d_mass_info", self._h, C.byref(i))
         return dict(name=self.name, N=i.N, nGrid=i.nGrid, NF=i.NF, M=i.NF, MOverNF=1.0,
                     epsMin=i.eps_min, epsMax=i.eps_max, epsMean=i.eps_mean,
-                    tiles=dict(a64=i.tile_a64, b64=i.tile_b64, a32=i.tile_a32, b32=i.tile_b32))
+                    tiles=dict(a64=i.tile_a64, b64=i.tile_b64, a32=i.tile_a32, b32=i.tile_b32),
+                    variants=dict(double=(i.variant_ac64, i.variant_b64), single=(i.variant_ac32, i.variant_b32)))
 
     def autotune(self, reps=5):
         """Project-written tile autotuner for both precisions."""
