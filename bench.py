@@ -260,8 +260,22 @@ def solve_timed(torch, pkg, sysq, v0, precision, reps=1):
     return best, out
 
 
+def vector_errors(torch, lam, Y64, Ymx, cluster_rel=1e-3):
+    """Per-mode eigenvector error of the mixed run against FP64: sin of the angle between
+    the mixed Ritz vector and the span of the FP64 Ritz vectors of its eigenvalue cluster
+    (modes within cluster_rel relative; the Bloch modes come in near-degenerate pairs whose
+    individual vectors are not determined).  Ritz vectors are orthonormal."""
+    G = (Y64.conj() @ Ymx.transpose(0, 1)).abs().pow(2).cpu().numpy()  # G[j, i] = |<y64_j, ymx_i>|^2
+    out = []
+    for i in range(len(lam)):
+        cl = [j for j in range(len(lam)) if abs(lam[j] - lam[i]) <= cluster_rel * abs(lam[i])]
+        out.append(float(np.sqrt(max(0.0, 1.0 - G[cl, i].sum()))))
+    return out
+
+
 def mixed_comparison(torch, pkg, N, amp, reps):
-    """Mixed vs FP64 on the amplitude-0.8 configuration: per-mode relative differences."""
+    """Mixed vs FP64 on the amplitude-0.8 configuration: per-mode relative eigenvalue
+    differences and eigenvector angles."""
     eps, _ = pkg.cosine_material(N, amp, EPS_SEED)
     mass = pkg.MassBackend(N, eps)
     mass.autotune(reps=5)
@@ -271,10 +285,16 @@ def mixed_comparison(torch, pkg, N, amp, reps):
     t64, o64 = solve_timed(torch, pkg, sysq, v0, "double", reps)
     tmx, omx = solve_timed(torch, pkg, sysq, v0, "mixed", reps)
     rel = (np.abs(omx["lambda_"] - o64["lambda_"]) / np.abs(o64["lambda_"])).tolist()
+    # Ritz vectors of both forms (untimed repeats of the same solves)
+    v64 = pkg.run_method_C(sysq, v0=v0, precision="double", want_vectors=True, **LANCZOS)
+    vmx = pkg.run_method_C(sysq, v0=v0, precision="mixed", want_vectors=True, **LANCZOS)
+    vec = vector_errors(torch, v64["lambda_"], v64["Y"], vmx["Y"])
+    del v64, vmx
     sysq.close()
     mass.close()
     return dict(N=N, amp=amp, fp64_seconds=t64, mixed_seconds=tmx, speedup=t64 / tmx,
-                rel_err_per_mode=rel, max_rel_err=max(rel), lambda_fp64=o64["lambda_"].tolist(),
+                rel_err_per_mode=rel, max_rel_err=max(rel), vec_sin_per_mode=vec, max_vec_sin=max(vec),
+                lambda_fp64=o64["lambda_"].tolist(),
                 fp64_krinv=o64["stats"]["krinv"], mixed_krinv=omx["stats"]["krinv"],
                 outer_steps=o64["stats"]["outer_steps"])
 
