@@ -407,7 +407,7 @@ int fame6d_qsys_destroy(fame6d_qsys* s) {
   if (!s) return FAME6D_OK;
   (void)hipStreamSynchronize(s->m->stream);
   for (DevBuf* b : {&s->Vd, &s->Wd, &s->Wf, &s->Qd, &s->kr64, &s->kr32, &s->st, &s->part, &s->part2,
-                    &s->ints, &s->sc})
+                    &s->ints, &s->sc, &s->ref})
     b->release();
   delete s;
   F6_API_END

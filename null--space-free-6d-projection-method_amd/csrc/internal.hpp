@@ -236,11 +236,12 @@ struct fame6d_qsys {
   // diagonal blocks, each G*NF: V11 V12 V21 V22 V31 V32 | W1 W2 invC | q1 q2 q3
   fame6d::DevBuf Vd, Wd, Wf, Qd;
   // Method-C workspaces (grown to the number of active groups)
-  fame6d::DevBuf kr64, kr32, st, part, part2, ints, sc;
+  fame6d::DevBuf kr64, kr32, st, part, part2, ints, sc, ref;
   double mixed_inner_tol = 3e-5;
   bool mixed_precond = true;  // FP32 inner PCG preconditioned by Mhat evaluated with 1/eps
   int mixed_max_cycles = 30;
-  int chunk_hint = 0, chunk_hint32 = 0;
+  int chunk_hint = 0;                 // first chunk of the FP64 PCG (iterations of the last call + 1)
+  std::vector<int> chunk_hint32 = std::vector<int>(4, 0);  // per refinement cycle of the mixed form
   std::vector<int> qmap_uploaded;  // q-slot map last copied to sc (skips repeated uploads)
   fame6d_krinv_stats_t stats{};
   const double* V(int i) const { return Vd.as<double>() + (size_t)i * G * NF; }

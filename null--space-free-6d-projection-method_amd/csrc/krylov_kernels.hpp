@@ -400,6 +400,88 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_resume(CgState* st, c
   }
 }
 
+// A solve whose recurrence stopped with flag 0 is accepted without the true-residual check
+// (inner solves of the mixed form, whose residual the FP64 refinement measures itself).
+static __global__ void k_cg_accept(CgState* st, int G) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
+    if (st[g].done) st[g].finished = 1;
+}
+
+// ------------------------------------------------------------------ FP64 refinement (mixed form)
+// Per-group state of M u = b solved by FP64 iterative refinement around FP32 inner solves.
+struct RefState {
+  double nb;      // ||b||
+  double nr;      // ||b - M u|| of the current iterate
+  int inactive;   // 1: converged (or ||b|| = 0); the group's refinement kernels are skipped
+  int nonfinite;  // 1: NaN/Inf met in ||b|| or a residual
+  int pad[2];
+};
+static_assert(sizeof(RefState) == 32, "RefState layout");
+__host__ inline Skip skip_inactive(const RefState* rs) {
+  return Skip{reinterpret_cast<const int*>(reinterpret_cast<const char*>(rs) + offsetof(RefState, inactive)),
+              (int)(sizeof(RefState) / sizeof(int))};
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_ref_init(RefState* rs, const double* partials, int nb) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  const double s = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    RefState r{};
+    r.nb = sqrt(s);
+    r.nr = r.nb;
+    r.nonfinite = isfinite(s) ? 0 : 1;
+    r.inactive = (r.nb > 0.0 && !r.nonfinite) ? 0 : 1;
+    rs[g] = r;
+  }
+}
+
+// After a refinement cycle: ||r||^2 partials -> nr; converged when nr <= tol*nb.
+static __global__ __launch_bounds__(kRedThreads) void k_ref_check(RefState* rs, const double* partials, int nb,
+                                                                  double tol) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (rs[g].inactive) return;
+  const double s = sum_partials(partials + (long long)g * nb, nb, red);
+  if (threadIdx.x == 0) {
+    const double nr = sqrt(s);
+    rs[g].nr = nr;
+    if (!isfinite(s)) {
+      rs[g].nonfinite = 1;
+      rs[g].inactive = 1;
+    } else if (nr <= tol * rs[g].nb) {
+      rs[g].inactive = 1;
+    }
+  }
+}
+
+// y = x / nr for the groups still refining, 0 for the others (grid.y = group).
+template <class To, class Ti>
+__global__ void k_convert_ref(long long n, const cpx<Ti>* x, long long ldx, const RefState* rs, cpx<To>* y,
+                              long long ldy) {
+  const int g = blockIdx.y;
+  const bool on = !rs[g].inactive;
+  const double scale = on ? 1.0 / rs[g].nr : 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cpx<Ti> v = on ? x[g * ldx + i] : mk<Ti>(0, 0);
+    y[g * ldy + i] = mk(static_cast<To>(scale * v.re), static_cast<To>(scale * v.im));
+  }
+}
+
+// acc += nr * x for the groups still refining (grid.y = group).
+template <class Ti>
+__global__ void k_accumulate_ref(long long n, const cpx<Ti>* x, long long ldx, const RefState* rs,
+                                 cpx<double>* acc, long long lda) {
+  const int g = blockIdx.y;
+  if (rs[g].inactive) return;
+  const double scale = rs[g].nr;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    acc[g * lda + i] = acc[g * lda + i] + scale * cvt<double>(x[g * ldx + i]);
+  }
+}
+
 // ------------------------------------------------------------------ vectors
 template <class T>
 __global__ void k_axpby(long long n, cpx<double> a, const cpx<T>* x, cpx<double> b,

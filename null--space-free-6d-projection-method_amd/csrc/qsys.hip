@@ -235,10 +235,13 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 // With `precond` the iteration is preconditioned by P^{-1} = Mhat evaluated with
 // 1/eps (exact inverse for constant eps); the reference's solves are unpreconditioned,
 // and this build uses `precond` only for the FP32 inner solves of the mixed form.
+// `true_check` false accepts the recurrence residual without the true-residual check (for
+// the inner solves of the mixed form).  `hint` is the first chunk length, updated to the
+// iterations this call needed plus one.
 template <class T>
 static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
                      cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
-                     std::vector<PcgOut>& out) {
+                     bool true_check, int& hint_ref, std::vector<PcgOut>& out) {
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const long long NF = m->NF;
@@ -313,7 +316,6 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const double bytesC = gb * (8 * cs + 3 * rs);  // 3 vectors, invC, W1, W2, x, p, r in; x, r out
   const int timer_first = m->timer.pending;
 
-  int& hint_ref = f32 ? S->chunk_hint32 : S->chunk_hint;
   int chunk = hint_ref > 0 ? hint_ref : 8;
   std::vector<CgState> hs(nact);
   int issued = 0, rounds = 0;
@@ -337,17 +339,21 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       }
       F6_CHECK_LAUNCH();
     }
-    // true residual r = b - Mhat x of the groups that stopped on the recurrence
-    k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
-    const Skip sk{need, 1};
-    mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
-    k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
-    k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr, sk);
-    k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, need, resume, precond ? 1 : 0);
-    if (precond) {
-      // z = P^{-1} r_true and rho = r'z for the groups that resume
-      mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{resume, 1}, s, true, part);
-      k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbE, resume);
+    if (true_check) {
+      // true residual r = b - Mhat x of the groups that stopped on the recurrence
+      k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
+      const Skip sk{need, 1};
+      mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
+      k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
+      k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr, sk);
+      k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, need, resume, precond ? 1 : 0);
+      if (precond) {
+        // z = P^{-1} r_true and rho = r'z for the groups that resume
+        mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{resume, 1}, s, true, part);
+        k_cg_resume<<<nact, kRedThreads, 0, s>>>(st, part, nbE, resume);
+      }
+    } else {
+      k_cg_accept<<<1, 64, 0, s>>>(st, nact);
     }
     F6_CHECK_LAUNCH();
     d2h_sync(m, hs.data(), st, sizeof(CgState) * nact);
@@ -423,14 +429,17 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
 
   std::vector<PcgOut> res;
   if (precision == FAME6D_FP64) {
-    pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, res);
+    pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, true,
+                     S->chunk_hint, res);
     for (int g = 0; g < nact; ++g) {
       S->stats.total_iterations += res[g].iter;
       S->stats.total_fp64_iterations += res[g].iter;
       S->stats.mass_applies_fp64 += 3LL * (res[g].iter + res[g].checks);
     }
   } else if (precision == FAME6D_MIXED) {
-    // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.
+    // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.  The refinement
+    // state (||b||, ||r||, converged) lives on the device; the host reads it once the
+    // cycles that cannot have converged yet are done (after every cycle from then on).
     S->kr32.ensure(sizeof(cpx<float>) * gn * 6);
     cpx<float>* b32 = S->kr32.as<cpx<float>>();
     cpx<float>* x32 = b32 + gn;
@@ -438,78 +447,66 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     cpx<float>* p32 = b32 + 3 * gn;
     cpx<float>* q32 = b32 + 4 * gn;
     cpx<float>* z32 = b32 + 5 * gn;
+    S->ref.ensure(sizeof(RefState) * nact);
+    RefState* rs = S->ref.as<RefState>();
+    const int nbD = red_blocks(NF);
+    S->part2.ensure(sizeof(double) * ((size_t)nbD * nact + 8));
+    double* part2 = S->part2.as<double>();
     k_fill_zero<double><<<grid_for(gn), 256, 0, s>>>((long long)gn, w.u);
     F6_HIP(hipMemcpyAsync(w.res, w.bhat, sizeof(cpx<double>) * gn, hipMemcpyDeviceToDevice, s));
-    std::vector<double> nb(nact), nr(nact);
-    dot_groups<double>(m, nact, w.bhat, NF, w.bhat, NF, NF, nb.data(), nullptr);
-    for (int g = 0; g < nact; ++g) {
-      nb[g] = std::sqrt(nb[g]);
-      nr[g] = nb[g];
-      if (!std::isfinite(nb[g])) throw Error(FAME6D_ENONFINITE, "Mhat right-hand side contains NaN/Inf.");
-    }
+    k_dot_partials<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, NF, w.bhat, NF, NF, part2, nullptr);
+    k_ref_init<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD);
+    F6_CHECK_LAUNCH();
     res.assign(nact, PcgOut{});
-    std::vector<int> active(nact);
-    for (int g = 0; g < nact; ++g) active[g] = (nb[g] > 0) ? 1 : 0;
     const double tol_in = S->mixed_inner_tol;
+    // cycles before the first possible convergence: each reduces the residual by about tol_in
+    const int blind = (tol_in > tol && tol_in < 1.0)
+                          ? std::max(1, (int)std::floor(std::log(tol) / std::log(tol_in)))
+                          : 1;
+    const Skip skr = skip_inactive(rs);
+    std::vector<RefState> hr(nact);
     int cycles = 0;
     while (true) {
-      bool any = false;
-      for (int g = 0; g < nact; ++g) any = any || active[g];
-      if (!any) break;
       if (cycles >= S->mixed_max_cycles)
         throw Error(FAME6D_ENOCONV, "Mhat iterative refinement did not converge");
-      ++cycles;
-      // pack active groups: b32 = res / ||res||
-      std::vector<int> act, slots;
-      for (int g = 0; g < nact; ++g)
-        if (active[g]) {
-          act.push_back(g);
-          slots.push_back(qm[g]);
-        }
-      const int na = (int)act.size();
-      for (int i = 0; i < na; ++i) {
-        const int g = act[i];
-        k_convert<float, double><<<gridm, 256, 0, s>>>(NF, w.res + (size_t)g * NF, 1.0 / nr[g],
-                                                      b32 + (size_t)i * NF);
-      }
+      // b32 = r / ||r|| (0 for converged groups, whose inner solve then stops at once)
+      k_convert_ref<float, double><<<dim3(gridm, nact), 256, 0, s>>>(NF, w.res, NF, rs, b32, NF);
       F6_CHECK_LAUNCH();
-      int* qmap_a = qmap + 128;
-      h2d_async(m, qmap_a, slots.data(), sizeof(int) * na);
       std::vector<PcgOut> inner;
-      pcg_mhat<float>(S, na, qmap_a, b32, x32, r32, p32, q32, z32, S->mixed_precond, tol_in, maxit, inner);
-      for (int i = 0; i < na; ++i) {
-        const int g = act[i];
-        if (inner[i].flag != 0 && inner[i].flag != 1 && inner[i].flag != 3)
-          throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(inner[i].flag) + ")");
-        S->stats.total_iterations += inner[i].iter;
-        S->stats.mass_applies_fp32 +=
-            3LL * ((S->mixed_precond ? 2 : 1) * inner[i].iter + inner[i].checks + (S->mixed_precond ? 1 : 0));
-        res[g].iter += inner[i].iter;
-        k_accumulate<float><<<gridm, 256, 0, s>>>(NF, x32 + (size_t)i * NF, nr[g], w.u + (size_t)g * NF);
-      }
-      F6_CHECK_LAUNCH();
-      // FP64 residual of every group still refining
-      std::vector<int> skiph(nact);
-      for (int g = 0; g < nact; ++g) skiph[g] = active[g] ? 0 : 1;
-      int* mask = qmap + 192;
-      h2d_async(m, mask, skiph.data(), sizeof(int) * nact);
-      Skip sk{mask, 1};
-      mhat_apply<double>(S, nact, qmap, w.u, w.y, sk, s);
-      k_residual<double><<<dim3(gridm, nact), 256, 0, s>>>(w.bhat, w.y, w.res, NF, NF, sk);
-      F6_CHECK_LAUNCH();
-      S->stats.mass_applies_fp64 += 3LL * na;
-      std::vector<double> rr(nact);
-      dot_groups<double>(m, nact, w.res, NF, w.res, NF, NF, rr.data(), nullptr);
+      int& hint = S->chunk_hint32[std::min(cycles, (int)S->chunk_hint32.size() - 1)];
+      pcg_mhat<float>(S, nact, qmap, b32, x32, r32, p32, q32, z32, S->mixed_precond, tol_in, maxit, false,
+                      hint, inner);
+      ++cycles;
       for (int g = 0; g < nact; ++g) {
-        if (!active[g]) continue;
-        nr[g] = std::sqrt(rr[g]);
-        if (!std::isfinite(nr[g])) throw Error(FAME6D_ENONFINITE, "Mhat refinement residual is NaN/Inf.");
-        if (nr[g] <= tol * nb[g]) {
-          active[g] = 0;
-          res[g].flag = 0;
-          res[g].relres = nr[g] / nb[g];
-        }
+        if (inner[g].flag != 0 && inner[g].flag != 1 && inner[g].flag != 3)
+          throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(inner[g].flag) + ")");
+        S->stats.total_iterations += inner[g].iter;
+        if (inner[g].iter > 0)
+          S->stats.mass_applies_fp32 += 3LL * ((S->mixed_precond ? 2 : 1) * inner[g].iter + inner[g].checks +
+                                               (S->mixed_precond ? 1 : 0));
+        res[g].iter += inner[g].iter;
       }
+      k_accumulate_ref<float><<<dim3(gridm, nact), 256, 0, s>>>(NF, x32, NF, rs, w.u, NF);
+      // FP64 residual of every group still refining
+      mhat_apply<double>(S, nact, qmap, w.u, w.y, skr, s);
+      k_residual<double><<<dim3(gridm, nact), 256, 0, s>>>(w.bhat, w.y, w.res, NF, NF, skr);
+      k_dot_partials<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.res, NF, w.res, NF, NF, part2, nullptr, skr);
+      k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD, tol);
+      F6_CHECK_LAUNCH();
+      for (int g = 0; g < nact; ++g)
+        if (inner[g].iter > 0) S->stats.mass_applies_fp64 += 3;
+      if (cycles < blind) continue;
+      d2h_sync(m, hr.data(), rs, sizeof(RefState) * nact);
+      bool any = false;
+      for (int g = 0; g < nact; ++g) {
+        if (hr[g].nonfinite) throw Error(FAME6D_ENONFINITE, "Mhat refinement residual is NaN/Inf.");
+        if (!hr[g].inactive) any = true;
+      }
+      if (!any) break;
+    }
+    for (int g = 0; g < nact; ++g) {
+      res[g].flag = 0;
+      res[g].relres = hr[g].nb > 0 ? hr[g].nr / hr[g].nb : 0.0;
     }
     S->stats.refinement_cycles += cycles;
   } else {
