@@ -217,6 +217,32 @@ int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_resta
                          double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
                          double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
 
+/* Method A statistics (make_A_eigs_top_preconditioned.m:53-54: totalMINRESIterations etc.). */
+typedef struct {
+  long long solves;      /* shifted systems solved (topVectorApplications) */
+  long long iterations;  /* MINRES iterations over all segments */
+  long long segments;    /* MINRES segments (totalMINRESCalls) */
+  long long restarts;    /* segments started from the current iterate after a true-residual miss */
+  int max_iterations;
+  int plateau_accepts;   /* solves accepted at the 1e-9 plateau ceiling */
+  double max_relres;     /* largest true relative residual accepted */
+  double seconds;
+} fame6d_minres_stats_t;
+/* Method A Top (make_A_eigs_top_preconditioned.m:73-113): y = (K - sigma M)^{-1} M x by
+ * preconditioned MINRES with P = K + tau I, tau = sigma * mean(eps)
+ * (build_original_shift_preconditioner_A_only.m:1-56), true relative residual <= tol.
+ * x, y: 3NF complex128 (component-first). */
+int fame6d_method_a_apply(fame6d_qsys* s, double sigma, double tol, int maxit, const void* x, void* y,
+                          fame6d_minres_stats_t* stats);
+/* run_method_A_fig1a (exp1/run_method_A_fig1a.m:6-50): the k largest theta of
+ * T = (K - sigma M)^{-1} M, lambda = sigma + 1/theta ascending.  The outer solver is
+ * thick-restart Lanczos in the M inner product (T is M-self-adjoint) where the reference
+ * calls eigs; k, p, tol, max_restart as cfg.eigs (make_config_exp1bc.m:36-47).
+ * v0_host: 3NF start vector; Y (device, 3NF x k) or NULL. */
+int fame6d_eigs_method_a(fame6d_qsys* s, int k, int p, double tol, int max_restart, double sigma,
+                         double tol_minres, int maxit_minres, const void* v0_host, double* lambda_host, void* Y,
+                         fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats);
+
 typedef struct {
   int iterations[64];
   double relres[64];

@@ -10,7 +10,7 @@ from ._lib import FP32, FP64, MIXED, Fame6dError, load  # noqa: F401
 from .mass import (MassBackend, P_DEFAULT, build_mass_backend_large_scalar, cosine_material,  # noqa: F401
                    smooth_scalar_material, start_vector)
 from .eigs import (QSystem, build_experiment1_q_system_C_compact, compute_fourier_diagnostics,  # noqa: F401
-                   krinv_nested, make_explicit_krinv_compact, recover_original_eigenvectors,
-                   run_method_B, run_method_C, run_method_C_batch)
+                   krinv_nested, make_explicit_krinv_compact, method_a_top, recover_original_eigenvectors,
+                   run_method_A, run_method_B, run_method_C, run_method_C_batch)
 from .taylor import (block_partition, build_multi_index_list_3d, direct_reconstruct,  # noqa: F401
                      taylor_evaluate, taylor_moments, yee_points)

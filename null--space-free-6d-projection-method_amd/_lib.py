@@ -66,6 +66,15 @@ class RecoverStats(C.Structure):
     _fields_ = [("iterations", C.c_int * 64), ("relres", C.c_double * 64), ("mass_norm2", C.c_double * 64)]
 
 
+class MinresStats(C.Structure):
+    _fields_ = [("solves", C.c_longlong), ("iterations", C.c_longlong), ("segments", C.c_longlong),
+                ("restarts", C.c_longlong), ("max_iterations", C.c_int), ("plateau_accepts", C.c_int),
+                ("max_relres", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class KernelTime(C.Structure):
     _fields_ = [("launches", C.c_longlong), ("milliseconds", C.c_double), ("bytes", C.c_double)]
 
@@ -107,6 +116,8 @@ SIGNATURES = {
     "fame6d_eigs_method_c_batch": [P, I, P, P, I, I, I, D, I, D, I, P, P, P, C.POINTER(LanczosStats)],
     "fame6d_krinv_nested_apply": [P, P, P, D, D, I, I, D, C.POINTER(KrinvStats)],
     "fame6d_eigs_method_b": [P, I, I, D, I, D, D, I, I, D, P, P, P, C.POINTER(LanczosStats)],
+    "fame6d_method_a_apply": [P, D, D, I, P, P, C.POINTER(MinresStats)],
+    "fame6d_eigs_method_a": [P, I, I, D, I, D, D, I, P, P, P, C.POINTER(LanczosStats), C.POINTER(MinresStats)],
     "fame6d_recover_eigenvectors": [P, P, I, D, I, P, P, C.POINTER(RecoverStats)],
     "fame6d_fourier_diagnostics": [P, P, P, P, I, P, P, P],
     "fame6d_taylor_moments": [LL, P, P, I, I, P, P, P],

@@ -18,6 +18,11 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
 void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, double tolKr, double tolM,
                   int maxitKr, int maxitM, double epsC, fame6d_krinv_stats_t* st);
 void set_nested(bool on, double tolM, int maxitM, double epsC);
+void method_a_apply(fame6d_qsys* S, double sigma, double tol, int maxit, const cpx<double>* x, cpx<double>* y,
+                    fame6d_minres_stats_t* stats);
+void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, double sigma, double tol_mr,
+                   int maxit_mr, const cpx<double>* v0, double* lambda, cpx<double>* Y,
+                   fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats);
 void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double tol, int maxit,
                           cpx<double>* X, cpx<double>* MX, fame6d_recover_stats_t* stats);
 void fourier_diagnostics(fame6d_qsys* S, const cpx<double>* X, const cpx<double>* MX,
@@ -545,6 +550,28 @@ int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_resta
     throw;
   }
   set_nested(false, 0, 0, 0);
+  F6_API_END
+}
+
+int fame6d_method_a_apply(fame6d_qsys* s, double sigma, double tol, int maxit, const void* x, void* y,
+                          fame6d_minres_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && x && y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
+  F6_REQUIRE(sigma > 0 && tol > 0 && maxit > 0, FAME6D_EARG, "sigma, tol and maxit must be positive");
+  method_a_apply(s, sigma, tol, maxit, (const cpx<double>*)x, (cpx<double>*)y, stats);
+  F6_API_END
+}
+
+int fame6d_eigs_method_a(fame6d_qsys* s, int k, int p, double tol, int max_restart, double sigma,
+                         double tol_minres, int maxit_minres, const void* v0_host, double* lambda_host, void* Y,
+                         fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
+  F6_REQUIRE(sigma > 0 && tol_minres > 0 && maxit_minres > 0, FAME6D_EARG, "bad Method-A parameters");
+  eigs_method_a(s, k, p, tol, max_restart, sigma, tol_minres, maxit_minres, (const cpx<double>*)v0_host,
+                lambda_host, (cpx<double>*)Y, stats, mstats);
   F6_API_END
 }
 

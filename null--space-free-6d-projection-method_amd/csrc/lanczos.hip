@@ -21,6 +21,7 @@
 
 #include "internal.hpp"
 #include "krylov_kernels.hpp"
+#include "minres_kernels.hpp"
 
 namespace fame6d {
 
@@ -178,6 +179,7 @@ struct LzState {
   int nconv = 0;
   double maxres = 0;
   std::vector<double> theta;
+  cpx<double>* Z = nullptr;  // M-weighted basis Z = M Q (Method A), restarted with Q
 };
 
 // Builds H, solves it, tests convergence and restarts (Lanczos_hybrid_identity_gpuop.m:79-120).
@@ -250,6 +252,7 @@ static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, 
   F6_HIP(hipMemcpyAsync(ubuf.ptr, Uk.data(), sizeof(double) * Uk.size(), hipMemcpyHostToDevice, s));
   const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
   k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n2, n2, m, k, ubuf.as<double>());
+  if (L.Z) k_restart<<<blocks, 256, 0, s>>>(L.Z, n2, n2, m, k, ubuf.as<double>());
   F6_CHECK_LAUNCH();
   F6_HIP(hipStreamSynchronize(s));  // ubuf is reused by the next restart
   L.d.resize(k);
@@ -732,6 +735,340 @@ void fourier_diagnostics(fame6d_qsys* S, const cpx<double>* X, const cpx<double>
     if (div) div[j] = std::sqrt(nd) / std::max(nM, 2.220446049250313e-16);
   }
   ws.release();
+}
+
+// ================================================================== Method A
+// Shift-invert on the full 3NF Fourier space: T x = (K - sigma M)^{-1} M x, whose largest
+// eigenvalues theta give lambda = sigma + 1/theta (make_A_eigs_top_preconditioned.m:1-60,
+// run_method_A_fig1a.m:29-50).  T is self-adjoint in the M inner product, so the outer
+// solver is the thick-restart Lanczos above run in that inner product, with Z = M Q kept
+// beside the basis (the M q of each step is the right-hand side of its shifted solve); the
+// reference hands T to MATLAB eigs (Krylov-Schur) instead.  Each shifted system is solved by
+// preconditioned MINRES with P = K + tau I, tau = sigma * mean(eps).
+
+// y = M x for the three components of a 3NF vector (skip: stride 0 over the components).
+static void mass3(fame6d_mass* m, const cpx<double>* x, cpx<double>* y, Skip skip, hipStream_t s) {
+  const long long NF = m->NF;
+  DevBuf& w = work_buf<double>(m);
+  w.ensure(sizeof(cpx<double>) * (size_t)NF * 3);
+  cpx<double>* t = w.as<cpx<double>>();
+  PassAArgs<double> a{};
+  a.x = x;
+  a.ldx = NF;
+  a.t = t;
+  a.ldt = NF;
+  a.skip = skip;
+  launch_pass_a<double>(m, PRE_COPY, 3, a, s);
+  launch_pass_b<double>(m, 1, 0, 3, t, NF, skip, s);
+  PassCArgs<double> c{};
+  c.t = t;
+  c.ldt = NF;
+  c.y = y;
+  c.ldy = NF;
+  c.skip = skip;
+  launch_pass_c<double>(m, POST_COPY, 3, c, s);
+}
+
+// Workspace of one MINRES solve (3NF complex vectors) and its device scalars.
+struct MrWs {
+  cpx<double>*x, *v, *vn, *Mv, *r1, *r2, *y, *z, *w, *w1, *w2, *res;
+  MrState* st;
+  double* part;
+  double* red;  // reduced scalars
+};
+
+struct MrAcc {
+  long long solves = 0, iterations = 0, segments = 0, restarts = 0;
+  int max_iterations = 0, plateau_accepts = 0;
+  double max_relres = 0.0, seconds = 0.0;
+};
+
+// sum over the partials, read to the host (one synchronisation)
+static double reduce_read(fame6d_mass* m, const double* part, int nb, double* red) {
+  k_reduce_final<<<1, kRedThreads, 0, m->stream>>>(part, nb, red);
+  F6_CHECK_LAUNCH();
+  double h = 0.0;
+  d2h_sync(m, &h, red, sizeof(double));
+  return h;
+}
+
+// Solves (K - sigma M) x = b by preconditioned MINRES segments (make_A_eigs_top_preconditioned.m:212-330):
+// a segment runs the Paige-Saunders recurrence until its preconditioned residual estimate
+// meets the segment tolerance, then the true residual ||b - A x||/||b|| is checked.  A miss
+// starts a new segment from the current iterate with the tolerance the miss calls for; a
+// residual that stops decreasing is accepted at the reference's plateau ceiling 1e-9 and
+// otherwise ends the solve with FAME6D_ENOCONV after 50 recovery events (the reference's
+// guard, make_config_exp1bc.m:50,61-72).  Not restated: the residual-correction tolerance
+// schedule of the reference's recovery path.
+static void minres_solve(fame6d_qsys* S, double sigma, double tau, const cpx<double>* b, double tol, int maxit,
+                         MrWs& W, int& hint, MrAcc& acc) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long NF = m->NF, n3 = 3 * NF;
+  const double *q1 = S->Qm(0), *q2 = S->Qm(1), *q3 = S->Qm(2);
+  const int nb = red_blocks(NF);
+  const int gr = grid_for(n3);
+  const auto t0 = std::chrono::steady_clock::now();
+  k_dot_partials<double><<<dim3(red_blocks(n3), 1), kRedThreads, 0, s>>>(b, n3, b, n3, n3, W.part, nullptr);
+  const double bn = std::sqrt(reduce_read(m, W.part, red_blocks(n3), W.red));
+  if (!std::isfinite(bn)) throw Error(FAME6D_ENONFINITE, "Invalid shifted-system right-hand side.");
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.x);
+  F6_CHECK_LAUNCH();
+  acc.solves += 1;
+  if (!(bn > 0)) return;
+  double rel_prev = INFINITY, tol_seg = tol;
+  int events = 0, plateau = 0, total = 0;
+  bool first = true;
+  while (true) {
+    // r1 = b - A x, z = P^{-1} r1, beta1
+    if (first) {
+      F6_HIP(hipMemcpyAsync(W.r1, b, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+    } else {
+      mass3(m, W.x, W.Mv, no_skip(), s);
+      k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.x, W.Mv, nullptr, nullptr, 0, W.r1, b, NF,
+                                               nullptr);
+    }
+    k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.r1, nullptr, nullptr, W.z, NF, W.part);
+    k_mr_init<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb, tol_seg, std::max(0, maxit - total));
+    F6_HIP(hipMemcpyAsync(W.r2, W.r1, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+    k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w);
+    k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w2);
+    k_mr_first_v<<<gr, 256, 0, s>>>(W.st, W.z, W.v, n3);
+    F6_CHECK_LAUNCH();
+    const Skip sk = skip_mr(W.st);
+    int issued = 0, chunk = std::max(4, hint);
+    MrState hs{};
+    while (true) {
+      for (int c = 0; c < chunk; ++c, ++issued) {
+        mass3(m, W.v, W.Mv, sk, s);
+        k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.v, W.Mv, W.r1, W.st, 1, W.y, nullptr, NF,
+                                                 W.part);
+        k_mr_alpha<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
+        k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.y, W.r2, W.st, W.z, NF, W.part);
+        k_mr_givens<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
+        // r1 <- r2, r2 <- y (updated), y <- old r1 (free); w1 <- w2, w2 <- w, w <- old w1
+        std::swap(W.r1, W.r2);
+        std::swap(W.r2, W.y);
+        cpx<double>* ow1 = W.w1;
+        W.w1 = W.w2;
+        W.w2 = W.w;
+        W.w = ow1;
+        k_mr_update<<<gr, 256, 0, s>>>(W.st, issued, W.v, W.w1, W.w2, W.w, W.x, W.z, W.vn, n3);
+        std::swap(W.v, W.vn);
+        F6_CHECK_LAUNCH();
+      }
+      d2h_sync(m, &hs, W.st, sizeof(MrState));
+      if (hs.done) break;
+      chunk = std::max(4, chunk / 2);
+    }
+    total += hs.iter;
+    acc.segments += 1;
+    if (first) hint = hs.iter + 1;
+    if (hs.flag == -1) throw Error(FAME6D_ENONFINITE, "MINRES produced nonfinite scalars.");
+    // true residual
+    mass3(m, W.x, W.Mv, no_skip(), s);
+    k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.x, W.Mv, nullptr, nullptr, 0, W.res, b, NF,
+                                             W.part);
+    const double rel = std::sqrt(reduce_read(m, W.part, nb, W.red)) / bn;
+    if (!std::isfinite(rel)) throw Error(FAME6D_ENONFINITE, "MINRES true residual is NaN/Inf.");
+    if (rel <= tol) {
+      acc.max_relres = std::max(acc.max_relres, rel);
+      break;
+    }
+    // recovery: a new segment from the current iterate
+    ++events;
+    if (std::isfinite(rel_prev) && rel >= rel_prev * (1.0 - 1e-2)) ++plateau;
+    else plateau = 0;
+    const bool stalled = std::isfinite(rel_prev) && !(rel < rel_prev * (1.0 - 1e-8));
+    if ((plateau >= 2 && rel <= 1e-9) || ((events > 50 || stalled || total >= maxit) && rel <= 1e-9)) {
+      acc.plateau_accepts += 1;
+      acc.max_relres = std::max(acc.max_relres, rel);
+      break;
+    }
+    if (events > 50 || stalled || total >= maxit)
+      throw Error(FAME6D_ENOCONV, "Method A MINRES did not converge: true relres=" + std::to_string(rel));
+    acc.restarts += 1;
+    tol_seg = std::max(0.25 * tol / rel, 1e-15);
+    rel_prev = rel;
+    first = false;
+  }
+  acc.iterations += total;
+  acc.max_iterations = std::max(acc.max_iterations, total);
+  acc.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static void mr_alloc(fame6d_mass* m, DevBuf& vec, DevBuf& aux, MrWs& W) {
+  const size_t n3 = (size_t)3 * m->NF;
+  vec.ensure(sizeof(cpx<double>) * n3 * 12);
+  cpx<double>* b0 = vec.as<cpx<double>>();
+  cpx<double>** slots[12] = {&W.x, &W.v, &W.vn, &W.Mv, &W.r1, &W.r2, &W.y, &W.z, &W.w, &W.w1, &W.w2, &W.res};
+  for (int i = 0; i < 12; ++i) *slots[i] = b0 + n3 * i;
+  const int nb = red_blocks((long long)n3);
+  aux.ensure(sizeof(MrState) + sizeof(double) * ((size_t)nb + 64));
+  W.st = aux.as<MrState>();
+  W.part = reinterpret_cast<double*>(reinterpret_cast<char*>(aux.ptr) + sizeof(MrState));
+  W.red = W.part + nb;
+}
+
+static void mr_stats_out(const MrAcc& a, fame6d_minres_stats_t* o) {
+  if (!o) return;
+  o->solves = a.solves;
+  o->iterations = a.iterations;
+  o->segments = a.segments;
+  o->restarts = a.restarts;
+  o->max_iterations = a.max_iterations;
+  o->plateau_accepts = a.plateau_accepts;
+  o->max_relres = a.max_relres;
+  o->seconds = a.seconds;
+}
+
+// y = T x = (K - sigma M)^{-1} M x  (the reference's Top, make_A_eigs_top_preconditioned.m:73-113)
+void method_a_apply(fame6d_qsys* S, double sigma, double tol, int maxit, const cpx<double>* x, cpx<double>* y,
+                    fame6d_minres_stats_t* stats) {
+  fame6d_mass* m = S->m;
+  const double tau = sigma * m->eps_mean;
+  if (!(sigma > 0) || !(tau > 0)) throw Error(FAME6D_EARG, "sigma and tau must be positive");
+  DevBuf vec, aux, rhs;
+  MrWs W{};
+  mr_alloc(m, vec, aux, W);
+  rhs.ensure(sizeof(cpx<double>) * 3 * (size_t)m->NF);
+  mass3(m, x, rhs.as<cpx<double>>(), no_skip(), m->stream);
+  MrAcc acc;
+  int hint = 0;
+  minres_solve(S, sigma, tau, rhs.as<cpx<double>>(), tol, maxit, W, hint, acc);
+  F6_HIP(hipMemcpyAsync(y, W.x, sizeof(cpx<double>) * 3 * m->NF, hipMemcpyDeviceToDevice, m->stream));
+  F6_HIP(hipStreamSynchronize(m->stream));
+  mr_stats_out(acc, stats);
+}
+
+void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, double sigma, double tol_mr,
+                   int maxit_mr, const cpx<double>* v0, double* lambda, cpx<double>* Y,
+                   fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long n3 = 3 * m->NF;
+  if (k0 < 1 || p <= k0 || p > 64) throw Error(FAME6D_EARG, "need 1 <= k < p <= 64");
+  const double tau = sigma * m->eps_mean;
+  if (!(sigma > 0) || !(tau > 0)) throw Error(FAME6D_EARG, "sigma and tau must be positive");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int gr = grid_for(n3);
+  const int nb3 = red_blocks(n3);
+  DevBuf basis, zb, scal, part, ubuf, vec, aux, wout;
+  basis.ensure(sizeof(cpx<double>) * (size_t)n3 * (p + 2));  // Q(:,0..p-1), v, r
+  zb.ensure(sizeof(cpx<double>) * (size_t)n3 * (p + 2));     // Z(:,0..p-1), zv, zr
+  scal.ensure(sizeof(double) * 512);
+  part.ensure(sizeof(double) * (size_t)std::max(nb3 * 256, 2048));
+  wout.ensure(sizeof(cpx<double>) * (size_t)n3);
+  MrWs W{};
+  mr_alloc(m, vec, aux, W);
+  LzState L;
+  L.B.Q = basis.as<cpx<double>>();
+  L.B.v = L.B.Q + (size_t)n3 * p;
+  L.B.r = L.B.Q + (size_t)n3 * (p + 1);
+  L.B.sc = scal.as<double>();
+  L.B.part = part.as<double>();
+  L.Z = zb.as<cpx<double>>();
+  L.k = k0;
+  cpx<double>* zv = L.Z + (size_t)n3 * p;
+  cpx<double>* zr = L.Z + (size_t)n3 * (p + 1);
+  Basis BZ = L.B;
+  BZ.Q = L.Z;
+  const int gn = (int)std::min<long long>(2048, (n3 + 255) / 256);
+
+  // v = v0 / ||v0||_M, zv = M v  (run_method_A_fig1a.m:12-14 normalises in the 2-norm; the
+  // M-norm normalisation here is the same direction)
+  F6_HIP(hipMemcpyAsync(L.B.v, v0, sizeof(cpx<double>) * n3, hipMemcpyHostToDevice, s));
+  mass3(m, L.B.v, zv, no_skip(), s);
+  k_dot_partials<double><<<dim3(nb3, 1), kRedThreads, 0, s>>>(L.B.v, n3, zv, n3, n3, L.B.part, nullptr);
+  const double b0 = reduce_read(m, L.B.part, nb3, L.B.sc + 1);
+  if (!(std::isfinite(b0) && b0 > 0)) throw Error(FAME6D_EARG, "Invalid Lanczos start vector.");
+  k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n3, L.B.v, L.B.v, L.B.sc + 1);
+  k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n3, zv, zv, L.B.sc + 1);
+  F6_CHECK_LAUNCH();
+
+  MrAcc acc;
+  int hint = 0;
+  while (!L.done) {
+    const int jj = L.jj;
+    F6_HIP(hipMemcpyAsync(L.B.Q + (size_t)n3 * (jj - 1), L.B.v, sizeof(cpx<double>) * n3,
+                          hipMemcpyDeviceToDevice, s));
+    F6_HIP(hipMemcpyAsync(L.Z + (size_t)n3 * (jj - 1), zv, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice,
+                          s));
+    // w = (K - sigma M)^{-1} (M q_j)
+    minres_solve(S, sigma, tau, zv, tol_mr, maxit_mr, W, hint, acc);
+    F6_HIP(hipMemcpyAsync(wout.ptr, W.x, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+    const cpx<double>* w = wout.as<cpx<double>>();
+    // classical Gram-Schmidt twice in the M inner product: h = Z^H w, r = w - Q h
+    gemv_c(m, n3, BZ, jj, w, L.B.sc + kScH, L.B.sc + kScHim, s);
+    k_gemv_n_out<<<gn, 256, 0, s>>>(L.B.Q, n3, jj, L.B.sc + kScH, L.B.sc + kScHim, w, L.B.r, n3, nullptr);
+    gemv_c(m, n3, BZ, jj, L.B.r, L.B.sc + kScH2, L.B.sc + kScH2im, s);
+    k_gemv_n_out<<<gn, 256, 0, s>>>(L.B.Q, n3, jj, L.B.sc + kScH2, L.B.sc + kScH2im, L.B.r, L.B.r, n3,
+                                    nullptr);
+    // beta^2 = r^H M r
+    mass3(m, L.B.r, zr, no_skip(), s);
+    k_dot_partials<double><<<dim3(nb3, 1), kRedThreads, 0, s>>>(L.B.r, n3, zr, n3, n3, L.B.part, nullptr);
+    k_reduce_final<<<1, kRedThreads, 0, s>>>(L.B.part, nb3, L.B.sc + 1);
+    F6_CHECK_LAUNCH();
+    std::vector<double> sc(kScH + jj);
+    d2h_sync(m, sc.data(), L.B.sc, sizeof(double) * (kScH + jj));
+    const double alpha = sc[kScH + jj - 1], beta2 = sc[1];
+    if (!std::isfinite(alpha) || !std::isfinite(beta2))
+      throw Error(FAME6D_ENONFINITE, "Method A operator returned invalid data.");
+    L.justRestarted = false;
+    bool end_cycle = false;
+    if (beta2 <= 0 || std::sqrt(beta2) < 1e-14) {
+      L.beta = std::sqrt(std::max(beta2, 0.0));
+      L.breakdown = true;
+      end_cycle = true;
+    } else {
+      L.beta = std::sqrt(beta2);
+      k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n3, L.B.v, L.B.r, L.B.sc + 1);
+      k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n3, zv, zr, L.B.sc + 1);
+      F6_CHECK_LAUNCH();
+      L.Alpha.push_back(alpha);
+      L.Beta.push_back(L.beta);
+      L.kk += 1;
+      if (L.jj >= p) end_cycle = true;
+      else L.jj += 1;
+    }
+    if (end_cycle && restart_step(L, k0, p, tol, maxRestart, n3, ubuf, s)) L.done = true;
+  }
+  if ((int)L.d.size() < k0) throw Error(FAME6D_ENOCONV, "Method A ended without k Ritz values.");
+  // lambda = sigma + 1/theta, ascending (run_method_A_fig1a.m:47-49)
+  std::vector<std::pair<double, int>> lam(k0);
+  for (int i = 0; i < k0; ++i) {
+    const double th = L.d[i];
+    if (!std::isfinite(th) || th == 0) throw Error(FAME6D_ENOCONV, "Method A invalid Ritz values.");
+    lam[i] = {sigma + 1.0 / th, i};
+  }
+  std::stable_sort(lam.begin(), lam.end());
+  for (int i = 0; i < k0; ++i) lambda[i] = lam[i].first;
+  if (Y) {
+    const int mq = L.qcols;
+    std::vector<double> Uy((size_t)mq * k0);
+    for (int i = 0; i < k0; ++i) {
+      const int col = L.ind[lam[i].second];
+      for (int r = 0; r < mq; ++r) Uy[(size_t)i * mq + r] = L.Us[(size_t)col * mq + r];
+    }
+    ubuf.ensure(sizeof(double) * Uy.size());
+    F6_HIP(hipMemcpyAsync(ubuf.ptr, Uy.data(), sizeof(double) * Uy.size(), hipMemcpyHostToDevice, s));
+    const int blocks = (int)((n3 + kRestartRows - 1) / kRestartRows);
+    k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n3, n3, mq, k0, ubuf.as<double>());
+    F6_CHECK_LAUNCH();
+    F6_HIP(hipMemcpyAsync(Y, L.B.Q, sizeof(cpx<double>) * n3 * k0, hipMemcpyDeviceToDevice, s));
+  }
+  F6_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    *stats = fame6d_lanczos_stats_t{};
+    stats->converged = L.converged ? 1 : 0;
+    stats->num_converged = L.nconv;
+    stats->outer_steps = L.kk;
+    stats->restarts = L.mm;
+    stats->hit_max_restart = L.hitmax ? 1 : 0;
+    stats->max_ritz_residual = L.maxres;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  mr_stats_out(acc, mstats);
 }
 
 }  // namespace fame6d

@@ -1,6 +1,8 @@
 """Method-C eigensolver on the MI355X: q-system, Mhat, K_r^{-1}, Lanczos, recovery.
 
 Mirrors build_experiment1_q_system_C_compact (exp1/build_experiment1_q_system_C_compact.m:7-38),
+make_A_eigs_top_preconditioned / run_method_A_fig1a (exp1/make_A_eigs_top_preconditioned.m:1-113,
+exp1/run_method_A_fig1a.m:1-50),
 make_explicit_krinv_compact (exp1/make_explicit_krinv_compact.m:1-98),
 run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:1-36),
 recover_original_eigenvectors_exp2 (exp2/recover_original_eigenvectors_exp2.m:1-57) and
@@ -180,6 +182,36 @@ def run_method_B(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_kr=1e-
          float(tol_m), int(maxit_kr), int(maxit_m), float(eps_c), v0.ctypes.data_as(C.c_void_p),
          lam.ctypes.data_as(C.c_void_p), ptr(Y), C.byref(st))
     return dict(lambda_=lam, Y=Y, stats=st.as_dict())
+
+
+def method_a_top(sys: QSystem, x, sigma=1e-4, tol=1e-10, maxit=4000):
+    """The Method-A operator y = (K - sigma M)^{-1} M x (make_A_eigs_top_preconditioned applyT)
+    by preconditioned MINRES; x, y: 3NF."""
+    x = sys._dev(x, sys.ndof)
+    y = torch.empty_like(x)
+    st = _lib.MinresStats()
+    call("fame6d_method_a_apply", sys.handle, float(sigma), float(tol), int(maxit), ptr(x), ptr(y), C.byref(st))
+    return y, st.as_dict()
+
+
+def run_method_A(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=300, sigma=1e-4, tol_minres=1e-10,
+                 maxit_minres=4000, seed=20260906, v0=None, want_vectors=False):
+    """run_method_A_fig1a: the k largest theta of T = (K - sigma M)^{-1} M on the full 3NF space,
+    lambda = sigma + 1/theta ascending (cfg.eigs / cfg.shift of exp1/make_config_exp1bc.m:36-72).
+    Returns dict(lambda (ascending), Y ((k, 3NF) Ritz vectors or None), stats, minres)."""
+    if v0 is None:
+        v0 = start_vector(sys.ndof, seed)
+    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    if v0.size != sys.ndof:
+        raise _lib.Fame6dError(1, "start vector must have 3NF entries")
+    lam = np.empty(k)
+    Y = torch.empty((k, sys.ndof), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
+    st = _lib.LanczosStats()
+    ms = _lib.MinresStats()
+    call("fame6d_eigs_method_a", sys.handle, int(k), int(p), float(tol), int(max_restart), float(sigma),
+         float(tol_minres), int(maxit_minres), v0.ctypes.data_as(C.c_void_p), lam.ctypes.data_as(C.c_void_p),
+         ptr(Y), C.byref(st), C.byref(ms))
+    return dict(lambda_=lam, Y=Y, stats=st.as_dict(), minres=ms.as_dict())
 
 
 def run_method_C_batch(mass: MassBackend, qs, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10,

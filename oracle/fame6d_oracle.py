@@ -488,6 +488,100 @@ def eigs_method_c(N, eps, q, v0, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1
     return (1.0 / theta)[order], Y[:, order], info, sys
 
 
+# ----------------------------------------------------------------------------- Method A
+def shift_precond_apply(sys, tau, x):
+    """P_A^{-1} x with P_A = K + tau I, exactly per mode
+    (exp1/build_original_shift_preconditioner_A_only.m:30-56):
+    (K_q + tau I)^{-1} = I/(Q+tau) + (1/tau - 1/(Q+tau)) q q^*/Q, Q = |q|^2."""
+    n = sys["NF"]
+    a, b, c = sys["q1"], sys["q2"], sys["q3"]
+    Q = a * a + b * b + c * c
+    inv_t = 1.0 / (Q + tau)
+    corr = (1.0 / tau - inv_t) / Q
+    x1, x2, x3 = x[:n], x[n:2 * n], x[2 * n:]
+    cqx = corr * (a * x1 + b * x2 + c * x3)
+    return np.concatenate([inv_t * x1 + a * cqx, inv_t * x2 + b * cqx, inv_t * x3 + c * cqx])
+
+
+def minres(A, b, tol, maxit, Minv=None, x0=None):
+    """Preconditioned MINRES (Paige-Saunders; MATLAB minres as called at
+    exp1/make_A_eigs_top_preconditioned.m:284-286) for Hermitian A and HPD Minv, stopping
+    when the recurrence residual estimate phibar <= tol * beta1.  Returns (x, iterations)."""
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    Minv = (lambda v: v) if Minv is None else Minv
+    r1 = b - A(x) if x0 is not None else b.copy()
+    y = Minv(r1)
+    beta1 = math.sqrt(max(np.real(np.vdot(r1, y)), 0.0))
+    if beta1 == 0.0:
+        return x, 0
+    oldb, beta, dbar, epsln, phibar = 0.0, beta1, 0.0, 0.0, beta1
+    cs, sn = -1.0, 0.0
+    w = np.zeros_like(b)
+    w2 = np.zeros_like(b)
+    r2 = r1.copy()
+    it = 0
+    while it < maxit:
+        it += 1
+        v = y / beta
+        y = A(v)
+        if it >= 2:
+            y = y - (beta / oldb) * r1
+        alfa = np.real(np.vdot(v, y))
+        y = y - (alfa / beta) * r2
+        r1, r2 = r2, y
+        y = Minv(r2)
+        oldb, beta = beta, math.sqrt(max(np.real(np.vdot(r2, y)), 0.0))
+        oldeps = epsln
+        delta = cs * dbar + sn * alfa
+        gbar = sn * dbar - cs * alfa
+        epsln = sn * beta
+        dbar = -cs * beta
+        gamma = max(math.hypot(gbar, beta), EPS)
+        cs, sn = gbar / gamma, beta / gamma
+        phi, phibar = cs * phibar, sn * phibar
+        w1, w2 = w2, w
+        w = (v - oldeps * w1 - delta * w2) / gamma
+        x = x + phi * w
+        if abs(phibar) <= tol * beta1 or beta == 0.0:
+            break
+    return x, it
+
+
+def method_a_top(sys, eps, x, N, sigma=1e-4, tol=1e-10, maxit=4000):
+    """T x = (K - sigma M)^{-1} M x (exp1/make_A_eigs_top_preconditioned.m:59-60,98-113) with
+    P_A = K + tau I, tau = sigma * mean(eps); MINRES segments restart from the current
+    iterate until the true relative residual meets tol."""
+    tau = sigma * float(np.mean(eps))
+
+    def A(v):
+        return apply_K(sys, v) - sigma * mass3_apply(eps, v, N)
+
+    b = mass3_apply(eps, x, N)
+    nb = np.linalg.norm(b)
+    y, its = minres(A, b, tol, maxit, lambda v: shift_precond_apply(sys, tau, v))
+    seg_tol = tol
+    for _ in range(20):
+        rel = np.linalg.norm(b - A(y)) / nb
+        if rel <= tol:
+            break
+        seg_tol = 0.25 * tol / rel
+        y, k = minres(A, b, seg_tol, maxit, lambda v: shift_precond_apply(sys, tau, v), x0=y)
+        its += k
+    return y, its
+
+
+def method_a_dense_lambda(sys, eps, N, sigma, k):
+    """The k largest eigenvalues theta of the dense T = (K - sigma M)^{-1} M (small N), mapped
+    to lambda = sigma + 1/theta ascending (exp1/run_method_A_fig1a.m:47-49)."""
+    n3 = 3 * sys["NF"]
+    I = np.eye(n3, dtype=complex)
+    K = np.stack([apply_K(sys, I[:, j]) for j in range(n3)], axis=1)
+    M = np.stack([mass3_apply(eps, I[:, j], N) for j in range(n3)], axis=1)
+    theta = np.linalg.eigvals(np.linalg.solve(K - sigma * M, M))
+    th = np.sort(np.real(theta))[::-1][:k]
+    return np.sort(sigma + 1.0 / th)
+
+
 def recover_eigenvectors(sys, eps, Y, N, tol=1e-12, maxit=3000):
     """x_j = M^{-1} V y_j by PCG, M-normalised (exp2/recover_original_eigenvectors_exp2.m:18-51)."""
     X, MX, its = [], [], []

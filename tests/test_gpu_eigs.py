@@ -182,3 +182,36 @@ def test_kernel_family_timer(n2, pkg):
         assert kt[fam]["launches"] > 0 and kt[fam]["bytes"] > 0, fam
     # skipped launches after convergence are discarded, so the three passes agree
     assert kt["mhat_pass_a"]["launches"] == kt["mhat_pass_b"]["launches"] == kt["mhat_pass_c"]["launches"]
+
+
+def test_method_a_top(n2, oracle, pkg):
+    """Method-A operator y = (K - sigma M)^{-1} M x: the GPU solution meets the true relative
+    residual 1e-10 of the shifted system, checked in numpy (make_A_eigs_top_preconditioned.m)."""
+    print("case: Method A top N=2", flush=True)
+    N, eps, s, osys = n2["N"], n2["eps"], n2["s"], n2["osys"]
+    sigma = 1e-4
+    rng = np.random.default_rng(51)
+    x = rand_c(rng, 3 * s.NF)
+    y, st = pkg.method_a_top(s, torch.as_tensor(x, device="cuda"), sigma, 1e-10, 4000)
+    y = y.cpu().numpy()
+    b = oracle.mass3_apply(eps, x, N)
+    r = oracle.apply_K(osys, y) - sigma * oracle.mass3_apply(eps, y, N) - b
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.5e-10
+    ref, its = oracle.method_a_top(osys, eps, x, N, sigma, 1e-10, 4000)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-5  # both at residual 1e-10, cond(A) ~ 1e5
+    assert st["solves"] == 1 and st["iterations"] > 0
+
+
+def test_method_a_eigs_golden_n2(n2, pkg):
+    """Method A reaches the reference-held eigenvalues (exp3/make_config_experiment4.m:40) and
+    agrees with Method C (exp1/check_fig1bc_results.m:23-25 warns above 1e-7)."""
+    print("case: Method A eigs N=2", flush=True)
+    s = n2["s"]
+    outA = pkg.run_method_A(s, k=10, p=40, tol=1e-10, max_restart=300, sigma=1e-4, tol_minres=1e-10,
+                            maxit_minres=4000, v0=pkg.start_vector(s.ndof, 20260906))
+    lamA = outA["lambda_"]
+    assert outA["stats"]["converged"] == 1
+    assert abs(lamA[0] - GOLDEN_N2[0]) / GOLDEN_N2[0] < 1e-9
+    assert abs(lamA[9] - GOLDEN_N2[1]) / GOLDEN_N2[1] < 1e-9
+    lamC = pkg.run_method_C(s, v0=pkg.start_vector(s.nr, 7), precision="double", maxit_cg=3000)["lambda_"]
+    assert np.max(np.abs(lamA - lamC) / lamC) < 1e-9

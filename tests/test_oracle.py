@@ -257,3 +257,40 @@ def test_method_b_matches_method_c(oracle):
     xc, stc = oracle.krinv_apply(s, eps, b, N, 1e-12, 3000)
     assert np.linalg.norm(xb - xc) / np.linalg.norm(xc) < 1e-9
     assert np.linalg.norm(oracle.kr_apply(s, eps, xb, N) - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_shift_preconditioner_closed_form(oracle):
+    # P_A^{-1} = (K + tau I)^{-1} per mode (build_original_shift_preconditioner_A_only.m:5-12)
+    N = 1
+    s = oracle.qsystem(N, Q_EXP4)
+    n = s["NF"]
+    tau = 3e-4
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
+    y = oracle.shift_precond_apply(s, tau, x)
+    for i in (0, 7, n - 1):
+        q = np.array([s["q1"][i], s["q2"][i], s["q3"][i]])
+        Kq = (q @ q) * np.eye(3) - np.outer(q, q)
+        ref = np.linalg.solve(Kq + tau * np.eye(3), x[[i, n + i, 2 * n + i]])
+        assert np.allclose(y[[i, n + i, 2 * n + i]], ref, rtol=1e-9, atol=0)  # cond(K_q + tau I) ~ 1e4
+
+
+def test_method_a_minres_and_dense_lambda_n1(oracle):
+    # T = (K - sigma M)^{-1} M at N = 1 (192 unknowns): MINRES meets the true residual, and
+    # sigma + 1/theta of the 10 largest theta are the 10 smallest eigenvalues of the dense
+    # K_r of Method C (the reference's cross-method check, exp1/check_fig1bc_results.m:23-25)
+    N = 1
+    eps = oracle.eps_reference(N)
+    s = oracle.qsystem(N, Q_EXP4)
+    sigma = 1e-4
+    rng = np.random.default_rng(43)
+    x = rng.standard_normal(3 * s["NF"]) + 1j * rng.standard_normal(3 * s["NF"])
+    y, its = oracle.method_a_top(s, eps, x, N, sigma, 1e-10, 4000)
+    b = oracle.mass3_apply(eps, x, N)
+    r = oracle.apply_K(s, y) - sigma * oracle.mass3_apply(eps, y, N) - b
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10 and its > 0
+    lamA = oracle.method_a_dense_lambda(s, eps, N, sigma, 10)
+    n2 = 2 * s["NF"]
+    Kr = np.stack([oracle.kr_apply(s, eps, np.eye(n2)[:, j].astype(complex), N) for j in range(n2)], 1)
+    lamC = np.linalg.eigvalsh(Kr)[:10]
+    assert np.allclose(lamA, lamC, rtol=1e-9)
