@@ -16,6 +16,8 @@
  *   s  = fame6d_mex('qsys_create', h, q, P)             % q: 3x1, P: 6x3
  *   y  = fame6d_mex('krinv', s, x, tol, maxit)          % KrInvOp (Method C), x: 2NF x 1
  *   [lambda, Y] = fame6d_mex('eigs', s, k, p, tol, maxRestart, tolCG, maxitCG, v0, precision)
+ *   y  = fame6d_mex('topA', s, x, sigma, tol, maxit)    % Method-A Top (make_A_eigs_top_preconditioned.m)
+ *   [lambda, Y] = fame6d_mex('eigsA', s, k, p, tol, maxRestart, sigma, tolMINRES, maxitMINRES, v0)
  *   fame6d_mex('qsys_destroy', s); fame6d_mex('mass_destroy', h)
  */
 #include <hip/hip_runtime_api.h>
@@ -130,6 +132,39 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                               mxGetDoubles(plhs[0]), Y, &st));
     if (nlhs > 1) {
       plhs[1] = from_device(Y, n2, k);
+      hipFree(Y);
+    }
+  } else if (!strcmp(cmd, "topA")) {
+    fame6d_qsys* s = (fame6d_qsys*)handle_in(prhs[1]);
+    size_t n;
+    void* x = to_device(prhs[2], &n);
+    void* y = NULL;
+    hipMalloc(&y, n * 16);
+    fame6d_minres_stats_t st;
+    fail(fame6d_method_a_apply(s, mxGetScalar(prhs[3]), mxGetScalar(prhs[4]), (int)mxGetScalar(prhs[5]), x, y,
+                               &st));
+    plhs[0] = from_device(y, n, 1);
+    hipFree(x);
+    hipFree(y);
+  } else if (!strcmp(cmd, "eigsA")) {
+    fame6d_qsys* s = (fame6d_qsys*)handle_in(prhs[1]);
+    int k = (int)mxGetScalar(prhs[2]), p = (int)mxGetScalar(prhs[3]);
+    double tol = mxGetScalar(prhs[4]);
+    int maxRestart = (int)mxGetScalar(prhs[5]);
+    double sigma = mxGetScalar(prhs[6]), tolM = mxGetScalar(prhs[7]);
+    int maxitM = (int)mxGetScalar(prhs[8]);
+    const mxArray* v0 = prhs[9];
+    size_t n3 = mxGetNumberOfElements(v0);
+    if (!mxIsComplex(v0)) mexErrMsgIdAndTxt("fame6d:usage", "v0 must be complex");
+    plhs[0] = mxCreateDoubleMatrix(k, 1, mxREAL);
+    void* Y = NULL;
+    if (nlhs > 1) hipMalloc(&Y, n3 * k * 16);
+    fame6d_lanczos_stats_t st;
+    fame6d_minres_stats_t ms;
+    fail(fame6d_eigs_method_a(s, k, p, tol, maxRestart, sigma, tolM, maxitM, mxGetComplexDoubles(v0),
+                              mxGetDoubles(plhs[0]), Y, &st, &ms));
+    if (nlhs > 1) {
+      plhs[1] = from_device(Y, n3, k);
       hipFree(Y);
     }
   } else {
