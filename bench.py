@@ -205,6 +205,34 @@ def extras(torch, pkg):
                 batched_vs_sequential_max_rel_diff=maxdiff, taylor=rows)
 
 
+def method_comparison(torch, pkg, N, lam_c):
+    """Methods A (MINRES shift-invert on the full 3NF space) and B (nested PCG) on the
+    benchmark configuration, against the Method-C eigenvalues (the reference warns above
+    1e-7, exp1/check_fig1bc_results.m:23-25)."""
+    eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
+    mass = pkg.MassBackend(N, eps)
+    mass.autotune(reps=3)
+    sysq = pkg.QSystem(mass, Q)
+    out = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a = pkg.run_method_A(sysq, k=10, p=40, tol=1e-10, max_restart=300, sigma=1e-4, tol_minres=1e-10,
+                         maxit_minres=4000, v0=pkg.start_vector(sysq.ndof, 20260906))
+    torch.cuda.synchronize()
+    out["A"] = dict(seconds=time.perf_counter() - t0, outer_steps=a["stats"]["outer_steps"],
+                    minres=a["minres"], max_rel_diff_vs_C=float(np.max(np.abs(a["lambda_"] - lam_c) / lam_c)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = pkg.run_method_B(sysq, v0=pkg.start_vector(sysq.nr, 20260905), **{k: v for k, v in LANCZOS.items()
+                                                                          if k in ("k", "p", "tol", "max_restart")})
+    torch.cuda.synchronize()
+    out["B"] = dict(seconds=time.perf_counter() - t0, outer_steps=b["stats"]["outer_steps"],
+                    max_rel_diff_vs_C=float(np.max(np.abs(b["lambda_"] - lam_c) / lam_c)))
+    sysq.close()
+    mass.close()
+    return out
+
+
 # kernel family (library timer slot) -> kernel-name pattern in rocprofv3 output (%d: nGrid)
 FAMILY_KERNEL = {
     "mhat_pass_a": r"k_pass_a3<double, %d, \d+, 2>",
@@ -402,6 +430,8 @@ def main():
     if not a.no_cpu:
         cpu = cpu_baseline(pkg, eps, N, st64)
     ext = extras(torch, pkg) if a.extras else None
+    if ext is not None:
+        ext["methods_vs_C"] = method_comparison(torch, pkg, N, lam64)
 
     line = {
         "metric": "N=5 time-to-10-eigenpairs + M-apply HBM GB/s; mixed vs FP64 at 1e-8 rel",

@@ -213,6 +213,16 @@ static __global__ void k_mr_update(const MrState* st, int issued_iter, const cpx
   }
 }
 
+// Continues a converged recurrence: the stopping threshold becomes factor*|phibar| and the
+// iteration kernels run again from the state the last iteration left.
+static __global__ void k_mr_rearm(MrState* st, double factor) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->tolb = fabs(st->phibar) * factor;
+    st->flag = 0;
+    st->done = 0;
+  }
+}
+
 // v = z / beta1 at the start of a segment.
 static __global__ void k_mr_first_v(const MrState* st, const cpx<double>* z, cpx<double>* v, long long n3) {
   if (st->done) return;
