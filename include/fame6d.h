@@ -184,6 +184,7 @@ typedef struct {
   double max_ritz_residual;
   double seconds;
   fame6d_krinv_stats_t krinv;
+  int second_sweeps;    /* steps whose DGKS test ran a second Gram-Schmidt sweep (Methods B, C) */
 } fame6d_lanczos_stats_t;
 
 /* run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:4-34) driving

@@ -54,7 +54,7 @@ class KrinvStats(C.Structure):
 class LanczosStats(C.Structure):
     _fields_ = [("converged", C.c_int), ("num_converged", C.c_int), ("outer_steps", C.c_int),
                 ("restarts", C.c_int), ("hit_max_restart", C.c_int), ("max_ritz_residual", C.c_double),
-                ("seconds", C.c_double), ("krinv", KrinvStats)]
+                ("seconds", C.c_double), ("krinv", KrinvStats), ("second_sweeps", C.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "krinv"}

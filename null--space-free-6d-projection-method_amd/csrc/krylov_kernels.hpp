@@ -535,8 +535,9 @@ __global__ void k_fill_zero_real(long long n, T* x) {
 template <int KB>
 __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
                                                             const cpx<double>* r, long long n, int nrb,
-                                                            double* pre, double* pim) {
+                                                            double* pre, double* pim, Skip sk) {
   __shared__ double red[kRedThreads];
+  if (skipped(sk, 0)) return;
   const int nch = (jj + KB - 1) / KB;
   const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
   const int rb = logical / nch, ch = logical - rb * nch;
@@ -609,9 +610,10 @@ static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long lon
 static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
                                                            const double* hre, const double* him,
                                                            const cpx<double>* in, cpx<double>* out,
-                                                           long long n, double* norm_partials) {
+                                                           long long n, double* norm_partials, Skip sk) {
   __shared__ cpx<double> hs[128];
   __shared__ double red[256];
+  if (skipped(sk, 0)) return;
   for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);
   __syncthreads();
   double s = 0.0;
@@ -625,6 +627,50 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
   if (norm_partials != nullptr) {
     s = block_sum<256>(s, red);
     if (threadIdx.x == 0) norm_partials[blockIdx.x] = s;
+  }
+}
+
+// Three-term Lanczos update r = w - alpha q_j - beta q_{j-1} (alpha = *alpha on the device,
+// qjm1 null on the first step) with ||r||^2 per workgroup
+// (Lanczos_hybrid_identity_gpuop.m:58-62).
+static __global__ __launch_bounds__(256) void k_lz_three_term(const cpx<double>* w, const cpx<double>* qj,
+                                                             const cpx<double>* qjm1, const double* alpha,
+                                                             double beta, cpx<double>* r, long long n,
+                                                             double* partials) {
+  __shared__ double red[256];
+  const double a = *alpha;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    cpx<double> v = w[i] - a * qj[i];
+    if (qjm1) v = v - beta * qjm1[i];
+    r[i] = v;
+    s += v.re * v.re + v.im * v.im;
+  }
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Norm bookkeeping of the re-orthogonalisation (one workgroup).  sc[1] = ||r||^2 of the
+// current vector, sc[2] = ||r||^2 before the first sweep, the int at sc + 4 = 1 when the
+// second sweep is not needed.  stage 0: after the three-term update; stage 1: after the
+// first sweep, which decides the second one by the DGKS test ||r_after|| < ||r_before||/sqrt(2);
+// stage 2: after the second sweep (taken only when it ran).
+static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, const double* partials, int nb,
+                                                                int stage) {
+  __shared__ double red[kRedThreads];
+  int* skip = reinterpret_cast<int*>(sc + 4);
+  if (stage == 2 && *skip) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) {
+    if (stage == 0) {
+      sc[2] = s;
+    } else {
+      sc[1] = s;
+      if (stage == 1) skip[0] = (s < 0.5 * sc[2]) ? 0 : 1;
+    }
   }
 }
 

@@ -15,6 +15,7 @@
 // K_r^{-1} per step.
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -135,15 +136,17 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
 // h1 in sc[kScH..], h2 in sc[kScH2..], ||r||^2 in sc[1].
 constexpr int kScH2 = 8 + 256, kScH2im = 8 + 384;
 constexpr int kGemvKB = 8;
+// With `sk` set the projection is skipped (and not timed: its launch does no work).
 static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* v, double* dre,
-                   double* dim, hipStream_t s) {
+                   double* dim, hipStream_t s, Skip sk = no_skip()) {
   const int nb = red_blocks(n);
   const int nch = (jj + kGemvKB - 1) / kGemvKB;
   double* pre = B.part;
   double* pim = B.part + (size_t)nb * 128;
-  timer_begin(m, TS_GEMV_C);
-  launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim);
-  timer_end(m, (jj + 1) * 16.0 * n);
+  if (sk.p == nullptr) timer_begin(m, TS_GEMV_C);
+  launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim,
+           sk);
+  if (sk.p == nullptr) timer_end(m, (jj + 1) * 16.0 * n);
   k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, dre);
   k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, dim);
   F6_CHECK_LAUNCH();
@@ -156,14 +159,46 @@ static void cgs2_step(fame6d_mass* m, long long n, const Basis& B, int jj, const
   gemv_c(m, n, B, jj, w, B.sc + kScH, B.sc + kScHim, s);
   timer_begin(m, TS_GEMV_N);
   launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n,
-           nullptr);
+           nullptr, no_skip());
   timer_end(m, (jj + 2) * vb);
   gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s);
   timer_begin(m, TS_GEMV_N);
   launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n,
-           B.part);
+           B.part, no_skip());
   timer_end(m, (jj + 2) * vb);
   k_reduce_final<<<1, kRedThreads, 0, s>>>(B.part, gn, B.sc + 1);
+  F6_CHECK_LAUNCH();
+}
+
+// A Lanczos step away from a restart (Lanczos_hybrid_identity_gpuop.m:55-63): alpha =
+// Re q_j^H w into sc[0]; the three-term update r = w - alpha q_j - beta q_{j-1}; one
+// classical Gram-Schmidt sweep r -= Q (Q^H r) over the whole basis; and a second sweep only
+// when the first removed more than 1 - 1/sqrt(2) of ||r|| (the DGKS test, decided on the
+// device, so the step never waits for the host).  The three-term update removes the two
+// large components of w, so the full sweep only cleans rounding-level ones and seldom
+// needs repeating; the reference always runs two modified Gram-Schmidt sweeps.
+// ||r||^2 ends in sc[1].
+static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* w,
+                              double beta, hipStream_t s) {
+  const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
+  const double vb = 16.0 * n;
+  const cpx<double>* qj = B.Q + (size_t)n * (jj - 1);
+  const cpx<double>* qjm1 = jj >= 2 ? B.Q + (size_t)n * (jj - 2) : nullptr;
+  dot_dev(n, qj, w, B.part, B.sc, nullptr, s);
+  k_lz_three_term<<<gn, 256, 0, s>>>(w, qj, qjm1, B.sc, beta, B.r, n, B.part);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 0);
+  F6_CHECK_LAUNCH();
+  gemv_c(m, n, B, jj, B.r, B.sc + kScH, B.sc + kScHim, s);
+  timer_begin(m, TS_GEMV_N);
+  launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, B.r, B.r, n,
+           B.part, no_skip());
+  timer_end(m, (jj + 2) * vb);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1);
+  F6_CHECK_LAUNCH();
+  const Skip sk{reinterpret_cast<const int*>(B.sc + 4), 0};
+  gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s, sk);
+  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n, B.part, sk);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 2);
   F6_CHECK_LAUNCH();
 }
 
@@ -181,6 +216,8 @@ struct LzState {
   double maxres = 0;
   std::vector<double> theta;
   cpx<double>* Z = nullptr;  // M-weighted basis Z = M Q (Method A), restarted with Q
+  int alpha_at = 0;          // index of alpha in the device scalars of the last step
+  int second_sweeps = 0;     // steps whose DGKS test asked for a second Gram-Schmidt sweep
 };
 
 // Builds H, solves it, tests convergence and restarts (Lanczos_hybrid_identity_gpuop.m:79-120).
@@ -365,8 +402,14 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       const Basis& B = Lg.B;
       const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
       const cpx<double>* w = qout + (size_t)n2 * i;
-      // CGS2: h1 = Q^H w (alpha = Re h1[j]); r = w - Q h1 and h2 = Q^H r; r -= Q h2 with ||r||^2
-      cgs2_step(m, n2, B, jj, w, s);
+      if (Lg.justRestarted) {
+        // w couples to every kept Ritz vector: CGS2 over the whole basis (alpha = Re h1[j])
+        cgs2_step(m, n2, B, jj, w, s);
+        Lg.alpha_at = kScH + jj - 1;
+      } else {
+        lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s);
+        Lg.alpha_at = 0;
+      }
       Lg.justRestarted = false;
     }
     // alpha = Re h1[jj-1] and beta^2 of every active run: one staged read, one sync
@@ -381,8 +424,13 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       m->pin.used = 0;
       for (int i = 0; i < na; ++i) {
         const double* v = reinterpret_cast<const double*>(stage + per * i);
-        ab[2 * i] = v[kScH + (L[act[i]].jj - 1)];
+        ab[2 * i] = v[L[act[i]].alpha_at];
         ab[2 * i + 1] = v[1];
+        if (L[act[i]].alpha_at == 0) {
+          int second = 0;
+          std::memcpy(&second, v + 4, sizeof(int));
+          L[act[i]].second_sweeps += second ? 0 : 1;
+        }
       }
     }
     for (int i = 0; i < na; ++i) {
@@ -446,6 +494,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       o.converged = Lg.converged ? 1 : 0;
       o.num_converged = Lg.nconv;
       o.outer_steps = Lg.kk;
+      o.second_sweeps = Lg.second_sweeps;
       o.restarts = Lg.mm;
       o.hit_max_restart = Lg.hitmax ? 1 : 0;
       o.max_ritz_residual = Lg.maxres;
@@ -1027,10 +1076,11 @@ void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, do
     const cpx<double>* w = wout.as<cpx<double>>();
     // classical Gram-Schmidt twice in the M inner product: h = Z^H w, r = w - Q h
     gemv_c(m, n3, BZ, jj, w, L.B.sc + kScH, L.B.sc + kScHim, s);
-    k_gemv_n_out<<<gn, 256, 0, s>>>(L.B.Q, n3, jj, L.B.sc + kScH, L.B.sc + kScHim, w, L.B.r, n3, nullptr);
+    k_gemv_n_out<<<gn, 256, 0, s>>>(L.B.Q, n3, jj, L.B.sc + kScH, L.B.sc + kScHim, w, L.B.r, n3, nullptr,
+                                    no_skip());
     gemv_c(m, n3, BZ, jj, L.B.r, L.B.sc + kScH2, L.B.sc + kScH2im, s);
     k_gemv_n_out<<<gn, 256, 0, s>>>(L.B.Q, n3, jj, L.B.sc + kScH2, L.B.sc + kScH2im, L.B.r, L.B.r, n3,
-                                    nullptr);
+                                    nullptr, no_skip());
     // beta^2 = r^H M r
     mass3(m, L.B.r, zr, no_skip(), s);
     k_dot_partials<double><<<dim3(nb3, 1), kRedThreads, 0, s>>>(L.B.r, n3, zr, n3, n3, L.B.part, nullptr);
