@@ -26,6 +26,7 @@
 #define FAME6D_H
 
 #include <stdint.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -285,6 +286,62 @@ int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, in
  * u = exp(i R q^T) coeff. */
 int fame6d_direct_reconstruct(long long NF, const double* qmodes, const void* coeff, long long M,
                               const double* points, void* u, void* stream);
+
+/* ----------------------------------------------------------------- Yee closure */
+
+/* Window of half-width n*h: plus-grid fields u1, u2, u3 are (2n+3)^3 complex128 arrays
+ * vectorised column-major (x fastest; the output of fame6d_taylor_evaluate_yee), the
+ * interior and every density or rho array is (2n+1)^3, column-major.  Entry points that
+ * take `work` use a device workspace of fame6d_yee_workspace_bytes(n) bytes, or allocate
+ * one for the call when it is NULL. */
+size_t fame6d_yee_workspace_bytes(int n);
+
+/* apply_cropped_A_exp3 (exp3/apply_cropped_A_exp3.m:8-29): cropped Yee curl-curl of the
+ * plus-grid field on the interior.  A1..A3: (2n+1)^3 complex128 (device). */
+int fame6d_yee_apply_cropped_A(int n, double h, const void* u1, const void* u2, const void* u3, void* A1,
+                               void* A2, void* A3, void* stream);
+
+/* sample_scalar_yee_material_exp4 (exp3/sample_scalar_yee_material_exp4.m:4-21):
+ * eps_c = eps_c * exp(alpha/6 sum_l cos(P_l . x)) on the three edge families of the interior.
+ * P_host: nrows x 3 row-major (the 6 x 3 cfg.problem.P); e1..e3: (2n+1)^3 float64 (device). */
+int fame6d_yee_sample_eps(int n, double h, const double* P_host, int nrows, double eps_c, double alpha, void* e1,
+                          void* e2, void* e3, void* work, void* stream);
+
+typedef struct {
+  double lambda;                 /* sum E / sum M */
+  double V, nu;                  /* weighted variance of rho = E/M about lambda, sqrt(V) */
+  double partition_error;        /* max of the two below */
+  double energy_partition_error; /* |sum E - sum a| / |sum a| */
+  double mass_partition_error;   /* |sum M - sum b| / |sum b| */
+  double sum_a, sum_b;
+} fame6d_lwrq_t;
+
+/* compute_lwrq_exp4 (exp3/compute_lwrq_exp4.m:12-56): E = conv(a ./ den), M = conv(b ./ den)
+ * with the separable Gaussian of radius `radius` cells and width sigma_factor*h,
+ * den = conv(ones); FAME6D_ENOCONV when M is non-finite or <= mass_floor_rel * max M.
+ * a, b: (2n+1)^3 float64 (device); rho: optional (2n+1)^3 float64 output E./M. */
+int fame6d_lwrq(int n, double h, const double* a, const double* b, int radius, double sigma_factor,
+                double mass_floor_rel, double* rho, void* work, fame6d_lwrq_t* out, void* stream);
+
+typedef struct {
+  double lambda_crop, lambda_3d;      /* sum a / sum b; LWRQ lambda */
+  double eta_3d, eta_6d;              /* ||Au - l Bu|| / (||Au|| + |l| ||Bu||) at lambda_3d, lambda6d */
+  double delta_lambda;                /* |lambda_3d - lambda6d| / |lambda6d| */
+  double V, nu;                       /* LWRQ variance and its square root */
+  double delta_part, delta_identity;  /* partition error; |lambda_3d - lambda_crop| / |lambda_crop| */
+  double cropped_mass;                /* sum b */
+  double energy_partition_error, mass_partition_error;
+} fame6d_yee_metrics_t;
+
+/* compute_yee_lwrq_metrics_exp4 (exp3/compute_yee_lwrq_metrics_exp4.m:16-70): the cropped
+ * curl-curl, the Yee material (P_host, eps_c, alpha as above), the densities
+ * a = Re(u' A u), b = Re(u' B u), the LWRQ (radius, sigma_factor, mass_floor_rel as
+ * cfg.lwrq, exp3/make_config_experiment4.m:72-74) and the residuals at lambda_3d and the
+ * 6D eigenvalue lambda6d.  rho: optional (2n+1)^3 float64 output of the local quotient. */
+int fame6d_yee_lwrq_metrics(int n, double h, const void* u1, const void* u2, const void* u3, const double* P_host,
+                            int nrows, double eps_c, double alpha, double lambda6d, int radius,
+                            double sigma_factor, double mass_floor_rel, double* rho, void* work,
+                            fame6d_yee_metrics_t* out, void* stream);
 
 /* ----------------------------------------------------------------- adjoint */
 

@@ -3,7 +3,7 @@
 The compute path is libfame6d.so (hand-written HIP kernels for gfx950 behind the C ABI
 of include/fame6d.h); this package is its host-side mirror of the reference's
 operator interface (the `mass` struct of build_mass_backend_large_scalar, the
-q-system, K_r^{-1}, Lanczos, recovery and Taylor reconstruction).  There is no
+q-system, K_r^{-1}, Lanczos, recovery, Taylor reconstruction and the Yee/LWRQ closure).  There is no
 CPU fallback: every operation raises when the library or the GPU is missing.
 """
 from ._lib import FP32, FP64, MIXED, Fame6dError, load  # noqa: F401
@@ -14,3 +14,5 @@ from .eigs import (QSystem, build_experiment1_q_system_C_compact, compute_fourie
                    run_method_A, run_method_B, run_method_C, run_method_C_batch)
 from .taylor import (block_partition, build_multi_index_list_3d, direct_reconstruct,  # noqa: F401
                      taylor_evaluate, taylor_moments, yee_points)
+from .yee import (apply_cropped_A, compute_lwrq, compute_yee_lwrq_metrics, crop_plus_field,  # noqa: F401
+                  sample_scalar_yee_material)

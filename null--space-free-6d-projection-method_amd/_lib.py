@@ -79,6 +79,25 @@ class KernelTime(C.Structure):
     _fields_ = [("launches", C.c_longlong), ("milliseconds", C.c_double), ("bytes", C.c_double)]
 
 
+class LwrqResult(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("V", C.c_double), ("nu", C.c_double), ("partition_error", C.c_double),
+                ("energy_partition_error", C.c_double), ("mass_partition_error", C.c_double),
+                ("sum_a", C.c_double), ("sum_b", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class YeeMetrics(C.Structure):
+    _fields_ = [("lambda_crop", C.c_double), ("lambda_3d", C.c_double), ("eta_3d", C.c_double),
+                ("eta_6d", C.c_double), ("delta_lambda", C.c_double), ("V", C.c_double), ("nu", C.c_double),
+                ("delta_part", C.c_double), ("delta_identity", C.c_double), ("cropped_mass", C.c_double),
+                ("energy_partition_error", C.c_double), ("mass_partition_error", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 NUM_TIMER_SLOTS = 5
 TIMER_SLOTS = ("mhat_pass_a", "mhat_pass_b", "mhat_pass_c", "lanczos_gemv_c", "lanczos_gemv_n")
 
@@ -126,7 +145,13 @@ SIGNATURES = {
     "fame6d_direct_reconstruct": [LL, P, P, LL, P, P, P],
     "fame6d_mass_adjoint_eps": [P, P, P, I, P],
     "fame6d_eig_grad_eps": [P, P, I, P, P, P],
+    "fame6d_yee_workspace_bytes": [I],
+    "fame6d_yee_apply_cropped_A": [I, D, P, P, P, P, P, P, P],
+    "fame6d_yee_sample_eps": [I, D, P, I, D, D, P, P, P, P, P],
+    "fame6d_lwrq": [I, D, P, P, I, D, D, P, P, C.POINTER(LwrqResult), P],
+    "fame6d_yee_lwrq_metrics": [I, D, P, P, P, P, I, D, D, D, I, D, D, P, P, C.POINTER(YeeMetrics), P],
 }
+RESTYPES = {"fame6d_yee_workspace_bytes": C.c_size_t}
 
 _lib = None
 
@@ -144,7 +169,7 @@ def load(path=LIB_PATH):
     for name, args in SIGNATURES.items():
         f = getattr(lib, name)
         f.argtypes = args
-        f.restype = C.c_int
+        f.restype = RESTYPES.get(name, C.c_int)
     _lib = lib
     return lib
 

@@ -29,6 +29,10 @@ __host__ __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) {
   return mk(a.re - b.re, a.im - b.im);
 }
 template <class T>
+__host__ __device__ __forceinline__ cpx<T> operator-(cpx<T> a) {
+  return mk(-a.re, -a.im);
+}
+template <class T>
 __host__ __device__ __forceinline__ cpx<T> operator*(cpx<T> a, cpx<T> b) {
   return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }

@@ -714,3 +714,140 @@ def eig_grad_eps(X, lam, N):
             g += np.abs(np.fft.ifftn(unvec(X[c * n:(c + 1) * n, j], N))) ** 2
         out.append(vec(-lam[j] * n * g))
     return np.stack(out, 1)
+
+
+# ----------------------------------------------------------------------------- Yee closure
+# Fields on the plus grid Omega_n^+ are (2n+3)^3 arrays indexed [x, y, z] (the device
+# layout is their column-major vectorisation, x fastest); the interior (2n+1)^3 is [1:-1]^3.
+def _shift(u, dx, dy, dz):
+    """u at interior point + (dx, dy, dz), each offset in {-1, 0, 1}."""
+    sl = {-1: slice(0, -2), 0: slice(1, -1), 1: slice(2, None)}
+    return u[sl[dx], sl[dy], sl[dz]]
+
+
+def apply_cropped_A(u1, u2, u3, h):
+    """Cropped Yee curl-curl action (exp3/apply_cropped_A_exp3.m:8-29), with the nine
+    difference operators of :32-56 written as offsets of the interior point."""
+    c = lambda u: _shift(u, 0, 0, 0)
+    Dxx = lambda u: -_shift(u, -1, 0, 0) + 2 * c(u) - _shift(u, 1, 0, 0)
+    Dyy = lambda u: -_shift(u, 0, -1, 0) + 2 * c(u) - _shift(u, 0, 1, 0)
+    Dzz = lambda u: -_shift(u, 0, 0, -1) + 2 * c(u) - _shift(u, 0, 0, 1)
+    DyT_DxP = lambda u: _shift(u, 1, -1, 0) - _shift(u, 0, -1, 0) - _shift(u, 1, 0, 0) + c(u)
+    DzT_DxP = lambda u: _shift(u, 1, 0, -1) - _shift(u, 0, 0, -1) - _shift(u, 1, 0, 0) + c(u)
+    DxT_DyP = lambda u: -_shift(u, -1, 0, 0) + c(u) + _shift(u, -1, 1, 0) - _shift(u, 0, 1, 0)
+    DzT_DyP = lambda u: -_shift(u, 0, 0, -1) + _shift(u, 0, 1, -1) + c(u) - _shift(u, 0, 1, 0)
+    DxT_DzP = lambda u: -_shift(u, -1, 0, 0) + c(u) + _shift(u, -1, 0, 1) - _shift(u, 0, 0, 1)
+    DyT_DzP = lambda u: -_shift(u, 0, -1, 0) + c(u) + _shift(u, 0, -1, 1) - _shift(u, 0, 0, 1)
+    h2inv = (1.0 / h) ** 2
+    A1 = h2inv * (((Dyy(u1) + Dzz(u1)) - DyT_DxP(u2)) - DzT_DxP(u3))
+    A2 = h2inv * (((Dxx(u2) + Dzz(u2)) - DxT_DyP(u1)) - DzT_DyP(u3))
+    A3 = h2inv * (((Dxx(u3) + Dyy(u3)) - DxT_DzP(u1)) - DyT_DzP(u2))
+    return A1, A2, A3
+
+
+def sample_scalar_yee_material(P, epsC, alpha, h, n):
+    """Scalar eps on the three Yee edge families of the (2n+1)^3 interior
+    (exp3/sample_scalar_yee_material_exp4.m:4-21): eps_c = epsC exp(alpha/6 sum_l cos(P_l . x))
+    with x shifted by h/2 along axis c."""
+    idx = np.arange(-n, n + 1) * h
+    out = []
+    for comp in range(3):
+        ax = [idx + (h / 2 if comp == d else 0.0) for d in range(3)]
+        X, Y, Z = ax[0][:, None, None], ax[1][None, :, None], ax[2][None, None, :]
+        S = np.zeros((2 * n + 1,) * 3)
+        for l in range(P.shape[0]):
+            S = S + np.cos(P[l, 0] * X + P[l, 1] * Y + P[l, 2] * Z)
+        out.append(epsC * np.exp(alpha / 6.0 * S))
+    return out
+
+
+def lwrq_kernel(h, R=3, sigma_factor=2.0):
+    """k = exp(-(off/sigma)^2), off = (-R:R) h, sigma = sigma_factor h (exp3/compute_lwrq_exp4.m:12-15)."""
+    off = np.arange(-R, R + 1) * h
+    sigma = sigma_factor * h
+    return np.exp(-(off ** 2) / sigma ** 2)
+
+
+def separable_conv3_same(X, k):
+    """Zero-padded separable 3D convolution, same size (exp3/separable_conv3_same_exp4.m:3-6)."""
+    R = (len(k) - 1) // 2
+    Y = X
+    for axis in range(3):
+        Z = np.zeros_like(Y)
+        m = Y.shape[axis]
+        for t in range(-R, R + 1):
+            # Z[i] += k[t + R] * Y[i - t] (convn 'same' with the centred kernel)
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            lo, hi = max(0, t), min(m, m + t)
+            if lo >= hi:
+                continue
+            dst[axis] = slice(lo, hi)
+            src[axis] = slice(lo - t, hi - t)
+            Z[tuple(dst)] += k[t + R] * Y[tuple(src)]
+        Y = Z
+    return Y
+
+
+def compute_lwrq(a, b, h, R=3, sigma_factor=2.0, mass_floor_rel=1e-14, return_rho=False):
+    """Column-normalised local Gaussian Rayleigh quotient (exp3/compute_lwrq_exp4.m:12-56)."""
+    k = lwrq_kernel(h, R, sigma_factor)
+    den = separable_conv3_same(np.ones_like(a), k)
+    if not (np.all(np.isfinite(den)) and np.all(den > 0)):
+        raise ValueError("Invalid LWRQ column-normalization denominator.")
+    E = separable_conv3_same(a / den, k)
+    M = separable_conv3_same(b / den, k)
+    maxM = M.max()
+    if not np.all(np.isfinite(M)) or np.any(M <= mass_floor_rel * maxM):
+        raise ValueError("Nonpositive or numerically tiny local LWRQ mass encountered.")
+    sumE, sumM = E.sum(), M.sum()
+    lam = sumE / sumM
+    rho = E / M
+    V = np.sum(M * np.abs(rho - lam) ** 2) / (sumM * abs(lam) ** 2)
+    sumA, sumB = a.sum(), b.sum()
+    e_part = abs(sumE - sumA) / max(abs(sumA), EPS)
+    m_part = abs(sumM - sumB) / max(abs(sumB), EPS)
+    return dict(lambda_=lam, V=float(np.real(V)), nu=math.sqrt(max(float(np.real(V)), 0.0)),
+                partition_error=max(e_part, m_part), energy_partition_error=e_part,
+                mass_partition_error=m_part, rho=rho if return_rho else None)
+
+
+def compute_yee_lwrq_metrics(u1, u2, u3, lambda6d, h, n, P=P_REF, epsC=4.0, alpha=1.2, R=3,
+                             sigma_factor=2.0, mass_floor_rel=1e-14, return_rho=False):
+    """compute_yee_lwrq_metrics_exp4 (exp3/compute_yee_lwrq_metrics_exp4.m:16-70) on plus-grid fields."""
+    m = 2 * n + 3
+    for u in (u1, u2, u3):
+        if u.shape != (m, m, m):
+            raise ValueError("fieldPlus size does not match n")
+    A1, A2, A3 = apply_cropped_A(u1, u2, u3, h)
+    v1, v2, v3 = (_shift(u, 0, 0, 0) for u in (u1, u2, u3))
+    e1, e2, e3 = sample_scalar_yee_material(P, epsC, alpha, h, n)
+    B1, B2, B3 = e1 * v1, e2 * v2, e3 * v3
+    a = np.real(np.conj(v1) * A1 + np.conj(v2) * A2 + np.conj(v3) * A3)
+    b = np.real(np.conj(v1) * B1 + np.conj(v2) * B2 + np.conj(v3) * B3)
+    sumB = b.sum()
+    if sumB <= 0:
+        raise ValueError("Cropped Yee mass is not positive.")
+    lam_crop = a.sum() / sumB
+    L = compute_lwrq(a, b, h, R, sigma_factor, mass_floor_rel, return_rho)
+    lam3 = float(np.real(L["lambda_"]))
+    d_id = abs(lam3 - lam_crop) / max(abs(lam_crop), EPS)
+    nA = math.sqrt(sum(np.sum(np.abs(x) ** 2) for x in (A1, A2, A3)))
+    nB = math.sqrt(sum(np.sum(np.abs(x) ** 2) for x in (B1, B2, B3)))
+    res = lambda lam: sum(np.sum(np.abs(A - lam * B) ** 2) for A, B in ((A1, B1), (A2, B2), (A3, B3)))
+    eta3 = math.sqrt(res(lam3)) / (nA + abs(lam3) * nB)
+    eta6 = math.sqrt(res(lambda6d)) / (nA + abs(lambda6d) * nB)
+    return dict(lambda_crop=float(lam_crop), lambda_3d=lam3, eta_3d=eta3, eta_6d=eta6,
+                delta_lambda=abs(lam3 - lambda6d) / abs(lambda6d), V=L["V"], nu=L["nu"],
+                delta_part=L["partition_error"], delta_identity=d_id, cropped_mass=float(sumB),
+                energy_partition_error=L["energy_partition_error"],
+                mass_partition_error=L["mass_partition_error"], rho=L["rho"])
+
+
+def crop_plus_field(u, n_large, n_small):
+    """Centred crop of a same-h plus-grid field (exp3/crop_plus_field_exp4.m:3-6)."""
+    if n_small > n_large:
+        raise ValueError("nSmall must not exceed nLarge")
+    c = n_large + 1  # 0-based index of physical grid index zero
+    r = slice(c - (n_small + 1), c + (n_small + 1) + 1)
+    return u[r, r, r]

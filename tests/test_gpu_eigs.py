@@ -78,6 +78,11 @@ def test_eigs_golden_n2(pkg, n2):
     assert out["stats"]["converged"]
     assert abs(lam[0] - GOLDEN_N2[0]) / GOLDEN_N2[0] < 1e-10
     assert abs(lam[9] - GOLDEN_N2[1]) / GOLDEN_N2[1] < 1e-10
+    # Ritz vectors of an orthonormal basis: one Gram-Schmidt sweep per step (plus the
+    # DGKS-triggered second ones) keeps Q^H Q = I to rounding
+    Y = out["Y"].cpu().numpy()
+    assert np.max(np.abs(Y.conj() @ Y.T - np.eye(10))) < 1e-12
+    assert 0 <= out["stats"]["second_sweeps"] <= out["stats"]["outer_steps"]
     n2["lam"] = lam
     n2["Y"] = out["Y"]
 

@@ -294,3 +294,138 @@ def test_method_a_minres_and_dense_lambda_n1(oracle):
     Kr = np.stack([oracle.kr_apply(s, eps, np.eye(n2)[:, j].astype(complex), N) for j in range(n2)], 1)
     lamC = np.linalg.eigvalsh(Kr)[:10]
     assert np.allclose(lamA, lamC, rtol=1e-9)
+
+
+# ----------------------------------------------------------------------------- Yee closure
+def _yee_random_fields(rng, n):
+    m = 2 * n + 3
+    return [rng.standard_normal((m, m, m)) + 1j * rng.standard_normal((m, m, m)) for _ in range(3)]
+
+
+def _curl_transpose_curl(u1, u2, u3, h):
+    """Independent form of the cropped operator: face curls w = C u of the plus-grid field,
+    then C^T w on the interior edges (u1 at (i+1/2, j, k), u2 at (i, j+1/2, k), u3 at
+    (i, j, k+1/2); w1 at (i, j+1/2, k+1/2), w2 at (i+1/2, j, k+1/2), w3 at (i+1/2, j+1/2, k),
+    each stored at its lower corner index)."""
+    m = u1.shape[0]
+    W1 = np.zeros_like(u1)
+    W2 = np.zeros_like(u1)
+    W3 = np.zeros_like(u1)
+    W1[:, :-1, :-1] = (u3[:, 1:, :-1] - u3[:, :-1, :-1]) - (u2[:, :-1, 1:] - u2[:, :-1, :-1])
+    W2[:-1, :, :-1] = (u1[:-1, :, 1:] - u1[:-1, :, :-1]) - (u3[1:, :, :-1] - u3[:-1, :, :-1])
+    W3[:-1, :-1, :] = (u2[1:, :-1, :] - u2[:-1, :-1, :]) - (u1[:-1, 1:, :] - u1[:-1, :-1, :])
+
+    def S(W, di, dj, dk):
+        return W[1 + di:m - 1 + di, 1 + dj:m - 1 + dj, 1 + dk:m - 1 + dk]
+
+    A1 = -S(W2, 0, 0, 0) + S(W2, 0, 0, -1) + S(W3, 0, 0, 0) - S(W3, 0, -1, 0)
+    A2 = S(W1, 0, 0, 0) - S(W1, 0, 0, -1) - S(W3, 0, 0, 0) + S(W3, -1, 0, 0)
+    A3 = -S(W1, 0, 0, 0) + S(W1, 0, -1, 0) + S(W2, 0, 0, 0) - S(W2, -1, 0, 0)
+    return A1 / h ** 2, A2 / h ** 2, A3 / h ** 2
+
+
+def test_cropped_A_is_yee_curl_transpose_curl(oracle):
+    rng = np.random.default_rng(41)
+    n, h = 3, 0.1
+    u = _yee_random_fields(rng, n)
+    A = oracle.apply_cropped_A(*u, h)
+    B = _curl_transpose_curl(*u, h)
+    for a, b in zip(A, B):
+        assert a.shape == (2 * n + 1,) * 3
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+def test_cropped_A_annihilates_discrete_gradients(oracle):
+    rng = np.random.default_rng(42)
+    n, h = 4, 0.05
+    m = 2 * n + 3
+    phi = rng.standard_normal((m + 1,) * 3) + 1j * rng.standard_normal((m + 1,) * 3)
+    u1 = (phi[1:, :m, :m] - phi[:m, :m, :m]) / h
+    u2 = (phi[:m, 1:, :m] - phi[:m, :m, :m]) / h
+    u3 = (phi[:m, :m, 1:] - phi[:m, :m, :m]) / h
+    scale = max(np.max(np.abs(x)) for x in (u1, u2, u3)) / h ** 2
+    for a in oracle.apply_cropped_A(u1, u2, u3, h):
+        assert np.max(np.abs(a)) <= 1e-13 * scale
+
+
+def _plane_wave(n, h, kvec, p):
+    """Staggered plane wave u_c = p_c exp(i k . x_c) on the plus grid and its discrete symbol
+    sigma_d = (2/h) sin(k_d h/2)."""
+    I = np.arange(-(n + 1), n + 2) * h
+    out = []
+    for c in range(3):
+        ax = [I + (h / 2 if d == c else 0.0) for d in range(3)]
+        ph = kvec[0] * ax[0][:, None, None] + kvec[1] * ax[1][None, :, None] + kvec[2] * ax[2][None, None, :]
+        out.append(p[c] * np.exp(1j * ph))
+    sigma = 2.0 / h * np.sin(np.asarray(kvec) * h / 2)
+    return out, sigma
+
+
+def test_plane_wave_is_exact_yee_eigenpair(oracle):
+    """A transverse staggered plane wave in constant eps: A u = |sigma|^2 u exactly, so the
+    metrics reduce to lambda = |sigma|^2/epsC, rho = lambda, V = 0 and zero residuals."""
+    n, h = 5, 0.1
+    kvec = np.array([1.3, -0.7, 2.1])
+    sigma = 2.0 / h * np.sin(kvec * h / 2)
+    p = np.cross(sigma, [0.3, 1.0, -0.4]) * (1.0 + 0.5j)
+    u, sigma = _plane_wave(n, h, kvec, p)
+    lam = float(sigma @ sigma) / 4.0
+    A = oracle.apply_cropped_A(*u, h)
+    for a, x in zip(A, u):
+        xi = x[1:-1, 1:-1, 1:-1]
+        assert np.max(np.abs(a - 4.0 * lam * xi)) <= 1e-11 * 4.0 * lam * np.max(np.abs(xi))
+    r = oracle.compute_yee_lwrq_metrics(*u, lam, h, n, epsC=4.0, alpha=0.0, return_rho=True)
+    assert abs(r["lambda_3d"] - lam) <= 1e-12 * lam
+    assert r["eta_3d"] < 1e-12 and r["eta_6d"] < 1e-12 and r["delta_lambda"] < 1e-12
+    assert r["V"] < 1e-24 and r["delta_identity"] < 1e-13 and r["delta_part"] < 1e-13
+    assert np.max(np.abs(r["rho"] - lam)) <= 1e-12 * lam
+
+
+def test_separable_convolution_is_3d_convolution(oracle):
+    from scipy import ndimage
+    rng = np.random.default_rng(43)
+    X = rng.standard_normal((9, 11, 10))
+    k = oracle.lwrq_kernel(0.05, 3, 2.0)
+    K3 = k[:, None, None] * k[None, :, None] * k[None, None, :]
+    ref = ndimage.convolve(X, K3, mode="constant", cval=0.0)
+    assert np.max(np.abs(oracle.separable_conv3_same(X, k) - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # den = conv(ones) is the product of one profile per axis
+    s = np.array([sum(k[t + 3] for t in range(-3, 4) if 0 <= i - t < 13) for i in range(13)])
+    den = oracle.separable_conv3_same(np.ones((13, 13, 13)), k)
+    assert np.max(np.abs(den - s[:, None, None] * s[None, :, None] * s[None, None, :])) < 1e-14 * den.max()
+
+
+def test_lwrq_constant_quotient_and_partition(oracle):
+    rng = np.random.default_rng(44)
+    b = 1.0 + rng.random((9, 9, 9))
+    r = oracle.compute_lwrq(2.5 * b, b, 0.1, return_rho=True)
+    assert abs(r["lambda_"] - 2.5) < 1e-14 and r["V"] < 1e-28
+    assert np.max(np.abs(r["rho"] - 2.5)) < 1e-13
+    a = rng.standard_normal((9, 9, 9))
+    r = oracle.compute_lwrq(a, b, 0.1)
+    assert r["partition_error"] < 1e-13  # column normalisation preserves both sums
+    with pytest.raises(ValueError):
+        oracle.compute_lwrq(a, np.zeros_like(b), 0.1)
+
+
+def test_yee_material_closed_forms(oracle):
+    n, h = 3, 0.2
+    e = oracle.sample_scalar_yee_material(oracle.P_REF, 4.0, 0.0, h, n)
+    assert all(np.all(x == 4.0) for x in e)
+    e = oracle.sample_scalar_yee_material(oracle.P_REF, 4.0, 1.2, h, n)
+    for comp, (i, j, k) in [(0, (0, 3, 6)), (1, (5, 1, 2)), (2, (6, 6, 0))]:
+        x = [(i - n) * h, (j - n) * h, (k - n) * h]
+        x[comp] += h / 2
+        S = sum(math.cos(sum(oracle.P_REF[l, d] * x[d] for d in range(3))) for l in range(6))
+        assert abs(e[comp][i, j, k] - 4.0 * math.exp(0.2 * S)) < 1e-14 * 4.0 * math.exp(0.2 * S)
+
+
+def test_crop_plus_field(oracle):
+    n_large, n_small = 6, 2
+    m = 2 * n_large + 3
+    I = np.arange(-(n_large + 1), n_large + 2)
+    u = I[:, None, None] * 10000 + I[None, :, None] * 100 + I[None, None, :] + 0.0
+    c = oracle.crop_plus_field(u, n_large, n_small)
+    J = np.arange(-(n_small + 1), n_small + 2)
+    assert np.array_equal(c, J[:, None, None] * 10000 + J[None, :, None] * 100 + J[None, None, :])
+    assert m == u.shape[0]
