@@ -205,6 +205,84 @@ def extras(torch, pkg):
                 batched_vs_sequential_max_rel_diff=maxdiff, taylor=rows)
 
 
+def plane_wave_fields(torch, n, h, kvec, p):
+    """Staggered plane wave u_c = p_c exp(i k . x_c) on the plus grid, flat column-major."""
+    I = torch.arange(-(n + 1), n + 2, dtype=torch.float64, device="cuda") * h
+    u = []
+    for c in range(3):
+        ax = [I + (h / 2 if d == c else 0.0) for d in range(3)]
+        ph = kvec[0] * ax[0][None, None, :] + kvec[1] * ax[1][None, :, None] + kvec[2] * ax[2][:, None, None]
+        u.append((complex(p[c]) * torch.exp(1j * ph)).reshape(-1))
+    return u
+
+
+def yee_closure(torch, pkg):
+    """The mesh study of the physical-space closure (exp3/run_experiment4.m:40-95 with
+    exp3/make_config_experiment4.m:17-32,58-85): N = 5, q = (0.13, 0.21, 0.17),
+    eps = 4 exp(1.2/6 sum cos), k = 12 eigenpairs (mode 11 is the reference's tracked N=5
+    mode, lambda = 1.200090550277e-02), Taylor p = 10 with the automatic phase-radius
+    partition, cropped Yee / LWRQ metrics for h = 0.1, 0.05, 0.025, 0.0125 at half-width 0.5.
+    Then the closure kernels alone on the largest window of the window study (n = 160,
+    h = 0.025) with an exact discrete plane wave: algorithmic bytes 192 per interior point
+    (fields 48 + densities 16 in pass 1, 32 + 32 for the x and y convolutions, 16 for z,
+    48 for the residual pass)."""
+    N = 5
+    eps = pkg.smooth_scalar_material(N)
+    mass = pkg.MassBackend(N, eps)
+    mass.autotune(reps=3)
+    sq = pkg.QSystem(mass, (0.13, 0.21, 0.17))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = pkg.run_method_C(sq, k=12, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
+                           v0=pkg.start_vector(sq.nr, 20260831), want_vectors=True)
+    torch.cuda.synchronize()
+    t_eig = time.perf_counter() - t0
+    lam = out["lambda_"]
+    modes = [0, 10]
+    X, _, _ = pkg.recover_original_eigenvectors(out["Y"][modes], sq)
+    T = pkg.taylor
+    q_d = torch.as_tensor(T.projected_q_modes(N, sq.info["kLift"], pkg.P_DEFAULT).T.copy(), device="cuda")
+    rows = []
+    for h, n in ((0.1, 5), (0.05, 10), (0.025, 20), (0.0125, 40)):
+        for i, mode in enumerate(modes):
+            torch.cuda.synchronize()
+            a0 = time.perf_counter()
+            u, nb = T.reconstruct_yee_field(X[i], q_d, n, h)
+            torch.cuda.synchronize()
+            a1 = time.perf_counter()
+            met = pkg.compute_yee_lwrq_metrics(*u, float(lam[mode]), n, h)
+            torch.cuda.synchronize()
+            a2 = time.perf_counter()
+            rows.append(dict(mode=mode + 1, h=h, n=n, blocks=nb, reconstruct_s=a1 - a0, metrics_s=a2 - a1,
+                             lambda_6d=float(lam[mode]), lambda_3d=met["lambda_3d"], eta_3d=met["eta_3d"],
+                             eta_6d=met["eta_6d"], nu=met["nu"], delta_lambda=met["delta_lambda"],
+                             delta_identity=met["delta_identity"], delta_part=met["delta_part"]))
+            del u
+    sq.close()
+    mass.close()
+    # closure kernels on the n = 160 window
+    n, h = 160, 0.025
+    kvec = np.array([1.3, -0.7, 2.1])
+    sig = 2.0 / h * np.sin(kvec * h / 2)
+    pw = np.cross(sig, [0.3, 1.0, -0.4])
+    u = plane_wave_fields(torch, n, h, kvec, pw / np.linalg.norm(pw))
+    work = pkg.yee.workspace(n)
+    lam_pw = float(sig @ sig) / 4.0
+    pkg.compute_yee_lwrq_metrics(*u, lam_pw, n, h, eps_c=4.0, alpha=0.0, work=work)
+    reps = 5
+    torch.cuda.synchronize()
+    a0 = time.perf_counter()
+    for _ in range(reps):
+        met = pkg.compute_yee_lwrq_metrics(*u, lam_pw, n, h, eps_c=4.0, alpha=1.2, work=work)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - a0) / reps
+    q3 = (2 * n + 1) ** 3
+    return dict(eigensolve_k12_seconds=t_eig, lambda_11=float(lam[10]),
+                lambda_11_rel_diff_vs_reference=abs(float(lam[10]) - 1.200090550277e-02) / 1.200090550277e-02,
+                mesh=rows, window_n160=dict(n=n, h=h, interior_points=q3, seconds=dt,
+                                           algorithmic_GBs=192.0 * q3 / dt / 1e9))
+
+
 def method_comparison(torch, pkg, N, lam_c):
     """Methods A (MINRES shift-invert on the full 3NF space) and B (nested PCG) on the
     benchmark configuration, against the Method-C eigenvalues (the reference warns above
@@ -432,6 +510,7 @@ def main():
     ext = extras(torch, pkg) if a.extras else None
     if ext is not None:
         ext["methods_vs_C"] = method_comparison(torch, pkg, N, lam64)
+        ext["yee_closure"] = yee_closure(torch, pkg)
 
     line = {
         "metric": "N=5 time-to-10-eigenpairs + M-apply HBM GB/s; mixed vs FP64 at 1e-8 rel",

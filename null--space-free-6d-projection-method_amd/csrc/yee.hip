@@ -7,14 +7,23 @@
 //
 // Layout: a plus-grid field is the column-major vectorisation of a (2n+3)^3 array (x
 // fastest), the window interior is its (2n+1)^3 core; densities and rho are column-major
-// (2n+1)^3 real arrays.  All work is HBM-bound stencil and reduction traffic:
-//   pass 1   u (3 complex fields, 48 B/point) -> a, b (16 B/point) + sums of a, b
-//   x, y, z  separable 7-tap convolution of a/den and b/den (den = conv(ones) is the
-//            product of one 1-D profile per axis, evaluated in closed form), 16+16 B/point
-//            for x and y; the z pass reduces E, M on the fly and writes nothing (rho only
-//            when asked)
-//   pass 3   u again (48 B/point) -> |Au - lambda Bu|^2 for lambda_3D and lambda_6D,
-//            ||Au||^2, ||Bu||^2 (recomputed rather than stored: 48 B read < 96 B write+read)
+// (2n+1)^3 real arrays.  The work is HBM-bound stencil and reduction traffic (per interior
+// point):
+//   pass 1   u (3 complex fields, 48 B) -> a/den, b/den (16 B); sums of a, b and of the
+//            residual terms about the 6D eigenvalue (below)
+//   x        convolution of the two densities along x (16 B in, 16 B out)
+//   y        the same along y, marched through registers (16 + 16 B)
+//   z        the same along z with the LWRQ reductions, nothing written (16 B; rho when asked)
+// den = conv(ones) is the product of one 1-D profile per axis and is applied in closed form.
+// The residuals ||Au - lam Bu||^2 for lam_3D (known only after the LWRQ) follow from sums
+// taken in pass 1 about lam_0 = lam_6D:
+//   ||Au - lam Bu||^2 = S0 - 2 d S1 + d^2 S2,  d = lam - lam_0,
+//   S0 = ||Au - lam_0 Bu||^2, S1 = Re <Au - lam_0 Bu, Bu>, S2 = ||Bu||^2,
+// which is exact algebra; when the three terms cancel to less than 1e-6 of their size the
+// residual is recomputed by a second pass over u (48 B) instead.
+//
+// Launch shape: workgroups of 64 x 4 threads own (x, y) tiles of 64 x 4 points (one wave per
+// row segment, coalesced) and loop over the third index, so no index is ever divided.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -24,27 +33,37 @@
 
 namespace fame6d {
 
-constexpr int kYeeThreads = 256;
+constexpr int kYeeTX = 64, kYeeTY = 4, kYeeThreads = kYeeTX * kYeeTY;
 constexpr int kYeeMaxBlocks = 2048;
-constexpr int kYeeMaxTaps = 17;   // LWRQ neighbour radius <= 8 cells
-constexpr int kYeeMaxRows = 16;   // rows of the 3D-to-6D projection P
+constexpr int kYeeMaxR = 8;        // LWRQ neighbour radius <= 8 cells
+constexpr int kYeeMaxRows = 16;    // rows of the 3D-to-6D projection P
 
 struct YeeGeom {
   long long q, m;  // interior and plus-grid extents
   double h2inv;
 };
 
-// Permittivity eps_c(x) = epsC exp(alpha/6 sum_l cos(P_l . x)) from per-axis tables:
-// tab[((s*3 + d)*L + l)*q + i] = (cos, sin)(P(l,d) * x_d(i)), x_d(i) = (i - n) h + s h/2.
-// cos(a + b + c) is expanded into the products of the three axis factors.
+// Permittivity eps_c(x) = epsC exp(alpha/6 sum_l cos(P_l . x)), x_d = (i_d - n) h (+ h/2 on
+// axis c).  When every row of P has at most one non-zero the sum splits over the axes and so
+// does the exponential: eps_c = epsC E[s_0][0][i] E[s_1][1][j] E[s_2][2][k] with
+// E[s][d][i] = exp(alpha/6 sum_{rows l on axis d} cos(P(l,d) x_d(i))), s_d = (d == c).
+// Otherwise tab[((s*3 + d)*L + l)*q + i] = (cos, sin)(P(l,d) x_d(i)) and cos(a + b + c) is
+// expanded into products of the three axis factors.
 struct YeeMat {
-  const double2* tab;
+  const double* sep;    // 6 x q separable factors (null: general form)
+  const double2* tab;   // 6 x L x q (cos, sin)
   int L;
   double epsC, a6;
 };
 
 __device__ __forceinline__ double yee_eps(const YeeMat& mt, long long q, int comp, long long i, long long j,
                                           long long k) {
+  if (mt.sep) {
+    const double ex = mt.sep[((comp == 0 ? 3 : 0) + 0) * q + i];
+    const double ey = mt.sep[((comp == 1 ? 3 : 0) + 1) * q + j];
+    const double ez = mt.sep[((comp == 2 ? 3 : 0) + 2) * q + k];
+    return mt.epsC * (ex * ey * ez);
+  }
   const long long axis_stride = (long long)mt.L * q;
   const double2* tx = mt.tab + ((comp == 0 ? 3 : 0) + 0) * axis_stride;
   const double2* ty = mt.tab + ((comp == 1 ? 3 : 0) + 1) * axis_stride;
@@ -57,57 +76,57 @@ __device__ __forceinline__ double yee_eps(const YeeMat& mt, long long q, int com
   return mt.epsC * exp(mt.a6 * S);
 }
 
-// Cropped curl-curl at interior point (i, j, k); c is its plus-grid offset.  Operator
-// order and signs as exp3/apply_cropped_A_exp3.m:11-28 with the difference operators of
-// :32-56 (each a combination of the point and its neighbours at offsets in {-1, 0, 1}).
-__device__ __forceinline__ void yee_curlcurl(const cpx<double>* u1, const cpx<double>* u2, const cpx<double>* u3,
-                                             long long c, long long sy, long long sz, double h2inv,
-                                             cpx<double>& A1, cpx<double>& A2, cpx<double>& A3) {
-  const long long sx = 1;
-  // u1: Dyy, Dzz, DxT_DyP, DxT_DzP
+// Cropped curl-curl at the interior point with plus-grid offset c.  Operator order and signs
+// as exp3/apply_cropped_A_exp3.m:11-28 with the difference operators of :32-56 (each a
+// combination of the point and its neighbours at offsets in {-1, 0, 1}).
+__device__ __forceinline__ void yee_curlcurl(const cpx<double>* __restrict__ u1, const cpx<double>* __restrict__ u2,
+                                             const cpx<double>* __restrict__ u3, long long c, long long sy,
+                                             long long sz, double h2inv, cpx<double>& A1, cpx<double>& A2,
+                                             cpx<double>& A3) {
+  const cpx<double> u1c = u1[c], u2c = u2[c], u3c = u3[c];
+  const cpx<double> u1xm = u1[c - 1], u2xm = u2[c - 1], u3xm = u3[c - 1];
+  const cpx<double> u2xp = u2[c + 1], u3xp = u3[c + 1];
+  const cpx<double> u1ym = u1[c - sy], u1yp = u1[c + sy], u2ym = u2[c - sy], u3ym = u3[c - sy], u3yp = u3[c + sy];
+  const cpx<double> u1zm = u1[c - sz], u1zp = u1[c + sz], u2zm = u2[c - sz], u2zp = u2[c + sz], u3zm = u3[c - sz];
   {
-    const cpx<double> uc = u1[c];
-    const cpx<double> dyy = (-u1[c - sy] + 2.0 * uc) - u1[c + sy];
-    const cpx<double> dzz = (-u1[c - sz] + 2.0 * uc) - u1[c + sz];
-    const cpx<double> u2c = u2[c];
-    const cpx<double> dytdxp = ((u2[c + sx - sy] - u2[c - sy]) - u2[c + sx]) + u2c;
-    const cpx<double> u3c = u3[c];
-    const cpx<double> dztdxp = ((u3[c + sx - sz] - u3[c - sz]) - u3[c + sx]) + u3c;
+    const cpx<double> dyy = (-u1ym + 2.0 * u1c) - u1yp;
+    const cpx<double> dzz = (-u1zm + 2.0 * u1c) - u1zp;
+    const cpx<double> dytdxp = ((u2[c + 1 - sy] - u2ym) - u2xp) + u2c;
+    const cpx<double> dztdxp = ((u3[c + 1 - sz] - u3zm) - u3xp) + u3c;
     A1 = h2inv * (((dyy + dzz) - dytdxp) - dztdxp);
   }
   {
-    const cpx<double> uc = u2[c];
-    const cpx<double> dxx = (-u2[c - sx] + 2.0 * uc) - u2[c + sx];
-    const cpx<double> dzz = (-u2[c - sz] + 2.0 * uc) - u2[c + sz];
-    const cpx<double> u1c = u1[c];
-    const cpx<double> dxtdyp = ((-u1[c - sx] + u1c) + u1[c - sx + sy]) - u1[c + sy];
-    const cpx<double> u3c = u3[c];
-    const cpx<double> dztdyp = ((-u3[c - sz] + u3[c + sy - sz]) + u3c) - u3[c + sy];
+    const cpx<double> dxx = (-u2xm + 2.0 * u2c) - u2xp;
+    const cpx<double> dzz = (-u2zm + 2.0 * u2c) - u2zp;
+    const cpx<double> dxtdyp = ((-u1xm + u1c) + u1[c - 1 + sy]) - u1yp;
+    const cpx<double> dztdyp = ((-u3zm + u3[c + sy - sz]) + u3c) - u3yp;
     A2 = h2inv * (((dxx + dzz) - dxtdyp) - dztdyp);
   }
   {
-    const cpx<double> uc = u3[c];
-    const cpx<double> dxx = (-u3[c - sx] + 2.0 * uc) - u3[c + sx];
-    const cpx<double> dyy = (-u3[c - sy] + 2.0 * uc) - u3[c + sy];
-    const cpx<double> u1c = u1[c];
-    const cpx<double> dxtdzp = ((-u1[c - sx] + u1c) + u1[c - sx + sz]) - u1[c + sz];
-    const cpx<double> u2c = u2[c];
-    const cpx<double> dytdzp = ((-u2[c - sy] + u2c) + u2[c - sy + sz]) - u2[c + sz];
+    const cpx<double> dxx = (-u3xm + 2.0 * u3c) - u3xp;
+    const cpx<double> dyy = (-u3ym + 2.0 * u3c) - u3yp;
+    const cpx<double> dxtdzp = ((-u1xm + u1c) + u1[c - 1 + sz]) - u1zp;
+    const cpx<double> dytdzp = ((-u2ym + u2c) + u2[c - sy + sz]) - u2zp;
     A3 = h2inv * (((dxx + dyy) - dxtdzp) - dytdzp);
   }
 }
 
 __device__ __forceinline__ double abs2(cpx<double> z) { return z.re * z.re + z.im * z.im; }
+__device__ __forceinline__ double re_dot(cpx<double> a, cpx<double> b) { return a.re * b.re + a.im * b.im; }
+__device__ __forceinline__ int yee_tid() { return threadIdx.x + kYeeTX * threadIdx.y; }
+__device__ __forceinline__ int yee_block() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ __forceinline__ int yee_nblocks() { return gridDim.x * gridDim.y * gridDim.z; }
 
 // NV block sums at once; the totals end in v[] of thread 0.  red: NV * kYeeThreads.
 template <int NV>
 __device__ __forceinline__ void yee_block_sums(double (&v)[NV], double* red) {
-  for (int k = 0; k < NV; ++k) red[k * kYeeThreads + threadIdx.x] = v[k];
+  const int tid = yee_tid();
+  for (int k = 0; k < NV; ++k) red[k * kYeeThreads + tid] = v[k];
   __syncthreads();
-  if (threadIdx.x < 64) {
+  if (tid < 64) {
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int i = threadIdx.x; i < kYeeThreads; i += 64) s += red[k * kYeeThreads + i];
+      for (int i = tid; i < kYeeThreads; i += 64) s += red[k * kYeeThreads + i];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
       v[k] = s;
@@ -117,66 +136,74 @@ __device__ __forceinline__ void yee_block_sums(double (&v)[NV], double* red) {
 
 enum YeeMode : int { YEE_APPLY_A = 0, YEE_DENSITIES = 1, YEE_RESIDUALS = 2 };
 
-// One pass over the window interior.
+// One pass over the window interior; thread (i, j) loops over k.
 //   YEE_APPLY_A:   A1..A3 = cropped curl-curl of u
-//   YEE_DENSITIES: a = Re(u' A u), b = Re(u' B u) per point; partials [sum a, sum b]
-//   YEE_RESIDUALS: partials [|Au - l3 Bu|^2, |Au - l6 Bu|^2, |Au|^2, |Bu|^2]
+//   YEE_DENSITIES: a/den, b/den per point (a = Re(u' A u), b = Re(u' B u)); partials
+//                  [sum a, sum b, |Au - l0 Bu|^2, Re<Au - l0 Bu, Bu>, |Bu|^2, |Au|^2]
+//   YEE_RESIDUALS: partials [|Au - l0 Bu|^2]
 template <int MODE>
-__global__ __launch_bounds__(kYeeThreads) void k_yee_points(YeeGeom g, const cpx<double>* u1,
-                                                            const cpx<double>* u2, const cpx<double>* u3,
-                                                            YeeMat mt, cpx<double>* A1o, cpx<double>* A2o,
-                                                            cpx<double>* A3o, double* a_out, double* b_out,
-                                                            double lam3, double lam6, double* partials) {
-  constexpr int NV = MODE == YEE_RESIDUALS ? 4 : 2;
+__global__ __launch_bounds__(kYeeThreads) void k_yee_points(YeeGeom g, const cpx<double>* __restrict__ u1,
+                                                            const cpx<double>* __restrict__ u2,
+                                                            const cpx<double>* __restrict__ u3, YeeMat mt,
+                                                            cpx<double>* A1o, cpx<double>* A2o, cpx<double>* A3o,
+                                                            const double* prof, double* a_out, double* b_out,
+                                                            double lam0, double* partials) {
+  constexpr int NV = MODE == YEE_DENSITIES ? 6 : 1;
   __shared__ double red[NV * kYeeThreads];
-  const long long q = g.q, m = g.m, sy = m, sz = m * m, Q3 = q * q * q;
+  const long long q = g.q, m = g.m, sy = m, sz = m * m;
+  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, j = blockIdx.y * (long long)kYeeTY + threadIdx.y;
   double acc[NV];
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (long long t = blockIdx.x * (long long)kYeeThreads + threadIdx.x; t < Q3;
-       t += (long long)gridDim.x * kYeeThreads) {
-    const long long i = t % q, r = t / q, j = r % q, k = r / q;
-    const long long c = (i + 1) + m * ((j + 1) + m * (k + 1));
-    cpx<double> A1, A2, A3;
-    yee_curlcurl(u1, u2, u3, c, sy, sz, g.h2inv, A1, A2, A3);
-    if constexpr (MODE == YEE_APPLY_A) {
-      A1o[t] = A1;
-      A2o[t] = A2;
-      A3o[t] = A3;
-    } else {
-      const cpx<double> v1 = u1[c], v2 = u2[c], v3 = u3[c];
-      const double e1 = yee_eps(mt, q, 0, i, j, k), e2 = yee_eps(mt, q, 1, i, j, k), e3 = yee_eps(mt, q, 2, i, j, k);
-      const cpx<double> B1 = e1 * v1, B2 = e2 * v2, B3 = e3 * v3;
-      if constexpr (MODE == YEE_DENSITIES) {
-        const double a = (v1.re * A1.re + v1.im * A1.im) + (v2.re * A2.re + v2.im * A2.im) +
-                         (v3.re * A3.re + v3.im * A3.im);
-        const double b = (v1.re * B1.re + v1.im * B1.im) + (v2.re * B2.re + v2.im * B2.im) +
-                         (v3.re * B3.re + v3.im * B3.im);
-        a_out[t] = a;
-        b_out[t] = b;
-        acc[0] += a;
-        acc[1] += b;
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  if (i < q && j < q) {
+    const double pij = prof ? prof[i] * prof[j] : 1.0;
+    for (long long k = blockIdx.z; k < q; k += gridDim.z) {
+      const long long t = i + q * (j + q * k);
+      const long long c = (i + 1) + m * ((j + 1) + m * (k + 1));
+      cpx<double> A1, A2, A3;
+      yee_curlcurl(u1, u2, u3, c, sy, sz, g.h2inv, A1, A2, A3);
+      if constexpr (MODE == YEE_APPLY_A) {
+        A1o[t] = A1;
+        A2o[t] = A2;
+        A3o[t] = A3;
       } else {
-        acc[0] += abs2(A1 - lam3 * B1) + abs2(A2 - lam3 * B2) + abs2(A3 - lam3 * B3);
-        acc[1] += abs2(A1 - lam6 * B1) + abs2(A2 - lam6 * B2) + abs2(A3 - lam6 * B3);
-        acc[2] += abs2(A1) + abs2(A2) + abs2(A3);
-        acc[3] += abs2(B1) + abs2(B2) + abs2(B3);
+        const cpx<double> v1 = u1[c], v2 = u2[c], v3 = u3[c];
+        const cpx<double> B1 = yee_eps(mt, q, 0, i, j, k) * v1, B2 = yee_eps(mt, q, 1, i, j, k) * v2,
+                          B3 = yee_eps(mt, q, 2, i, j, k) * v3;
+        const cpx<double> R1 = A1 - lam0 * B1, R2 = A2 - lam0 * B2, R3 = A3 - lam0 * B3;
+        if constexpr (MODE == YEE_DENSITIES) {
+          const double a = (re_dot(v1, A1) + re_dot(v2, A2)) + re_dot(v3, A3);
+          const double b = (re_dot(v1, B1) + re_dot(v2, B2)) + re_dot(v3, B3);
+          const double den = pij * prof[k];
+          a_out[t] = a / den;
+          b_out[t] = b / den;
+          acc[0] += a;
+          acc[1] += b;
+          acc[2] += abs2(R1) + abs2(R2) + abs2(R3);
+          acc[3] += re_dot(R1, B1) + re_dot(R2, B2) + re_dot(R3, B3);
+          acc[4] += abs2(B1) + abs2(B2) + abs2(B3);
+          acc[5] += abs2(A1) + abs2(A2) + abs2(A3);
+        } else {
+          acc[0] += abs2(R1) + abs2(R2) + abs2(R3);
+        }
       }
     }
   }
   if constexpr (MODE != YEE_APPLY_A) {
     yee_block_sums<NV>(acc, red);
-    if (threadIdx.x == 0)
-      for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = acc[k];
+    if (yee_tid() == 0) {
+      const int b = yee_block(), nb = yee_nblocks();
+      for (int v = 0; v < NV; ++v) partials[v * nb + b] = acc[v];
+    }
   }
 }
 
 // The three permittivity families on the interior (exp3/sample_scalar_yee_material_exp4.m:7-9).
 __global__ __launch_bounds__(kYeeThreads) void k_yee_eps(long long q, YeeMat mt, double* e1, double* e2,
                                                          double* e3) {
-  const long long Q3 = q * q * q;
-  for (long long t = blockIdx.x * (long long)kYeeThreads + threadIdx.x; t < Q3;
-       t += (long long)gridDim.x * kYeeThreads) {
-    const long long i = t % q, r = t / q, j = r % q, k = r / q;
+  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, j = blockIdx.y * (long long)kYeeTY + threadIdx.y;
+  if (i >= q || j >= q) return;
+  for (long long k = blockIdx.z; k < q; k += gridDim.z) {
+    const long long t = i + q * (j + q * k);
     e1[t] = yee_eps(mt, q, 0, i, j, k);
     e2[t] = yee_eps(mt, q, 1, i, j, k);
     e3[t] = yee_eps(mt, q, 2, i, j, k);
@@ -184,107 +211,147 @@ __global__ __launch_bounds__(kYeeThreads) void k_yee_eps(long long q, YeeMat mt,
 }
 
 struct Taps {
-  double k[kYeeMaxTaps];
+  double k[2 * kYeeMaxR + 1];
   int R;
 };
 
-// Zero-padded 'same' convolution along one axis (stride s, extent q) of two arrays.
-// With `prof`: each source value is first divided by den = prof(i) prof(j) prof(k) (the
-// column normalisation of exp3/compute_lwrq_exp4.m:22-26) and partials [sum in_a, sum in_b]
-// of the raw inputs are written when `partials` is set.
-__global__ __launch_bounds__(kYeeThreads) void k_lwrq_conv(long long q, long long s, Taps tp, const double* ina,
-                                                           const double* inb, double* outa, double* outb,
-                                                           const double* prof, double* partials) {
+// Zero-padded 'same' convolution of two arrays along x; thread (i, j) loops over k.  With
+// `prof` every source value is first divided by den = prof(i') prof(j) prof(k) (the column
+// normalisation of exp3/compute_lwrq_exp4.m:22-26) and partials [sum in_a, sum in_b] of the
+// raw inputs are written.
+__global__ __launch_bounds__(kYeeThreads) void k_lwrq_conv_x(long long q, Taps tp, const double* __restrict__ ina,
+                                                             const double* __restrict__ inb, double* outa,
+                                                             double* outb, const double* prof, double* partials) {
   __shared__ double red[2 * kYeeThreads];
-  const long long Q3 = q * q * q;
+  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, j = blockIdx.y * (long long)kYeeTY + threadIdx.y;
   double acc[2] = {0.0, 0.0};
-  for (long long t = blockIdx.x * (long long)kYeeThreads + threadIdx.x; t < Q3;
-       t += (long long)gridDim.x * kYeeThreads) {
-    const long long i = t % q, r = t / q, j = r % q, k = r / q;
-    const long long pos = (s == 1) ? i : (s == q ? j : k);
-    double sa = 0.0, sb = 0.0;
-    for (int tau = -tp.R; tau <= tp.R; ++tau) {
-      const long long src = pos - tau;
-      if (src < 0 || src >= q) continue;
-      const long long ts = t - tau * s;
-      double va = ina[ts], vb = inb[ts];
-      if (prof) {
-        const double den = (s == 1 ? prof[src] * prof[j] * prof[k]
-                                   : (s == q ? prof[i] * prof[src] * prof[k] : prof[i] * prof[j] * prof[src]));
-        va = va / den;
-        vb = vb / den;
+  if (i < q && j < q) {
+    for (long long k = blockIdx.z; k < q; k += gridDim.z) {
+      const long long t = i + q * (j + q * k);
+      const double pjk = prof ? prof[j] * prof[k] : 1.0;
+      double sa = 0.0, sb = 0.0;
+      for (int tau = -tp.R; tau <= tp.R; ++tau) {
+        const long long src = i - tau;
+        if (src < 0 || src >= q) continue;
+        double va = ina[t - tau], vb = inb[t - tau];
+        if (prof) {
+          const double den = prof[src] * pjk;
+          va = va / den;
+          vb = vb / den;
+        }
+        sa += tp.k[tau + tp.R] * va;
+        sb += tp.k[tau + tp.R] * vb;
       }
-      sa += tp.k[tau + tp.R] * va;
-      sb += tp.k[tau + tp.R] * vb;
-    }
-    outa[t] = sa;
-    outb[t] = sb;
-    if (partials) {
-      acc[0] += ina[t];
-      acc[1] += inb[t];
+      outa[t] = sa;
+      outb[t] = sb;
+      if (partials) {
+        acc[0] += ina[t];
+        acc[1] += inb[t];
+      }
     }
   }
   if (partials) {
     yee_block_sums<2>(acc, red);
-    if (threadIdx.x == 0) {
-      partials[blockIdx.x] = acc[0];
-      partials[gridDim.x + blockIdx.x] = acc[1];
+    if (yee_tid() == 0) {
+      const int b = yee_block(), nb = yee_nblocks();
+      partials[b] = acc[0];
+      partials[nb + b] = acc[1];
     }
   }
 }
 
-// Last (z) convolution with the LWRQ reductions: E, M at every point, rho = E/M (written
-// when asked) and partials [sum E, sum M, sum (E - lam_a M)^2 / M, sum (E - lam_a M),
-// max M, min M, non-finite count], lam_a = sc[2].
-__global__ __launch_bounds__(kYeeThreads) void k_lwrq_conv_z(long long q, Taps tp, const double* ina,
-                                                             const double* inb, const double* sc, double* rho,
-                                                             double* partials) {
-  __shared__ double red[5 * kYeeThreads];
-  __shared__ double redm[2 * kYeeThreads];
-  const long long s = q * q, Q3 = q * q * q;
-  const double lam = sc[2];
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // E, M, dev2, dev1, bad
-  double mx = -INFINITY, mn = INFINITY;
-  for (long long t = blockIdx.x * (long long)kYeeThreads + threadIdx.x; t < Q3;
-       t += (long long)gridDim.x * kYeeThreads) {
-    const long long k = t / s;
-    double E = 0.0, M = 0.0;
-    for (int tau = -tp.R; tau <= tp.R; ++tau) {
-      const long long src = k - tau;
-      if (src < 0 || src >= q) continue;
-      const long long ts = t - tau * s;
-      E += tp.k[tau + tp.R] * ina[ts];
-      M += tp.k[tau + tp.R] * inb[ts];
-    }
-    const double d = E - lam * M;
-    acc[0] += E;
-    acc[1] += M;
-    acc[2] += d * d / M;
-    acc[3] += d;
-    if (!isfinite(M) || !isfinite(E)) acc[4] += 1.0;
-    mx = fmax(mx, M);
-    mn = fmin(mn, M);
-    if (rho) rho[t] = E / M;
-  }
-  yee_block_sums<5>(acc, red);
-  redm[threadIdx.x] = mx;
-  redm[kYeeThreads + threadIdx.x] = mn;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    for (int i = threadIdx.x + 64; i < kYeeThreads; i += 64) {
-      mx = fmax(mx, redm[i]);
-      mn = fmin(mn, redm[kYeeThreads + i]);
-    }
+// Convolution along y (AXIS 1) or z (AXIS 2) of two arrays: thread (i, o) walks the axis
+// through chunk blockIdx.z (length `chunk`) with the 2R+1 taps' inputs held in registers,
+// one load per array per point.  o is k for AXIS 1 and j for AXIS 2.
+// REDUCE (the z pass of the LWRQ): instead of writing, E and M are reduced into partials
+// [sum E, sum M, sum (E - lam_a M)^2 / M, sum (E - lam_a M), non-finite count, max M, min M]
+// (lam_a = sc[2]) and rho = E/M is written when asked.
+template <int R, int AXIS, bool REDUCE>
+__global__ __launch_bounds__(kYeeThreads) void k_lwrq_march(long long q, Taps tp, const double* __restrict__ ina,
+                                                            const double* __restrict__ inb, double* outa,
+                                                            double* outb, long long chunk, const double* sc,
+                                                            double* rho, double* partials) {
+  constexpr int W = 2 * R + 1;
+  constexpr int NV = 5;
+  __shared__ double red[REDUCE ? NV * kYeeThreads : 1];
+  __shared__ double redm[REDUCE ? 2 * kYeeThreads : 1];
+  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, o = blockIdx.y * (long long)kYeeTY + threadIdx.y;
+  const long long s = AXIS == 1 ? q : q * q;
+  const long long base = AXIS == 1 ? i + q * q * o : i + q * o;
+  const long long p0 = blockIdx.z * chunk, p1 = p0 + chunk < q ? p0 + chunk : q;
+  double kk[W];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      mx = fmax(mx, __shfl_down(mx, o, 64));
-      mn = fmin(mn, __shfl_down(mn, o, 64));
+  for (int t = 0; t < W; ++t) kk[t] = tp.k[t];
+  double acc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double mx = -INFINITY, mn = INFINITY;
+  const double lam = REDUCE ? sc[2] : 0.0;
+  if (i < q && o < q && p0 < q) {
+    // window w[t] = in(p - R + t); before the first step it holds in(p0 - R - 1 .. p0 + R - 1)
+    double wa[W], wb[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+      const long long src = p0 - R - 1 + t;
+      const bool ok = src >= 0 && src < q;
+      wa[t] = ok ? ina[base + src * s] : 0.0;
+      wb[t] = ok ? inb[base + src * s] : 0.0;
+    }
+    for (long long p = p0; p < p1; ++p) {
+#pragma unroll
+      for (int t = 0; t + 1 < W; ++t) {
+        wa[t] = wa[t + 1];
+        wb[t] = wb[t + 1];
+      }
+      const long long src = p + R;
+      const bool ok = src < q;
+      wa[W - 1] = ok ? ina[base + src * s] : 0.0;
+      wb[W - 1] = ok ? inb[base + src * s] : 0.0;
+      double E = 0.0, M = 0.0;
+#pragma unroll
+      for (int t = 0; t < W; ++t) {
+        // in(p - R + t) carries tap index R - (R - t) ... the taps are symmetric: k[t] = k[W-1-t]
+        E += kk[W - 1 - t] * wa[t];
+        M += kk[W - 1 - t] * wb[t];
+      }
+      const long long pt = base + p * s;
+      if constexpr (REDUCE) {
+        const double d = E - lam * M;
+        acc[0] += E;
+        acc[1] += M;
+        acc[2] += d * d / M;
+        acc[3] += d;
+        if (!isfinite(M) || !isfinite(E)) acc[4] += 1.0;
+        mx = fmax(mx, M);
+        mn = fmin(mn, M);
+        if (rho) rho[pt] = E / M;
+      } else {
+        outa[pt] = E;
+        outb[pt] = M;
+      }
     }
   }
-  if (threadIdx.x == 0) {
-    for (int v = 0; v < 5; ++v) partials[v * gridDim.x + blockIdx.x] = acc[v];
-    partials[5 * gridDim.x + blockIdx.x] = mx;
-    partials[6 * gridDim.x + blockIdx.x] = mn;
+  if constexpr (REDUCE) {
+    yee_block_sums<NV>(acc, red);
+    const int tid = yee_tid();
+    redm[tid] = mx;
+    redm[kYeeThreads + tid] = mn;
+    __syncthreads();
+    if (tid < 64) {
+      for (int t = tid + 64; t < kYeeThreads; t += 64) {
+        mx = fmax(mx, redm[t]);
+        mn = fmin(mn, redm[kYeeThreads + t]);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, off, 64));
+        mn = fmin(mn, __shfl_down(mn, off, 64));
+      }
+    }
+    if (tid == 0) {
+      const int b = yee_block(), nb = yee_nblocks();
+      for (int v = 0; v < NV; ++v) partials[v * nb + b] = acc[v];
+      partials[5 * nb + b] = mx;
+      partials[6 * nb + b] = mn;
+    }
   }
 }
 
@@ -293,12 +360,11 @@ struct Kinds {
 };
 
 // out[v] = reduction of partials[v * nb ...]: kind 0 sum, 1 max, 2 min (one workgroup per value).
-__global__ __launch_bounds__(kYeeThreads) void k_yee_finish(const double* partials, int nb, Kinds kinds,
-                                                            double* out) {
-  __shared__ double red[kYeeThreads];
+__global__ __launch_bounds__(256) void k_yee_finish(const double* partials, int nb, Kinds kinds, double* out) {
+  __shared__ double red[256];
   const int v = blockIdx.x, kind = kinds.k[v];
   double s = kind == 0 ? 0.0 : (kind == 1 ? -INFINITY : INFINITY);
-  for (int i = threadIdx.x; i < nb; i += kYeeThreads) {
+  for (int i = threadIdx.x; i < nb; i += 256) {
     const double x = partials[(long long)v * nb + i];
     s = kind == 0 ? s + x : (kind == 1 ? fmax(s, x) : fmin(s, x));
   }
@@ -306,7 +372,7 @@ __global__ __launch_bounds__(kYeeThreads) void k_yee_finish(const double* partia
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = red[0];
-    for (int i = 1; i < kYeeThreads; ++i) r = kind == 0 ? r + red[i] : (kind == 1 ? fmax(r, red[i]) : fmin(r, red[i]));
+    for (int i = 1; i < 256; ++i) r = kind == 0 ? r + red[i] : (kind == 1 ? fmax(r, red[i]) : fmin(r, red[i]));
     out[v] = r;
   }
 }
@@ -317,16 +383,46 @@ __global__ void k_yee_ratio(double* sc) {
 }
 
 // ----------------------------------------------------------------------------- host side
-static int yee_blocks(long long Q3) {
-  long long b = (Q3 + kYeeThreads - 1) / kYeeThreads;
-  return (int)std::max<long long>(1, std::min<long long>(b, kYeeMaxBlocks));
+// Workgroup grid over (x, y) tiles with gz workgroups along the looped axis: at most
+// kYeeMaxBlocks workgroups when the tile count allows (their partial sums are reduced after).
+struct YeeLaunch {
+  dim3 grid, block;
+  int nb;
+};
+
+static YeeLaunch yee_launch(long long q, long long along) {
+  const long long gx = (q + kYeeTX - 1) / kYeeTX, gy = (q + kYeeTY - 1) / kYeeTY;
+  long long gz = kYeeMaxBlocks / (gx * gy);
+  gz = std::max<long long>(1, std::min<long long>(gz, along));
+  YeeLaunch l;
+  l.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)gz);
+  l.block = dim3(kYeeTX, kYeeTY, 1);
+  l.nb = (int)(gx * gy * gz);
+  return l;
+}
+
+// The marching kernels split the walked axis into chunks; chunk >= R keeps the reloaded halo
+// a small fraction of each chunk.
+static YeeLaunch yee_march_launch(long long q, int R, long long& chunk) {
+  YeeLaunch l = yee_launch(q, q);
+  chunk = (q + l.grid.z - 1) / l.grid.z;
+  chunk = std::max<long long>(chunk, std::max(4 * R, 1));
+  const long long gz = (q + chunk - 1) / chunk;
+  l.grid.z = (unsigned)gz;
+  l.nb = (int)(l.grid.x * l.grid.y * gz);
+  return l;
+}
+
+static long long yee_max_partials(long long q) {
+  const long long gx = (q + kYeeTX - 1) / kYeeTX, gy = (q + kYeeTY - 1) / kYeeTY;
+  return std::max<long long>(kYeeMaxBlocks, gx * gy);
 }
 
 // Workspace layout (every part 256-byte aligned): 4 real (2n+1)^3 arrays | partials
-// (8 x kYeeMaxBlocks) | scalars (64) | convolution profile (2n+1) | material tables
-// 6 x kYeeMaxRows x (2n+1) (cos, sin).
+// (8 x the largest workgroup count) | scalars (64) | convolution profile (2n+1) | separable
+// permittivity factors 6 x (2n+1) | general tables 6 x kYeeMaxRows x (2n+1) (cos, sin).
 struct YeeWs {
-  double *w0, *w1, *w2, *w3, *part, *sc, *prof;
+  double *w0, *w1, *w2, *w3, *part, *sc, *prof, *sep;
   double2* tab;
 };
 
@@ -334,8 +430,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static size_t yee_ws_bytes(long long q) {
   const size_t Q3 = (size_t)q * q * q;
-  return 4 * align256(sizeof(double) * Q3) + align256(sizeof(double) * 8 * kYeeMaxBlocks) +
-         align256(sizeof(double) * 64) + align256(sizeof(double) * q) +
+  return 4 * align256(sizeof(double) * Q3) + align256(sizeof(double) * 8 * yee_max_partials(q)) +
+         align256(sizeof(double) * 64) + align256(sizeof(double) * q) + align256(sizeof(double) * 6 * q) +
          align256(sizeof(double2) * 6 * kYeeMaxRows * q);
 }
 
@@ -349,11 +445,13 @@ static YeeWs yee_ws_map(void* base, long long q) {
     p += align256(sizeof(double) * Q3);
   }
   w.part = reinterpret_cast<double*>(p);
-  p += align256(sizeof(double) * 8 * kYeeMaxBlocks);
+  p += align256(sizeof(double) * 8 * yee_max_partials(q));
   w.sc = reinterpret_cast<double*>(p);
   p += align256(sizeof(double) * 64);
   w.prof = reinterpret_cast<double*>(p);
   p += align256(sizeof(double) * q);
+  w.sep = reinterpret_cast<double*>(p);
+  p += align256(sizeof(double) * 6 * q);
   w.tab = reinterpret_cast<double2*>(p);
   return w;
 }
@@ -362,7 +460,7 @@ static YeeWs yee_ws_map(void* base, long long q) {
 // (exp3/compute_lwrq_exp4.m:12-15), and the 'same' convolution of ones along one axis,
 // prof(i) = sum of the taps that land inside 0..q-1; den = prof(i) prof(j) prof(k).
 static Taps lwrq_taps(double h, int R, double sigma_factor, long long q, std::vector<double>& prof) {
-  if (R < 0 || 2 * R + 1 > kYeeMaxTaps) throw Error(FAME6D_EARG, "LWRQ neighbour radius must be in 0..8");
+  if (R < 1 || R > kYeeMaxR) throw Error(FAME6D_EARG, "LWRQ neighbour radius must be in 1..8");
   if (!(h > 0) || !(sigma_factor > 0)) throw Error(FAME6D_EARG, "h and sigma factor must be positive");
   Taps tp{};
   tp.R = R;
@@ -382,48 +480,105 @@ static Taps lwrq_taps(double h, int R, double sigma_factor, long long q, std::ve
   return tp;
 }
 
-// Per-axis (cos, sin) tables of the permittivity (see YeeMat).  `tab` must stay alive
-// until the stream has been synchronised.
+// Permittivity tables (see YeeMat).  `sep` and `tab` must stay alive until the stream has
+// been synchronised.
 static YeeMat upload_material(const YeeWs& w, int n, double h, long long q, const double* P, int L,
-                              double epsC, double alpha, hipStream_t s, std::vector<double2>& tab) {
+                              double epsC, double alpha, hipStream_t s, std::vector<double>& sep,
+                              std::vector<double2>& tab) {
   if (L < 1 || L > kYeeMaxRows) throw Error(FAME6D_EARG, "the projection must have 1..16 rows");
+  const double a6 = alpha / 6.0;
+  bool separable = true;
+  for (int l = 0; l < L; ++l) {
+    int nz = 0;
+    for (int d = 0; d < 3; ++d) nz += P[l * 3 + d] != 0.0;
+    separable = separable && nz <= 1;
+  }
+  auto coord = [&](int st, long long i) { return (double)(i - n) * h + (st ? h / 2 : 0.0); };
+  if (separable) {
+    sep.assign((size_t)6 * q, 0.0);
+    for (int st = 0; st < 2; ++st)
+      for (int d = 0; d < 3; ++d)
+        for (long long i = 0; i < q; ++i) {
+          double S = 0.0;
+          for (int l = 0; l < L; ++l) {
+            int ax = 0;  // axis of row l (rows of zeros are constant and go to axis 0)
+            for (int e = 0; e < 3; ++e)
+              if (P[l * 3 + e] != 0.0) ax = e;
+            if (ax == d) S += std::cos(P[l * 3 + d] * coord(st, i));
+          }
+          sep[((size_t)st * 3 + d) * q + i] = std::exp(a6 * S);
+        }
+    F6_HIP(hipMemcpyAsync(w.sep, sep.data(), sizeof(double) * sep.size(), hipMemcpyHostToDevice, s));
+    return YeeMat{w.sep, nullptr, L, epsC, a6};
+  }
   tab.assign((size_t)6 * L * q, double2{0.0, 0.0});
   for (int st = 0; st < 2; ++st)
     for (int d = 0; d < 3; ++d)
       for (int l = 0; l < L; ++l)
         for (long long i = 0; i < q; ++i) {
-          const double x = (double)(i - n) * h + (st ? h / 2 : 0.0);
-          const double ph = P[l * 3 + d] * x;  // P row-major, L x 3
+          const double ph = P[l * 3 + d] * coord(st, i);  // P row-major, L x 3
           tab[(((size_t)st * 3 + d) * L + l) * q + i] = double2{std::cos(ph), std::sin(ph)};
         }
   F6_HIP(hipMemcpyAsync(w.tab, tab.data(), sizeof(double2) * tab.size(), hipMemcpyHostToDevice, s));
-  return YeeMat{w.tab, L, epsC, alpha / 6.0};
+  return YeeMat{nullptr, w.tab, L, epsC, a6};
 }
 
 struct LwrqSums {
-  double sum_a, sum_b, sum_E, sum_M, dev2, dev1, bad, max_M, min_M;
+  double sum_a, sum_b, S0, S1, S2, S3;           // pass-1 sums (fused path)
+  double sum_E, sum_M, dev2, dev1, bad, max_M, min_M;
 };
 
-// x and y passes of the separable convolution (the x pass divides by den) and the z pass
-// with the LWRQ reductions.  a, b are read; w2/w3 and w0/w1 are ping-ponged.  With
-// sums_from_x the x pass also sums a and b into sc[0], sc[1]; otherwise the caller has
-// put them there.  One synchronisation.
-static LwrqSums lwrq_device(const YeeWs& w, long long q, const Taps& tp, const double* a, const double* b,
-                            double* rho, hipStream_t s, bool sums_from_x) {
-  const long long Q3 = q * q * q;
-  const int nb = yee_blocks(Q3);
-  k_lwrq_conv<<<nb, kYeeThreads, 0, s>>>(q, 1, tp, a, b, w.w2, w.w3, w.prof, sums_from_x ? w.part : nullptr);
+template <int R>
+static void launch_march(long long q, const Taps& tp, const double* ina, const double* inb, double* outa,
+                         double* outb, const double* sc, double* rho, double* part, bool reduce, hipStream_t s) {
+  long long chunk = 0;
+  const YeeLaunch l = yee_march_launch(q, R, chunk);
+  if (reduce)
+    k_lwrq_march<R, 2, true><<<l.grid, l.block, 0, s>>>(q, tp, ina, inb, nullptr, nullptr, chunk, sc, rho, part);
+  else
+    k_lwrq_march<R, 1, false><<<l.grid, l.block, 0, s>>>(q, tp, ina, inb, outa, outb, chunk, nullptr, nullptr,
+                                                         nullptr);
   F6_CHECK_LAUNCH();
-  if (sums_from_x) k_yee_finish<<<2, kYeeThreads, 0, s>>>(w.part, nb, Kinds{{0, 0}}, w.sc);
+}
+
+static void march(int R, long long q, const Taps& tp, const double* ina, const double* inb, double* outa,
+                  double* outb, const double* sc, double* rho, double* part, bool reduce, hipStream_t s) {
+  switch (R) {
+    case 1: launch_march<1>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 2: launch_march<2>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 3: launch_march<3>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 4: launch_march<4>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 5: launch_march<5>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 6: launch_march<6>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 7: launch_march<7>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    case 8: launch_march<8>(q, tp, ina, inb, outa, outb, sc, rho, part, reduce, s); break;
+    default: throw Error(FAME6D_EARG, "LWRQ neighbour radius must be in 1..8");
+  }
+}
+
+// The three convolution passes of the LWRQ and its reductions; one synchronisation.
+// normalise: the x pass divides a, b by den and sums them into sc[0], sc[1] (the standalone
+// LWRQ); otherwise a, b are already a/den, b/den and pass 1 has put sc[0..5].  w2/w3 and
+// w0/w1 are ping-ponged (a, b may be w0, w1: the x pass reads them before the y pass writes).
+static LwrqSums lwrq_device(const YeeWs& w, long long q, const Taps& tp, const double* a, const double* b,
+                            double* rho, hipStream_t s, bool normalise) {
+  const YeeLaunch lx = yee_launch(q, q);
+  k_lwrq_conv_x<<<lx.grid, lx.block, 0, s>>>(q, tp, a, b, w.w2, w.w3, normalise ? w.prof : nullptr,
+                                             normalise ? w.part : nullptr);
+  F6_CHECK_LAUNCH();
+  if (normalise) k_yee_finish<<<2, 256, 0, s>>>(w.part, lx.nb, Kinds{{0, 0}}, w.sc);
   k_yee_ratio<<<1, 64, 0, s>>>(w.sc);
-  k_lwrq_conv<<<nb, kYeeThreads, 0, s>>>(q, q, tp, w.w2, w.w3, w.w0, w.w1, nullptr, nullptr);
-  k_lwrq_conv_z<<<nb, kYeeThreads, 0, s>>>(q, tp, w.w0, w.w1, w.sc, rho, w.part);
-  k_yee_finish<<<7, kYeeThreads, 0, s>>>(w.part, nb, Kinds{{0, 0, 0, 0, 0, 1, 2}}, w.sc + 8);
+  F6_CHECK_LAUNCH();
+  march(tp.R, q, tp, w.w2, w.w3, w.w0, w.w1, nullptr, nullptr, nullptr, false, s);
+  long long chunk = 0;
+  const YeeLaunch lz = yee_march_launch(q, tp.R, chunk);
+  march(tp.R, q, tp, w.w0, w.w1, nullptr, nullptr, w.sc, rho, w.part, true, s);
+  k_yee_finish<<<7, 256, 0, s>>>(w.part, lz.nb, Kinds{{0, 0, 0, 0, 0, 1, 2}}, w.sc + 8);
   F6_CHECK_LAUNCH();
   double h[16];
   F6_HIP(hipMemcpyAsync(h, w.sc, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
   F6_HIP(hipStreamSynchronize(s));
-  return LwrqSums{h[0], h[1], h[8], h[9], h[10], h[11], h[12], h[13], h[14]};
+  return LwrqSums{h[0], h[1], h[3], h[4], h[5], h[6], h[8], h[9], h[10], h[11], h[12], h[13], h[14]};
 }
 
 // compute_lwrq_exp4 scalars from the reduced sums (exp3/compute_lwrq_exp4.m:28-48).  The
@@ -482,9 +637,10 @@ int fame6d_yee_apply_cropped_A(int n, double h, const void* u1, const void* u2, 
     hipStream_t s = (hipStream_t)stream;
     const long long q = 2LL * n + 1;
     const YeeGeom g{q, q + 2, (1.0 / h) * (1.0 / h)};
-    k_yee_points<YEE_APPLY_A><<<yee_blocks(q * q * q), kYeeThreads, 0, s>>>(
+    const YeeLaunch l = yee_launch(q, q);
+    k_yee_points<YEE_APPLY_A><<<l.grid, l.block, 0, s>>>(
         g, (const cpx<double>*)u1, (const cpx<double>*)u2, (const cpx<double>*)u3, YeeMat{}, (cpx<double>*)A1,
-        (cpx<double>*)A2, (cpx<double>*)A3, nullptr, nullptr, 0.0, 0.0, nullptr);
+        (cpx<double>*)A2, (cpx<double>*)A3, nullptr, nullptr, nullptr, 0.0, nullptr);
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
   } catch (const Error& e) {
@@ -503,9 +659,11 @@ int fame6d_yee_sample_eps(int n, double h, const double* P_host, int nrows, doub
     const long long q = 2LL * n + 1;
     YeeScratch scr;
     const YeeWs w = yee_ws_map(scr.get(work, yee_ws_bytes(q)), q);
+    std::vector<double> sep;
     std::vector<double2> tab;
-    const YeeMat mt = upload_material(w, n, h, q, P_host, nrows, eps_c, alpha, s, tab);
-    k_yee_eps<<<yee_blocks(q * q * q), kYeeThreads, 0, s>>>(q, mt, (double*)e1, (double*)e2, (double*)e3);
+    const YeeMat mt = upload_material(w, n, h, q, P_host, nrows, eps_c, alpha, s, sep, tab);
+    const YeeLaunch l = yee_launch(q, q);
+    k_yee_eps<<<l.grid, l.block, 0, s>>>(q, mt, (double*)e1, (double*)e2, (double*)e3);
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
   } catch (const Error& e) {
@@ -543,27 +701,31 @@ int fame6d_yee_lwrq_metrics(int n, double h, const void* u1, const void* u2, con
   try {
     check_window(n, h);
     if (!u1 || !u2 || !u3 || !P_host || !out) throw Error(FAME6D_EARG, "null argument");
+    if (!std::isfinite(lambda6d) || lambda6d == 0.0) throw Error(FAME6D_EARG, "lambda6d must be finite and non-zero");
     hipStream_t s = (hipStream_t)stream;
-    const long long q = 2LL * n + 1, Q3 = q * q * q;
+    const long long q = 2LL * n + 1;
     std::vector<double> prof;
     const Taps tp = lwrq_taps(h, radius, sigma_factor, q, prof);
     YeeScratch scr;
     const YeeWs w = yee_ws_map(scr.get(work, yee_ws_bytes(q)), q);
     F6_HIP(hipMemcpyAsync(w.prof, prof.data(), sizeof(double) * q, hipMemcpyHostToDevice, s));
+    std::vector<double> sep;
     std::vector<double2> tab;
-    const YeeMat mt = upload_material(w, n, h, q, P_host, nrows, eps_c, alpha, s, tab);
+    const YeeMat mt = upload_material(w, n, h, q, P_host, nrows, eps_c, alpha, s, sep, tab);
     const YeeGeom g{q, q + 2, (1.0 / h) * (1.0 / h)};
     const auto* f1 = (const cpx<double>*)u1;
     const auto* f2 = (const cpx<double>*)u2;
     const auto* f3 = (const cpx<double>*)u3;
-    const int nb = yee_blocks(Q3);
-    // pass 1: a, b and their sums
-    k_yee_points<YEE_DENSITIES><<<nb, kYeeThreads, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, w.w0,
-                                                           w.w1, 0.0, 0.0, w.part);
+    const YeeLaunch l = yee_launch(q, q);
+    // pass 1: a/den, b/den; sums of a, b and of the residual terms about lambda6d
+    k_yee_points<YEE_DENSITIES><<<l.grid, l.block, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, w.prof,
+                                                           w.w0, w.w1, lambda6d, w.part);
     F6_CHECK_LAUNCH();
-    k_yee_finish<<<2, kYeeThreads, 0, s>>>(w.part, nb, Kinds{{0, 0}}, w.sc);
+    k_yee_finish<<<1, 256, 0, s>>>(w.part, l.nb, Kinds{{0}}, w.sc);
+    k_yee_finish<<<1, 256, 0, s>>>(w.part + l.nb, l.nb, Kinds{{0}}, w.sc + 1);
+    k_yee_finish<<<4, 256, 0, s>>>(w.part + 2 * l.nb, l.nb, Kinds{{0, 0, 0, 0}}, w.sc + 3);
     F6_CHECK_LAUNCH();
-    // LWRQ on (a, b); w0/w1 are free once the x pass has read them
+    // LWRQ on the normalised densities; w0/w1 are free once the x pass has read them
     const LwrqSums r = lwrq_device(w, q, tp, w.w0, w.w1, rho, s, false);
     if (!(r.sum_b > 0)) throw Error(FAME6D_ENOCONV, "Cropped Yee mass is not positive.");
     const fame6d_lwrq_t L = lwrq_finish(r, mass_floor_rel);
@@ -572,18 +734,22 @@ int fame6d_yee_lwrq_metrics(int n, double h, const void* u1, const void* u2, con
     o.lambda_crop = r.sum_a / r.sum_b;
     o.lambda_3d = L.lambda;
     o.delta_identity = std::fabs(o.lambda_3d - o.lambda_crop) / std::max(std::fabs(o.lambda_crop), eps);
-    // pass 3: residual norms at lambda_3D and lambda_6D
-    k_yee_points<YEE_RESIDUALS><<<nb, kYeeThreads, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, nullptr,
-                                                           nullptr, o.lambda_3d, lambda6d, w.part);
-    F6_CHECK_LAUNCH();
-    k_yee_finish<<<4, kYeeThreads, 0, s>>>(w.part, nb, Kinds{{0, 0, 0, 0}}, w.sc + 16);
-    F6_CHECK_LAUNCH();
-    double hr[4];
-    F6_HIP(hipMemcpyAsync(hr, w.sc + 16, sizeof(hr), hipMemcpyDeviceToHost, s));
-    F6_HIP(hipStreamSynchronize(s));
-    const double nA = std::sqrt(hr[2]), nB = std::sqrt(hr[3]);
-    o.eta_3d = std::sqrt(hr[0]) / (nA + std::fabs(o.lambda_3d) * nB);
-    o.eta_6d = std::sqrt(hr[1]) / (nA + std::fabs(lambda6d) * nB);
+    // residual at lambda_3d from the sums about lambda6d, or a second pass when they cancel
+    const double d = o.lambda_3d - lambda6d;
+    double r3 = r.S0 - 2.0 * d * r.S1 + d * d * r.S2;
+    const double size = r.S0 + 2.0 * std::fabs(d * r.S1) + d * d * r.S2;
+    if (!(r3 >= 1e-6 * size)) {
+      k_yee_points<YEE_RESIDUALS><<<l.grid, l.block, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, nullptr,
+                                                             nullptr, nullptr, o.lambda_3d, w.part);
+      F6_CHECK_LAUNCH();
+      k_yee_finish<<<1, 256, 0, s>>>(w.part, l.nb, Kinds{{0}}, w.sc + 16);
+      F6_CHECK_LAUNCH();
+      F6_HIP(hipMemcpyAsync(&r3, w.sc + 16, sizeof(double), hipMemcpyDeviceToHost, s));
+      F6_HIP(hipStreamSynchronize(s));
+    }
+    const double nA = std::sqrt(r.S3), nB = std::sqrt(r.S2);
+    o.eta_3d = std::sqrt(std::max(r3, 0.0)) / (nA + std::fabs(o.lambda_3d) * nB);
+    o.eta_6d = std::sqrt(r.S0) / (nA + std::fabs(lambda6d) * nB);
     o.delta_lambda = std::fabs(o.lambda_3d - lambda6d) / std::fabs(lambda6d);
     o.V = L.V;
     o.nu = L.nu;

@@ -89,3 +89,83 @@ def direct_reconstruct(qmodes, coeff, points, device="cuda"):
     call("fame6d_direct_reconstruct", c.numel(), ptr(q), ptr(c), M, ptr(pts), ptr(u),
          C.c_void_p(torch.cuda.current_stream().cuda_stream))
     return u
+
+
+def projected_q_modes(N, k_lift, P):
+    """q_xi = P^T (xi + k_lift) for every mode of the (2N)^6 grid in column-major order, (NF, 3)
+    (exp3/build_projected_q_modes.m:14-20)."""
+    f = np.concatenate([np.arange(N), np.arange(-N, 0)]).astype(float)
+    NF = (2 * N) ** 6
+    qm = np.zeros((NF, 3))
+    P = np.asarray(P, dtype=float)
+    for ell in range(6):
+        # xi_ell of mode j: digit ell of j in base 2N (column-major vectorisation)
+        xi = np.tile(np.repeat(f, (2 * N) ** ell), (2 * N) ** (5 - ell)) + k_lift[ell]
+        for c in range(3):
+            qm[:, c] += P[ell, c] * xi
+    return qm
+
+
+def auto_taylor_blocks(m, h, q_l1_max, target=5.20, max_blocks=32, slack=2e-12):
+    """Blocks per dimension of build_auto_taylor_partition_exp4
+    (exp3/build_auto_taylor_partition_exp4.m:9-25): the smallest nb whose conservative phase
+    radius max||q||_1 * (ceil(m/nb) - 1) h/2 is within the target (cfg.taylor,
+    exp3/make_config_experiment4.m:65-67)."""
+    for nb in range(1, max_blocks + 1):
+        half = max(0.0, (-(-m // nb) - 1) * h / 2)
+        if q_l1_max * half <= target * (1 + slack):
+            return nb, q_l1_max * half
+    raise ValueError("No Taylor partition up to %d^3 centers satisfies rho<=%.3f (m=%d, h=%g, qL1Max=%g)"
+                     % (max_blocks, target, m, h, q_l1_max))
+
+
+def yee_block_centers(n, h, comp, nb):
+    """Midpoints of the bounding boxes of the nb^3 blocks of component comp's plus-grid points
+    (exp2/build_yee_block_partition_exp3.m:5-30: the first mod(m, nb) ranges one longer; block
+    order i fastest), (nb^3, 3)."""
+    m = 2 * n + 3
+    base, rem = divmod(m, nb)
+    mid, s = [], 0
+    for b in range(nb):
+        ln = base + (1 if b < rem else 0)
+        mid.append(h * ((s + s + ln - 1) / 2.0 - (n + 1)))
+        s += ln
+    sh = [0.5 * h if comp == d else 0.0 for d in range(3)]
+    return np.array([[mid[i] + sh[0], mid[j] + sh[1], mid[k] + sh[2]]
+                     for k in range(nb) for j in range(nb) for i in range(nb)])
+
+
+def reconstruct_yee_field(X, qmodes, n, h, p=10, nblocks=None, target=5.20):
+    """The three plus-grid components of the 6D field X (3NF complex128, component-first) on
+    the Yee window of half-width n h, by Taylor moments and Yee evaluation per component
+    (exp3/reconstruct_tracked_modes_exp4.m:10-30).  qmodes: (NF, 3) array or a (3, NF)
+    device tensor.  nblocks None chooses the automatic phase-radius partition.  Returns
+    ([u1, u2, u3] flat (2n+3)^3 device tensors, nblocks)."""
+    if isinstance(qmodes, torch.Tensor):
+        q_d = qmodes.contiguous()
+        q_l1 = float(q_d.abs().sum(0).max())
+    else:
+        qm = np.asarray(qmodes, dtype=np.float64)
+        q_d = torch.as_tensor(qm.T.copy(), device="cuda")
+        q_l1 = float(np.abs(qm).sum(1).max())
+    NF = q_d.shape[1]
+    m = 2 * n + 3
+    if nblocks is None:
+        nblocks, _ = auto_taylor_blocks(m, h, q_l1, target)
+    L = (p + 1) * (p + 2) * (p + 3) // 6
+    X = X.reshape(-1)
+    if X.numel() != 3 * NF:
+        raise ValueError("X must hold 3 NF values")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    mom = torch.empty((nblocks ** 3, L), dtype=torch.complex128, device=q_d.device)
+    out = []
+    for comp in range(3):
+        cen = np.ascontiguousarray(yee_block_centers(n, h, comp, nblocks))
+        coeff = X[comp * NF:(comp + 1) * NF].contiguous()
+        call("fame6d_taylor_moments", NF, ptr(q_d), ptr(coeff), int(p), nblocks ** 3, cen.ctypes.data_as(C.c_void_p),
+             ptr(mom), stream)
+        u = torch.empty(m ** 3, dtype=torch.complex128, device=q_d.device)
+        call("fame6d_taylor_evaluate_yee", int(n), float(h), comp, int(nblocks), int(p), nblocks ** 3,
+             cen.ctypes.data_as(C.c_void_p), ptr(mom), ptr(u), stream)
+        out.append(u)
+    return out, nblocks

@@ -121,6 +121,29 @@ def test_metrics_plane_wave_window_n160(pkg):
     assert g["V"] < 1e-22 and g["delta_identity"] < 1e-13 and g["delta_part"] < 1e-13
 
 
+def test_metrics_residual_far_from_6d_eigenvalue(pkg, oracle):
+    """lambda6d far from lambda_3d on an exact eigenpair: the residual sums taken about
+    lambda6d cancel to nothing at lambda_3d, so the residual is recomputed directly; eta_6d of
+    a plane wave with lambda6d = 2 lambda is exactly 1/3 (||Au|| = lambda ||Bu||)."""
+    n, h = 8, 0.05
+    print("case: Yee metrics residual pass n=%d" % n, flush=True)
+    kvec = np.array([0.9, 1.7, -1.1])
+    sigma = 2.0 / h * np.sin(kvec * h / 2)
+    p = np.cross(sigma, [1.0, -0.2, 0.5])
+    I = np.arange(-(n + 1), n + 2) * h
+    u = []
+    for c in range(3):
+        ax = [I + (h / 2 if d == c else 0.0) for d in range(3)]
+        ph = kvec[0] * ax[0][:, None, None] + kvec[1] * ax[1][None, :, None] + kvec[2] * ax[2][None, None, :]
+        u.append(p[c] * np.exp(1j * ph))
+    lam = float(sigma @ sigma) / 4.0
+    g = pkg.compute_yee_lwrq_metrics(*[flat(x) for x in u], 2 * lam, n, h, eps_c=4.0, alpha=0.0)
+    r = oracle.compute_yee_lwrq_metrics(*u, 2 * lam, h, n, epsC=4.0, alpha=0.0)
+    assert g["eta_3d"] < 1e-12 and r["eta_3d"] < 1e-12
+    assert abs(g["eta_6d"] - 1.0 / 3.0) < 1e-12 and abs(r["eta_6d"] - 1.0 / 3.0) < 1e-12
+    assert abs(g["lambda_3d"] - lam) < 1e-12 * lam
+
+
 def test_crop_plus_field(pkg, oracle):
     print("case: crop", flush=True)
     rng = np.random.default_rng(10)
