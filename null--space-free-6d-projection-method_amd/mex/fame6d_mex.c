@@ -18,6 +18,9 @@
  *   [lambda, Y] = fame6d_mex('eigs', s, k, p, tol, maxRestart, tolCG, maxitCG, v0, precision)
  *   y  = fame6d_mex('topA', s, x, sigma, tol, maxit)    % Method-A Top (make_A_eigs_top_preconditioned.m)
  *   [lambda, Y] = fame6d_mex('eigsA', s, k, p, tol, maxRestart, sigma, tolMINRES, maxitMINRES, v0)
+ *   m  = fame6d_mex('yee_metrics', n, h, u1, u2, u3, lambda6D, P, epsC, alpha)
+ *                                                     % compute_yee_lwrq_metrics_exp4 (exp3/...:16-70),
+ *                                                     % u*: (2n+3)^3 complex plus-grid fields
  *   fame6d_mex('qsys_destroy', s); fame6d_mex('mass_destroy', h)
  */
 #include <hip/hip_runtime_api.h>
@@ -167,6 +170,32 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       plhs[1] = from_device(Y, n3, k);
       hipFree(Y);
     }
+  } else if (!strcmp(cmd, "yee_metrics")) {
+    /* cfg.lwrq of exp3/make_config_experiment4.m:72-74 */
+    const int n = (int)mxGetScalar(prhs[1]);
+    const double h = mxGetScalar(prhs[2]);
+    size_t m3[3];
+    void* u[3];
+    for (int c = 0; c < 3; ++c) u[c] = to_device(prhs[3 + c], &m3[c]);
+    const size_t mm = (size_t)(2 * n + 3);
+    if (m3[0] != mm * mm * mm || m3[1] != m3[0] || m3[2] != m3[0])
+      mexErrMsgIdAndTxt("fame6d:usage", "fieldPlus size does not match requested n");
+    const double* P = mxGetDoubles(prhs[7]); /* 6 x 3 column-major in MATLAB */
+    double Prow[48];
+    const int rows = (int)mxGetM(prhs[7]);
+    if (rows < 1 || rows > 16 || mxGetN(prhs[7]) != 3) mexErrMsgIdAndTxt("fame6d:usage", "P must be rows x 3");
+    for (int r = 0; r < rows; ++r)
+      for (int d = 0; d < 3; ++d) Prow[r * 3 + d] = P[r + rows * d];
+    fame6d_yee_metrics_t o;
+    fail(fame6d_yee_lwrq_metrics(n, h, u[0], u[1], u[2], Prow, rows, mxGetScalar(prhs[8]), mxGetScalar(prhs[9]),
+                                 mxGetScalar(prhs[6]), 3, 2.0, 1e-14, NULL, NULL, &o, NULL));
+    for (int c = 0; c < 3; ++c) hipFree(u[c]);
+    const char* names[] = {"lambdaCrop", "lambda3D", "eta3D", "etaUsing6DLambda", "deltaLambda", "VLWRQ", "nuLWRQ",
+                           "deltaPart", "deltaLWRQIdentity", "croppedMass"};
+    const double vals[] = {o.lambda_crop, o.lambda_3d, o.eta_3d, o.eta_6d, o.delta_lambda,
+                           o.V, o.nu, o.delta_part, o.delta_identity, o.cropped_mass};
+    plhs[0] = mxCreateStructMatrix(1, 1, 10, names);
+    for (int i = 0; i < 10; ++i) mxSetField(plhs[0], 0, names[i], mxCreateDoubleScalar(vals[i]));
   } else {
     mexErrMsgIdAndTxt("fame6d:usage", "unknown command %s", cmd);
   }
