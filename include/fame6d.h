@@ -197,6 +197,21 @@ int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol
                          double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host,
                          void* Y, fame6d_lanczos_stats_t* stats);
 
+
+/* Lanczos_tracking (exp3/Lanczos_tracking.m:1-188) as solve_tracking_scan runs it
+ * (exp3/solve_tracking_scan.m:14-40): the Lanczos of fame6d_eigs_method_c for the k lowest
+ * eigenvalues (k = 140, p = k + 48 for the N = 5 scan, exp3/make_config_experiment4.m:48-50;
+ * p <= 256), after which only the selected Ritz vectors are formed.  Selection: the nfixed
+ * 1-based indices fixed_host (i-th smallest eigenvalue), or, with nfixed = 0, for each of
+ * the ntarget values target_host the nearest eigenvalue widened by half_width on each side
+ * (at least two candidates, as the reference requires).  Outputs: lambda_host (k, ascending),
+ * rel_res_host (k relative inverse-Ritz residuals, optional), sel_host (the selected indices,
+ * capacity sel_capacity), *nsel, Y (device, 2NF x *nsel complex128, optional). */
+int fame6d_eigs_tracking(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart, double tol_cg,
+                         int maxit_cg, const void* v0_host, int nfixed, const int* fixed_host, int ntarget,
+                         const double* target_host, int half_width, double* lambda_host, double* rel_res_host,
+                         int* sel_host, int sel_capacity, int* nsel, void* Y, fame6d_lanczos_stats_t* stats);
+
 /* The same for nq Bloch points sharing one mass backend, solved together: every
  * Lanczos step issues one batched K_r^{-1} over all still-active points, so each
  * kernel launch covers all of them (q_host: 3*nq; v0_host: nq x 2NF; lambda_host:

@@ -11,7 +11,8 @@ from .mass import (MassBackend, P_DEFAULT, build_mass_backend_large_scalar, cosi
                    smooth_scalar_material, start_vector)
 from .eigs import (QSystem, build_experiment1_q_system_C_compact, compute_fourier_diagnostics,  # noqa: F401
                    krinv_nested, make_explicit_krinv_compact, method_a_top, recover_original_eigenvectors,
-                   run_method_A, run_method_B, run_method_C, run_method_C_batch)
+                   run_method_A, run_method_B, run_method_C, run_method_C_batch,
+                   solve_tracked_modes, solve_tracking_scan)
 from .taylor import (auto_taylor_blocks, block_partition, build_multi_index_list_3d,  # noqa: F401
                      direct_reconstruct, projected_q_modes, reconstruct_yee_field, taylor_evaluate,
                      taylor_moments, yee_block_centers, yee_points)

@@ -132,6 +132,7 @@ SIGNATURES = {
     "fame6d_apply_k": [P, P, P],
     "fame6d_krinv_apply": [P, I, P, P, D, I, C.POINTER(KrinvStats)],
     "fame6d_eigs_method_c": [P, I, I, I, D, I, D, I, P, P, P, C.POINTER(LanczosStats)],
+    "fame6d_eigs_tracking": [P, I, I, I, D, I, D, I, P, I, P, I, P, I, P, P, P, I, P, P, C.POINTER(LanczosStats)],
     "fame6d_eigs_method_c_batch": [P, I, P, P, I, I, I, D, I, D, I, P, P, P, C.POINTER(LanczosStats)],
     "fame6d_krinv_nested_apply": [P, P, P, D, D, I, I, D, C.POINTER(KrinvStats)],
     "fame6d_eigs_method_b": [P, I, I, D, I, D, D, I, I, D, P, P, P, C.POINTER(LanczosStats)],

@@ -14,7 +14,7 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
                    double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
-                   fame6d_lanczos_stats_t* stats);
+                   fame6d_lanczos_stats_t* stats, const RitzSelect* sel = nullptr);
 void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, double tolKr, double tolM,
                   int maxitKr, int maxitM, double epsC, fame6d_krinv_stats_t* st);
 void set_nested(bool on, double tolM, int maxitM, double epsC);
@@ -513,6 +513,34 @@ int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
   eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
                 lambda_host, (cpx<double>*)Y, stats);
+  F6_API_END
+}
+
+int fame6d_eigs_tracking(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart, double tol_cg,
+                         int maxit_cg, const void* v0_host, int nfixed, const int* fixed_host, int ntarget,
+                         const double* target_host, int half_width, double* lambda_host, double* rel_res_host,
+                         int* sel_host, int sel_capacity, int* nsel, void* Y, fame6d_lanczos_stats_t* stats) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && v0_host && lambda_host && sel_host && nsel, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
+  F6_REQUIRE((nfixed > 0 && fixed_host) || (ntarget > 0 && target_host), FAME6D_EARG,
+             "Either fixedIndices or targetLambda must be supplied.");
+  F6_REQUIRE(half_width >= 0 && sel_capacity > 0, FAME6D_EARG, "bad selection sizes");
+  RitzSelect sel;
+  if (nfixed > 0 && fixed_host) {
+    sel.idx = fixed_host;
+    sel.nidx = nfixed;
+  } else {
+    sel.target = target_host;
+    sel.ntarget = ntarget;
+    sel.half_width = half_width;
+  }
+  sel.idx_out = sel_host;
+  sel.idx_capacity = sel_capacity;
+  sel.nidx_out = nsel;
+  sel.rel_res = rel_res_host;
+  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host, lambda_host,
+                (cpx<double>*)Y, stats, &sel);
   F6_API_END
 }
 

@@ -268,6 +268,20 @@ template <class T>
 void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, const cpx<T>* b,
                 long long ldb, long long n, double* out_re_host, double* out_im_host);
 
+// Ritz-vector selection of the tracking scan (exp3/Lanczos_tracking.m:134-165): fixed
+// 1-based indices, or the nearest eigenvalue to each target widened by half_width.
+struct RitzSelect {
+  const int* idx = nullptr;
+  int nidx = 0;
+  const double* target = nullptr;
+  int ntarget = 0;
+  int half_width = 0;
+  int* idx_out = nullptr;     // selected 1-based indices (capacity given by the caller)
+  int idx_capacity = 0;
+  int* nidx_out = nullptr;
+  double* rel_res = nullptr;  // k relative inverse-Ritz residuals |beta U(end, i)| / |theta_i|
+};
+
 inline int grid_for(long long n, int threads = 256) {
   long long b = (n + threads - 1) / threads;
   if (b > 65536) b = 65536;

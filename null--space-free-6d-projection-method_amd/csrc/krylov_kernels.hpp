@@ -10,6 +10,7 @@ namespace fame6d {
 
 constexpr int kRedThreads = 256;
 constexpr int kMaxRedBlocks = 1024;
+constexpr int kLzMaxP = 256;  // largest Lanczos basis (Krylov dimension p)
 
 __host__ inline int red_blocks(long long n) {
   long long b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q
 // r -= Q(:, 0:j) h   (h complex, on device)
 static __global__ void k_gemv_n_sub(const cpx<double>* Q, long long ld, int j, const double* hre,
                              const double* him, cpx<double>* r, long long n) {
-  __shared__ cpx<double> hs[128];
+  __shared__ cpx<double> hs[kLzMaxP];
   for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);
   __syncthreads();
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -606,12 +607,49 @@ static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long lon
   }
 }
 
+// out(:, c) = Q(:, 0:m) U(:, c) for c < kk, out-of-place: complex Q (ld complex rows) times
+// real U (m x kk, column-major), i.e. the real (2n x m) matrix of Q's interleaved parts times
+// U.  A thread owns one real row and CB output columns in registers; U is staged through LDS
+// in slices of 32 rows, so Q is read ceil(kk / CB) times and U comes from LDS.  Used for the
+// thick restart when the basis is too wide for the in-place kernel above, and for the Ritz
+// vectors Y = Q U.
+template <int CB>
+__global__ __launch_bounds__(256) void k_basis_gemm(const double* __restrict__ Q, long long ldq, long long nr, int m,
+                                                    int kk, const double* __restrict__ U, double* __restrict__ out,
+                                                    long long ldo) {
+  __shared__ double Us[32 * CB];
+  const long long row = blockIdx.x * 256LL + threadIdx.x;
+  const int c0 = blockIdx.y * CB;
+  const int kc = min(CB, kk - c0);
+  double acc[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+  for (int l0 = 0; l0 < m; l0 += 32) {
+    const int lc = min(32, m - l0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 32 * CB; t += 256) {
+      const int l = t / CB, c = t - (t / CB) * CB;
+      Us[t] = (l < lc && c < kc) ? U[(long long)(c0 + c) * m + l0 + l] : 0.0;
+    }
+    __syncthreads();
+    if (row < nr) {
+      for (int l = 0; l < lc; ++l) {
+        const double qv = Q[(long long)(l0 + l) * ldq + row];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[c] += qv * Us[l * CB + c];
+      }
+    }
+  }
+  if (row < nr)
+    for (int c = 0; c < kc; ++c) out[(long long)(c0 + c) * ldo + row] = acc[c];
+}
+
 // out = in - Q(:, 0:j) h (h complex, on device); optional ||out||^2 partials per workgroup.
 static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
                                                            const double* hre, const double* him,
                                                            const cpx<double>* in, cpx<double>* out,
                                                            long long n, double* norm_partials, Skip sk) {
-  __shared__ cpx<double> hs[128];
+  __shared__ cpx<double> hs[kLzMaxP];
   __shared__ double red[256];
   if (skipped(sk, 0)) return;
   for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);

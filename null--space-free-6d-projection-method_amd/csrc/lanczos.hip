@@ -21,6 +21,7 @@
 #include <functional>
 #include <numeric>
 
+#include "dense_eig.hpp"
 #include "internal.hpp"
 #include "krylov_kernels.hpp"
 #include "minres_kernels.hpp"
@@ -65,60 +66,19 @@ __global__ void k_div(const double* q1, const double* q2, const double* q3, cons
     y[i] = q1[i] * x[i] + q2[i] * x[n + i] + q3[i] * x[2 * n + i];
 }
 
-// ------------------------------------------------------------------ small dense eig
-// Cyclic Jacobi for a real symmetric m x m matrix (column-major); eigenvectors
-// in the columns of V.
-static void jacobi_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
-  V.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
-  auto a = [&](int r, int c) -> double& { return A[(size_t)c * m + r]; };
-  double fro = 0;
-  for (double v : A) fro += v * v;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0;
-    for (int c = 0; c < m; ++c)
-      for (int r = 0; r < m; ++r)
-        if (r != c) off += a(r, c) * a(r, c);
-    if (off <= 1e-32 * fro || off == 0.0) break;
-    for (int p = 0; p < m - 1; ++p) {
-      for (int q = p + 1; q < m; ++q) {
-        const double apq = a(p, q);
-        if (apq == 0.0) continue;
-        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < m; ++k) {  // columns p, q
-          const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
-        }
-        for (int k = 0; k < m; ++k) {  // rows p, q
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
-        for (int k = 0; k < m; ++k) {
-          const double vkp = V[(size_t)p * m + k], vkq = V[(size_t)q * m + k];
-          V[(size_t)p * m + k] = c * vkp - s * vkq;
-          V[(size_t)q * m + k] = s * vkp + c * vkq;
-        }
-      }
-    }
-  }
-  w.resize(m);
-  for (int i = 0; i < m; ++i) w[i] = a(i, i);
-}
-
 // ------------------------------------------------------------------ device basis ops
 struct Basis {
-  cpx<double>* Q = nullptr;  // n x p
-  cpx<double>* v = nullptr;  // next Lanczos vector
-  cpx<double>* r = nullptr;  // residual work
-  double* sc = nullptr;      // device scalars: [0] alpha, [1] beta2, [2..3] corr, [8..] h_re, [136..] h_im
-  double* part = nullptr;    // partials
+  cpx<double>* Q = nullptr;     // n x p
+  cpx<double>* v = nullptr;     // next Lanczos vector
+  cpx<double>* r = nullptr;     // residual work
+  cpx<double>* Qalt = nullptr;  // second n x p basis for out-of-place restarts (p > 64)
+  double* sc = nullptr;         // device scalars: [0] alpha, [1] beta2, [2] ||r||^2 before the
+                                // sweep, [4] DGKS flag, then h, h2 (re, im) of kLzMaxP each
+  double* part = nullptr;       // partials
 };
 
-constexpr int kScH = 8, kScHim = 8 + 128;
+constexpr int kScH = 8, kScHim = 8 + kLzMaxP;
+constexpr int kScStride = 8 + 4 * kLzMaxP;  // doubles of device scalars per Lanczos run
 
 // dst_re/dst_im = sum(conj(a) .* b)
 static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, double* part,
@@ -134,7 +94,7 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
 // One Lanczos orthogonalisation step: h1 = Q^H w (its entry jj-1 is alpha = q_j^H w),
 // r = w - Q h1, h2 = Q^H r, r -= Q h2 with ||r||^2 fused into the last update.
 // h1 in sc[kScH..], h2 in sc[kScH2..], ||r||^2 in sc[1].
-constexpr int kScH2 = 8 + 256, kScH2im = 8 + 384;
+constexpr int kScH2 = 8 + 2 * kLzMaxP, kScH2im = 8 + 3 * kLzMaxP;
 constexpr int kGemvKB = 8;
 // With `sk` set the projection is skipped (and not timed: its launch does no work).
 static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* v, double* dre,
@@ -142,7 +102,7 @@ static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cp
   const int nb = red_blocks(n);
   const int nch = (jj + kGemvKB - 1) / kGemvKB;
   double* pre = B.part;
-  double* pim = B.part + (size_t)nb * 128;
+  double* pim = B.part + (size_t)nb * kLzMaxP;
   if (sk.p == nullptr) timer_begin(m, TS_GEMV_C);
   launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim,
            sk);
@@ -243,7 +203,11 @@ static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, 
   L.Alpha.clear();
   L.Beta.clear();
   std::vector<double> w, U;
-  jacobi_eig(m, H, w, U);
+  try {
+    sym_eig(m, H, w, U);
+  } catch (const std::runtime_error& e) {
+    throw Error(FAME6D_ENOCONV, e.what());
+  }
   std::vector<int> ind(m);
   std::iota(ind.begin(), ind.end(), 0);
   std::stable_sort(ind.begin(), ind.end(), [&](int a, int b) { return w[a] > w[b]; });
@@ -288,9 +252,18 @@ static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, 
   }
   ubuf.ensure(sizeof(double) * Uk.size());
   F6_HIP(hipMemcpyAsync(ubuf.ptr, Uk.data(), sizeof(double) * Uk.size(), hipMemcpyHostToDevice, s));
-  const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
-  k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n2, n2, m, k, ubuf.as<double>());
-  if (L.Z) k_restart<<<blocks, 256, 0, s>>>(L.Z, n2, n2, m, k, ubuf.as<double>());
+  if (m <= 64) {
+    const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
+    k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n2, n2, m, k, ubuf.as<double>());
+    if (L.Z) k_restart<<<blocks, 256, 0, s>>>(L.Z, n2, n2, m, k, ubuf.as<double>());
+  } else {
+    // wide basis: out-of-place product into the second basis, then swap the two
+    if (!L.B.Qalt || L.Z) throw Error(FAME6D_EARG, "Lanczos restart of a basis wider than 64 needs the second basis");
+    const dim3 grid((unsigned)((2 * n2 + 255) / 256), (unsigned)((k + 63) / 64));
+    k_basis_gemm<64><<<grid, 256, 0, s>>>(reinterpret_cast<const double*>(L.B.Q), 2 * n2, 2 * n2, m, k,
+                                          ubuf.as<double>(), reinterpret_cast<double*>(L.B.Qalt), 2 * n2);
+    std::swap(L.B.Q, L.B.Qalt);
+  }
   F6_CHECK_LAUNCH();
   F6_HIP(hipStreamSynchronize(s));  // ubuf is reused by the next restart
   L.d.resize(k);
@@ -323,12 +296,13 @@ void set_nested(bool on, double tolM, int maxitM, double epsC) {
 
 void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
                    double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
-                   fame6d_lanczos_stats_t* stats) {
+                   fame6d_lanczos_stats_t* stats, const RitzSelect* sel) {
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const int G = S->G;
   const long long NF = m->NF, n2 = 2 * NF;
-  if (k0 < 1 || p <= k0 || p > 64) throw Error(FAME6D_EARG, "need 1 <= k < p <= 64");
+  if (k0 < 1 || p <= k0 || p > kLzMaxP) throw Error(FAME6D_EARG, "need 1 <= k < p <= 256");
+  if (sel && G != 1) throw Error(FAME6D_EARG, "Ritz selection needs a single Bloch point");
   if (G > 64) throw Error(FAME6D_EARG, "at most 64 Bloch points");
   if (precision != FAME6D_FP64 && precision != FAME6D_MIXED)
     throw Error(FAME6D_EARG, "eigensolver precision must be FAME6D_FP64 or FAME6D_MIXED");
@@ -338,19 +312,22 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   // device memory: per group Q (n2 x p) + v + r; scalars; partials; packed in/out
   const int nb = red_blocks(n2);
   DevBuf basis, scal, part, io, ubuf;
-  basis.ensure(sizeof(cpx<double>) * (size_t)n2 * (p + 2) * G);
-  scal.ensure(sizeof(double) * 512 * G);
-  part.ensure(sizeof(double) * (size_t)std::max(nb * 256, 2048) * G);
+  const int cols = p + 2 + (p > 64 ? p : 0);  // Q, v, r (+ the second basis of wide restarts)
+  const size_t part_per = (size_t)std::max(nb * 2 * kLzMaxP, 2048);
+  basis.ensure(sizeof(cpx<double>) * (size_t)n2 * cols * G);
+  scal.ensure(sizeof(double) * kScStride * G);
+  part.ensure(sizeof(double) * part_per * G);
   io.ensure(sizeof(cpx<double>) * (size_t)n2 * 2 * G);
   std::vector<LzState> L(G);
   for (int g = 0; g < G; ++g) {
     L[g].slot = g;
-    cpx<double>* b0 = basis.as<cpx<double>>() + (size_t)n2 * (p + 2) * g;
+    cpx<double>* b0 = basis.as<cpx<double>>() + (size_t)n2 * cols * g;
     L[g].B.Q = b0;
     L[g].B.v = b0 + (size_t)n2 * p;
     L[g].B.r = b0 + (size_t)n2 * (p + 1);
-    L[g].B.sc = scal.as<double>() + 512 * g;
-    L[g].B.part = part.as<double>() + (size_t)std::max(nb * 256, 2048) * g;
+    L[g].B.Qalt = p > 64 ? b0 + (size_t)n2 * (p + 2) : nullptr;
+    L[g].B.sc = scal.as<double>() + (size_t)kScStride * g;
+    L[g].B.part = part.as<double>() + part_per * g;
     L[g].k = k0;
   }
   cpx<double>* qin = io.as<cpx<double>>();
@@ -472,21 +449,65 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     }
     std::stable_sort(lam.begin(), lam.end());
     for (int i = 0; i < k0; ++i) lambda[(size_t)g * k0 + i] = lam[i].first;
+    // Ritz vectors to form: all k0 (in the order of lambda), or the tracking selection
+    // (exp3/Lanczos_tracking.m:134-165), whose indices count the inverse Ritz values from the
+    // largest, i.e. the eigenvalues from the smallest
+    std::vector<int> want;  // positions in d order
+    if (sel) {
+      if (sel->rel_res)
+        for (int i = 0; i < k0; ++i) sel->rel_res[i] = Lg.res[i] / std::max(std::fabs(Lg.d[i]), 2.220446049250313e-16);
+      std::vector<int> pick;
+      auto add = [&](int v) {
+        if (std::find(pick.begin(), pick.end(), v) == pick.end()) pick.push_back(v);
+      };
+      if (sel->nidx > 0) {
+        for (int t = 0; t < sel->nidx; ++t) {
+          if (sel->idx[t] < 1 || sel->idx[t] > k0)
+            throw Error(FAME6D_EARG, "Requested fixed tracking index is outside the scan.");
+          add(sel->idx[t]);
+        }
+      } else {
+        for (int t = 0; t < sel->ntarget; ++t) {
+          const double tg = sel->target[t];
+          int ic = 0;
+          double best = INFINITY;
+          for (int i = 0; i < k0; ++i) {
+            const double dv = std::fabs(lam[i].first - tg) / std::max(std::fabs(tg), 2.220446049250313e-16);
+            if (dv < best) {
+              best = dv;
+              ic = i;
+            }
+          }
+          for (int i = std::max(0, ic - sel->half_width); i <= std::min(k0 - 1, ic + sel->half_width); ++i)
+            add(i + 1);
+        }
+        if ((int)pick.size() < 2)
+          throw Error(FAME6D_EARG, "Tracking candidate extraction produced only " + std::to_string(pick.size()) +
+                                       " candidate(s).");
+      }
+      if ((int)pick.size() > sel->idx_capacity) throw Error(FAME6D_EARG, "selection exceeds its output capacity");
+      for (size_t t = 0; t < pick.size(); ++t) {
+        sel->idx_out[t] = pick[t];
+        want.push_back(lam[pick[t] - 1].second);
+      }
+      *sel->nidx_out = (int)pick.size();
+    } else {
+      for (int i = 0; i < k0; ++i) want.push_back(lam[i].second);
+    }
     if (Y) {
-      const int mq = Lg.qcols;
-      std::vector<double> Uy((size_t)mq * k0);
-      for (int i = 0; i < k0; ++i) {
-        const int col = Lg.ind[lam[i].second];
+      const int mq = Lg.qcols, ny = (int)want.size();
+      std::vector<double> Uy((size_t)mq * ny);
+      for (int i = 0; i < ny; ++i) {
+        const int col = Lg.ind[want[i]];
         for (int r = 0; r < mq; ++r) Uy[(size_t)i * mq + r] = Lg.Us[(size_t)col * mq + r];
       }
       ubuf.ensure(sizeof(double) * Uy.size());
       F6_HIP(hipMemcpyAsync(ubuf.ptr, Uy.data(), sizeof(double) * Uy.size(), hipMemcpyHostToDevice, s));
-      // out-of-place product through a k0-column staging area inside the basis
-      const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
-      k_restart<<<blocks, 256, 0, s>>>(Lg.B.Q, n2, n2, mq, k0, ubuf.as<double>());
+      const dim3 grid((unsigned)((2 * n2 + 255) / 256), (unsigned)((ny + 63) / 64));
+      k_basis_gemm<64><<<grid, 256, 0, s>>>(reinterpret_cast<const double*>(Lg.B.Q), 2 * n2, 2 * n2, mq, ny,
+                                            ubuf.as<double>(), reinterpret_cast<double*>(Y + (size_t)n2 * ny * g),
+                                            2 * n2);
       F6_CHECK_LAUNCH();
-      F6_HIP(hipMemcpyAsync(Y + (size_t)n2 * k0 * g, Lg.B.Q, sizeof(cpx<double>) * n2 * k0,
-                            hipMemcpyDeviceToDevice, s));
       F6_HIP(hipStreamSynchronize(s));
     }
     if (stats) {
@@ -1032,8 +1053,8 @@ void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, do
   DevBuf basis, zb, scal, part, ubuf, vec, aux, wout;
   basis.ensure(sizeof(cpx<double>) * (size_t)n3 * (p + 2));  // Q(:,0..p-1), v, r
   zb.ensure(sizeof(cpx<double>) * (size_t)n3 * (p + 2));     // Z(:,0..p-1), zv, zr
-  scal.ensure(sizeof(double) * 512);
-  part.ensure(sizeof(double) * (size_t)std::max(nb3 * 256, 2048));
+  scal.ensure(sizeof(double) * kScStride);
+  part.ensure(sizeof(double) * (size_t)std::max(nb3 * 2 * kLzMaxP, 2048));
   wout.ensure(sizeof(cpx<double>) * (size_t)n3);
   MrWs W{};
   mr_alloc(m, vec, aux, W);

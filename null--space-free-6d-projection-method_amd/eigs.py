@@ -156,6 +156,68 @@ def run_method_C(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-
     return dict(lambda_=lam, Y=Y, stats=st.as_dict())
 
 
+def solve_tracking_scan(sys: QSystem, k=140, p=None, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
+                        fixed_indices=None, target_lambda=None, half_width=0, seed=None, v0=None,
+                        precision="double", want_vectors=True):
+    """solve_tracking_scan + Lanczos_tracking (exp3/solve_tracking_scan.m:1-56,
+    exp3/Lanczos_tracking.m:1-188): the k lowest eigenvalues (p = k + 48 by default, the
+    N = 5 scan of exp3/make_config_experiment4.m:48-50) and only the selected Ritz vectors.
+    Select by fixed_indices (1-based) or by target_lambda (+/- half_width).  The default start
+    vector seed is the reference's rngSeed + 1000 N + k (exp3/solve_tracking_scan.m:4; the
+    generator itself is this package's)."""
+    p = k + 48 if p is None else p
+    if v0 is None:
+        v0 = start_vector(sys.nr, (20260831 + 1000 * sys.mass.N + k) if seed is None else seed)
+    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    if v0.size != sys.nr:
+        raise _lib.Fame6dError(1, "start vector must have 2NF entries")
+    fixed = None if fixed_indices is None else np.ascontiguousarray(np.asarray(fixed_indices, dtype=np.int32))
+    tgt = None if target_lambda is None else np.ascontiguousarray(np.asarray(target_lambda, dtype=np.float64))
+    cap = len(fixed) if fixed is not None else len(tgt) * (2 * half_width + 1)
+    lam = np.empty(k)
+    rel = np.empty(k)
+    sel = np.zeros(cap, dtype=np.int32)
+    nsel = C.c_int(0)
+    Y = torch.empty((cap, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
+    st = _lib.LanczosStats()
+    call("fame6d_eigs_tracking", sys.handle, precision_code(precision), int(k), int(p), float(tol), int(max_restart),
+         float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p), 0 if fixed is None else len(fixed),
+         None if fixed is None else fixed.ctypes.data_as(C.c_void_p), 0 if tgt is None else len(tgt),
+         None if tgt is None else tgt.ctypes.data_as(C.c_void_p), int(half_width), lam.ctypes.data_as(C.c_void_p),
+         rel.ctypes.data_as(C.c_void_p), sel.ctypes.data_as(C.c_void_p), cap, C.byref(nsel), ptr(Y), C.byref(st))
+    ns = nsel.value
+    idx = sel[:ns].copy()
+    return dict(kScan=k, pScan=p, lambda_=lam, Yselected=None if Y is None else Y[:ns], selectedIndices=idx,
+                selectedLambda=lam[idx - 1], selectedRelRitzResidual=rel[idx - 1], relRitzResiduals=rel,
+                stats=st.as_dict())
+
+
+def solve_tracked_modes(sys: QSystem, modes=(11, 114), lambda_reference=(1.200090550277e-02, 4.462939906027e-02),
+                        lambda_check_rel_tol=5e-7, required_rel_ritz_residual=1e-10, k=140, p_extra=48,
+                        tol_recover=1e-12, maxit_recover=3000, **kw):
+    """solve_tracked_modes_exp4 (exp3/solve_tracked_modes_exp4.m:1-58): the scan, its checks
+    (convergence, the selected indices, the selected relative inverse-Ritz residuals, the
+    eigenvalues against the reference values, cfg.tracking of exp3/make_config_experiment4.m:37-45)
+    and the recovery of the selected eigenvectors (exp3/recover_fixed_vectors_tracking.m)."""
+    if max(modes) > k:
+        raise ValueError("kScan=%d does not include tracked mode %d." % (k, max(modes)))
+    scan = solve_tracking_scan(sys, k=k, p=k + p_extra, fixed_indices=modes, **kw)
+    if not scan["stats"]["converged"]:
+        raise _lib.Fame6dError(3, "selected-mode Lanczos did not satisfy the requested convergence test")
+    if list(scan["selectedIndices"]) != list(modes):
+        raise _lib.Fame6dError(3, "Selected Ritz indices do not match requested tracked modes.")
+    if np.any(scan["selectedRelRitzResidual"] > required_rel_ritz_residual):
+        raise _lib.Fame6dError(3, "Selected relative inverse-Ritz residual exceeds required threshold.")
+    X, MX, info = recover_original_eigenvectors(scan["Yselected"], sys, tol_recover, maxit_recover)
+    lam = scan["selectedLambda"]
+    if lambda_reference is not None:
+        ref = np.asarray(lambda_reference, dtype=float)
+        rel = np.abs(lam - ref) / np.maximum(np.abs(ref), np.finfo(float).eps)
+        if np.any(rel > lambda_check_rel_tol):
+            raise _lib.Fame6dError(3, "Tracked-mode eigenvalue sanity check failed: %s" % rel)
+    return dict(modes=list(modes), lambda6D=lam, X=X, MX=MX, scan=scan, recovery=info)
+
+
 def krinv_nested(sys: QSystem, b, tol_kr=1e-10, tol_m=1e-12, maxit_kr=600, maxit_m=1600, eps_c=None):
     """Method-B K_r^{-1} b (make_nested_krinv_instrumented); eps_c defaults to the mean permittivity."""
     eps_c = sys.mass.info["epsMean"] if eps_c is None else eps_c

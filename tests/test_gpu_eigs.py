@@ -220,3 +220,53 @@ def test_method_a_eigs_golden_n2(n2, pkg):
     assert abs(lamA[9] - GOLDEN_N2[1]) / GOLDEN_N2[1] < 1e-9
     lamC = pkg.run_method_C(s, v0=pkg.start_vector(s.nr, 7), precision="double", maxit_cg=3000)["lambda_"]
     assert np.max(np.abs(lamA - lamC) / lamC) < 1e-9
+
+
+def test_tracking_scan_n2_selection_and_wide_basis(pkg):
+    """Lanczos_tracking's selected-vector extraction on the narrow (p <= 64, in-place restart)
+    and wide (out-of-place restart into a second basis) paths, against the full extraction of
+    the same recurrence; target selection widened by one index on each side."""
+    print("case: tracking scan N=2", flush=True)
+    N = 2
+    mass = pkg.MassBackend(N, pkg.smooth_scalar_material(N))
+    sq = pkg.QSystem(mass, Q_EXP4)
+    v0 = pkg.start_vector(sq.nr, 7)
+    full = pkg.run_method_C(sq, k=12, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000, v0=v0,
+                            want_vectors=True)
+    assert abs(full["lambda_"][0] - GOLDEN_N2[0]) / GOLDEN_N2[0] < 1e-10
+    assert abs(full["lambda_"][9] - GOLDEN_N2[1]) / GOLDEN_N2[1] < 1e-10
+    Yf = full["Y"].cpu().numpy()
+    for k, p in ((12, 40), (70, 118)):
+        t = pkg.solve_tracking_scan(sq, k=k, p=p, v0=v0, fixed_indices=[1, 10])
+        assert list(t["selectedIndices"]) == [1, 10]
+        assert np.max(np.abs(t["lambda_"][:12] - full["lambda_"]) / full["lambda_"]) < 1e-12
+        assert np.all(t["selectedRelRitzResidual"] < 1e-10)
+        Ys = t["Yselected"].cpu().numpy()
+        for a, i in enumerate((1, 10)):
+            cos = abs(np.vdot(Ys[a], Yf[i - 1])) / np.linalg.norm(Ys[a]) / np.linalg.norm(Yf[i - 1])
+            assert abs(cos - 1.0) < 1e-10
+    t = pkg.solve_tracking_scan(sq, k=70, p=118, v0=v0, target_lambda=list(GOLDEN_N2), half_width=1,
+                                want_vectors=False)
+    assert list(t["selectedIndices"]) == [1, 2, 9, 10, 11]
+    with pytest.raises(pkg.Fame6dError):
+        pkg.solve_tracking_scan(sq, k=12, p=40, v0=v0, fixed_indices=[13], want_vectors=False)
+    sq.close()
+    mass.close()
+
+
+def test_tracked_modes_n5_reference_eigenvalues(pkg):
+    """The N = 5 selected-mode scan of experiment 4 (k = 140, p = 188; modes 11 and 114,
+    exp3/make_config_experiment4.m:37-50): both eigenvalues against the values the reference
+    holds (1.200090550277e-02, 4.462939906027e-02, quoted to 13 digits; the reference accepts
+    5e-7), and the recovered eigenvectors' M-norm Rayleigh quotients."""
+    print("case: tracked modes N=5 k=140", flush=True)
+    N = 5
+    mass = pkg.MassBackend(N, pkg.smooth_scalar_material(N))
+    sq = pkg.QSystem(mass, Q_EXP4)
+    sel = pkg.solve_tracked_modes(sq, lambda_check_rel_tol=1e-11)
+    ref = np.array([1.200090550277e-02, 4.462939906027e-02])
+    assert np.max(np.abs(sel["lambda6D"] - ref) / ref) < 1e-11
+    assert np.allclose(sel["recovery"]["massNorm2"], sel["lambda6D"], rtol=1e-10)
+    assert sel["scan"]["stats"]["num_converged"] == 140
+    sq.close()
+    mass.close()
