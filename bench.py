@@ -217,15 +217,14 @@ def plane_wave_fields(torch, n, h, kvec, p):
 
 
 def yee_closure(torch, pkg):
-    """The mesh study of the physical-space closure (exp3/run_experiment4.m:40-95 with
-    exp3/make_config_experiment4.m:17-32,58-85): N = 5, q = (0.13, 0.21, 0.17),
-    eps = 4 exp(1.2/6 sum cos), k = 12 eigenpairs (mode 11 is the reference's tracked N=5
-    mode, lambda = 1.200090550277e-02), Taylor p = 10 with the automatic phase-radius
-    partition, cropped Yee / LWRQ metrics for h = 0.1, 0.05, 0.025, 0.0125 at half-width 0.5.
-    Then the closure kernels alone on the largest window of the window study (n = 160,
-    h = 0.025) with an exact discrete plane wave: algorithmic bytes 192 per interior point
-    (fields 48 + densities 16 in pass 1, 32 + 32 for the x and y convolutions, 16 for z,
-    48 for the residual pass)."""
+    """Experiment 4 (exp3/run_experiment4.m:40-95, exp3/make_config_experiment4.m:17-85): N = 5,
+    q = (0.13, 0.21, 0.17), eps = 4 exp(1.2/6 sum cos); the selected-mode scan (k = 140,
+    p = 188) for modes 11 and 114 with the reference's eigenvalue check; Taylor p = 10 with the
+    automatic phase-radius partition; the cropped Yee / LWRQ metrics of the mesh study
+    (h = 0.1, 0.05, 0.025, 0.0125 at half-width 0.5).  Then the closure kernels alone on the
+    largest window of the window study (n = 160, h = 0.025) with an exact discrete plane wave:
+    algorithmic bytes 144 per interior point (fields 48 + normalised densities 16 in pass 1,
+    32 + 32 for the x and y convolutions, 16 for z)."""
     N = 5
     eps = pkg.smooth_scalar_material(N)
     mass = pkg.MassBackend(N, eps)
@@ -233,13 +232,12 @@ def yee_closure(torch, pkg):
     sq = pkg.QSystem(mass, (0.13, 0.21, 0.17))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = pkg.run_method_C(sq, k=12, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
-                           v0=pkg.start_vector(sq.nr, 20260831), want_vectors=True)
+    sel = pkg.solve_tracked_modes(sq)
     torch.cuda.synchronize()
     t_eig = time.perf_counter() - t0
-    lam = out["lambda_"]
-    modes = [0, 10]
-    X, _, _ = pkg.recover_original_eigenvectors(out["Y"][modes], sq)
+    modes = sel["modes"]
+    lam = sel["lambda6D"]
+    X = sel["X"]
     T = pkg.taylor
     q_d = torch.as_tensor(T.projected_q_modes(N, sq.info["kLift"], pkg.P_DEFAULT).T.copy(), device="cuda")
     rows = []
@@ -250,11 +248,11 @@ def yee_closure(torch, pkg):
             u, nb = T.reconstruct_yee_field(X[i], q_d, n, h)
             torch.cuda.synchronize()
             a1 = time.perf_counter()
-            met = pkg.compute_yee_lwrq_metrics(*u, float(lam[mode]), n, h)
+            met = pkg.compute_yee_lwrq_metrics(*u, float(lam[i]), n, h)
             torch.cuda.synchronize()
             a2 = time.perf_counter()
-            rows.append(dict(mode=mode + 1, h=h, n=n, blocks=nb, reconstruct_s=a1 - a0, metrics_s=a2 - a1,
-                             lambda_6d=float(lam[mode]), lambda_3d=met["lambda_3d"], eta_3d=met["eta_3d"],
+            rows.append(dict(mode=mode, h=h, n=n, blocks=nb, reconstruct_s=a1 - a0, metrics_s=a2 - a1,
+                             lambda_6d=float(lam[i]), lambda_3d=met["lambda_3d"], eta_3d=met["eta_3d"],
                              eta_6d=met["eta_6d"], nu=met["nu"], delta_lambda=met["delta_lambda"],
                              delta_identity=met["delta_identity"], delta_part=met["delta_part"]))
             del u
@@ -277,10 +275,13 @@ def yee_closure(torch, pkg):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - a0) / reps
     q3 = (2 * n + 1) ** 3
-    return dict(eigensolve_k12_seconds=t_eig, lambda_11=float(lam[10]),
-                lambda_11_rel_diff_vs_reference=abs(float(lam[10]) - 1.200090550277e-02) / 1.200090550277e-02,
+    ref = np.array([1.200090550277e-02, 4.462939906027e-02])
+    st = sel["scan"]["stats"]
+    return dict(tracking_scan=dict(k=140, p=188, seconds=t_eig, outer_steps=st["outer_steps"],
+                                   restarts=st["restarts"], lambda_11_114=[float(x) for x in lam],
+                                   rel_diff_vs_reference=[float(x) for x in np.abs(lam - ref) / ref]),
                 mesh=rows, window_n160=dict(n=n, h=h, interior_points=q3, seconds=dt,
-                                           algorithmic_GBs=192.0 * q3 / dt / 1e9))
+                                           algorithmic_GBs=144.0 * q3 / dt / 1e9))
 
 
 def method_comparison(torch, pkg, N, lam_c):

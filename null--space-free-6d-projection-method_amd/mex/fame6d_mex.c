@@ -18,6 +18,8 @@
  *   [lambda, Y] = fame6d_mex('eigs', s, k, p, tol, maxRestart, tolCG, maxitCG, v0, precision)
  *   y  = fame6d_mex('topA', s, x, sigma, tol, maxit)    % Method-A Top (make_A_eigs_top_preconditioned.m)
  *   [lambda, Y] = fame6d_mex('eigsA', s, k, p, tol, maxRestart, sigma, tolMINRES, maxitMINRES, v0)
+ *   [lambda, Y, idx, relRes] = fame6d_mex('tracking', s, k, p, tol, maxRestart, tolCG, maxitCG, v0, fixedIdx)
+ *                                                     % Lanczos_tracking (exp3/Lanczos_tracking.m)
  *   m  = fame6d_mex('yee_metrics', n, h, u1, u2, u3, lambda6D, P, epsC, alpha)
  *                                                     % compute_yee_lwrq_metrics_exp4 (exp3/...:16-70),
  *                                                     % u*: (2n+3)^3 complex plus-grid fields
@@ -170,6 +172,35 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       plhs[1] = from_device(Y, n3, k);
       hipFree(Y);
     }
+  } else if (!strcmp(cmd, "tracking")) {
+    fame6d_qsys* s = (fame6d_qsys*)handle_in(prhs[1]);
+    const int k = (int)mxGetScalar(prhs[2]), p = (int)mxGetScalar(prhs[3]);
+    const double tol = mxGetScalar(prhs[4]);
+    const int maxRestart = (int)mxGetScalar(prhs[5]);
+    const double tolCG = mxGetScalar(prhs[6]);
+    const int maxitCG = (int)mxGetScalar(prhs[7]);
+    const mxArray* v0 = prhs[8];
+    if (!mxIsComplex(v0)) mexErrMsgIdAndTxt("fame6d:usage", "v0 must be complex");
+    const size_t n2 = mxGetNumberOfElements(v0);
+    const int nfix = (int)mxGetNumberOfElements(prhs[9]);
+    if (nfix < 1 || nfix > 256) mexErrMsgIdAndTxt("fame6d:usage", "fixedIdx must hold 1..256 indices");
+    int fixed[256], sel[256];
+    for (int i = 0; i < nfix; ++i) fixed[i] = (int)mxGetDoubles(prhs[9])[i];
+    plhs[0] = mxCreateDoubleMatrix(k, 1, mxREAL);
+    mxArray* rel = mxCreateDoubleMatrix(k, 1, mxREAL);
+    void* Y = NULL;
+    hipMalloc(&Y, n2 * nfix * 16);
+    int nsel = 0;
+    fame6d_lanczos_stats_t st;
+    fail(fame6d_eigs_tracking(s, FAME6D_FP64, k, p, tol, maxRestart, tolCG, maxitCG, mxGetComplexDoubles(v0), nfix,
+                              fixed, 0, NULL, 0, mxGetDoubles(plhs[0]), mxGetDoubles(rel), sel, nfix, &nsel, Y, &st));
+    if (nlhs > 1) plhs[1] = from_device(Y, n2, nsel);
+    hipFree(Y);
+    if (nlhs > 2) {
+      plhs[2] = mxCreateDoubleMatrix(nsel, 1, mxREAL);
+      for (int i = 0; i < nsel; ++i) mxGetDoubles(plhs[2])[i] = sel[i];
+    }
+    if (nlhs > 3) plhs[3] = rel;
   } else if (!strcmp(cmd, "yee_metrics")) {
     /* cfg.lwrq of exp3/make_config_experiment4.m:72-74 */
     const int n = (int)mxGetScalar(prhs[1]);
