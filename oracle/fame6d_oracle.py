@@ -33,6 +33,29 @@ P_REF = np.array([[1.0, 0.0, 0.0],
 
 EPS = np.finfo(float).eps
 
+# FFT threads of the oracle: None uses numpy's FFT (one thread); an integer uses scipy.fft
+# with that many workers (the timed CPU baseline; the results agree to rounding).
+_FFT_WORKERS = [None]
+
+
+def set_fft_workers(workers):
+    """Threads for the oracle's n-dimensional FFTs (None: numpy, one thread)."""
+    _FFT_WORKERS[0] = workers
+
+
+def _fftn(a):
+    if _FFT_WORKERS[0] is None:
+        return np.fft.fftn(a)
+    import scipy.fft
+    return scipy.fft.fftn(a, workers=_FFT_WORKERS[0])
+
+
+def _ifftn(a):
+    if _FFT_WORKERS[0] is None:
+        return np.fft.ifftn(a)
+    import scipy.fft
+    return scipy.fft.ifftn(a, workers=_FFT_WORKERS[0])
+
 
 # ----------------------------------------------------------------------------- grid
 def fourier_freq(N):
@@ -83,14 +106,14 @@ def mass_apply(eps, x, N):
     x = np.asarray(x, dtype=complex)
     E = unvec(eps, N)
     if x.ndim == 1:
-        return vec(np.fft.fftn(E * np.fft.ifftn(unvec(x, N))))
+        return vec(_fftn(E * _ifftn(unvec(x, N))))
     return np.stack([mass_apply(eps, x[:, j], N) for j in range(x.shape[1])], axis=1)
 
 
 def mass_inverse_apply(eps, x, N):
     """M_s^{-1} x = fftn(ifftn(x) ./ eps): closed form of the inverse circulant operator."""
     E = unvec(eps, N)
-    return vec(np.fft.fftn(np.fft.ifftn(unvec(x, N)) / E))
+    return vec(_fftn(_ifftn(unvec(x, N)) / E))
 
 
 def rank1_plan(N, dim=6):
@@ -119,7 +142,7 @@ def rank1_kernel_fft(eps, plan):
     """kernelFFT = fft(h), h(code) = epsHat[residue(code)]/NF
     (exp1/build_exact_rank1_scalar_kernel_fft.m:9-37)."""
     N, n, B, M = plan["N"], plan["n"], plan["B"], plan["M"]
-    Ehat = vec(np.fft.fftn(unvec(eps, N)))
+    Ehat = vec(_fftn(unvec(eps, N)))
     code = np.arange(M)
     tmp = code.copy()
     res = np.zeros(M, dtype=np.int64)
@@ -699,8 +722,8 @@ def block_centers(points, bid, nb3):
 def mass_adjoint_eps(eps, x, gy, N):
     """d/d eps of Re<gy, M_s x> = NF * Re(conj(ifftn gy) .* ifftn x) (derivative of exp1/applyMij_fft.m:11)."""
     n = (2 * N) ** 6
-    u = np.fft.ifftn(unvec(x, N))
-    v = np.fft.ifftn(unvec(gy, N))
+    u = _ifftn(unvec(x, N))
+    v = _ifftn(unvec(gy, N))
     return vec(n * np.real(np.conj(v) * u))
 
 
@@ -711,7 +734,7 @@ def eig_grad_eps(X, lam, N):
     for j in range(X.shape[1]):
         g = np.zeros(grid_shape(N))
         for c in range(3):
-            g += np.abs(np.fft.ifftn(unvec(X[c * n:(c + 1) * n, j], N))) ** 2
+            g += np.abs(_ifftn(unvec(X[c * n:(c + 1) * n, j], N))) ** 2
         out.append(vec(-lam[j] * n * g))
     return np.stack(out, 1)
 
