@@ -25,37 +25,56 @@ __host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * 
 // v_mfma_f64_16x16x4f64 (wave64): lane l supplies A[m = l%16][k = l/16] and
 // B[k = l/16][n = l%16]; accumulator double r of lane l is D[(l/16) + 4*r][l%16]
 // (row-interleaved; measured on gfx950 with a one-wave probe).
-// Workgroup = 4 waves, one 16-column tile (8 centers) and one slice of the mode
-// range; the operands are generated in LDS per chunk of kTChunk modes: powers
-// q_x^s, q_y^s, q_z^s (A = products of three of them) and B = exp(i q.r_c) coeff.
-// Wave w owns row tiles w, w+4, ... of the (L rounded up to 16) alpha rows.
+// Workgroup = 4 waves, CT 16-column tiles (8 CT centers) and one slice of the mode range;
+// the operands are generated in LDS per chunk of kTChunk modes: powers q_x^s, q_y^s, q_z^s
+// (A = products of three of them) and B = exp(i q.r_c) coeff.  Wave w owns row tiles
+// w, w+4, ..., w+4(TPW-1) of the (L rounded up to 16) alpha rows, with TPW x CT accumulators
+// in AGPRs: each A fragment feeds CT matrix-core ops and each B fragment TPW of them.  The
+// power-row offsets of a lane's alphas are fixed for the whole launch and kept in registers;
+// alpha rows beyond L read an all-zero power row instead of branching.  Up to 20 accumulators
+// two workgroups share a CU (registers capped at 256 per lane, MINB = 2), so one's operand
+// generation (sincos, powers) overlaps the other's matrix-core work; more accumulators do not
+// fit that cap without spilling and run one workgroup per CU.
 constexpr int kTChunk = 64;
-constexpr int kTMaxTilesPerWave = 16;  // L <= 1024 (p <= 16)
+constexpr int kTZeroRow = kTaylorMaxP + 1;  // row of zeros in the power tables
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_taylor_moments_mfma(long long NF, const double* qx, const double* qy,
+template <int TPW, int CT, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF, const double* qx, const double* qy,
                                                              const double* qz, const cpx<double>* coeff, int p,
                                                              int L, int C, const double* centers,
                                                              const unsigned char* alist, long long span,
-                                                             double* part) {
+                                                             int Cpad2, double* part) {
+  constexpr int NCOL = 16 * CT;  // real columns of this workgroup
   const int Lpad = (L + 15) & ~15;
   const int T = Lpad / 16;
-  const int Cpad2 = ((C + 7) / 8) * 16;  // real columns
-  __shared__ double pw[3][(kTaylorMaxP + 1) * kTChunk];
-  __shared__ double bt[kTChunk * 16];
-  __shared__ unsigned char al[3 * 1024];
+  __shared__ double pw[3][(kTZeroRow + 1) * kTChunk];
+  __shared__ double bt[kTChunk * NCOL];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ct = blockIdx.x;  // column tile (8 centers)
+  const int kl = lane >> 4, nl = lane & 15;
+  const int cg = blockIdx.x;  // column group (8 CT centers)
   const long long j_begin = (long long)blockIdx.y * span;
   const long long j_end = min(NF, j_begin + span);
-  for (int i = tid; i < 3 * Lpad; i += 256) al[i] = (i < 3 * L) ? alist[i] : 0;
 
-  double4_t acc[kTMaxTilesPerWave];
+  int o0[TPW], o1[TPW], o2[TPW];
 #pragma unroll
-  for (int t = 0; t < kTMaxTilesPerWave; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < TPW; ++t) {
+    const int alpha = (wave + 4 * t) * 16 + nl;
+    const bool ok = alpha < L;
+    o0[t] = (ok ? (int)alist[3 * alpha] : kTZeroRow) * kTChunk;
+    o1[t] = (ok ? (int)alist[3 * alpha + 1] : 0) * kTChunk;
+    o2[t] = (ok ? (int)alist[3 * alpha + 2] : 0) * kTChunk;
+  }
+  for (int i = tid; i < kTChunk; i += 256) pw[0][kTZeroRow * kTChunk + i] = 0.0;
+
+  double4_t acc[TPW][CT];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[t][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
   for (long long j0 = j_begin; j0 < j_end; j0 += kTChunk) {
-    __syncthreads();  // previous chunk fully consumed (and the alpha table staged)
+    __syncthreads();  // previous chunk fully consumed
     for (int jj = tid; jj < kTChunk; jj += 256) {
       const long long j = j0 + jj;
       const bool ok = j < j_end;
@@ -70,10 +89,10 @@ __global__ __launch_bounds__(256) void k_taylor_moments_mfma(long long NF, const
         z *= d;
       }
     }
-    for (int e = tid; e < kTChunk * 8; e += 256) {
-      const int jj = e >> 3, cl = e & 7;
+    for (int e = tid; e < kTChunk * 8 * CT; e += 256) {
+      const int jj = e / (8 * CT), cl = e - jj * (8 * CT);
       const long long j = j0 + jj;
-      const int c = ct * 8 + cl;
+      const int c = cg * 8 * CT + cl;
       double re = 0.0, im = 0.0;
       if (j < j_end && c < C) {
         double sn, cs;
@@ -82,47 +101,45 @@ __global__ __launch_bounds__(256) void k_taylor_moments_mfma(long long NF, const
         re = v.re;
         im = v.im;
       }
-      bt[jj * 16 + 2 * cl] = re;
-      bt[jj * 16 + 2 * cl + 1] = im;
+      bt[jj * NCOL + 2 * cl] = re;
+      bt[jj * NCOL + 2 * cl + 1] = im;
     }
     __syncthreads();
-    const int kl = lane >> 4, nl = lane & 15;
-#pragma unroll 2
+#pragma unroll 1
     for (int k4 = 0; k4 < kTChunk; k4 += 4) {
       const int kk = k4 + kl;
-      const double bv = bt[kk * 16 + nl];
+      double bv[CT];
 #pragma unroll
-      for (int t = 0; t < kTMaxTilesPerWave; ++t) {
-        const int tile = wave + 4 * t;
-        if (tile < T) {
-          const int alpha = tile * 16 + nl;
-          const double av = pw[0][al[3 * alpha] * kTChunk + kk] * pw[1][al[3 * alpha + 1] * kTChunk + kk] *
-                            pw[2][al[3 * alpha + 2] * kTChunk + kk];
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(alpha < L ? av : 0.0, bv, acc[t], 0, 0, 0);
-        }
+      for (int c = 0; c < CT; ++c) bv[c] = bt[kk * NCOL + c * 16 + nl];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        const double av = pw[0][o0[t] + kk] * pw[1][o1[t] + kk] * pw[2][o2[t] + kk];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[t][c], 0, 0, 0);
       }
     }
   }
   // partial sums of this mode slice: part[(slice*Lpad + alpha)*Cpad2 + col]
 #pragma unroll
-  for (int t = 0; t < kTMaxTilesPerWave; ++t) {
+  for (int t = 0; t < TPW; ++t) {
     const int tile = wave + 4 * t;
     if (tile < T) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int alpha = tile * 16 + (lane >> 4) + 4 * r;
-        const int col = ct * 16 + (lane & 15);
-        part[((long long)blockIdx.y * Lpad + alpha) * Cpad2 + col] = acc[t][r];
-      }
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int alpha = tile * 16 + kl + 4 * r;
+          const int col = cg * NCOL + c * 16 + nl;
+          part[((long long)blockIdx.y * Lpad + alpha) * Cpad2 + col] = acc[t][c][r];
+        }
     }
   }
 }
 
 // Final moments: sum the mode slices, form re + i*im and multiply by i^|alpha|/alpha!.
-__global__ void k_taylor_finish(int L, int C, int nslices, const double* part, const cpx<double>* calpha,
+__global__ void k_taylor_finish(int L, int C, int Cpad2, int nslices, const double* part, const cpx<double>* calpha,
                                 cpx<double>* mom) {
   const int Lpad = (L + 15) & ~15;
-  const int Cpad2 = ((C + 7) / 8) * 16;
   const long long tot = (long long)L * C;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
        i += (long long)gridDim.x * blockDim.x) {
@@ -264,12 +281,28 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
     alpha_tables(p, al, ca);
     const int L = (int)ca.size();
     const int Lpad = (L + 15) & ~15;
-    const int ctiles = (C + 7) / 8;
-    const int Cpad2 = ctiles * 16;
-    // mode slices: enough workgroups to fill the chip, each slice a multiple of the chunk
-    int nslices = (int)std::max<long long>(1, std::min<long long>(NF / kTChunk, (1024 + ctiles - 1) / ctiles));
-    long long span = (NF + nslices - 1) / nslices;
-    span = ((span + kTChunk - 1) / kTChunk) * kTChunk;
+    const int T = Lpad / 16;
+    const int tpw = (T + 3) / 4;           // row tiles per wave
+    const int ct = tpw <= 8 ? 4 : 2;       // 16-column tiles per workgroup (tpw * ct <= 32 accumulators)
+    const int cgroups = (C + 8 * ct - 1) / (8 * ct);
+    const int Cpad2 = cgroups * 16 * ct;
+    // mode slices, each a multiple of the chunk: the grid is kept at whole rounds of the
+    // resident workgroups (at most four rounds) rather than just over a multiple, which would
+    // add a round for a few stragglers
+    int cus = 256;
+    {
+      int dev = 0;
+      hipDeviceProp_t prop;
+      if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+          prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    }
+    const int minb = tpw * ct <= 20 ? 2 : 1;  // workgroups per CU of the instance below
+    const int resident = cus * minb;
+    const long long chunks = (NF + kTChunk - 1) / kTChunk;
+    long long target = std::max<long long>(1, (4LL * resident) / cgroups);
+    int nslices = (int)std::max<long long>(1, std::min<long long>(chunks, target));
+    long long span = ((chunks + nslices - 1) / nslices) * kTChunk;
     nslices = (int)((NF + span - 1) / span);
     std::vector<unsigned char> alb(al.begin(), al.end());
     DevBuf tabs, part;
@@ -281,12 +314,27 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
     F6_HIP(hipMemcpyAsync(b + off_ca, ca.data(), sizeof(cpx<double>) * ca.size(), hipMemcpyHostToDevice, s));
     F6_HIP(hipMemcpyAsync(b + off_cen, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
     part.ensure(sizeof(double) * (size_t)nslices * Lpad * Cpad2);
-    dim3 grid(ctiles, nslices);
-    k_taylor_moments_mfma<<<grid, 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF,
-                                               (const cpx<double>*)coeff, p, L, C, (const double*)(b + off_cen),
-                                               (const unsigned char*)b, span, part.as<double>());
+    dim3 grid(cgroups, nslices);
+    const auto* cf = (const cpx<double>*)coeff;
+    const auto* cen = (const double*)(b + off_cen);
+    const auto* alp = (const unsigned char*)b;
+    switch (tpw) {
+#define F6_TAYLOR_CASE(TP, CTV)                                                                                   \
+  case TP:                                                                                                       \
+    k_taylor_moments_mfma<TP, CTV, (TP * CTV <= 20 ? 2 : 1)><<<grid, 256, 0, s>>>(                              \
+        NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C,                                                   \
+                                                        cen, alp, span, Cpad2, part.as<double>());               \
+    break;
+      F6_TAYLOR_CASE(1, 4) F6_TAYLOR_CASE(2, 4) F6_TAYLOR_CASE(3, 4) F6_TAYLOR_CASE(4, 4)
+      F6_TAYLOR_CASE(5, 4) F6_TAYLOR_CASE(6, 4) F6_TAYLOR_CASE(7, 4) F6_TAYLOR_CASE(8, 4)
+      F6_TAYLOR_CASE(9, 2) F6_TAYLOR_CASE(10, 2) F6_TAYLOR_CASE(11, 2) F6_TAYLOR_CASE(12, 2)
+      F6_TAYLOR_CASE(13, 2) F6_TAYLOR_CASE(14, 2) F6_TAYLOR_CASE(15, 2) F6_TAYLOR_CASE(16, 2)
+#undef F6_TAYLOR_CASE
+      default:
+        throw Error(FAME6D_EARG, "Taylor order too large");
+    }
     F6_CHECK_LAUNCH();
-    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, nslices, part.as<double>(),
+    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, Cpad2, nslices, part.as<double>(),
                                                                (const cpx<double>*)(b + off_ca),
                                                                (cpx<double>*)moments);
     F6_CHECK_LAUNCH();
