@@ -18,3 +18,4 @@ from .taylor import (auto_taylor_blocks, block_partition, build_multi_index_list
                      taylor_moments, yee_block_centers, yee_points)
 from .yee import (apply_cropped_A, compute_lwrq, compute_yee_lwrq_metrics, crop_plus_field,  # noqa: F401
                   sample_scalar_yee_material)
+from . import results  # noqa: F401,E402
