@@ -633,11 +633,76 @@ static PcgOut pcg_host(fame6d_mass* m, long long n, const Matvec& A, const Matve
   return o;
 }
 
+static void mass3(fame6d_mass* m, const cpx<double>* x, cpx<double>* y, Skip skip, hipStream_t s);
+
+// p = r on the first iteration, p = r + beta p after it (MATLAB pcg without preconditioner);
+// skipped once the solve is done.
+static __global__ void k_cg_direction(const CgState* st, const cpx<double>* r, cpx<double>* p, long long n) {
+  if (st->done) return;
+  const bool first = st->iter == 0;
+  const double beta = st->beta;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = first ? r[i] : r[i] + beta * p[i];
+}
+
+// MATLAB pcg (x0 = 0, no preconditioner) on M = I3 (x) M_s with its control on the device:
+// the iteration kernels read and update one CgState and are skipped once it is done; the
+// host issues `hint` iterations at a time and reads the state once per chunk, and the
+// true-residual confirmation and "moresteps" continuation run on the device too.  `hint`
+// carries the last solve's iteration count to the next one.
 static PcgOut pcg_mass3(fame6d_mass* m, const cpx<double>* b, cpx<double>* x, cpx<double>* r,
-                        cpx<double>* p, cpx<double>* q, double tol, int maxit) {
-  const long long NF = m->NF;
-  Matvec A = [&](const cpx<double>* in, cpx<double>* out) { mass_apply<double>(m, in, NF, out, NF, 3, m->stream); };
-  return pcg_host(m, 3 * NF, A, nullptr, b, x, r, p, q, nullptr, tol, maxit);
+                        cpx<double>* p, cpx<double>* q, double tol, int maxit, int& hint) {
+  hipStream_t s = m->stream;
+  const long long n = 3 * m->NF;
+  const int nb = red_blocks(n);
+  const int gr = grid_for(n);
+  DevBuf ctl;
+  ctl.ensure(256 + sizeof(double) * (size_t)nb + 256);
+  CgState* st = ctl.as<CgState>();
+  double* part = reinterpret_cast<double*>(ctl.as<char>() + 256);
+  int* need = reinterpret_cast<int*>(part + nb);
+  int* resume = need + 1;
+  const Skip sk{reinterpret_cast<const int*>(reinterpret_cast<const char*>(st) + offsetof(CgState, done)), 0};
+  const int maxmsteps = (int)std::min<long long>({n / 50, 5, std::max<long long>(n - maxit, 0)});
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n, x);
+  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<double>) * n, hipMemcpyDeviceToDevice, s));
+  k_dot_partials<double><<<dim3(nb, 1), kRedThreads, 0, s>>>(b, n, b, n, n, part, nullptr);
+  k_cg_init<<<1, kRedThreads, 0, s>>>(st, part, nb, tol, maxit, maxmsteps);
+  F6_CHECK_LAUNCH();
+  int chunk = std::max(2, hint);
+  CgState hs{};
+  for (int rounds = 0;; ++rounds) {
+    if (rounds > maxit + 8) throw Error(FAME6D_ENOCONV, "M CG control loop did not terminate");
+    for (int c = 0; c < chunk; ++c) {
+      k_cg_direction<<<gr, 256, 0, s>>>(st, r, p, n);
+      mass3(m, p, q, sk, s);
+      k_dot_partials<double><<<dim3(nb, 1), kRedThreads, 0, s>>>(p, n, q, n, n, part, nullptr, sk);
+      k_cg_alpha<<<1, kRedThreads, 0, s>>>(st, part, nb);
+      k_cg_update<double><<<dim3(nb, 1), kRedThreads, 0, s>>>(x, p, r, q, n, n, st, part);
+      k_cg_beta<<<1, kRedThreads, 0, s>>>(st, part, nb);
+      F6_CHECK_LAUNCH();
+    }
+    // true residual r = b - M x once the recurrence has stopped (MATLAB pcg's confirmation)
+    k_cg_need_check<<<1, 64, 0, s>>>(st, 1, need);
+    const Skip skn{need, 0};
+    mass3(m, x, q, skn, s);
+    k_residual<double><<<dim3(gr, 1), 256, 0, s>>>(b, q, r, n, n, skn);
+    k_dot_partials<double><<<dim3(nb, 1), kRedThreads, 0, s>>>(r, n, r, n, n, part, nullptr, skn);
+    k_cg_true_check<<<1, kRedThreads, 0, s>>>(st, part, nb, need, resume, 0);
+    F6_CHECK_LAUNCH();
+    d2h_sync(m, &hs, st, sizeof(CgState));
+    if (hs.finished) break;
+    chunk = hs.check_every ? 1 : 2;
+  }
+  hint = std::max(2, hs.iter + 1);
+  PcgOut o;
+  o.flag = hs.flag;
+  o.iter = hs.iter;
+  o.checks = hs.checks;
+  const double nr = hs.checks > 0 ? hs.normr_true : hs.normr;
+  o.relres = hs.n2b > 0 ? nr / hs.n2b : (std::isnan(hs.n2b) ? hs.n2b : 0.0);
+  return o;
 }
 
 // ------------------------------------------------------------------ Method B
@@ -687,13 +752,13 @@ void krinv_nested(fame6d_qsys* S, const cpx<double>* bin, cpx<double>* xout, dou
   cpx<double>* tmp = iq + n3;
   const int gr = grid_for(NF);
   long long inner_its = 0, inner_calls = 0;
-  int inner_max = 0;
+  int inner_max = 0, inner_hint = 0;
   double inner_relres = 0;
   Matvec Kr = [&](const cpx<double>* zin, cpx<double>* yo) {
     k_apply_v<<<dim3(gr, 1), 256, 0, s>>>(zin, n2, NF, S->V(0), S->V(1), S->V(2), S->V(3), S->V(4), S->V(5),
                                           vz, n3, no_skip(), nullptr);
     F6_CHECK_LAUNCH();
-    PcgOut o = pcg_mass3(m, vz, mi, ir, ip, iq, tolM, maxitM);
+    PcgOut o = pcg_mass3(m, vz, mi, ir, ip, iq, tolM, maxitM, inner_hint);
     if (!std::isfinite(o.relres) || (o.flag != 0 && o.relres > 10 * tolM))
       throw Error(FAME6D_ENOCONV, "Nested M solve failed: flag=" + std::to_string(o.flag));
     inner_its += o.iter;
@@ -744,13 +809,14 @@ void recover_eigenvectors(fame6d_qsys* S, const cpx<double>* Y, int k, double to
   cpx<double>* p = rhs + 2 * n3;
   cpx<double>* q = rhs + 3 * n3;
   const int gr = grid_for(NF);
+  int hint = 0;
   for (int j = 0; j < k; ++j) {
     k_apply_v<<<dim3(gr, 1), 256, 0, s>>>(Y + (size_t)2 * NF * j, 2 * NF, NF, S->V(0), S->V(1), S->V(2),
                                           S->V(3), S->V(4), S->V(5), rhs, n3, no_skip(), nullptr);
     F6_CHECK_LAUNCH();
     cpx<double>* xj = X + (size_t)n3 * j;
     cpx<double>* mxj = MX + (size_t)n3 * j;
-    PcgOut o = pcg_mass3(m, rhs, xj, r, p, q, tol, maxit);
+    PcgOut o = pcg_mass3(m, rhs, xj, r, p, q, tol, maxit, hint);
     if (!std::isfinite(o.relres) || (o.flag != 0 && o.relres > 10 * tol))
       throw Error(FAME6D_ENOCONV, "Recovery CG failed for eigenvector " + std::to_string(j + 1));
     mass_apply<double>(m, xj, NF, mxj, NF, 3, s);
