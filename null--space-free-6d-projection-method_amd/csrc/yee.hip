@@ -23,7 +23,9 @@
 // residual is recomputed by a second pass over u (48 B) instead.
 //
 // Launch shape: workgroups of 64 x 4 threads own (x, y) tiles of 64 x 4 points (one wave per
-// row segment, coalesced) and loop over the third index, so no index is ever divided.
+// row segment, coalesced) and loop over a chunk of the third index, so no grid index is ever
+// divided.  The point passes keep the plus-grid planes k-1, k, k+1 of their tile (with a
+// one-point halo) in LDS; the convolutions march a register window.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -37,6 +39,7 @@ constexpr int kYeeTX = 64, kYeeTY = 4, kYeeThreads = kYeeTX * kYeeTY;
 constexpr int kYeeMaxBlocks = 2048;
 constexpr int kYeeMaxR = 8;        // LWRQ neighbour radius <= 8 cells
 constexpr int kYeeMaxRows = 16;    // rows of the 3D-to-6D projection P
+constexpr int kYeeMinChunk = 16;   // planes per workgroup of the point passes (2 more are loaded)
 
 struct YeeGeom {
   long long q, m;  // interior and plus-grid extents
@@ -76,37 +79,38 @@ __device__ __forceinline__ double yee_eps(const YeeMat& mt, long long q, int com
   return mt.epsC * exp(mt.a6 * S);
 }
 
-// Cropped curl-curl at the interior point with plus-grid offset c.  Operator order and signs
-// as exp3/apply_cropped_A_exp3.m:11-28 with the difference operators of :32-56 (each a
-// combination of the point and its neighbours at offsets in {-1, 0, 1}).
-__device__ __forceinline__ void yee_curlcurl(const cpx<double>* __restrict__ u1, const cpx<double>* __restrict__ u2,
-                                             const cpx<double>* __restrict__ u3, long long c, long long sy,
-                                             long long sz, double h2inv, cpx<double>& A1, cpx<double>& A2,
+// Cropped curl-curl at one interior point from an accessor U(comp, dx, dy, dz) of the
+// plus-grid fields at offsets in {-1, 0, 1}.  Operator order and signs as
+// exp3/apply_cropped_A_exp3.m:11-28 with the difference operators of :32-56.
+template <class Get>
+__device__ __forceinline__ void yee_curlcurl(const Get& U, double h2inv, cpx<double>& A1, cpx<double>& A2,
                                              cpx<double>& A3) {
-  const cpx<double> u1c = u1[c], u2c = u2[c], u3c = u3[c];
-  const cpx<double> u1xm = u1[c - 1], u2xm = u2[c - 1], u3xm = u3[c - 1];
-  const cpx<double> u2xp = u2[c + 1], u3xp = u3[c + 1];
-  const cpx<double> u1ym = u1[c - sy], u1yp = u1[c + sy], u2ym = u2[c - sy], u3ym = u3[c - sy], u3yp = u3[c + sy];
-  const cpx<double> u1zm = u1[c - sz], u1zp = u1[c + sz], u2zm = u2[c - sz], u2zp = u2[c + sz], u3zm = u3[c - sz];
+  const cpx<double> u1c = U(0, 0, 0, 0), u2c = U(1, 0, 0, 0), u3c = U(2, 0, 0, 0);
+  const cpx<double> u1xm = U(0, -1, 0, 0), u2xm = U(1, -1, 0, 0), u3xm = U(2, -1, 0, 0);
+  const cpx<double> u2xp = U(1, 1, 0, 0), u3xp = U(2, 1, 0, 0);
+  const cpx<double> u1ym = U(0, 0, -1, 0), u1yp = U(0, 0, 1, 0), u2ym = U(1, 0, -1, 0), u3ym = U(2, 0, -1, 0),
+                    u3yp = U(2, 0, 1, 0);
+  const cpx<double> u1zm = U(0, 0, 0, -1), u1zp = U(0, 0, 0, 1), u2zm = U(1, 0, 0, -1), u2zp = U(1, 0, 0, 1),
+                    u3zm = U(2, 0, 0, -1);
   {
     const cpx<double> dyy = (-u1ym + 2.0 * u1c) - u1yp;
     const cpx<double> dzz = (-u1zm + 2.0 * u1c) - u1zp;
-    const cpx<double> dytdxp = ((u2[c + 1 - sy] - u2ym) - u2xp) + u2c;
-    const cpx<double> dztdxp = ((u3[c + 1 - sz] - u3zm) - u3xp) + u3c;
+    const cpx<double> dytdxp = ((U(1, 1, -1, 0) - u2ym) - u2xp) + u2c;
+    const cpx<double> dztdxp = ((U(2, 1, 0, -1) - u3zm) - u3xp) + u3c;
     A1 = h2inv * (((dyy + dzz) - dytdxp) - dztdxp);
   }
   {
     const cpx<double> dxx = (-u2xm + 2.0 * u2c) - u2xp;
     const cpx<double> dzz = (-u2zm + 2.0 * u2c) - u2zp;
-    const cpx<double> dxtdyp = ((-u1xm + u1c) + u1[c - 1 + sy]) - u1yp;
-    const cpx<double> dztdyp = ((-u3zm + u3[c + sy - sz]) + u3c) - u3yp;
+    const cpx<double> dxtdyp = ((-u1xm + u1c) + U(0, -1, 1, 0)) - u1yp;
+    const cpx<double> dztdyp = ((-u3zm + U(2, 0, 1, -1)) + u3c) - u3yp;
     A2 = h2inv * (((dxx + dzz) - dxtdyp) - dztdyp);
   }
   {
     const cpx<double> dxx = (-u3xm + 2.0 * u3c) - u3xp;
     const cpx<double> dyy = (-u3ym + 2.0 * u3c) - u3yp;
-    const cpx<double> dxtdzp = ((-u1xm + u1c) + u1[c - 1 + sz]) - u1zp;
-    const cpx<double> dytdzp = ((-u2ym + u2c) + u2[c - sy + sz]) - u2zp;
+    const cpx<double> dxtdzp = ((-u1xm + u1c) + U(0, -1, 0, 1)) - u1zp;
+    const cpx<double> dytdzp = ((-u2ym + u2c) + U(1, 0, -1, 1)) - u2zp;
     A3 = h2inv * (((dxx + dyy) - dxtdzp) - dytdzp);
   }
 }
@@ -136,61 +140,163 @@ __device__ __forceinline__ void yee_block_sums(double (&v)[NV], double* red) {
 
 enum YeeMode : int { YEE_APPLY_A = 0, YEE_DENSITIES = 1, YEE_RESIDUALS = 2 };
 
-// One pass over the window interior; thread (i, j) loops over k.
+// Plus-grid planes held in LDS by the point passes: the 64 x 4 interior tile of a workgroup
+// with a one-point halo in x and y, three components; a plane is kYPE values, loaded as
+// kYPR rounds of the 256 threads.
+constexpr int kYPX = kYeeTX + 2, kYPY = kYeeTY + 2, kYPC = kYPX * kYPY, kYPE = 3 * kYPC;
+constexpr int kYPR = (kYPE + kYeeThreads - 1) / kYeeThreads;
+
+// One pass over the window interior, a 2.5D stencil: the workgroup owns a 64 x 4 tile of
+// (i, j) and walks k through chunk blockIdx.z, holding the plus-grid planes below, at and
+// above the current one (halo included) in an LDS ring, so every field value is read from
+// memory about once.  The plane two steps ahead is in flight in registers while a plane is
+// computed (two register sets, the loop unrolled by two), so the load latency overlaps the
+// compute of two planes.
 //   YEE_APPLY_A:   A1..A3 = cropped curl-curl of u
 //   YEE_DENSITIES: a/den, b/den per point (a = Re(u' A u), b = Re(u' B u)); partials
 //                  [sum a, sum b, |Au - l0 Bu|^2, Re<Au - l0 Bu, Bu>, |Bu|^2, |Au|^2]
 //   YEE_RESIDUALS: partials [|Au - l0 Bu|^2]
 template <int MODE>
-__global__ __launch_bounds__(kYeeThreads) void k_yee_points(YeeGeom g, const cpx<double>* __restrict__ u1,
-                                                            const cpx<double>* __restrict__ u2,
-                                                            const cpx<double>* __restrict__ u3, YeeMat mt,
-                                                            cpx<double>* A1o, cpx<double>* A2o, cpx<double>* A3o,
-                                                            const double* prof, double* a_out, double* b_out,
-                                                            double lam0, double* partials) {
+__global__ __launch_bounds__(kYeeThreads, 2) void k_yee_points(YeeGeom g, const cpx<double>* __restrict__ u1,
+                                                               const cpx<double>* __restrict__ u2,
+                                                               const cpx<double>* __restrict__ u3, YeeMat mt,
+                                                               cpx<double>* A1o, cpx<double>* A2o,
+                                                               cpx<double>* A3o, const double* prof,
+                                                               double* a_out, double* b_out, double lam0,
+                                                               long long chunk, double* partials) {
   constexpr int NV = MODE == YEE_DENSITIES ? 6 : 1;
+  __shared__ cpx<double> pl[3][kYPE];
   __shared__ double red[NV * kYeeThreads];
-  const long long q = g.q, m = g.m, sy = m, sz = m * m;
-  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, j = blockIdx.y * (long long)kYeeTY + threadIdx.y;
+  const long long q = g.q, m = g.m, mm = g.m * g.m;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = yee_tid();
+  const long long i0 = blockIdx.x * (long long)kYeeTX, j0 = blockIdx.y * (long long)kYeeTY;
+  const long long i = i0 + tx, j = j0 + ty;
+  const bool act = i < q && j < q;
+  const long long k0 = blockIdx.z * chunk, k1 = min(q, k0 + chunk), kend = k1 + 1;  // last plus plane
+  // This thread's share of a plane: element e = tid + 256 r (component, halo row, halo column).
+  // Halo cells beyond the plus grid are read clamped to it: active points never use them
+  // (i, j < q touch plus indices <= q + 1 = m - 1).
+  const cpx<double>* src[kYPR];
+#pragma unroll
+  for (int r = 0; r < kYPR; ++r) {
+    const int e = min(tid + kYeeThreads * r, kYPE - 1);
+    const int comp = e / kYPC, rr = e - comp * kYPC, yy = rr / kYPX, xx = rr - yy * kYPX;
+    const long long gx = min(i0 + xx, m - 1), gy = min(j0 + yy, m - 1);
+    src[r] = (comp == 0 ? u1 : (comp == 1 ? u2 : u3)) + gx + m * gy;
+  }
+  // The in-flight planes are clang vectors: struct copies into a local array become memcpys
+  // that keep the array in scratch memory instead of registers.
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  auto gload = [&](d2(&v)[kYPR], long long z) {  // planes beyond kend: kend again, unused
+    const long long off = min(z, kend) * mm;
+#pragma unroll
+    for (int r = 0; r < kYPR; ++r) v[r] = *reinterpret_cast<const d2*>(src[r] + off);
+  };
+  auto lstore = [&](int slot, const d2(&v)[kYPR], long long z) {
+    if (z <= kend) {
+#pragma unroll
+      for (int r = 0; r < kYPR; ++r)
+        if (tid + kYeeThreads * r < kYPE) *reinterpret_cast<d2*>(&pl[slot][tid + kYeeThreads * r]) = v[r];
+    }
+  };
+  // Separable permittivity: the (i, j) factors of each component are fixed for the thread and
+  // the k factor is uniform over the workgroup.
+  double exy[3] = {0.0, 0.0, 0.0};
+  if (mt.sep && act) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) exy[c] = mt.sep[((c == 0 ? 3 : 0) + 0) * q + i] * mt.sep[((c == 1 ? 3 : 0) + 1) * q + j];
+  }
+  const double pij = (prof && act) ? prof[i] * prof[j] : 1.0;
   double acc[NV];
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  if (i < q && j < q) {
-    const double pij = prof ? prof[i] * prof[j] : 1.0;
-    for (long long k = blockIdx.z; k < q; k += gridDim.z) {
-      const long long t = i + q * (j + q * k);
-      const long long c = (i + 1) + m * ((j + 1) + m * (k + 1));
-      cpx<double> A1, A2, A3;
-      yee_curlcurl(u1, u2, u3, c, sy, sz, g.h2inv, A1, A2, A3);
-      if constexpr (MODE == YEE_APPLY_A) {
-        A1o[t] = A1;
-        A2o[t] = A2;
-        A3o[t] = A3;
+
+  // Interior plane k from the slots holding plus planes k, k + 1, k + 2.
+  auto compute = [&](long long k, int s0, int s1, int s2) {
+    if (!act) return;
+    const int c = (ty + 1) * kYPX + (tx + 1);
+    const cpx<double>* Pm = pl[s0];
+    const cpx<double>* Pc = pl[s1];
+    const cpx<double>* Pp = pl[s2];
+    auto U = [&](int comp, int dx, int dy, int dz) -> cpx<double> {
+      const cpx<double>* P = dz < 0 ? Pm : (dz == 0 ? Pc : Pp);
+      return P[comp * kYPC + c + dx + kYPX * dy];
+    };
+    cpx<double> A1, A2, A3;
+    yee_curlcurl(U, g.h2inv, A1, A2, A3);
+    const long long t = i + q * (j + q * k);
+    if constexpr (MODE == YEE_APPLY_A) {
+      A1o[t] = A1;
+      A2o[t] = A2;
+      A3o[t] = A3;
+    } else {
+      const cpx<double> v1 = Pc[c], v2 = Pc[kYPC + c], v3 = Pc[2 * kYPC + c];
+      double e1, e2, e3;
+      if (mt.sep) {
+        const double ez = mt.sep[2 * q + k], ez2 = mt.sep[5 * q + k];
+        e1 = mt.epsC * (exy[0] * ez);
+        e2 = mt.epsC * (exy[1] * ez);
+        e3 = mt.epsC * (exy[2] * ez2);
       } else {
-        const cpx<double> v1 = u1[c], v2 = u2[c], v3 = u3[c];
-        const cpx<double> B1 = yee_eps(mt, q, 0, i, j, k) * v1, B2 = yee_eps(mt, q, 1, i, j, k) * v2,
-                          B3 = yee_eps(mt, q, 2, i, j, k) * v3;
-        const cpx<double> R1 = A1 - lam0 * B1, R2 = A2 - lam0 * B2, R3 = A3 - lam0 * B3;
-        if constexpr (MODE == YEE_DENSITIES) {
-          const double a = (re_dot(v1, A1) + re_dot(v2, A2)) + re_dot(v3, A3);
-          const double b = (re_dot(v1, B1) + re_dot(v2, B2)) + re_dot(v3, B3);
-          const double den = pij * prof[k];
-          a_out[t] = a / den;
-          b_out[t] = b / den;
-          acc[0] += a;
-          acc[1] += b;
-          acc[2] += abs2(R1) + abs2(R2) + abs2(R3);
-          acc[3] += re_dot(R1, B1) + re_dot(R2, B2) + re_dot(R3, B3);
-          acc[4] += abs2(B1) + abs2(B2) + abs2(B3);
-          acc[5] += abs2(A1) + abs2(A2) + abs2(A3);
-        } else {
-          acc[0] += abs2(R1) + abs2(R2) + abs2(R3);
-        }
+        e1 = yee_eps(mt, q, 0, i, j, k);
+        e2 = yee_eps(mt, q, 1, i, j, k);
+        e3 = yee_eps(mt, q, 2, i, j, k);
       }
+      const cpx<double> B1 = e1 * v1, B2 = e2 * v2, B3 = e3 * v3;
+      const cpx<double> R1 = A1 - lam0 * B1, R2 = A2 - lam0 * B2, R3 = A3 - lam0 * B3;
+      if constexpr (MODE == YEE_DENSITIES) {
+        const double a = (re_dot(v1, A1) + re_dot(v2, A2)) + re_dot(v3, A3);
+        const double b = (re_dot(v1, B1) + re_dot(v2, B2)) + re_dot(v3, B3);
+        const double den = pij * prof[k];
+        a_out[t] = a / den;
+        b_out[t] = b / den;
+        acc[0] += a;
+        acc[1] += b;
+        acc[2] += abs2(R1) + abs2(R2) + abs2(R3);
+        acc[3] += re_dot(R1, B1) + re_dot(R2, B2) + re_dot(R3, B3);
+        acc[4] += abs2(B1) + abs2(B2) + abs2(B3);
+        acc[5] += abs2(A1) + abs2(A2) + abs2(A3);
+      } else {
+        acc[0] += abs2(R1) + abs2(R2) + abs2(R3);
+      }
+    }
+  };
+
+  if (k0 < k1) {
+    d2 va[kYPR], vb[kYPR];
+    gload(va, k0);  // plus planes k0 .. k0 + 2 (interior k0 - 1 .. k0 + 1); k0 + 2 <= kend
+    gload(vb, k0 + 1);
+    lstore(0, va, k0);
+    lstore(1, vb, k0 + 1);
+    gload(va, k0 + 2);
+    lstore(2, va, k0 + 2);
+    gload(va, k0 + 3);  // in flight: plus planes k0 + 3 (va) and k0 + 4 (vb)
+    gload(vb, k0 + 4);
+    __syncthreads();
+    int s0 = 0, s1 = 1, s2 = 2;
+    for (long long k = k0; k < k1; k += 2) {
+      // step k: slots (s0, s1, s2) hold plus planes k, k + 1, k + 2
+      compute(k, s0, s1, s2);
+      __syncthreads();
+      lstore(s0, va, k + 3);
+      gload(va, k + 5);
+      __syncthreads();
+      if (k + 1 < k1) {
+        // step k + 1: slots (s1, s2, s0)
+        compute(k + 1, s1, s2, s0);
+        __syncthreads();
+        lstore(s1, vb, k + 4);
+        gload(vb, k + 6);
+        __syncthreads();
+      }
+      const int sn = s0;  // two rotations of the ring
+      s0 = s2;
+      s2 = s1;
+      s1 = sn;
     }
   }
   if constexpr (MODE != YEE_APPLY_A) {
     yee_block_sums<NV>(acc, red);
-    if (yee_tid() == 0) {
+    if (tid == 0) {
       const int b = yee_block(), nb = yee_nblocks();
       for (int v = 0; v < NV; ++v) partials[v * nb + b] = acc[v];
     }
@@ -215,44 +321,111 @@ struct Taps {
   int R;
 };
 
-// Zero-padded 'same' convolution of two arrays along x; thread (i, j) loops over k.  With
-// `prof` every source value is first divided by den = prof(i') prof(j) prof(k) (the column
-// normalisation of exp3/compute_lwrq_exp4.m:22-26) and partials [sum in_a, sum in_b] of the
-// raw inputs are written.
+// Zero-padded 'same' convolution of two arrays along x; the workgroup's 64 x 4 tile of (i, j)
+// loops over k, staging the tile's rows with an R-point halo on both sides in LDS (one load
+// per value; the zero padding is stored there), double-buffered with the next plane's values
+// in flight in registers.  With `prof` every source value is first divided by
+// den = prof(i') prof(j) prof(k) (the column normalisation of exp3/compute_lwrq_exp4.m:22-26)
+// and partials [sum in_a, sum in_b] of the raw inputs are written.
+template <int R>
 __global__ __launch_bounds__(kYeeThreads) void k_lwrq_conv_x(long long q, Taps tp, const double* __restrict__ ina,
                                                              const double* __restrict__ inb, double* outa,
                                                              double* outb, const double* prof, double* partials) {
+  constexpr int W = 2 * R + 1, RW = kYeeTX + 2 * R, NE = kYeeTY * RW;
+  constexpr int NR = (NE + kYeeThreads - 1) / kYeeThreads;
+  __shared__ double ra[2][NE], rb[2][NE];
   __shared__ double red[2 * kYeeThreads];
-  const long long i = blockIdx.x * (long long)kYeeTX + threadIdx.x, j = blockIdx.y * (long long)kYeeTY + threadIdx.y;
-  double acc[2] = {0.0, 0.0};
-  if (i < q && j < q) {
-    for (long long k = blockIdx.z; k < q; k += gridDim.z) {
-      const long long t = i + q * (j + q * k);
-      const double pjk = prof ? prof[j] * prof[k] : 1.0;
-      double sa = 0.0, sb = 0.0;
-      for (int tau = -tp.R; tau <= tp.R; ++tau) {
-        const long long src = i - tau;
-        if (src < 0 || src >= q) continue;
-        double va = ina[t - tau], vb = inb[t - tau];
-        if (prof) {
-          const double den = prof[src] * pjk;
-          va = va / den;
-          vb = vb / den;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = yee_tid();
+  const long long i0 = blockIdx.x * (long long)kYeeTX, j0 = blockIdx.y * (long long)kYeeTY;
+  const long long i = i0 + tx, j = j0 + ty, qq = q * q, gz = gridDim.z;
+  const bool act = i < q && j < q;
+  // This thread's share of a staged plane: element e = tid + 256 r (row e / RW, column e % RW)
+  // is source point (i0 - R + column, j0 + row); outside the window it is the zero padding.
+  bool ok[NR];
+  long long off[NR];
+  double pi[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int e = tid + kYeeThreads * r, row = e / RW, col = e - row * RW;
+    const long long gi = i0 - R + col, gj = j0 + row;
+    ok[r] = e < NE && gi >= 0 && gi < q && gj < q;
+    off[r] = ok[r] ? gi + q * gj : 0;
+    pi[r] = (ok[r] && prof) ? prof[gi] * prof[gj] : 1.0;
+  }
+  auto gload = [&](double (&va)[NR], double (&vb)[NR], long long k) {
+    if (k >= q) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double x = 0.0, y = 0.0;
+      if (ok[r]) {
+        x = ina[off[r] + qq * k];
+        y = inb[off[r] + qq * k];
+      }
+      va[r] = x;
+      vb[r] = y;
+    }
+  };
+  auto lstore = [&](int slot, const double (&va)[NR], const double (&vb)[NR], long long k) {
+    if (k >= q) return;
+    const double pk = prof ? prof[k] : 1.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int e = tid + kYeeThreads * r;
+      if (e < NE) {
+        double x = va[r], y = vb[r];
+        if (prof && ok[r]) {
+          const double den = pi[r] * pk;
+          x = x / den;
+          y = y / den;
         }
-        sa += tp.k[tau + tp.R] * va;
-        sb += tp.k[tau + tp.R] * vb;
+        ra[slot][e] = x;
+        rb[slot][e] = y;
       }
-      outa[t] = sa;
-      outb[t] = sb;
+    }
+  };
+  double kk[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) kk[t] = tp.k[t];
+  double acc[2] = {0.0, 0.0};
+  // plane k is computed from LDS slot sl while planes k + gz (being stored into the other
+  // slot) and k + 2 gz are in registers; the loop is unrolled by two for the two register sets
+  auto step = [&](long long k, int sl, double (&xa)[NR], double (&xb)[NR]) {
+    if (act) {
+      const int c = ty * RW + tx + R;
+      double sa = 0.0, sb = 0.0;
+#pragma unroll
+      for (int t = 0; t < W; ++t) {  // tap tau = t - R reads source i - tau
+        sa += kk[t] * ra[sl][c - (t - R)];
+        sb += kk[t] * rb[sl][c - (t - R)];
+      }
+      const long long pt = i + q * j + qq * k;
+      outa[pt] = sa;
+      outb[pt] = sb;
       if (partials) {
-        acc[0] += ina[t];
-        acc[1] += inb[t];
+        acc[0] += ina[pt];
+        acc[1] += inb[pt];
       }
+    }
+    lstore(sl ^ 1, xa, xb, k + gz);  // the other slot was last read before the previous barrier
+    gload(xa, xb, k + 3 * gz);
+    __syncthreads();
+  };
+  const long long kb = blockIdx.z;
+  if (kb < q) {
+    double a0[NR], b0[NR], a1[NR], b1[NR];
+    gload(a0, b0, kb);
+    lstore(0, a0, b0, kb);
+    gload(a0, b0, kb + gz);
+    gload(a1, b1, kb + 2 * gz);
+    __syncthreads();
+    for (long long k = kb; k < q; k += 2 * gz) {
+      step(k, 0, a0, b0);
+      if (k + gz < q) step(k + gz, 1, a1, b1);
     }
   }
   if (partials) {
     yee_block_sums<2>(acc, red);
-    if (yee_tid() == 0) {
+    if (tid == 0) {
       const int b = yee_block(), nb = yee_nblocks();
       partials[b] = acc[0];
       partials[nb + b] = acc[1];
@@ -401,12 +574,13 @@ static YeeLaunch yee_launch(long long q, long long along) {
   return l;
 }
 
-// The marching kernels split the walked axis into chunks; chunk >= R keeps the reloaded halo
-// a small fraction of each chunk.
-static YeeLaunch yee_march_launch(long long q, int R, long long& chunk) {
+// The marching kernels split the walked axis into chunks; chunk >= min_chunk keeps the
+// reloaded halo (2R planes for the convolutions, 2 for the point passes) a small fraction of
+// each chunk.
+static YeeLaunch yee_march_launch(long long q, long long min_chunk, long long& chunk) {
   YeeLaunch l = yee_launch(q, q);
   chunk = (q + l.grid.z - 1) / l.grid.z;
-  chunk = std::max<long long>(chunk, std::max(4 * R, 1));
+  chunk = std::max<long long>(chunk, std::max<long long>(min_chunk, 1));
   const long long gz = (q + chunk - 1) / chunk;
   l.grid.z = (unsigned)gz;
   l.nb = (int)(l.grid.x * l.grid.y * gz);
@@ -532,7 +706,7 @@ template <int R>
 static void launch_march(long long q, const Taps& tp, const double* ina, const double* inb, double* outa,
                          double* outb, const double* sc, double* rho, double* part, bool reduce, hipStream_t s) {
   long long chunk = 0;
-  const YeeLaunch l = yee_march_launch(q, R, chunk);
+  const YeeLaunch l = yee_march_launch(q, 4 * R, chunk);
   if (reduce)
     k_lwrq_march<R, 2, true><<<l.grid, l.block, 0, s>>>(q, tp, ina, inb, nullptr, nullptr, chunk, sc, rho, part);
   else
@@ -556,6 +730,27 @@ static void march(int R, long long q, const Taps& tp, const double* ina, const d
   }
 }
 
+static void conv_x(int R, const YeeLaunch& l, long long q, const Taps& tp, const double* ina, const double* inb,
+                   double* outa, double* outb, const double* prof, double* part, hipStream_t s) {
+#define F6_CONV_X(RR)                                                                                     \
+  case RR:                                                                                                \
+    k_lwrq_conv_x<RR><<<l.grid, l.block, 0, s>>>(q, tp, ina, inb, outa, outb, prof, part);                \
+    break;
+  switch (R) {
+    F6_CONV_X(1)
+    F6_CONV_X(2)
+    F6_CONV_X(3)
+    F6_CONV_X(4)
+    F6_CONV_X(5)
+    F6_CONV_X(6)
+    F6_CONV_X(7)
+    F6_CONV_X(8)
+    default: throw Error(FAME6D_EARG, "LWRQ neighbour radius must be in 1..8");
+  }
+#undef F6_CONV_X
+  F6_CHECK_LAUNCH();
+}
+
 // The three convolution passes of the LWRQ and its reductions; one synchronisation.
 // normalise: the x pass divides a, b by den and sums them into sc[0], sc[1] (the standalone
 // LWRQ); otherwise a, b are already a/den, b/den and pass 1 has put sc[0..5].  w2/w3 and
@@ -563,15 +758,13 @@ static void march(int R, long long q, const Taps& tp, const double* ina, const d
 static LwrqSums lwrq_device(const YeeWs& w, long long q, const Taps& tp, const double* a, const double* b,
                             double* rho, hipStream_t s, bool normalise) {
   const YeeLaunch lx = yee_launch(q, q);
-  k_lwrq_conv_x<<<lx.grid, lx.block, 0, s>>>(q, tp, a, b, w.w2, w.w3, normalise ? w.prof : nullptr,
-                                             normalise ? w.part : nullptr);
-  F6_CHECK_LAUNCH();
+  conv_x(tp.R, lx, q, tp, a, b, w.w2, w.w3, normalise ? w.prof : nullptr, normalise ? w.part : nullptr, s);
   if (normalise) k_yee_finish<<<2, 256, 0, s>>>(w.part, lx.nb, Kinds{{0, 0}}, w.sc);
   k_yee_ratio<<<1, 64, 0, s>>>(w.sc);
   F6_CHECK_LAUNCH();
   march(tp.R, q, tp, w.w2, w.w3, w.w0, w.w1, nullptr, nullptr, nullptr, false, s);
   long long chunk = 0;
-  const YeeLaunch lz = yee_march_launch(q, tp.R, chunk);
+  const YeeLaunch lz = yee_march_launch(q, 4 * tp.R, chunk);
   march(tp.R, q, tp, w.w0, w.w1, nullptr, nullptr, w.sc, rho, w.part, true, s);
   k_yee_finish<<<7, 256, 0, s>>>(w.part, lz.nb, Kinds{{0, 0, 0, 0, 0, 1, 2}}, w.sc + 8);
   F6_CHECK_LAUNCH();
@@ -637,10 +830,11 @@ int fame6d_yee_apply_cropped_A(int n, double h, const void* u1, const void* u2, 
     hipStream_t s = (hipStream_t)stream;
     const long long q = 2LL * n + 1;
     const YeeGeom g{q, q + 2, (1.0 / h) * (1.0 / h)};
-    const YeeLaunch l = yee_launch(q, q);
+    long long chunk = 0;
+    const YeeLaunch l = yee_march_launch(q, kYeeMinChunk, chunk);
     k_yee_points<YEE_APPLY_A><<<l.grid, l.block, 0, s>>>(
         g, (const cpx<double>*)u1, (const cpx<double>*)u2, (const cpx<double>*)u3, YeeMat{}, (cpx<double>*)A1,
-        (cpx<double>*)A2, (cpx<double>*)A3, nullptr, nullptr, nullptr, 0.0, nullptr);
+        (cpx<double>*)A2, (cpx<double>*)A3, nullptr, nullptr, nullptr, 0.0, chunk, nullptr);
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
   } catch (const Error& e) {
@@ -716,10 +910,11 @@ int fame6d_yee_lwrq_metrics(int n, double h, const void* u1, const void* u2, con
     const auto* f1 = (const cpx<double>*)u1;
     const auto* f2 = (const cpx<double>*)u2;
     const auto* f3 = (const cpx<double>*)u3;
-    const YeeLaunch l = yee_launch(q, q);
+    long long chunk = 0;
+    const YeeLaunch l = yee_march_launch(q, kYeeMinChunk, chunk);
     // pass 1: a/den, b/den; sums of a, b and of the residual terms about lambda6d
     k_yee_points<YEE_DENSITIES><<<l.grid, l.block, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, w.prof,
-                                                           w.w0, w.w1, lambda6d, w.part);
+                                                           w.w0, w.w1, lambda6d, chunk, w.part);
     F6_CHECK_LAUNCH();
     k_yee_finish<<<1, 256, 0, s>>>(w.part, l.nb, Kinds{{0}}, w.sc);
     k_yee_finish<<<1, 256, 0, s>>>(w.part + l.nb, l.nb, Kinds{{0}}, w.sc + 1);
@@ -740,7 +935,7 @@ int fame6d_yee_lwrq_metrics(int n, double h, const void* u1, const void* u2, con
     const double size = r.S0 + 2.0 * std::fabs(d * r.S1) + d * d * r.S2;
     if (!(r3 >= 1e-6 * size)) {
       k_yee_points<YEE_RESIDUALS><<<l.grid, l.block, 0, s>>>(g, f1, f2, f3, mt, nullptr, nullptr, nullptr, nullptr,
-                                                             nullptr, nullptr, o.lambda_3d, w.part);
+                                                             nullptr, nullptr, o.lambda_3d, chunk, w.part);
       F6_CHECK_LAUNCH();
       k_yee_finish<<<1, 256, 0, s>>>(w.part, l.nb, Kinds{{0}}, w.sc + 16);
       F6_CHECK_LAUNCH();
