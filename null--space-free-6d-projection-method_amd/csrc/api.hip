@@ -150,19 +150,75 @@ static void autotune_t(fame6d_mass* m, int reps) {
       if (best[c] < best[arg]) arg = c;
     tc = cand[arg];
   };
+  auto mhat = [&]() {
+    launch_pass_a<T>(m, PRE_MHAT, 1, pa, m->stream);
+    launch_pass_b<T>(m, 3, 0, 1, t, NF, no_skip(), m->stream);
+    launch_pass_c<T>(m, POST_MHAT, 1, pc, m->stream);
+  };
   std::vector<TileChoice> cand;
   for (int vb = 0; vb < 2; ++vb)
     for (int v3 = 0; v3 < 2; ++v3) {
       TileChoice c = tc;
       c.b = vb;
       c.a3 = v3;
+      c.c3 = v3;
+      c.c3p = 0;
+      c.a3p = 0;
       cand.push_back(c);
     }
-  pick(cand, [&]() {
-    launch_pass_a<T>(m, PRE_MHAT, 1, pa, m->stream);
-    launch_pass_b<T>(m, 3, 0, 1, t, NF, no_skip(), m->stream);
-    launch_pass_c<T>(m, POST_MHAT, 1, pc, m->stream);
-  });
+  pick(cand, mhat);
+  // 1b, 1c) the Mhat passes A and C on their own terms (tile, persistent workgroups per CU),
+  // each timed alone in the conjugate-gradient form the solves launch: pass A forms
+  // p = r + beta p, pass C takes alpha from pass-B partials and updates x and r.  The state
+  // and partials are stand-ins (iter 1, beta 1/2, rho 1, unit partials) so that no branch
+  // is skipped; the vectors are zero.
+  CgState hst{};
+  hst.iter = 1;
+  hst.beta = 0.5;
+  hst.rho = 1.0;
+  DevBuf cgs, pqp, cpart;
+  cgs.ensure(sizeof(CgState));
+  F6_HIP(hipMemcpyAsync(cgs.ptr, &hst, sizeof(CgState), hipMemcpyHostToDevice, m->stream));
+  const int npq = 3 * pass_b_blocks(m, std::is_same<T, float>::value);
+  const std::vector<double> ones(npq, 1.0);
+  pqp.ensure(sizeof(double) * npq);
+  F6_HIP(hipMemcpyAsync(pqp.ptr, ones.data(), sizeof(double) * npq, hipMemcpyHostToDevice, m->stream));
+  cpart.ensure(sizeof(double) * (size_t)m->n * m->n * m->n);  // one partial per S-block at most
+  PassAArgs<T> pa_cg = pa;
+  pa_cg.p = x + NF;
+  pa_cg.st = cgs.as<CgState>();
+  PassCArgs<T> pc_cg = pc;
+  pc_cg.y = nullptr;
+  pc_cg.cg = cgs.as<CgState>();
+  pc_cg.pq_partials = pqp.as<double>();
+  pc_cg.npq = npq;
+  pc_cg.x = x + 2 * NF;
+  pc_cg.p = x + NF;
+  pc_cg.r = x + 3 * NF;
+  pc_cg.ldx = NF;
+  pc_cg.partials = cpart.as<double>();
+  F6_HIP(hipStreamSynchronize(m->stream));
+  cand.clear();
+  for (int v3 = 0; v3 < 2; ++v3)
+    for (int per = 0; per < 4; ++per) {
+      TileChoice c = tc;
+      c.a3 = v3;
+      c.a3p = per;
+      cand.push_back(c);
+    }
+  pick(cand, [&]() { launch_pass_a<T>(m, PRE_MHAT_CG, 1, pa_cg, m->stream); });
+  cand.clear();
+  for (int v3 = 0; v3 < 2; ++v3)
+    for (int per = 0; per < 4; ++per) {
+      TileChoice c = tc;
+      c.c3 = v3;
+      c.c3p = per;
+      cand.push_back(c);
+    }
+  pick(cand, [&]() { launch_pass_c<T>(m, POST_MHAT_CG, 1, pc_cg, m->stream); });
+  cgs.release();
+  pqp.release();
+  cpart.release();
   // 2) plain applies (two right-hand sides, as around each K_r^{-1}): pass A/C tile and I/O form
   cand.clear();
   for (int vio = 0; vio < 2; ++vio)
@@ -256,6 +312,8 @@ int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass*
     m->NF = NF;
     m->stream = static_cast<hipStream_t>(stream);
     F6_HIP(hipGetDevice(&m->device));
+    F6_HIP(hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, m->device));
+    if (m->cus <= 0) throw Error(FAME6D_EHIP, "device reports no compute units");
     m->pin.init(1 << 16);
     upload_eps(m, eps_host);
   } catch (...) {
@@ -309,7 +367,10 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
   tile_variant_desc(m->n, false, true, m->tile64.b, &info->tile_b64, &th);
   tile_variant_desc(m->n, true, false, m->tile32.a, &info->tile_a32, &th);
   tile_variant_desc(m->n, true, true, m->tile32.b, &info->tile_b32, &th);
-  auto code = [](const TileChoice& t) { return (t.a & 1) | ((t.io == 1 ? 0 : 1) << 1) | ((t.a3 & 1) << 2); };
+  auto code = [](const TileChoice& t) {
+    return (t.a & 1) | ((t.io == 1 ? 0 : 1) << 1) | ((t.a3 & 1) << 2) | ((t.c3p & 3) << 3) | ((t.c3 & 1) << 5) |
+           ((t.a3p & 3) << 6);
+  };
   info->variant_ac64 = code(m->tile64);
   info->variant_b64 = m->tile64.b;
   info->variant_ac32 = code(m->tile32);
@@ -320,11 +381,14 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   F6_API_BEGIN
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
-  F6_REQUIRE(va >= 0 && va < 8 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
+  F6_REQUIRE(va >= 0 && va < 256 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
   TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
   t.a = va & 1;
   t.io = ((va >> 1) & 1) ? 0 : 1;
   t.a3 = (va >> 2) & 1;
+  t.c3p = (va >> 3) & 3;
+  t.c3 = (va >> 5) & 1;
+  t.a3p = (va >> 6) & 3;
   t.b = vb;
   F6_API_END
 }

@@ -125,7 +125,10 @@ struct TileChoice {
   int a = 0;  // pass A / pass C variant
   int b = 0;  // pass B variant
   int io = 1;  // passes A/C: 1 = coalesced dim-3-first I/O, 0 = dim-1 register pencils
-  int a3 = 0;  // Mhat passes A/C (three thread groups) variant
+  int a3 = 0;  // Mhat pass A (three thread groups) variant
+  int c3 = 0;   // Mhat pass C (three thread groups) variant
+  int c3p = 0;  // Mhat pass C: 0 = one workgroup per tile, k > 0 = persistent, k workgroups per CU
+  int a3p = 0;  // Mhat pass A: the same
 };
 
 }  // namespace fame6d
@@ -136,6 +139,7 @@ struct fame6d_mass {
   long long NF = 0;
   hipStream_t stream = nullptr;
   int device = 0;
+  int cus = 0;  // compute units of the device
   double eps_min = 0, eps_max = 0, eps_mean = 0;
   fame6d::DevBuf eps64, eps32;  // eps / NF
   fame6d::DevBuf ieps64, ieps32;  // (1/eps) / NF: the inverse mass M_s^{-1} = F diag(1/eps) F^{-1}

@@ -55,17 +55,30 @@ static const TileChoice& tile_of(const fame6d_mass* m) {
   else return m->tile32;
 }
 
+// Workgroups per group of the persistent Mhat passes A and C: per_cu per compute unit, at
+// most one per tile.
+static int pass_c3p_blocks(const fame6d_mass* m, int tiles, int per_cu) {
+  return std::max(1, std::min(tiles, per_cu * m->cus));
+}
+
 template <class T>
 void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
   const int v = tile_of<T>(m).a;
   if (op == PRE_MHAT || op == PRE_MHAT_CG) {
     const int v3 = tile_of<T>(m).a3;
+    const int a3p = tile_of<T>(m).a3p;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
       constexpr int S = n * n * n;
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
+        if (a3p > 0) {
+          dim3 grid(pass_c3p_blocks(m, S / BPW, a3p), ngroups), block(3 * BPW * n * n);
+          if (op == PRE_MHAT) launch_k(m, k_pass_a3p<T, n, BPW, PRE_MHAT>, grid, block, s, a);
+          else launch_k(m, k_pass_a3p<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a);
+          return;
+        }
         dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
         if (op == PRE_MHAT) launch_k(m, k_pass_a3<T, n, BPW, PRE_MHAT>, grid, block, s, a);
         else launch_k(m, k_pass_a3<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a);
@@ -133,12 +146,19 @@ template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
   if (op == POST_MHAT || op == POST_MHAT_CG) {
-    const int v3 = tile_of<T>(m).a3;
+    const int v3 = tile_of<T>(m).c3;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
       constexpr int S = n * n * n;
+      const int c3p = tile_of<T>(m).c3p;
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
+        if (c3p > 0) {
+          dim3 grid(pass_c3p_blocks(m, S / BPW, c3p), ngroups), block(3 * BPW * n * n);
+          if (op == POST_MHAT_CG) launch_k(m, k_pass_c3p<T, n, BPW, true>, grid, block, s, a);
+          else launch_k(m, k_pass_c3p<T, n, BPW, false>, grid, block, s, a);
+          return;
+        }
         dim3 grid(S / BPW, ngroups), block(3 * BPW * n * n);
         if (op == POST_MHAT_CG) launch_k(m, k_pass_c3<T, n, BPW, true>, grid, block, s, a);
         else launch_k(m, k_pass_c3<T, n, BPW, false>, grid, block, s, a);
@@ -172,12 +192,12 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
 int pass_c_blocks(const fame6d_mass* m, bool fp32) {
   // workgroups per group of the Mhat pass C (the only pass C that writes partials)
   int out = 0;
-  const int v = fp32 ? m->tile32.a3 : m->tile64.a3;
+  const TileChoice& tc = fp32 ? m->tile32 : m->tile64;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
-    const int bpw = fp32 ? Tiles<n, float>::bpw3[v] : Tiles<n, double>::bpw3[v];
-    out = S / bpw;
+    const int bpw = fp32 ? Tiles<n, float>::bpw3[tc.c3] : Tiles<n, double>::bpw3[tc.c3];
+    out = tc.c3p > 0 ? pass_c3p_blocks(m, S / bpw, tc.c3p) : S / bpw;
   });
   return out;
 }

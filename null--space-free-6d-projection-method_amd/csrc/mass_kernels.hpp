@@ -657,6 +657,141 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
   for (int i = 0; i < n; ++i) tg[tl + TG * i] = R[tl + TG * i];
 }
 
+// Mhat pass A as a persistent, software-pipelined kernel (the counterpart of k_pass_c3p):
+// the workgroup walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ... of its group; the
+// input pencils (x, and p for the CG form) of the next tile are loaded into registers once
+// the current tile's first DFT has gone to LDS.  Same arithmetic per mode as k_pass_a3.
+template <class T, int n, int BPW, int OP>
+__global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int TG = BPW * P2;  // threads per group
+  constexpr int E = BPW * S;    // modes per tile
+  constexpr int NT = S / BPW;   // tiles per group
+  constexpr bool CG = OP == PRE_MHAT_CG;
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  __shared__ cpx<T> lds[3 * E];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int o = tid / TG, tl = tid - o * TG;
+  const int blk = tl / P2, pp = tl - blk * P2;
+  cpx<T>* R = lds + o * E;
+  cpx<T>* L = R + blk * S;
+  const long long boff = (long long)blk * S + pp;
+  const cpx<T>* xsrc = a.x + g * a.ldx + boff;
+  cpx<T>* psrc = CG ? a.p + g * a.ldx + boff : nullptr;
+  bool use_beta = false;
+  T beta = T(0);
+  if constexpr (CG) {
+    use_beta = a.st[g].iter > 0;
+    beta = static_cast<T>(a.st[g].beta);
+  }
+  const long long qg = qslot(a.qmap, g) * a.ldq + boff;
+  const T* wo = (o == 1) ? a.W1 : a.W2;
+  v2 px[n], pq[n];
+  auto fetch = [&](int tile) {
+    const long long to = (long long)tile * E;
+#pragma unroll
+    for (int k = 0; k < n; ++k) px[k] = *reinterpret_cast<const v2*>(xsrc + to + P2 * k);
+    if (CG && use_beta) {
+#pragma unroll
+      for (int k = 0; k < n; ++k) pq[k] = *reinterpret_cast<const v2*>(psrc + to + P2 * k);
+    }
+  };
+  int tile = blockIdx.x;
+  fetch(tile);  // the launch has gridDim.x <= NT, so every workgroup owns a tile
+  for (; tile < NT; tile += gridDim.x) {
+    const long long to = (long long)tile * E;
+    T w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      w[k] = a.invC[qg + to + P2 * k];
+      if (o > 0) w[k] *= wo[qg + to + P2 * k];
+    }
+    cpx<T> u[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) u[k] = mk(px[k].x, px[k].y);
+    if constexpr (CG) {
+      if (use_beta) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) u[k] = u[k] + beta * mk(pq[k].x, pq[k].y);
+      }
+      __syncthreads();  // every group holds its copy of the old p before group 0 overwrites it
+      if (o == 0) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) psrc[to + P2 * k] = u[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k) u[k] = w[k] * u[k];
+    dft<T, n, +1>(u);
+#pragma unroll
+    for (int k = 0; k < n; ++k) L[pp + P2 * k] = u[k];
+    if (tile + (int)gridDim.x < NT) fetch(tile + gridDim.x);
+    __syncthreads();
+    {
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> v[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) v[k] = L[base2 + n * k];
+      dft<T, n, +1>(v);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = v[k];
+    }
+    __syncthreads();
+    {
+      const int base1 = (pp % n) * n + (pp / n) * P2;
+      cpx<T> v[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) v[k] = L[base1 + k];
+      dft<T, n, +1>(v);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base1 + k] = v[k];
+    }
+    __syncthreads();
+    cpx<T>* tg = a.t + (g * 3 + o) * a.ldt + to;
+#pragma unroll
+    for (int i = 0; i < n; ++i) tg[tl + TG * i] = R[tl + TG * i];
+    __syncthreads();  // the next tile's first DFT rewrites LDS
+  }
+}
+
+// Epilogue of the Mhat pass C over the E modes of one tile, already transformed in LDS
+// (component o at lds + o*E): q = invC*(s0 + W1 s1 + W2 s2) in linear mode order.  CG: x +=
+// al*p, r -= al*q, s += |r_new|^2; otherwise y = q and s += Re(d^H q) when d is given.
+// qg: the group's offset in the diagonals (qslot * ldq), read once per workgroup: a load here
+// would make the tile wait for every memory operation issued before it.  (Issuing all of a
+// thread's loads before its stores measured slower than this loop.)
+template <class T, int E, int THREADS, bool CG>
+__device__ __forceinline__ void mhat_c_epilogue(const PassCArgs<T>& a, const cpx<T>* lds, int g, long long qg,
+                                                long long wg0, T al, double& s) {
+  const int tid = threadIdx.x;
+  const long long qbase = qg + wg0;
+  if constexpr (CG) {
+    const long long xo = g * a.ldx + wg0;
+    for (int e = tid; e < E; e += THREADS) {
+      const long long q = qbase + e;
+      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      a.x[xo + e] = a.x[xo + e] + al * a.p[xo + e];
+      const cpx<T> rn = a.r[xo + e] - al * v;
+      a.r[xo + e] = rn;
+      s += static_cast<double>(rn.re) * rn.re + static_cast<double>(rn.im) * rn.im;
+    }
+  } else {
+    cpx<T>* yg = a.y + g * a.ldy + wg0;
+    const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
+    for (int e = tid; e < E; e += THREADS) {
+      const long long q = qbase + e;
+      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      yg[e] = v;
+      if (dg) {
+        const cpx<T> d = dg[e];
+        s += static_cast<double>(d.re * v.re + d.im * v.im);
+      }
+    }
+  }
+}
+
 // CG = true fuses the conjugate-gradient step of MATLAB pcg into the epilogue: every
 // workgroup reduces the pass-B partials of p'q in the same fixed order, so all agree on
 // alpha = rho/p'q (and on a breakdown), workgroup 0 records it, and each workgroup updates
@@ -755,6 +890,108 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
         s += static_cast<double>(d.re * v.re + d.im * v.im);
       }
     }
+  }
+  if (a.partials != nullptr) {
+    s = block_sum<THREADS>(s, red);
+    if (tid == 0) a.partials[(long long)g * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// Mhat pass C as a persistent, software-pipelined kernel: the workgroup walks the tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... of its group, and the dim-3 pencils of the next
+// tile are loaded into registers as soon as the current tile's first DFT has gone to LDS,
+// so they are in flight through the current tile's LDS phases and epilogue.  With one
+// workgroup per tile the load, LDS and store phases of a workgroup run one after the other
+// and only other workgroups on the CU can fill the gaps.  Results are those of k_pass_c3
+// (same arithmetic per mode); partials are per workgroup (gridDim.x per group).
+template <class T, int n, int BPW, bool CG = false>
+__global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3p(PassCArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int TG = BPW * P2;
+  constexpr int THREADS = 3 * TG;
+  constexpr int E = BPW * S;   // modes per tile
+  constexpr int NT = S / BPW;  // tiles per group
+  // pencils in flight are clang vectors: a local array of structs copied from memory would be
+  // kept in scratch across the loop
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  __shared__ cpx<T> lds[3 * E];
+  __shared__ double red[THREADS];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int o = tid / TG, tl = tid - o * TG;
+  const int blk = tl / P2, pp = tl - blk * P2;
+  cpx<T>* L = lds + o * E + blk * S;
+  const cpx<T>* tsrc = a.t + (g * 3 + o) * a.ldt + (long long)blk * S + pp;
+  v2 pf[n];
+  auto fetch = [&](int tile) {
+    const cpx<T>* src = tsrc + (long long)tile * E;
+#pragma unroll
+    for (int k = 0; k < n; ++k) pf[k] = *reinterpret_cast<const v2*>(src + P2 * k);
+  };
+  int tile = blockIdx.x;
+  fetch(tile);  // the launch has gridDim.x <= NT, so every workgroup owns a tile
+  T al = T(0);
+  if constexpr (CG) {
+    double pq = 0.0;
+    {
+      const double* pqp = a.pq_partials + (long long)g * a.npq;
+      for (int i = tid; i < a.npq; i += THREADS) pq += pqp[i];
+      pq = block_sum<THREADS>(pq, red);
+      if (tid == 0) red[0] = pq;
+      __syncthreads();
+      pq = red[0];
+      __syncthreads();  // red is reused below
+    }
+    const double alpha = a.cg[g].rho / pq;
+    const bool ok = (pq > 0.0) && !isinf(pq) && !isinf(alpha) && !isnan(alpha);
+    if (blockIdx.x == 0 && tid == 0) {
+      a.cg[g].pq = pq;
+      if (ok) {
+        a.cg[g].alpha = alpha;
+      } else {
+        a.cg[g].flag = 4;
+        a.cg[g].done = 1;
+      }
+    }
+    if (!ok) return;
+    al = static_cast<T>(alpha);
+  }
+  const long long qg = qslot(a.qmap, g) * a.ldq;
+  double s = 0.0;
+  for (; tile < NT; tile += gridDim.x) {
+    {
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = mk(pf[k].x, pf[k].y);
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[pp + P2 * k] = w[k];
+    }
+    if (tile + (int)gridDim.x < NT) fetch(tile + gridDim.x);
+    __syncthreads();
+    {
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base2 + n * k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = w[k];
+    }
+    __syncthreads();
+    {
+      const int base1 = (pp % n) * n + (pp / n) * P2;
+      cpx<T> w[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) w[k] = L[base1 + k];
+      dft<T, n, -1>(w);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base1 + k] = w[k];
+    }
+    __syncthreads();
+    mhat_c_epilogue<T, E, THREADS, CG>(a, lds, g, qg, (long long)tile * E, al, s);
+    __syncthreads();  // the next tile's first DFT rewrites LDS
   }
   if (a.partials != nullptr) {
     s = block_sum<THREADS>(s, red);

@@ -41,14 +41,79 @@ def test_mhat(n2, oracle, v3):
     rng = np.random.default_rng(1)
     x = rand_c(rng, n2["m"].NF)
     ref = oracle.mhat_apply(n2["osys"], n2["eps"], x, n2["N"])
-    n2["m"].set_tiles("double", 4 * v3, 0)
-    n2["m"].set_tiles("single", 4 * v3, 1)
+    n2["m"].set_tiles("double", 36 * v3, 0)
+    n2["m"].set_tiles("single", 36 * v3, 1)
     y = n2["s"].Mhat(torch.as_tensor(x, device="cuda")).cpu().numpy()
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
     y32 = n2["s"].Mhat(torch.as_tensor(x, device="cuda").to(torch.complex64), "single").cpu().numpy()
     assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 5e-6
     n2["m"].set_tiles("double", 0, 0)
     n2["m"].set_tiles("single", 0, 0)
+
+
+@pytest.fixture(scope="module")
+def n5(pkg, oracle):
+    torch.cuda.set_device(0)
+    N = 5
+    eps = oracle.eps_reference(N)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, Q_EXP1)
+    yield dict(N=N, eps=eps, m=m, s=s)
+    s.close()
+    m.close()
+
+
+# Mhat pass A and C variants at N=5, where a persistent workgroup walks several tiles:
+# codes 36*v3 + 8*c3p + 64*a3p (v3: blocks per workgroup of passes A and C; c3p, a3p: 0 one
+# workgroup per tile, k > 0 persistent with k workgroups per compute unit).
+C3P_CODES = (0, 36, 8, 44, 16 + 64, 52 + 64, 24 + 128, 60 + 128, 192, 36 + 192)
+
+
+def test_mhat_pass_c_variants_n5(n5, oracle):
+    print("case: Mhat N=5 pass-C variants", flush=True)
+    rng = np.random.default_rng(5)
+    x = rand_c(rng, n5["m"].NF)
+    ref = oracle.mhat_apply(oracle.qsystem(5, Q_EXP1), n5["eps"], x, 5)
+    xd = torch.as_tensor(x, device="cuda")
+    try:
+        for code in C3P_CODES:
+            n5["m"].set_tiles("double", code, 0)
+            n5["m"].set_tiles("single", code, 0)
+            y = n5["s"].Mhat(xd).cpu().numpy()
+            assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13, code
+            y32 = n5["s"].Mhat(xd.to(torch.complex64), "single").cpu().numpy()
+            assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 5e-6, code
+    finally:
+        n5["m"].set_tiles("double", 0, 0)
+        n5["m"].set_tiles("single", 0, 0)
+
+
+def test_krinv_pass_c_variants_n5(n5):
+    """The CG form of the persistent pass C (alpha from the pass-B partials, x and r updated
+    over several tiles per workgroup, ||r||^2 partials per workgroup) against the
+    one-workgroup-per-tile kernel, FP64 and mixed."""
+    print("case: krinv N=5 pass-C variants", flush=True)
+    rng = np.random.default_rng(6)
+    q = torch.as_tensor(rand_c(rng, 2 * n5["m"].NF), device="cuda")
+    try:
+        n5["m"].set_tiles("double", 36, 0)
+        n5["m"].set_tiles("single", 36, 0)
+        y0, st0 = n5["s"].krinv(q, 1e-10, 1200)
+        # the mixed solve meets the same residual tolerance; its solution differs from the
+        # FP64 one by about cond * tol
+        ym0, stm0 = n5["s"].krinv(q, 1e-10, 1200, "mixed")
+        assert (torch.linalg.norm(ym0 - y0) / torch.linalg.norm(y0)).item() < 1e-8
+        for code in C3P_CODES[2:]:
+            n5["m"].set_tiles("double", code, 0)
+            n5["m"].set_tiles("single", code, 0)
+            y, st = n5["s"].krinv(q, 1e-10, 1200)
+            assert st["last_flag"] == 0 and abs(st["last_iterations"] - st0["last_iterations"]) <= 1, code
+            assert (torch.linalg.norm(y - y0) / torch.linalg.norm(y0)).item() < 1e-9, code
+            ym, stm = n5["s"].krinv(q, 1e-10, 1200, "mixed")
+            assert (torch.linalg.norm(ym - y0) / torch.linalg.norm(y0)).item() < 1e-8, code
+    finally:
+        n5["m"].set_tiles("double", 0, 0)
+        n5["m"].set_tiles("single", 0, 0)
 
 
 def test_krinv_fp64_and_mixed(n2, oracle):
