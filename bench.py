@@ -331,7 +331,7 @@ def method_comparison(torch, pkg, N):
 
 # kernel family (library timer slot) -> kernel-name pattern in rocprofv3 output (%d: nGrid)
 FAMILY_KERNEL = {
-    "mhat_pass_a": r"k_pass_a3<double, %d, \d+, 2>",
+    "mhat_pass_a": r"k_pass_a3[ps]?<double, %d, \d+, 2>",
     "mhat_pass_b": r"k_pass_b<double, %d, \d+, 3, 0>",
     "mhat_pass_c": r"k_pass_c3p?<double, %d, \d+, true>",
     "lanczos_gemv_c": r"k_gemv_c_all<\d+>",

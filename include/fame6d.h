@@ -83,13 +83,13 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
  * passes at this N and precision and keeps the fastest (no reference counterpart;
  * the reference has one implicit cuFFT plan). */
 int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
-/* Set tile variants explicitly: variant_ac in 0..255 (bit 0: blocks per workgroup of the
+/* Set tile variants explicitly: variant_ac in 0..511 (bit 0: blocks per workgroup of the
  * plain passes A/C, bit 1 set: their dim-1 register-pencil I/O instead of the coalesced
  * dim-3-first I/O, bit 2: blocks per workgroup of the three-group Mhat pass A, bits 3-4:
  * k > 0 runs the Mhat pass C persistent with k workgroups per compute unit, 0 one
  * workgroup per tile, bit 5: blocks per workgroup of the Mhat pass C, bits 6-7: the same
- * as bits 3-4 for the Mhat pass A), variant_b in 0..1 (inner positions per workgroup of
- * pass B). */
+ * as bits 3-4 for the Mhat pass A, bit 8: the persistent Mhat pass A reads its input once
+ * and shares it through LDS), variant_b in 0..1 (inner positions per workgroup of pass B). */
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
 
 /* Kernel-family timing (no reference counterpart; the reference times whole operator

@@ -164,6 +164,7 @@ static void autotune_t(fame6d_mass* m, int reps) {
       c.c3 = v3;
       c.c3p = 0;
       c.a3p = 0;
+      c.a3s = 0;
       cand.push_back(c);
     }
   pick(cand, mhat);
@@ -200,12 +201,14 @@ static void autotune_t(fame6d_mass* m, int reps) {
   F6_HIP(hipStreamSynchronize(m->stream));
   cand.clear();
   for (int v3 = 0; v3 < 2; ++v3)
-    for (int per = 0; per < 4; ++per) {
-      TileChoice c = tc;
-      c.a3 = v3;
-      c.a3p = per;
-      cand.push_back(c);
-    }
+    for (int per = 0; per < 4; ++per)
+      for (int sh = 0; sh < (per > 0 ? 2 : 1); ++sh) {
+        TileChoice c = tc;
+        c.a3 = v3;
+        c.a3p = per;
+        c.a3s = sh;
+        cand.push_back(c);
+      }
   pick(cand, [&]() { launch_pass_a<T>(m, PRE_MHAT_CG, 1, pa_cg, m->stream); });
   cand.clear();
   for (int v3 = 0; v3 < 2; ++v3)
@@ -369,7 +372,7 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
   tile_variant_desc(m->n, true, true, m->tile32.b, &info->tile_b32, &th);
   auto code = [](const TileChoice& t) {
     return (t.a & 1) | ((t.io == 1 ? 0 : 1) << 1) | ((t.a3 & 1) << 2) | ((t.c3p & 3) << 3) | ((t.c3 & 1) << 5) |
-           ((t.a3p & 3) << 6);
+           ((t.a3p & 3) << 6) | ((t.a3s & 1) << 8);
   };
   info->variant_ac64 = code(m->tile64);
   info->variant_b64 = m->tile64.b;
@@ -381,7 +384,7 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   F6_API_BEGIN
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
-  F6_REQUIRE(va >= 0 && va < 256 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
+  F6_REQUIRE(va >= 0 && va < 512 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
   TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
   t.a = va & 1;
   t.io = ((va >> 1) & 1) ? 0 : 1;
@@ -389,6 +392,7 @@ int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   t.c3p = (va >> 3) & 3;
   t.c3 = (va >> 5) & 1;
   t.a3p = (va >> 6) & 3;
+  t.a3s = (va >> 8) & 1;
   t.b = vb;
   F6_API_END
 }

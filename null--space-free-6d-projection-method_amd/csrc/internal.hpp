@@ -129,6 +129,7 @@ struct TileChoice {
   int c3 = 0;   // Mhat pass C (three thread groups) variant
   int c3p = 0;  // Mhat pass C: 0 = one workgroup per tile, k > 0 = persistent, k workgroups per CU
   int a3p = 0;  // Mhat pass A: the same
+  int a3s = 0;  // persistent Mhat pass A: 1 = input read once and shared through LDS
 };
 
 }  // namespace fame6d

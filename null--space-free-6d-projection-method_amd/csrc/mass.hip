@@ -67,7 +67,7 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
   const int v = tile_of<T>(m).a;
   if (op == PRE_MHAT || op == PRE_MHAT_CG) {
     const int v3 = tile_of<T>(m).a3;
-    const int a3p = tile_of<T>(m).a3p;
+    const int a3p = tile_of<T>(m).a3p, a3s = tile_of<T>(m).a3s;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
       constexpr int S = n * n * n;
@@ -75,7 +75,9 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
         constexpr int BPW = decltype(bc)::value;
         if (a3p > 0) {
           dim3 grid(pass_c3p_blocks(m, S / BPW, a3p), ngroups), block(3 * BPW * n * n);
-          if (op == PRE_MHAT) launch_k(m, k_pass_a3p<T, n, BPW, PRE_MHAT>, grid, block, s, a);
+          if (a3s && op == PRE_MHAT) launch_k(m, k_pass_a3s<T, n, BPW, PRE_MHAT>, grid, block, s, a);
+          else if (a3s) launch_k(m, k_pass_a3s<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a);
+          else if (op == PRE_MHAT) launch_k(m, k_pass_a3p<T, n, BPW, PRE_MHAT>, grid, block, s, a);
           else launch_k(m, k_pass_a3p<T, n, BPW, PRE_MHAT_CG>, grid, block, s, a);
           return;
         }
@@ -279,7 +281,9 @@ void h2d_async(fame6d_mass* m, void* dev, const void* host, size_t bytes) {
 void KernelTimer::enable(int e) {
   if (e > 0 && ev.empty()) {
     ev.resize(2 * kPool);
-    for (auto& x : ev) F6_HIP(hipEventCreate(&x));
+    // timing-only events: without the system-scope fence a bracketed kernel does not pay for
+    // an L2 writeback at its end that unbracketed launches do not have
+    for (auto& x : ev) F6_HIP(hipEventCreateWithFlags(&x, hipEventDisableSystemFence));
     slot.resize(kPool);
     bytes_pending.resize(kPool);
     tag.resize(kPool);

@@ -756,6 +756,114 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
   }
 }
 
+// Mhat pass A, persistent, with the pre-op shared: the tile's input (x, and p for the CG
+// form) is read once, in linear order by all threads (the three thread groups of k_pass_a3
+// each read every pencil), combined (p = r + beta p, stored), and staged in a fourth LDS
+// region from which every group reads its dim-3 pencils.  The next tile's input is loaded
+// into registers while the current tile goes through its DFTs.  Same arithmetic per mode as
+// k_pass_a3.
+template <class T, int n, int BPW, int OP>
+__global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3s(PassAArgs<T> a) {
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int TG = BPW * P2;  // threads per group
+  constexpr int THREADS = 3 * TG;
+  constexpr int E = BPW * S;    // modes per tile
+  constexpr int NT = S / BPW;   // tiles per group
+  constexpr int NE = (E + THREADS - 1) / THREADS;  // linear input modes per thread
+  constexpr bool CG = OP == PRE_MHAT_CG;
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  __shared__ cpx<T> lds[4 * E];
+  const int g = blockIdx.y;
+  if (skipped(a.skip, g)) return;
+  const int tid = threadIdx.x;
+  const int o = tid / TG, tl = tid - o * TG;
+  const int blk = tl / P2, pp = tl - blk * P2;
+  cpx<T>* R = lds + o * E;
+  cpx<T>* L = R + blk * S;
+  cpx<T>* X = lds + 3 * E;  // the shared pre-op result of the tile
+  const cpx<T>* xsrc = a.x + g * a.ldx;
+  cpx<T>* pdst = CG ? a.p + g * a.ldx : nullptr;
+  bool use_beta = false;
+  T beta = T(0);
+  if constexpr (CG) {
+    use_beta = a.st[g].iter > 0;
+    beta = static_cast<T>(a.st[g].beta);
+  }
+  const long long qg = qslot(a.qmap, g) * a.ldq + (long long)blk * S + pp;
+  const T* wo = (o == 1) ? a.W1 : a.W2;
+  v2 px[NE], pq[NE];
+  auto fetch = [&](int tile) {
+    const long long to = (long long)tile * E;
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+      const int e = tid + THREADS * r;
+      if (e < E) {
+        px[r] = *reinterpret_cast<const v2*>(xsrc + to + e);
+        if (CG && use_beta) pq[r] = *reinterpret_cast<const v2*>(pdst + to + e);
+      }
+    }
+  };
+  int tile = blockIdx.x;
+  fetch(tile);  // the launch has gridDim.x <= NT, so every workgroup owns a tile
+  for (; tile < NT; tile += gridDim.x) {
+    const long long to = (long long)tile * E;
+    T w[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      w[k] = a.invC[qg + to + P2 * k];
+      if (o > 0) w[k] *= wo[qg + to + P2 * k];
+    }
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+      const int e = tid + THREADS * r;
+      if (e < E) {
+        cpx<T> u = mk(px[r].x, px[r].y);
+        if constexpr (CG) {
+          if (use_beta) u = u + beta * mk(pq[r].x, pq[r].y);
+          pdst[to + e] = u;
+        }
+        X[e] = u;
+      }
+    }
+    if (tile + (int)gridDim.x < NT) fetch(tile + gridDim.x);
+    __syncthreads();
+    {
+      const cpx<T>* Xb = X + blk * S;
+      cpx<T> u[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) u[k] = w[k] * Xb[pp + P2 * k];
+      dft<T, n, +1>(u);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[pp + P2 * k] = u[k];
+    }
+    __syncthreads();
+    {
+      const int base2 = (pp % n) + (pp / n) * P2;
+      cpx<T> v[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) v[k] = L[base2 + n * k];
+      dft<T, n, +1>(v);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base2 + n * k] = v[k];
+    }
+    __syncthreads();
+    {
+      const int base1 = (pp % n) * n + (pp / n) * P2;
+      cpx<T> v[n];
+#pragma unroll
+      for (int k = 0; k < n; ++k) v[k] = L[base1 + k];
+      dft<T, n, +1>(v);
+#pragma unroll
+      for (int k = 0; k < n; ++k) L[base1 + k] = v[k];
+    }
+    __syncthreads();
+    cpx<T>* tg = a.t + (g * 3 + o) * a.ldt + to;
+#pragma unroll
+    for (int i = 0; i < n; ++i) tg[tl + TG * i] = R[tl + TG * i];
+    // the next tile's pre-op rewrites X only: X was last read before the barrier above
+  }
+}
+
 // Epilogue of the Mhat pass C over the E modes of one tile, already transformed in LDS
 // (component o at lds + o*E): q = invC*(s0 + W1 s1 + W2 s2) in linear mode order.  CG: x +=
 // al*p, r -= al*q, s += |r_new|^2; otherwise y = q and s += Re(d^H q) when d is given.

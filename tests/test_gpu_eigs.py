@@ -64,9 +64,10 @@ def n5(pkg, oracle):
 
 
 # Mhat pass A and C variants at N=5, where a persistent workgroup walks several tiles:
-# codes 36*v3 + 8*c3p + 64*a3p (v3: blocks per workgroup of passes A and C; c3p, a3p: 0 one
-# workgroup per tile, k > 0 persistent with k workgroups per compute unit).
-C3P_CODES = (0, 36, 8, 44, 16 + 64, 52 + 64, 24 + 128, 60 + 128, 192, 36 + 192)
+# codes 36*v3 + 8*c3p + 64*a3p + 256*a3s (v3: blocks per workgroup of passes A and C; c3p,
+# a3p: 0 one workgroup per tile, k > 0 persistent with k workgroups per compute unit; a3s:
+# the persistent pass A shares its input through LDS).
+C3P_CODES = (0, 36, 8, 44, 16 + 64, 52 + 64, 24 + 128, 60 + 128, 192, 36 + 192, 256 + 64, 256 + 36 + 128)
 
 
 def test_mhat_pass_c_variants_n5(n5, oracle):
