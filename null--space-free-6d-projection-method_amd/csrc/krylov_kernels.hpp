@@ -640,8 +640,11 @@ __global__ __launch_bounds__(256) void k_basis_gemm(const double* __restrict__ Q
       }
     }
   }
-  if (row < nr)
-    for (int c = 0; c < kc; ++c) out[(long long)(c0 + c) * ldo + row] = acc[c];
+  if (row < nr) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c)  // compile-time indices keep acc[] in registers
+      if (c < kc) out[(long long)(c0 + c) * ldo + row] = acc[c];
+  }
 }
 
 // out = in - Q(:, 0:j) h (h complex, on device); optional ||out||^2 partials per workgroup.

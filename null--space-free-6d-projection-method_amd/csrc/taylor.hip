@@ -32,12 +32,16 @@ __host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * 
 // in AGPRs: each A fragment feeds CT matrix-core ops and each B fragment TPW of them.  The
 // power-row offsets of a lane's alphas are fixed for the whole launch and kept in registers;
 // alpha rows beyond L read an all-zero power row instead of branching.  Up to 20 accumulators
-// two workgroups share a CU (registers capped at 256 per lane, MINB = 2), so one's operand
+// (and TPW < 10) two workgroups share a CU (registers capped at 256 per lane, MINB = 2), so one's operand
 // generation (sincos, powers) overlaps the other's matrix-core work; more accumulators do not
 // fit that cap without spilling and run one workgroup per CU.
 constexpr int kTChunk = 64;
 constexpr int kTZeroRow = kTaylorMaxP + 1;  // row of zeros in the power tables
 typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Workgroups per CU an instance is compiled for: two when its accumulators leave room in
+// 256 registers per lane (TPW = 10, CT = 2 would spill at two).
+constexpr int taylor_minb(int tpw, int ct) { return (tpw * ct <= 20 && tpw < 10) ? 2 : 1; }
 
 template <int TPW, int CT, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF, const double* qx, const double* qy,
@@ -297,7 +301,7 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
           prop.multiProcessorCount > 0)
         cus = prop.multiProcessorCount;
     }
-    const int minb = tpw * ct <= 20 ? 2 : 1;  // workgroups per CU of the instance below
+    const int minb = taylor_minb(tpw, ct);  // workgroups per CU of the instance below
     const int resident = cus * minb;
     const long long chunks = (NF + kTChunk - 1) / kTChunk;
     long long target = std::max<long long>(1, (4LL * resident) / cgroups);
@@ -321,7 +325,7 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
     switch (tpw) {
 #define F6_TAYLOR_CASE(TP, CTV)                                                                                   \
   case TP:                                                                                                       \
-    k_taylor_moments_mfma<TP, CTV, (TP * CTV <= 20 ? 2 : 1)><<<grid, 256, 0, s>>>(                              \
+    k_taylor_moments_mfma<TP, CTV, taylor_minb(TP, CTV)><<<grid, 256, 0, s>>>(                                  \
         NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C,                                                   \
                                                         cen, alp, span, Cpad2, part.as<double>());               \
     break;
