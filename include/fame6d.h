@@ -89,7 +89,8 @@ int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
  * k > 0 runs the Mhat pass C persistent with k workgroups per compute unit, 0 one
  * workgroup per tile, bit 5: blocks per workgroup of the Mhat pass C, bits 6-7: the same
  * as bits 3-4 for the Mhat pass A, bit 8: the persistent Mhat pass A reads its input once
- * and shares it through LDS), variant_b in 0..1 (inner positions per workgroup of pass B). */
+ * and shares it through LDS), variant_b in 0..3 (bit 0: inner positions per workgroup of
+ * pass B in the M-hat applies, bit 1: the same in the plain applies). */
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int variant_ac, int variant_b);
 
 /* Kernel-family timing (no reference counterpart; the reference times whole operator

@@ -222,16 +222,22 @@ static void autotune_t(fame6d_mass* m, int reps) {
   cgs.release();
   pqp.release();
   cpart.release();
-  // 2) plain applies (two right-hand sides, as around each K_r^{-1}): pass A/C tile and I/O form
+  // 2) plain applies: pass A/C tile and I/O form and the one-vector pass-B tile, timed on one
+  // and on two right-hand sides (the API's single applies and the pairs around each K_r^{-1})
   cand.clear();
   for (int vio = 0; vio < 2; ++vio)
-    for (int va = 0; va < 2; ++va) {
-      TileChoice c = tc;
-      c.io = vio;
-      c.a = va;
-      cand.push_back(c);
-    }
-  pick(cand, [&]() { mass_apply<T>(m, x, NF, x + 2 * NF, NF, 2, m->stream); });
+    for (int va = 0; va < 2; ++va)
+      for (int vb1 = 0; vb1 < 2; ++vb1) {
+        TileChoice c = tc;
+        c.io = vio;
+        c.a = va;
+        c.b1 = vb1;
+        cand.push_back(c);
+      }
+  pick(cand, [&]() {
+    mass_apply<T>(m, x, NF, x + 2 * NF, NF, 2, m->stream);
+    mass_apply<T>(m, x, NF, x + 2 * NF, NF, 1, m->stream);
+  });
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   vec.release();
@@ -375,16 +381,16 @@ int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info) {
            ((t.a3p & 3) << 6) | ((t.a3s & 1) << 8);
   };
   info->variant_ac64 = code(m->tile64);
-  info->variant_b64 = m->tile64.b;
+  info->variant_b64 = m->tile64.b | (m->tile64.b1 << 1);
   info->variant_ac32 = code(m->tile32);
-  info->variant_b32 = m->tile32.b;
+  info->variant_b32 = m->tile32.b | (m->tile32.b1 << 1);
   F6_API_END
 }
 
 int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   F6_API_BEGIN
   F6_REQUIRE(m, FAME6D_EARG, "null handle");
-  F6_REQUIRE(va >= 0 && va < 512 && vb >= 0 && vb < 2, FAME6D_EARG, "tile variant out of range");
+  F6_REQUIRE(va >= 0 && va < 512 && vb >= 0 && vb < 4, FAME6D_EARG, "tile variant out of range");
   TileChoice& t = (precision == FAME6D_FP32) ? m->tile32 : m->tile64;
   t.a = va & 1;
   t.io = ((va >> 1) & 1) ? 0 : 1;
@@ -393,7 +399,8 @@ int fame6d_mass_set_tiles(fame6d_mass* m, int precision, int va, int vb) {
   t.c3 = (va >> 5) & 1;
   t.a3p = (va >> 6) & 3;
   t.a3s = (va >> 8) & 1;
-  t.b = vb;
+  t.b = vb & 1;
+  t.b1 = (vb >> 1) & 1;
   F6_API_END
 }
 

@@ -123,7 +123,8 @@ struct KernelTimer {
 // Tile variant indices of the three passes (chosen by the autotuner or defaults).
 struct TileChoice {
   int a = 0;  // pass A / pass C variant
-  int b = 0;  // pass B variant
+  int b = 0;  // pass B variant (M-hat applies, three vectors per group)
+  int b1 = 0;  // pass B variant of the plain applies (one vector per group)
   int io = 1;  // passes A/C: 1 = coalesced dim-3-first I/O, 0 = dim-1 register pencils
   int a3 = 0;  // Mhat pass A (three thread groups) variant
   int c3 = 0;   // Mhat pass C (three thread groups) variant

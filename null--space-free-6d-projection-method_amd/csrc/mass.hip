@@ -30,8 +30,8 @@ F6_TILES(6, double, 8, 6, 8, 24, 4, 2)
 F6_TILES(6, float, 8, 6, 8, 24, 4, 2)
 F6_TILES(8, double, 4, 8, 8, 4, 2, 1)
 F6_TILES(8, float, 4, 8, 8, 16, 2, 4)
-F6_TILES(10, double, 4, 2, 4, 8, 1, 2)
-F6_TILES(10, float, 4, 2, 8, 4, 1, 2)
+F6_TILES(10, double, 2, 1, 4, 2, 1, 2)
+F6_TILES(10, float, 2, 1, 4, 2, 1, 2)
 F6_TILES(12, double, 2, 2, 4, 2, 1, 1)
 F6_TILES(12, float, 2, 2, 4, 6, 1, 2)
 #undef F6_TILES
@@ -126,7 +126,7 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
   const void* e = std::is_same<T, double>::value ? (inverse_eps ? m->ieps64.ptr : m->eps64.ptr)
                                                  : (inverse_eps ? m->ieps32.ptr : m->eps32.ptr);
   PassBArgs<T> a{t, ldt, (const T*)e, skip, partials};
-  const int v = tile_of<T>(m).b;
+  const int v = nr == 1 ? tile_of<T>(m).b1 : tile_of<T>(m).b;  // plain applies have their own tile
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;

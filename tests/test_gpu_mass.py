@@ -29,7 +29,7 @@ def test_mscalar_fp64_fp32(pkg, oracle, dev, N):
     y32 = m.MscalarOp(torch.as_tensor(x, device=dev).to(torch.complex64)).cpu().numpy()
     assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 2e-6
     for va in range(8):
-        for vb in (0, 1):
+        for vb in range(4):
             m.set_tiles("double", va, vb)
             y2 = m.MscalarOp(torch.as_tensor(x, device=dev)).cpu().numpy()
             assert np.linalg.norm(y2 - ref) / np.linalg.norm(ref) < 1e-13, (va, vb)
