@@ -397,6 +397,50 @@ def vector_errors(torch, lam, Y64, Ymx, cluster_rel=1e-3):
     return out
 
 
+def differentiation(torch, pkg, N=5, k=5, ndir=64, h=1e-6):
+    """Gradients of the k lowest eigenvalues at q = (0.05, 0.04, 0.03) with respect to all NF
+    permittivity samples (Hellmann-Feynman through the adjoint mass kernel, behind
+    torch.autograd), checked against central differences with step h on ndir seeded random
+    directions of the weighted sum of the eigenvalues."""
+    from importlib import import_module
+    ag = import_module("fame6d_amd.autograd")
+    eps0, _ = pkg.cosine_material(N, AMP, EPS_SEED)
+    mass = pkg.MassBackend(N, eps0)
+    mass.autotune(reps=3)
+    opts = dict(p=40, tol=1e-12, max_restart=60, tol_cg=1e-13, maxit_cg=5000)
+    wts = torch.tensor([1.0, 0.5, -0.25, 0.125, 2.0][:k], dtype=torch.float64, device="cuda")
+    eps = torch.as_tensor(eps0, device="cuda").requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam = ag.lowest_eigenvalues(eps, mass, Q, k=k, **opts)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    (lam * wts).sum().backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    g = eps.grad.cpu().numpy()
+    rng = np.random.default_rng(64)
+    devs, fds = [], []
+    t0 = time.perf_counter()
+    for _ in range(ndir):
+        d = rng.standard_normal(mass.NF)
+        lp = ag.lowest_eigenvalues(torch.as_tensor(eps0 + h * d, device="cuda"), mass, Q, k=k, **opts)
+        lm = ag.lowest_eigenvalues(torch.as_tensor(eps0 - h * d, device="cuda"), mass, Q, k=k, **opts)
+        fd = float(((lp - lm) / (2 * h) * wts).sum())
+        fds.append(fd)
+        devs.append(abs(fd - float(g @ d)))
+    t_fd = time.perf_counter() - t0
+    mass.close()
+    fds = np.array(fds)
+    devs = np.array(devs)
+    rms = float(np.sqrt(np.mean(fds ** 2)))
+    return dict(N=N, k=k, directions=ndir, step=h, lambda_=lam.detach().cpu().numpy().tolist(),
+                forward_seconds=t_fwd, backward_seconds=t_bwd, fd_seconds=t_fd,
+                median_rel_dev_fd=float(np.median(devs / np.abs(fds))),
+                max_dev_over_rms_fd=float(devs.max() / rms), rms_fd=rms)
+
+
 def mixed_comparison(torch, pkg, N, amp, reps):
     """Mixed vs FP64 on the amplitude-0.8 configuration: per-mode relative eigenvalue
     differences and eigenvector angles."""
@@ -519,8 +563,7 @@ def main():
     comparisons = []
     if not a.no_fp64:
         comparisons.append(mixed_comparison(torch, pkg, 5, 0.8, 1))
-        if a.extras:
-            comparisons.append(mixed_comparison(torch, pkg, 6, 0.8, 1))
+        comparisons.append(mixed_comparison(torch, pkg, 6, 0.8, 1))
 
     cpu = None
     if not a.no_cpu:
@@ -528,7 +571,8 @@ def main():
     ext = extras(torch, pkg) if a.extras else None
     if ext is not None:
         for key, fn in (("methods_vs_C", lambda: method_comparison(torch, pkg, N)),
-                        ("yee_closure", lambda: yee_closure(torch, pkg))):
+                        ("yee_closure", lambda: yee_closure(torch, pkg)),
+                        ("differentiation", lambda: differentiation(torch, pkg))):
             try:
                 ext[key] = fn()
             except Exception as e:  # an extra that fails is reported, not fatal to the line

@@ -165,6 +165,29 @@ def test_eigs_vs_oracle_and_mixed(pkg, oracle, n2):
     assert np.max(np.abs(outm["lambda_"] - lam_o) / lam_o) < 1e-8
 
 
+def test_mixed_vs_fp64_n6_amplitude_08(pkg):
+    """The mixed-precision configuration at its largest size (N = 6, 12^6 modes, cosine
+    permittivity of amplitude 0.8): the ten eigenvalues of the mixed solve within 1e-8 of the
+    FP64 solve, both converged with small Ritz residuals."""
+    print("case: mixed vs FP64 N=6 amplitude 0.8", flush=True)
+    N = 6
+    eps, _ = pkg.cosine_material(N, 0.8, 0)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, Q_EXP1)
+    try:
+        v0 = pkg.start_vector(s.nr, 20260905)
+        lz = dict(k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-10, maxit_cg=1200, v0=v0)
+        o64 = pkg.run_method_C(s, **lz)
+        omx = pkg.run_method_C(s, precision="mixed", **lz)
+        assert o64["stats"]["converged"] and omx["stats"]["converged"]
+        assert o64["stats"]["max_ritz_residual"] < 1e-10 and omx["stats"]["max_ritz_residual"] < 1e-10
+        assert np.all(o64["lambda_"] > 0)
+        assert np.max(np.abs(omx["lambda_"] - o64["lambda_"]) / o64["lambda_"]) < 1e-8
+    finally:
+        s.close()
+        m.close()
+
+
 def test_recovery_and_diagnostics(pkg, oracle, n2):
     print("case: recovery N=2", flush=True)
     if "Y" not in n2:
