@@ -29,6 +29,23 @@ def test_moments_vs_oracle(pkg, oracle, setup, p, C):
     assert err < 1e-12, err
 
 
+def test_moments_many_chunks_per_workgroup(pkg, oracle):
+    """N = 3 (46656 modes) with 216 centers: every workgroup walks several chunks of modes,
+    so the double-buffered tables of the next chunk are built while the current one is
+    consumed; p = 10 and p = 14 (the one-column-tile instances)."""
+    rng = np.random.default_rng(33)
+    qm = oracle.projected_q_modes(3, (0.05, 0.04, 0.03))
+    NF = qm.shape[0]
+    coeff = (rng.standard_normal(NF) + 1j * rng.standard_normal(NF)) * np.exp(-0.5 * (qm ** 2).sum(1))
+    cen = rng.uniform(-0.6, 0.6, (216, 3))
+    for p in (10, 14):
+        print("case: taylor moments N=3 p=%d C=216" % p, flush=True)
+        mom = pkg.taylor_moments(qm, coeff, p, cen).cpu().numpy()
+        ref = oracle.taylor_moments(qm, coeff, p, cen)
+        err = np.abs(mom - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, (p, err)
+
+
 def test_yee_evaluation_vs_oracle_and_direct(pkg, oracle, setup):
     n, h, nb, p = 4, 0.0125, 2, 10
     print("case: taylor yee n=%d nb=%d p=%d" % (n, nb, p), flush=True)
