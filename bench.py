@@ -229,8 +229,8 @@ def yee_closure(torch, pkg):
     automatic phase-radius partition; the cropped Yee / LWRQ metrics of the mesh study
     (h = 0.1, 0.05, 0.025, 0.0125 at half-width 0.5).  Then the closure kernels alone on the
     largest window of the window study (n = 160, h = 0.025) with an exact discrete plane wave:
-    algorithmic bytes 144 per interior point (fields 48 + normalised densities 16 in pass 1,
-    32 + 32 for the x and y convolutions, 16 for z)."""
+    algorithmic bytes 80 per interior point (fields 48 in and normalised densities 16 out in
+    pass 1, the densities 16 in for the fused x/y/z convolution and its reductions)."""
     N = 5
     eps = pkg.smooth_scalar_material(N)
     mass = pkg.MassBackend(N, eps)
@@ -287,7 +287,7 @@ def yee_closure(torch, pkg):
                                    restarts=st["restarts"], lambda_11_114=[float(x) for x in lam],
                                    rel_diff_vs_reference=[float(x) for x in np.abs(lam - ref) / ref]),
                 mesh=rows, window_n160=dict(n=n, h=h, interior_points=q3, seconds=dt,
-                                           algorithmic_GBs=144.0 * q3 / dt / 1e9))
+                                           algorithmic_GBs=80.0 * q3 / dt / 1e9))
 
 
 def method_comparison(torch, pkg, N):

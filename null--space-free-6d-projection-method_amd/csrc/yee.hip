@@ -11,9 +11,11 @@
 // point):
 //   pass 1   u (3 complex fields, 48 B) -> a/den, b/den (16 B); sums of a, b and of the
 //            residual terms about the 6D eigenvalue (below)
-//   x        convolution of the two densities along x (16 B in, 16 B out)
-//   y        the same along y, marched through registers (16 + 16 B)
-//   z        the same along z with the LWRQ reductions, nothing written (16 B; rho when asked)
+//   LWRQ     the convolution of the two densities along x, y and z with the LWRQ
+//            reductions in one z-march, nothing written (16 B; rho when asked).  The
+//            standalone LWRQ entry point (which also normalises its raw inputs) runs it as
+//            three passes: x (16 B in, 16 B out), y marched through registers (16 + 16 B),
+//            z with the reductions (16 B)
 // den = conv(ones) is the product of one 1-D profile per axis and is applied in closed form.
 // The residuals ||Au - lam Bu||^2 for lam_3D (known only after the LWRQ) follow from sums
 // taken in pass 1 about lam_0 = lam_6D:
@@ -551,6 +553,183 @@ __global__ __launch_bounds__(256) void k_yee_finish(const double* partials, int 
 }
 
 // sc[2] = sc[0] / sc[1] (lambda of the raw densities, the centre of the deviation sums)
+// The whole LWRQ in one pass over the two densities: the Gaussian convolution along x, y and
+// z and the reductions of the z pass (as k_lwrq_march<R, 2, true>), without writing the x or
+// y intermediate.  The workgroup owns a 64 x 8 tile of (i, j) (thread (tx, ty) the rows ty and
+// ty + 4) and marches k through chunk blockIdx.z: for each input plane it stages the tile with
+// an R-point halo in x and y in LDS (the next plane's values in flight in registers), convolves
+// along x into a second LDS array and along y into registers, and pushes the result into a
+// register window along z.  The halo rows and columns are read by the neighbouring tiles too
+// (from L2); HBM sees each density value about once instead of five times.
+constexpr int kLfTY = 8;  // output rows per workgroup (two per thread row)
+
+template <int R>
+__global__ __launch_bounds__(kYeeThreads, R <= 3 ? 3 : (R <= 6 ? 2 : 1)) void k_lwrq_fused(long long q, Taps tp, const double* __restrict__ ina,
+                                                            const double* __restrict__ inb, long long chunk,
+                                                            const double* sc, double* rho, double* partials) {
+  constexpr int W = 2 * R + 1, IX = kYeeTX + 2 * R, IY = kLfTY + 2 * R, NI = IX * IY;
+  constexpr int NRL = (NI + kYeeThreads - 1) / kYeeThreads;  // staged values per thread and array
+  constexpr int NV = 5;
+  __shared__ double sa[NI], sb[NI];                  // input plane with halo
+  __shared__ double xa[IY * kYeeTX], xb[IY * kYeeTX];  // convolved along x
+  __shared__ double red[NV * kYeeThreads];
+  __shared__ double redm[2 * kYeeThreads];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = yee_tid();
+  const long long i0 = blockIdx.x * (long long)kYeeTX, j0 = blockIdx.y * (long long)kLfTY;
+  const long long qq = q * q;
+  const long long p0 = blockIdx.z * chunk, p1 = min(q, p0 + chunk);
+  double kk[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) kk[t] = tp.k[t];
+  // This thread's share of a staged plane: element e = tid + 256 r, at (row e / IX, column e % IX)
+  // = source (i0 - R + column, j0 - R + row); outside the window it is the zero padding.
+  long long off[NRL];
+  bool ok[NRL];
+#pragma unroll
+  for (int r = 0; r < NRL; ++r) {
+    const int e = tid + kYeeThreads * r, row = e / IX, col = e - row * IX;
+    const long long gi = i0 - R + col, gj = j0 - R + row;
+    ok[r] = e < NI && gi >= 0 && gi < q && gj >= 0 && gj < q;
+    off[r] = ok[r] ? gi + q * gj : 0;
+  }
+  double va[NRL], vb[NRL];
+  auto gload = [&](long long pl) {  // plane pl (zeros outside 0..q-1)
+    const bool in = pl >= 0 && pl < q;
+#pragma unroll
+    for (int r = 0; r < NRL; ++r) {
+      double x = 0.0, y = 0.0;
+      if (in && ok[r]) {
+        x = ina[off[r] + qq * pl];
+        y = inb[off[r] + qq * pl];
+      }
+      va[r] = x;
+      vb[r] = y;
+    }
+  };
+  // Y = the staged plane convolved along x then y at rows ty and ty + 4 of the tile.  Two
+  // barriers: sa/sb are last read before the second, xa/xb before the next call's first.
+  auto plane_y = [&](double (&ya)[2], double (&yb)[2]) {
+#pragma unroll
+    for (int r = 0; r < NRL; ++r) {
+      const int e = tid + kYeeThreads * r;
+      if (e < NI) {
+        sa[e] = va[r];
+        sb[e] = vb[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int row = ty; row < IY; row += kYeeTY) {  // x: tap tau = t - R reads source i - tau
+      const int c = row * IX + tx + R;
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int t = 0; t < W; ++t) {
+        a += kk[t] * sa[c - (t - R)];
+        b += kk[t] * sb[c - (t - R)];
+      }
+      xa[row * kYeeTX + tx] = a;
+      xb[row * kYeeTX + tx] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // y: window rows (j - R .. j + R) = tile rows jl .. jl + 2R
+      const int jl = ty + kYeeTY * h;
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int t = 0; t < W; ++t) {
+        a += kk[W - 1 - t] * xa[(jl + t) * kYeeTX + tx];
+        b += kk[W - 1 - t] * xb[(jl + t) * kYeeTX + tx];
+      }
+      ya[h] = a;
+      yb[h] = b;
+    }
+  };
+  double acc[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double mx = -INFINITY, mn = INFINITY;
+  const double lam = sc[2];
+  const long long i = i0 + tx;
+  if (p0 < p1) {
+    // window w[h][t] = Y(p - R + t) of row h; the first step needs planes p0 - R .. p0 + R
+    double wa[2][W], wb[2][W];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < W; ++t) wa[h][t] = wb[h][t] = 0.0;
+    gload(p0 - R);
+#pragma unroll 1
+    for (int t = 1; t < W; ++t) {  // planes p0 - R .. p0 + R - 1 into w[.][1..W-1]
+      double ya[2], yb[2];
+      plane_y(ya, yb);
+      gload(p0 - R + t);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int u = 0; u + 1 < W; ++u) {
+          wa[h][u] = wa[h][u + 1];
+          wb[h][u] = wb[h][u + 1];
+        }
+        wa[h][W - 1] = ya[h];
+        wb[h][W - 1] = yb[h];
+      }
+    }
+#pragma unroll 1
+    for (long long pz = p0; pz < p1; ++pz) {  // step pz adds plane pz + R (loaded already)
+      double ya[2], yb[2];
+      plane_y(ya, yb);
+      gload(pz + R + 1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int u = 0; u + 1 < W; ++u) {
+          wa[h][u] = wa[h][u + 1];
+          wb[h][u] = wb[h][u + 1];
+        }
+        wa[h][W - 1] = ya[h];
+        wb[h][W - 1] = yb[h];
+        const long long j = j0 + ty + kYeeTY * h;
+        if (i < q && j < q) {
+          double E = 0.0, M = 0.0;
+#pragma unroll
+          for (int t = 0; t < W; ++t) {
+            E += kk[W - 1 - t] * wa[h][t];
+            M += kk[W - 1 - t] * wb[h][t];
+          }
+          const double d = E - lam * M;
+          acc[0] += E;
+          acc[1] += M;
+          acc[2] += d * d / M;
+          acc[3] += d;
+          if (!isfinite(M) || !isfinite(E)) acc[4] += 1.0;
+          mx = fmax(mx, M);
+          mn = fmin(mn, M);
+          if (rho) rho[i + q * j + qq * pz] = E / M;
+        }
+      }
+    }
+  }
+  yee_block_sums<NV>(acc, red);
+  redm[tid] = mx;
+  redm[kYeeThreads + tid] = mn;
+  __syncthreads();
+  if (tid < 64) {
+    for (int t = tid + 64; t < kYeeThreads; t += 64) {
+      mx = fmax(mx, redm[t]);
+      mn = fmin(mn, redm[kYeeThreads + t]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mx = fmax(mx, __shfl_down(mx, o, 64));
+      mn = fmin(mn, __shfl_down(mn, o, 64));
+    }
+  }
+  if (tid == 0) {
+    const int b = yee_block(), nb = yee_nblocks();
+    for (int v = 0; v < NV; ++v) partials[v * nb + b] = acc[v];
+    partials[5 * nb + b] = mx;
+    partials[6 * nb + b] = mn;
+  }
+}
+
 __global__ void k_yee_ratio(double* sc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sc[2] = sc[0] / sc[1];
 }
@@ -755,8 +934,57 @@ static void conv_x(int R, const YeeLaunch& l, long long q, const Taps& tp, const
 // normalise: the x pass divides a, b by den and sums them into sc[0], sc[1] (the standalone
 // LWRQ); otherwise a, b are already a/den, b/den and pass 1 has put sc[0..5].  w2/w3 and
 // w0/w1 are ping-ponged (a, b may be w0, w1: the x pass reads them before the y pass writes).
+// The fused LWRQ pass: (x, y) tiles of 64 x 8 and about 1024 workgroups in all, each z chunk
+// at least 4R planes long (its 2R warm-up planes are recomputed by the chunk).
+static YeeLaunch lwrq_fused_launch(long long q, int R, long long& chunk) {
+  const long long gx = (q + kYeeTX - 1) / kYeeTX, gy = (q + kLfTY - 1) / kLfTY;
+  long long gz = std::max<long long>(1, std::min<long long>(q, (1024 + gx * gy - 1) / (gx * gy)));
+  chunk = std::max<long long>((q + gz - 1) / gz, std::max(4 * R, 1));
+  gz = (q + chunk - 1) / chunk;
+  YeeLaunch l;
+  l.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)gz);
+  l.block = dim3(kYeeTX, kYeeTY, 1);
+  l.nb = (int)(gx * gy * gz);
+  return l;
+}
+
+static void lwrq_fused(int R, const YeeLaunch& l, long long q, const Taps& tp, const double* ina, const double* inb,
+                       long long chunk, const double* sc, double* rho, double* part, hipStream_t s) {
+#define F6_LWRQ_FUSED(RR)                                                                              \
+  case RR:                                                                                             \
+    k_lwrq_fused<RR><<<l.grid, l.block, 0, s>>>(q, tp, ina, inb, chunk, sc, rho, part);                \
+    break;
+  switch (R) {
+    F6_LWRQ_FUSED(1)
+    F6_LWRQ_FUSED(2)
+    F6_LWRQ_FUSED(3)
+    F6_LWRQ_FUSED(4)
+    F6_LWRQ_FUSED(5)
+    F6_LWRQ_FUSED(6)
+    F6_LWRQ_FUSED(7)
+    F6_LWRQ_FUSED(8)
+    default: throw Error(FAME6D_EARG, "LWRQ neighbour radius must be in 1..8");
+  }
+#undef F6_LWRQ_FUSED
+  F6_CHECK_LAUNCH();
+}
+
 static LwrqSums lwrq_device(const YeeWs& w, long long q, const Taps& tp, const double* a, const double* b,
                             double* rho, hipStream_t s, bool normalise) {
+  if (!normalise) {
+    // a, b already normalised (the metrics pass): one fused pass; sc[2] = sum a / sum b is set
+    k_yee_ratio<<<1, 64, 0, s>>>(w.sc);
+    long long chunk = 0;
+    const YeeLaunch lf = lwrq_fused_launch(q, tp.R, chunk);
+    if (lf.nb > yee_max_partials(q)) throw Error(FAME6D_EARG, "LWRQ partials exceed the workspace");
+    lwrq_fused(tp.R, lf, q, tp, a, b, chunk, w.sc, rho, w.part, s);
+    k_yee_finish<<<7, 256, 0, s>>>(w.part, lf.nb, Kinds{{0, 0, 0, 0, 0, 1, 2}}, w.sc + 8);
+    F6_CHECK_LAUNCH();
+    double h[16];
+    F6_HIP(hipMemcpyAsync(h, w.sc, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+    F6_HIP(hipStreamSynchronize(s));
+    return LwrqSums{h[0], h[1], h[3], h[4], h[5], h[6], h[8], h[9], h[10], h[11], h[12], h[13], h[14]};
+  }
   const YeeLaunch lx = yee_launch(q, q);
   conv_x(tp.R, lx, q, tp, a, b, w.w2, w.w3, normalise ? w.prof : nullptr, normalise ? w.part : nullptr, s);
   if (normalise) k_yee_finish<<<2, 256, 0, s>>>(w.part, lx.nb, Kinds{{0, 0}}, w.sc);

@@ -90,6 +90,22 @@ def test_metrics_vs_oracle(pkg, oracle, kind):
     assert np.max(np.abs(unflat(g["rho"], q) - r["rho"])) <= 1e-11 * np.max(np.abs(r["rho"]))
 
 
+@pytest.mark.parametrize("R", [1, 2, 5, 8])
+def test_metrics_vs_oracle_radius(pkg, oracle, R):
+    """The fused x/y/z LWRQ pass at other neighbour radii (one compiled instance each), on a
+    window whose z extent spans several chunks of the march for the small radii."""
+    print("case: Yee metrics n=9 radius %d" % R, flush=True)
+    n, h = 9, 0.05
+    u = rand_fields(np.random.default_rng(10 + R), n)
+    lam6 = 0.0123
+    g = pkg.compute_yee_lwrq_metrics(*[flat(x) for x in u], lam6, n, h, radius=R, return_rho=True)
+    r = oracle.compute_yee_lwrq_metrics(*u, lam6, h, n, R=R, return_rho=True)
+    for key in ("lambda_3d", "eta_3d", "V", "nu", "cropped_mass"):
+        assert abs(g[key] - r[key]) <= 1e-11 * abs(r[key]), (key, g[key], r[key])
+    q = 2 * n + 1
+    assert np.max(np.abs(unflat(g["rho"], q) - r["rho"])) <= 1e-11 * np.max(np.abs(r["rho"]))
+
+
 def test_metrics_plane_wave_window_n160(pkg):
     """The largest window of the reference's window study (n = 160, h = 0.025,
     exp3/make_config_experiment4.m:89-91) on an exact discrete eigenpair: a transverse
