@@ -152,6 +152,22 @@ static __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const doubl
   if (threadIdx.x == 0) out[g] = s;
 }
 
+// Two sets of jj reductions in one launch (grid 2*jj): out_re[g] from partials_re, out_im[g]
+// from partials_im; skipped as a whole with `sk`.
+static __global__ __launch_bounds__(kRedThreads) void k_reduce_final_ri(const double* pre, const double* pim,
+                                                                 int nb, int jj, double* out_re, double* out_im,
+                                                                 Skip sk) {
+  __shared__ double red[kRedThreads];
+  if (skipped(sk, 0)) return;
+  const bool im = blockIdx.x >= (unsigned)jj;
+  const int g = im ? blockIdx.x - jj : blockIdx.x;
+  const double* partials = im ? pim : pre;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[(long long)g * nb + i];
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) (im ? out_im : out_re)[g] = s;
+}
+
 // ------------------------------------------------------------------ CG scalars
 // Follows MATLAB pcg without preconditioner and x0 = 0 (call sites
 // make_explicit_krinv_compact.m:43): rho = r'r, p = r + (rho/rho_old) p,

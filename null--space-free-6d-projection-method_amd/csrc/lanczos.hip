@@ -107,8 +107,7 @@ static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cp
   launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim,
            sk);
   if (sk.p == nullptr) timer_end(m, (jj + 1) * 16.0 * n);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pre, nb, dre);
-  k_reduce_final<<<jj, kRedThreads, 0, s>>>(pim, nb, dim);
+  k_reduce_final_ri<<<2 * jj, kRedThreads, 0, s>>>(pre, pim, nb, jj, dre, dim, sk);
   F6_CHECK_LAUNCH();
 }
 
