@@ -358,19 +358,24 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     if (act.empty()) break;
     const int na = (int)act.size();
     std::vector<int> slots(na);
+    // K_r^{-1} reads its input only, so a single active run passes its basis column itself
+    const cpx<double>* kin = qin;
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       cpx<double>* Qj = Lg.B.Q + (size_t)n2 * (Lg.jj - 1);
       F6_HIP(hipMemcpyAsync(Qj, Lg.B.v, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
-      F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      if (na == 1)
+        kin = Qj;
+      else
+        F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
       slots[i] = Lg.slot;
     }
     if (g_nested.on) {
       for (int i = 0; i < na; ++i)
-        krinv_nested(S, qin + (size_t)n2 * i, qout + (size_t)n2 * i, tol_cg, g_nested.tolM, maxit_cg,
+        krinv_nested(S, kin + (size_t)n2 * i, qout + (size_t)n2 * i, tol_cg, g_nested.tolM, maxit_cg,
                      g_nested.maxitM, g_nested.epsC, nullptr);
     } else {
-      krinv_groups(S, precision, na, slots.data(), qin, qout, tol_cg, maxit_cg, nullptr);
+      krinv_groups(S, precision, na, slots.data(), kin, qout, tol_cg, maxit_cg, nullptr);
     }
     ++steps;
     for (int i = 0; i < na; ++i) {
