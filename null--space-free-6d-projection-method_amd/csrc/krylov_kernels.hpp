@@ -141,6 +141,26 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partials(const cpx<T>* a, l
   }
 }
 
+// Start of a conjugate-gradient solve with x0 = 0: x = 0, r = b and the partials of ||b||^2
+// (the same element order and sums as k_dot_partials(b, b)), one pass over b.
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_cg_start(const cpx<T>* b, cpx<T>* x, cpx<T>* r, long long n,
+                                                          double* partials) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.y;
+  const long long go = g * n;
+  double sr = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const cpx<T> u = b[go + i];
+    x[go + i] = mk(T(0), T(0));
+    r[go + i] = u;
+    sr += static_cast<double>(u.re) * u.re + static_cast<double>(u.im) * u.im;
+  }
+  sr = block_sum<kRedThreads>(sr, red);
+  if (threadIdx.x == 0) partials[(long long)g * gridDim.x + blockIdx.x] = sr;
+}
+
 // Fixed-order sum of nb partials per group; out[g] = sum.
 static __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* partials, int nb,
                                                               double* out) {

@@ -245,7 +245,6 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const long long NF = m->NF;
-  const long long total = NF * nact;
   constexpr bool f32 = std::is_same<T, float>::value;
   const int nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
   const int nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
@@ -265,9 +264,7 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const cpx<T>* zr = precond ? z : r;  // the vector the search direction is built from
   const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
 
-  k_fill_zero<T><<<grid_for(total), 256, 0, s>>>(total, x);
-  F6_HIP(hipMemcpyAsync(r, b, sizeof(cpx<T>) * total, hipMemcpyDeviceToDevice, s));
-  k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, NF, b, NF, NF, part2, nullptr);
+  k_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, x, r, NF, part2);  // x = 0, r = b, ||b||^2
   if (precond) {
     mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, part);
     k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbE, tol, maxit, maxmsteps);
