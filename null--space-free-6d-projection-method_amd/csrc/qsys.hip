@@ -449,9 +449,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     const int nbD = red_blocks(NF);
     S->part2.ensure(sizeof(double) * ((size_t)nbD * nact + 8));
     double* part2 = S->part2.as<double>();
-    k_fill_zero<double><<<grid_for(gn), 256, 0, s>>>((long long)gn, w.u);
-    F6_HIP(hipMemcpyAsync(w.res, w.bhat, sizeof(cpx<double>) * gn, hipMemcpyDeviceToDevice, s));
-    k_dot_partials<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, NF, w.bhat, NF, NF, part2, nullptr);
+    k_cg_start<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, w.u, w.res, NF, part2);  // u = 0, res = bhat
     k_ref_init<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD);
     F6_CHECK_LAUNCH();
     res.assign(nact, PcgOut{});
