@@ -327,35 +327,12 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_update(cpx<T>* x, const cpx<
 
 static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, const double* partials,
                                                          int nb) {
+  static_assert(kRedThreads == kCgSumThreads, "the pass-A fold sums the partials in the same order");
   __shared__ double red[kRedThreads];
   const int g = blockIdx.x;
   if (st[g].done) return;
   const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
-  if (threadIdx.x == 0) {
-    CgState c = st[g];
-    c.iter += 1;
-    c.normr = sqrt(rho_new);
-    if (!isfinite(rho_new)) {
-      c.flag = -1;
-      c.done = 1;
-    } else if (c.normr <= c.tolb || c.check_every) {
-      c.flag = 0;  // provisional: the host checks the true residual
-      c.done = 1;
-    } else if (c.iter >= c.maxit) {
-      c.flag = 1;
-      c.done = 1;
-    } else {
-      const double beta = rho_new / c.rho;
-      if (beta == 0.0 || isinf(beta) || rho_new == 0.0) {
-        c.flag = 4;
-        c.done = 1;
-      }
-      c.beta = beta;
-    }
-    c.rho_prev = c.rho;
-    c.rho = rho_new;
-    st[g] = c;
-  }
+  if (threadIdx.x == 0) cg_store_step(st + g, cg_beta_step(st + g, rho_new));
 }
 
 // True-residual check of MATLAB pcg, on the device.  k_cg_need_check marks the groups

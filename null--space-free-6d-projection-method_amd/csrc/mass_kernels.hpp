@@ -95,6 +95,14 @@ struct PassAArgs {
   const CgState* st;  // per group (PRE_MHAT_CG)
   Skip skip;
   const int* qmap;
+  // PRE_MHAT_CG, optional: the pcg step after the previous iteration (the work of
+  // k_cg_beta) is folded into this pass.  Every workgroup reduces the nrr pass-C partials
+  // of ||r||^2 of group g (rr + g*nrr) and updates a copy of st[g]; workgroup 0 stores the
+  // updated state in st_next[g], which the rest of the iteration uses.  No workgroup writes
+  // st, so all of them see the same old state.
+  CgState* st_next;
+  const double* rr;
+  int nrr;
 };
 
 template <class T>
@@ -173,6 +181,87 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
       for (int i = 0; i < THREADS; ++i) s += red[i];
   }
   return s;
+}
+
+// MATLAB pcg's bookkeeping after an unpreconditioned iteration: iteration count, recurrence
+// convergence test, maxit, breakdown, beta = rho_new / rho (make_explicit_krinv_compact.m:43
+// calls pcg with x0 = 0).  rho_new = ||r_new||^2 is the fixed-order sum of the pass-C
+// partials: thread stride kCgSumThreads, then a workgroup sum, the same value in k_cg_beta
+// and in the pass-A fold below (extra threads add zeros).
+constexpr int kCgSumThreads = 256;
+// The fields a step changes, as scalars (a local copy of the whole struct would be kept in
+// scratch memory).
+struct CgStep {
+  int iter, done, flag;
+  double normr, beta, rho, rho_prev;
+};
+__device__ __forceinline__ CgStep cg_beta_step(const CgState* st, double rho_new) {
+  CgStep c{st->iter + 1, st->done, st->flag, sqrt(rho_new), st->beta, rho_new, st->rho};
+  if (!isfinite(rho_new)) {
+    c.flag = -1;
+    c.done = 1;
+  } else if (c.normr <= st->tolb || st->check_every) {
+    c.flag = 0;  // provisional: the true residual is checked after the chunk
+    c.done = 1;
+  } else if (c.iter >= st->maxit) {
+    c.flag = 1;
+    c.done = 1;
+  } else {
+    const double beta = rho_new / st->rho;
+    if (beta == 0.0 || isinf(beta) || rho_new == 0.0) {
+      c.flag = 4;
+      c.done = 1;
+    }
+    c.beta = beta;
+  }
+  return c;
+}
+__device__ __forceinline__ void cg_store_step(CgState* d, const CgStep& c) {
+  d->iter = c.iter;
+  d->done = c.done;
+  d->flag = c.flag;
+  d->normr = c.normr;
+  d->beta = c.beta;
+  d->rho = c.rho;
+  d->rho_prev = c.rho_prev;
+}
+
+// State of group g for a PRE_MHAT_CG pass A: false when the group is done (the caller
+// returns).  With a.st_next the previous iteration's pcg step is applied here (see
+// PassAArgs); red is LDS scratch of THREADS doubles, free again on return.
+template <int THREADS, class T>
+__device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, double* red, bool& use_beta,
+                                                T& beta) {
+  if (a.st_next == nullptr) {
+    if (skipped(a.skip, g)) return false;
+    use_beta = a.st[g].iter > 0;
+    beta = static_cast<T>(a.st[g].beta);
+    return true;
+  }
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  if (tid < kCgSumThreads)
+    for (int i = tid; i < a.nrr; i += kCgSumThreads) v += a.rr[(long long)g * a.nrr + i];
+  v = block_sum<THREADS>(v, red);
+  if (tid == 0) {
+    const CgState* o = a.st + g;
+    const bool was_done = o->done != 0;
+    const CgStep c = was_done ? CgStep{o->iter, o->done, o->flag, o->normr, o->beta, o->rho, o->rho_prev}
+                              : cg_beta_step(o, v);
+    if (blockIdx.x == 0) {
+      a.st_next[g] = *o;
+      if (!was_done) cg_store_step(a.st_next + g, c);
+    }
+    red[0] = c.done ? 1.0 : 0.0;
+    red[1] = c.iter > 0 ? 1.0 : 0.0;
+    red[2] = c.beta;
+  }
+  __syncthreads();
+  const bool done = red[0] != 0.0;
+  use_beta = red[1] != 0.0;
+  beta = static_cast<T>(red[2]);
+  __syncthreads();  // red is LDS the caller reuses
+  return !done;
 }
 
 // ---------------------------------------------------------------- pass A
@@ -593,7 +682,13 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
   constexpr int TG = BPW * P2;  // threads per group
   __shared__ cpx<T> lds[3 * BPW * S];
   const int g = blockIdx.y;
-  if (skipped(a.skip, g)) return;
+  bool use_beta = false;
+  T beta = T(0);
+  if constexpr (OP == PRE_MHAT_CG) {
+    if (!cg_pass_a_state<3 * BPW * n * n>(a, g, reinterpret_cast<double*>(lds), use_beta, beta)) return;
+  } else {
+    if (skipped(a.skip, g)) return;
+  }
   const int tid = threadIdx.x;
   const int o = tid / TG, tl = tid - o * TG;
   const int blk = tl / P2, pp = tl - blk * P2;
@@ -608,8 +703,7 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
   for (int k = 0; k < n; ++k) pn[k] = xg[P2 * k];
   if constexpr (OP == PRE_MHAT_CG) {
     cpx<T>* pg = a.p + g * a.ldx + off + pp;
-    if (a.st[g].iter > 0) {
-      const T beta = static_cast<T>(a.st[g].beta);
+    if (use_beta) {
 #pragma unroll
       for (int k = 0; k < n; ++k) pn[k] = pn[k] + beta * pg[P2 * k];
     }
@@ -671,7 +765,13 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
   typedef T v2 __attribute__((ext_vector_type(2)));
   __shared__ cpx<T> lds[3 * E];
   const int g = blockIdx.y;
-  if (skipped(a.skip, g)) return;
+  bool use_beta = false;
+  T beta = T(0);
+  if constexpr (CG) {
+    if (!cg_pass_a_state<3 * TG>(a, g, reinterpret_cast<double*>(lds), use_beta, beta)) return;
+  } else {
+    if (skipped(a.skip, g)) return;
+  }
   const int tid = threadIdx.x;
   const int o = tid / TG, tl = tid - o * TG;
   const int blk = tl / P2, pp = tl - blk * P2;
@@ -680,12 +780,6 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
   const long long boff = (long long)blk * S + pp;
   const cpx<T>* xsrc = a.x + g * a.ldx + boff;
   cpx<T>* psrc = CG ? a.p + g * a.ldx + boff : nullptr;
-  bool use_beta = false;
-  T beta = T(0);
-  if constexpr (CG) {
-    use_beta = a.st[g].iter > 0;
-    beta = static_cast<T>(a.st[g].beta);
-  }
   const long long qg = qslot(a.qmap, g) * a.ldq + boff;
   const T* wo = (o == 1) ? a.W1 : a.W2;
   v2 px[n], pq[n];
@@ -774,7 +868,13 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3s(PassAArgs<T> a) {
   typedef T v2 __attribute__((ext_vector_type(2)));
   __shared__ cpx<T> lds[4 * E];
   const int g = blockIdx.y;
-  if (skipped(a.skip, g)) return;
+  bool use_beta = false;
+  T beta = T(0);
+  if constexpr (CG) {
+    if (!cg_pass_a_state<THREADS>(a, g, reinterpret_cast<double*>(lds), use_beta, beta)) return;
+  } else {
+    if (skipped(a.skip, g)) return;
+  }
   const int tid = threadIdx.x;
   const int o = tid / TG, tl = tid - o * TG;
   const int blk = tl / P2, pp = tl - blk * P2;
@@ -783,12 +883,6 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3s(PassAArgs<T> a) {
   cpx<T>* X = lds + 3 * E;  // the shared pre-op result of the tile
   const cpx<T>* xsrc = a.x + g * a.ldx;
   cpx<T>* pdst = CG ? a.p + g * a.ldx : nullptr;
-  bool use_beta = false;
-  T beta = T(0);
-  if constexpr (CG) {
-    use_beta = a.st[g].iter > 0;
-    beta = static_cast<T>(a.st[g].beta);
-  }
   const long long qg = qslot(a.qmap, g) * a.ldq + (long long)blk * S + pp;
   const T* wo = (o == 1) ? a.W1 : a.W2;
   v2 px[NE], pq[NE];

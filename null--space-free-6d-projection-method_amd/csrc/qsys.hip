@@ -249,11 +249,14 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const int nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
   const int nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
   const int nbD = red_blocks(NF);
-  S->st.ensure(sizeof(CgState) * nact);
+  S->st.ensure(sizeof(CgState) * nact * 2);  // two slots: the unpreconditioned loop alternates
   S->part.ensure(sizeof(double) * (size_t)nbE * nact);
   S->part2.ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
   S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
-  CgState* st = S->st.as<CgState>();
+  CgState* const st0 = S->st.as<CgState>();
+  int cur = 0;  // slot of the current state
+  auto slot = [&](int k) { return st0 + (size_t)k * nact; };
+  CgState* st = slot(0);
   double* part = S->part.as<double>();
   double* part2 = S->part2.as<double>();
   int* need = S->ints.as<int>();
@@ -319,9 +322,21 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   while (true) {
     if (++rounds > maxit + 8) throw Error(FAME6D_ENOCONV, "Mhat CG control loop did not terminate");
     for (int c = 0; c < chunk; ++c, ++issued) {
+      // unpreconditioned: from the second iteration of a chunk on, pass A applies the pcg step
+      // of the previous iteration (reading slot cur, writing slot cur^1)
+      const bool fold = !precond && c > 0;
+      pa.st = slot(cur);
+      pa.skip = skip_done(slot(cur));
+      pa.st_next = fold ? slot(cur ^ 1) : nullptr;
+      pa.rr = part2;
+      pa.nrr = nbC;
       timer_begin(m, TS_MHAT_PASS_A);
       launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
       timer_end(m, bytesA, issued);
+      if (fold) cur ^= 1;
+      st = slot(cur);
+      pc.cg = st;
+      pc.skip = skip_done(st);
       timer_begin(m, TS_MHAT_PASS_B);
       launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s, false, part);
       timer_end(m, bytesB, issued);
@@ -331,11 +346,11 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       if (precond) {
         mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
         k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, part, nbE);
-      } else {
-        k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC);
       }
       F6_CHECK_LAUNCH();
     }
+    // the pcg step of the chunk's last iteration (a group that stopped earlier is skipped)
+    if (!precond) k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC);
     if (true_check) {
       // true residual r = b - Mhat x of the groups that stopped on the recurrence
       k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
