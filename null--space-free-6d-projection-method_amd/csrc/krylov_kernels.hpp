@@ -545,11 +545,14 @@ __global__ void k_fill_zero_real(long long n, T* x) {
 // Partials of h_k = Q(:,k)' r for all k < jj in one launch.  Block (rb, c) of the
 // logical grid covers rows of row block rb for columns [c*KB, c*KB + KB); the logical
 // order (column chunk fastest) is mapped XCD-aware, so the chunks of one row block run
-// on one XCD and re-read r from its L2.  pre/pim[k*nrb + rb].
+// on one XCD and re-read r from its L2.  pre/pim[k*nrb + rb].  A thread takes two rows per
+// step and issues all 2(KB+1) loads before the first product: a short last chunk re-reads
+// its last column instead of branching around the loads, which would serialise them.
 template <int KB>
 __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
                                                             const cpx<double>* r, long long n, int nrb,
                                                             double* pre, double* pim, Skip sk) {
+  constexpr int RPT = 2;
   __shared__ double red[kRedThreads];
   if (skipped(sk, 0)) return;
   const int nch = (jj + KB - 1) / KB;
@@ -559,16 +562,34 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q
   double ar[KB], ai[KB];
 #pragma unroll
   for (int k = 0; k < KB; ++k) ar[k] = ai[k] = 0.0;
-  for (long long i = rb * (long long)kRedThreads + threadIdx.x; i < n; i += (long long)nrb * kRedThreads) {
-    const cpx<double> v = r[i];
+  auto accumulate = [&](const cpx<double>& v, const cpx<double>* q) {
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       if (k < kcount) {
-        const cpx<double> q = Q[(long long)(k0 + k) * ld + i];
-        ar[k] += q.re * v.re + q.im * v.im;
-        ai[k] += q.re * v.im - q.im * v.re;
+        ar[k] += q[k].re * v.re + q[k].im * v.im;
+        ai[k] += q[k].re * v.im - q[k].im * v.re;
       }
     }
+  };
+  const long long stride = (long long)nrb * kRedThreads;
+  long long i = rb * (long long)kRedThreads + threadIdx.x;
+  for (; i + (RPT - 1) * stride < n; i += RPT * stride) {
+    cpx<double> v[RPT], q[RPT][KB];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      v[u] = r[i + u * stride];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) q[u][k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) accumulate(v[u], q[u]);
+  }
+  if (i < n) {  // at most one row is left: the loop above stops within RPT strides of n
+    cpx<double> q[KB];
+    const cpx<double> v = r[i];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) q[k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i];
+    accumulate(v, q);
   }
   for (int k = 0; k < kcount; ++k) {
     const double s = block_sum<kRedThreads>(ar[k], red);

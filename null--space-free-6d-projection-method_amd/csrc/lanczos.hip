@@ -95,7 +95,7 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
 // r = w - Q h1, h2 = Q^H r, r -= Q h2 with ||r||^2 fused into the last update.
 // h1 in sc[kScH..], h2 in sc[kScH2..], ||r||^2 in sc[1].
 constexpr int kScH2 = 8 + 2 * kLzMaxP, kScH2im = 8 + 3 * kLzMaxP;
-constexpr int kGemvKB = 8;
+constexpr int kGemvKB = 4;
 // With `sk` set the projection is skipped (and not timed: its launch does no work).
 static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* v, double* dre,
                    double* dim, hipStream_t s, Skip sk = no_skip()) {
@@ -177,6 +177,7 @@ struct LzState {
   cpx<double>* Z = nullptr;  // M-weighted basis Z = M Q (Method A), restarted with Q
   int alpha_at = 0;          // index of alpha in the device scalars of the last step
   int second_sweeps = 0;     // steps whose DGKS test asked for a second Gram-Schmidt sweep
+  cpx<double>* vloc = nullptr;  // where the newest unit vector is: B.v or the next basis column
 };
 
 // Builds H, solves it, tests convergence and restarts (Lanczos_hybrid_identity_gpuop.m:79-120).
@@ -346,6 +347,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     for (int g = 0; g < G; ++g) {
       if (!(std::isfinite(nv[g]) && nv[g] > 0)) throw Error(FAME6D_EARG, "Invalid Lanczos start vector.");
       k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, L[g].B.v, L[g].B.v, L[g].B.sc + 1);
+      L[g].vloc = L[g].B.v;
     }
     F6_CHECK_LAUNCH();
   }
@@ -363,7 +365,8 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       cpx<double>* Qj = Lg.B.Q + (size_t)n2 * (Lg.jj - 1);
-      F6_HIP(hipMemcpyAsync(Qj, Lg.B.v, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      if (Lg.vloc != Qj)  // after the start and after restarts; a step's own v is written in place
+        F6_HIP(hipMemcpyAsync(Qj, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
       if (na == 1)
         kin = Qj;
       else
@@ -426,8 +429,12 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         end_cycle = true;
       } else {
         Lg.beta = std::sqrt(beta2);
-        k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, Lg.B.v, Lg.B.r, Lg.B.sc + 1);
+        // v = r / beta goes straight into the column it takes next, or into B.v when the
+        // basis is full and a restart follows
+        cpx<double>* dst = Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v;
+        k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, dst, Lg.B.r, Lg.B.sc + 1);
         F6_CHECK_LAUNCH();
+        Lg.vloc = dst;
         Lg.Alpha.push_back(alpha);
         Lg.Beta.push_back(Lg.beta);
         Lg.kk += 1;
@@ -435,6 +442,11 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         else Lg.jj += 1;
       }
       if (end_cycle) {
+        // the restart rewrites the basis columns in place: the newest v must be in B.v
+        if (Lg.vloc != Lg.B.v) {
+          F6_HIP(hipMemcpyAsync(Lg.B.v, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+          Lg.vloc = Lg.B.v;
+        }
         if (restart_step(Lg, k0, p, tol, maxRestart, n2, ubuf, s)) Lg.done = true;
       }
     }
