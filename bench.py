@@ -334,7 +334,7 @@ FAMILY_KERNEL = {
     "mhat_pass_a": r"k_pass_a3[ps]?<double, %d, \d+, 2>",
     "mhat_pass_b": r"k_pass_b<double, %d, \d+, 3, 0>",
     "mhat_pass_c": r"k_pass_c3p?<double, %d, \d+, true>",
-    "lanczos_gemv_c": r"k_gemv_c_all<\d+>",
+    "lanczos_gemv_c": r"k_gemv_c_all<\d+, (true|false)>",
     "lanczos_gemv_n": r"k_gemv_n_out",
 }
 FAMILY_DESC = {

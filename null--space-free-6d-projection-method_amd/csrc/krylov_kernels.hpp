@@ -548,10 +548,19 @@ __global__ void k_fill_zero_real(long long n, T* x) {
 // on one XCD and re-read r from its L2.  pre/pim[k*nrb + rb].  A thread takes two rows per
 // step and issues all 2(KB+1) loads before the first product: a short last chunk re-reads
 // its last column instead of branching around the loads, which would serialise them.
-template <int KB>
+// THREE: r is the three-term update w - alpha q_{jj-1} - beta q_{jj-2} (tt.r = w), formed
+// on the fly from columns the sweep reads anyway, and the chunk-0 blocks also write the
+// ||r||^2 partials to tt.norm[rb]; r itself is never stored.
+struct ThreeTerm {
+  const double* alpha;  // device scalar
+  double beta;          // 0 on the first step
+  double* norm;         // nrb partials of ||r||^2
+};
+template <int KB, bool THREE = false>
 __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
                                                             const cpx<double>* r, long long n, int nrb,
-                                                            double* pre, double* pim, Skip sk) {
+                                                            double* pre, double* pim, Skip sk,
+                                                            ThreeTerm tt = ThreeTerm{nullptr, 0.0, nullptr}) {
   constexpr int RPT = 2;
   __shared__ double red[kRedThreads];
   if (skipped(sk, 0)) return;
@@ -572,24 +581,51 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q
     }
   };
   const long long stride = (long long)nrb * kRedThreads;
+  // three-term form: alpha, beta and the two newest columns (jj >= 2 for beta)
+  double alpha = 0.0, nrm = 0.0;
+  const cpx<double>* qa = nullptr;
+  const cpx<double>* qb = nullptr;
+  if constexpr (THREE) {
+    alpha = *tt.alpha;
+    qa = Q + (long long)(jj - 1) * ld;
+    qb = jj >= 2 ? Q + (long long)(jj - 2) * ld : qa;
+  }
+  // beta = 0 before the second column exists: v - 0 q = v, so the loads stay unconditional
+  const double beta = (THREE && jj >= 2) ? tt.beta : 0.0;
+  auto row = [&](long long ii) {
+    cpx<double> v = r[ii];
+    if constexpr (THREE) v = (v - alpha * qa[ii]) - beta * qb[ii];
+    return v;
+  };
   long long i = rb * (long long)kRedThreads + threadIdx.x;
   for (; i + (RPT - 1) * stride < n; i += RPT * stride) {
     cpx<double> v[RPT], q[RPT][KB];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
-      v[u] = r[i + u * stride];
+      v[u] = row(i + u * stride);
 #pragma unroll
       for (int k = 0; k < KB; ++k) q[u][k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i + u * stride];
     }
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) accumulate(v[u], q[u]);
+    for (int u = 0; u < RPT; ++u) {
+      accumulate(v[u], q[u]);
+      if constexpr (THREE) nrm += v[u].re * v[u].re + v[u].im * v[u].im;
+    }
   }
   if (i < n) {  // at most one row is left: the loop above stops within RPT strides of n
     cpx<double> q[KB];
-    const cpx<double> v = r[i];
+    const cpx<double> v = row(i);
 #pragma unroll
     for (int k = 0; k < KB; ++k) q[k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i];
     accumulate(v, q);
+    if constexpr (THREE) nrm += v.re * v.re + v.im * v.im;
+  }
+  if constexpr (THREE) {
+    if (ch == 0) {  // uniform per block
+      const double s = block_sum<kRedThreads>(nrm, red);
+      __syncthreads();
+      if (threadIdx.x == 0) tt.norm[rb] = s;
+    }
   }
   for (int k = 0; k < kcount; ++k) {
     const double s = block_sum<kRedThreads>(ar[k], red);
@@ -682,14 +718,24 @@ __global__ __launch_bounds__(256) void k_basis_gemm(const double* __restrict__ Q
 }
 
 // out = in - Q(:, 0:j) h (h complex, on device); optional ||out||^2 partials per workgroup.
+// With `alpha` set, h_{j-1} += alpha and h_{j-2} += beta (j >= 2): out = in - alpha q_{j-1}
+// - beta q_{j-2} - Q h, the three-term update and the sweep in one pass.
 static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
                                                            const double* hre, const double* him,
                                                            const cpx<double>* in, cpx<double>* out,
-                                                           long long n, double* norm_partials, Skip sk) {
+                                                           long long n, double* norm_partials, Skip sk,
+                                                           const double* alpha = nullptr, double beta = 0.0) {
   __shared__ cpx<double> hs[kLzMaxP];
   __shared__ double red[256];
   if (skipped(sk, 0)) return;
-  for (int k = threadIdx.x; k < j; k += blockDim.x) hs[k] = mk(hre[k], him[k]);
+  for (int k = threadIdx.x; k < j; k += blockDim.x) {
+    cpx<double> h = mk(hre[k], him[k]);
+    if (alpha != nullptr) {
+      if (k == j - 1) h.re += *alpha;
+      if (k == j - 2) h.re += beta;
+    }
+    hs[k] = h;
+  }
   __syncthreads();
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -703,27 +749,6 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
     s = block_sum<256>(s, red);
     if (threadIdx.x == 0) norm_partials[blockIdx.x] = s;
   }
-}
-
-// Three-term Lanczos update r = w - alpha q_j - beta q_{j-1} (alpha = *alpha on the device,
-// qjm1 null on the first step) with ||r||^2 per workgroup
-// (Lanczos_hybrid_identity_gpuop.m:58-62).
-static __global__ __launch_bounds__(256) void k_lz_three_term(const cpx<double>* w, const cpx<double>* qj,
-                                                             const cpx<double>* qjm1, const double* alpha,
-                                                             double beta, cpx<double>* r, long long n,
-                                                             double* partials) {
-  __shared__ double red[256];
-  const double a = *alpha;
-  double s = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    cpx<double> v = w[i] - a * qj[i];
-    if (qjm1) v = v - beta * qjm1[i];
-    r[i] = v;
-    s += v.re * v.re + v.im * v.im;
-  }
-  s = block_sum<256>(s, red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 // Norm bookkeeping of the re-orthogonalisation (one workgroup).  sc[1] = ||r||^2 of the

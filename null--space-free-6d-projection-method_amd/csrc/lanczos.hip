@@ -74,7 +74,8 @@ struct Basis {
   cpx<double>* Qalt = nullptr;  // second n x p basis for out-of-place restarts (p > 64)
   double* sc = nullptr;         // device scalars: [0] alpha, [1] beta2, [2] ||r||^2 before the
                                 // sweep, [4] DGKS flag, then h, h2 (re, im) of kLzMaxP each
-  double* part = nullptr;       // partials
+  double* part = nullptr;       // partials: h (re, im) rows of nb each for kLzMaxP columns,
+                                // then nb of ||r||^2
 };
 
 constexpr int kScH = 8, kScHim = 8 + kLzMaxP;
@@ -97,15 +98,21 @@ static void dot_dev(long long n, const cpx<double>* a, const cpx<double>* b, dou
 constexpr int kScH2 = 8 + 2 * kLzMaxP, kScH2im = 8 + 3 * kLzMaxP;
 constexpr int kGemvKB = 4;
 // With `sk` set the projection is skipped (and not timed: its launch does no work).
+// With `tt` set, v is w and the projection is of the three-term update w - alpha q_{jj-1} -
+// beta q_{jj-2} (||.||^2 partials to tt->norm).
 static void gemv_c(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* v, double* dre,
-                   double* dim, hipStream_t s, Skip sk = no_skip()) {
+                   double* dim, hipStream_t s, Skip sk = no_skip(), const ThreeTerm* tt = nullptr) {
   const int nb = red_blocks(n);
   const int nch = (jj + kGemvKB - 1) / kGemvKB;
   double* pre = B.part;
   double* pim = B.part + (size_t)nb * kLzMaxP;
   if (sk.p == nullptr) timer_begin(m, TS_GEMV_C);
-  launch_k(m, k_gemv_c_all<kGemvKB>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre, pim,
-           sk);
+  if (tt)
+    launch_k(m, k_gemv_c_all<kGemvKB, true>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre,
+             pim, sk, *tt);
+  else
+    launch_k(m, k_gemv_c_all<kGemvKB, false>, dim3(nb * nch), dim3(kRedThreads), s, B.Q, n, jj, v, n, nb, pre,
+             pim, sk, ThreeTerm{nullptr, 0.0, nullptr});
   if (sk.p == nullptr) timer_end(m, (jj + 1) * 16.0 * n);
   k_reduce_final_ri<<<2 * jj, kRedThreads, 0, s>>>(pre, pim, nb, jj, dre, dim, sk);
   F6_CHECK_LAUNCH();
@@ -118,12 +125,12 @@ static void cgs2_step(fame6d_mass* m, long long n, const Basis& B, int jj, const
   gemv_c(m, n, B, jj, w, B.sc + kScH, B.sc + kScHim, s);
   timer_begin(m, TS_GEMV_N);
   launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n,
-           nullptr, no_skip());
+           nullptr, no_skip(), nullptr, 0.0);
   timer_end(m, (jj + 2) * vb);
   gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s);
   timer_begin(m, TS_GEMV_N);
   launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n,
-           B.part, no_skip());
+           B.part, no_skip(), nullptr, 0.0);
   timer_end(m, (jj + 2) * vb);
   k_reduce_final<<<1, kRedThreads, 0, s>>>(B.part, gn, B.sc + 1);
   F6_CHECK_LAUNCH();
@@ -142,15 +149,18 @@ static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int j
   const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
   const double vb = 16.0 * n;
   const cpx<double>* qj = B.Q + (size_t)n * (jj - 1);
-  const cpx<double>* qjm1 = jj >= 2 ? B.Q + (size_t)n * (jj - 2) : nullptr;
   dot_dev(n, qj, w, B.part, B.sc, nullptr, s);
-  k_lz_three_term<<<gn, 256, 0, s>>>(w, qj, qjm1, B.sc, beta, B.r, n, B.part);
-  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 0);
+  // the three-term update is formed inside the first sweep: Q^H r from w and the two newest
+  // columns (with ||r||^2 for the DGKS test), then r = w - Q (h + alpha e_j + beta e_{j-1})
+  const int nb = red_blocks(n);
+  double* npart = B.part + (size_t)2 * nb * kLzMaxP;
+  const ThreeTerm tt{B.sc, beta, npart};
+  gemv_c(m, n, B, jj, w, B.sc + kScH, B.sc + kScHim, s, no_skip(), &tt);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, npart, nb, 0);
   F6_CHECK_LAUNCH();
-  gemv_c(m, n, B, jj, B.r, B.sc + kScH, B.sc + kScHim, s);
   timer_begin(m, TS_GEMV_N);
-  launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, B.r, B.r, n,
-           B.part, no_skip());
+  launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n,
+           B.part, no_skip(), (const double*)B.sc, beta);
   timer_end(m, (jj + 2) * vb);
   k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1);
   F6_CHECK_LAUNCH();
@@ -313,7 +323,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   const int nb = red_blocks(n2);
   DevBuf basis, scal, part, io, ubuf;
   const int cols = p + 2 + (p > 64 ? p : 0);  // Q, v, r (+ the second basis of wide restarts)
-  const size_t part_per = (size_t)std::max(nb * 2 * kLzMaxP, 2048);
+  const size_t part_per = (size_t)std::max(nb * 2 * kLzMaxP, 2048) + nb;
   basis.ensure(sizeof(cpx<double>) * (size_t)n2 * cols * G);
   scal.ensure(sizeof(double) * kScStride * G);
   part.ensure(sizeof(double) * part_per * G);
