@@ -181,6 +181,11 @@ void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long 
 template <class T>
 DevBuf& work_buf(fame6d_mass* m);
 
+// y = M_s t with t the two outputs of the modewise step `op` (PRE_V1H or PRE_ZW) of each of
+// ngroups groups: the step runs in the M_s apply's first pass.
+void mass_apply_pre(fame6d_mass* m, int op, int ngroups, const PassAArgs<double>& a, cpx<double>* y,
+                    long long ldy, hipStream_t s);
+
 // Small transfers through the page-locked staging of m.  d2h_sync synchronises the
 // stream and recycles the staging; h2d_async returns at once.
 void d2h_sync(fame6d_mass* m, void* host, const void* dev, size_t bytes);

@@ -31,22 +31,6 @@ __host__ inline int red_blocks(long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;         \
        i += (long long)gridDim.x * blockDim.x)
 
-// r = V1^{-*} q   (solveV1H_blockdiag.m:7-8)
-template <class T>
-__global__ void k_v1h_solve(const cpx<double>* q, long long ldq, long long n, const double* V11,
-                            const double* V12, const double* V21, const double* V22, cpx<T>* r,
-                            long long ldr, Skip skip, const int* qmap) {
-  F6_MODEWISE_LOOP {
-    const long long j = qs + i;
-    const double det = V11[j] * V22[j] - V12[j] * V21[j];
-    const cpx<double> x1 = q[g * ldq + i], x2 = q[g * ldq + n + i];
-    const cpx<double> y1 = (1.0 / det) * (V22[j] * x1 - V21[j] * x2);
-    const cpx<double> y2 = (1.0 / det) * (V11[j] * x2 - V12[j] * x1);
-    r[g * ldr + i] = cvt<T>(y1);
-    r[g * ldr + n + i] = cvt<T>(y2);
-  }
-}
-
 // bhat = -invC .* (W1 .* mr1 + W2 .* mr2)   (make_explicit_krinv_compact.m:34-35)
 template <class T>
 __global__ void k_bhat(const cpx<T>* mr, long long ldm, long long n, const T* W1, const T* W2,
@@ -55,18 +39,6 @@ __global__ void k_bhat(const cpx<T>* mr, long long ldm, long long n, const T* W1
     const long long j = qs + i;
     const cpx<T> s = W1[j] * mr[g * ldm + i] + W2[j] * mr[g * ldm + n + i];
     bhat[g * ldb + i] = (-invC[j]) * s;
-  }
-}
-
-// z = invC .* u; out = [W1 .* z; W2 .* z]   (make_explicit_krinv_compact.m:60-62)
-template <class T>
-__global__ void k_zw(const cpx<T>* u, long long ldu, long long n, const T* W1, const T* W2,
-                     const T* invC, cpx<T>* out, long long ldo, Skip skip, const int* qmap) {
-  F6_MODEWISE_LOOP {
-    const long long j = qs + i;
-    const cpx<T> z = invC[j] * u[g * ldu + i];
-    out[g * ldo + i] = W1[j] * z;
-    out[g * ldo + n + i] = W2[j] * z;
   }
 }
 

@@ -97,6 +97,14 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
     auto go = [&](auto bc) {
       constexpr int BPW = decltype(bc)::value;
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
+      if (op == PRE_V1H || op == PRE_ZW) {  // FP64 only, coalesced form
+        if constexpr (std::is_same<T, double>::value) {
+          if (op == PRE_V1H) launch_k(m, k_pass_a_lin<T, n, BPW, PRE_V1H>, grid, block, s, a);
+          else launch_k(m, k_pass_a_lin<T, n, BPW, PRE_ZW>, grid, block, s, a);
+          return;
+        }
+        throw Error(FAME6D_EARG, "bad pass-A op");
+      }
       if (tile_of<T>(m).io == 1) {
         switch (op) {
           case PRE_COPY: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_COPY>, grid, block, s, a); break;
@@ -239,6 +247,29 @@ template <class T>
 DevBuf& work_buf(fame6d_mass* m) {
   if constexpr (std::is_same<T, double>::value) return m->work64;
   else return m->work32;
+}
+
+// y = M_s t for the two outputs t of a PRE_V1H or PRE_ZW pass A per group (2 * ngroups
+// vectors at y + v*ldy, group g's at v = 2g, 2g + 1).  `a` carries the op's inputs.
+void mass_apply_pre(fame6d_mass* m, int op, int ngroups, const PassAArgs<double>& a_in, cpx<double>* y,
+                    long long ldy, hipStream_t s) {
+  if (ngroups <= 0) return;
+  if (op != PRE_V1H && op != PRE_ZW) throw Error(FAME6D_EARG, "bad fused M_s apply");
+  const int nrhs = 2 * ngroups;
+  DevBuf& w = work_buf<double>(m);
+  w.ensure(sizeof(cpx<double>) * (size_t)m->NF * (size_t)nrhs);
+  cpx<double>* t = w.as<cpx<double>>();
+  PassAArgs<double> a = a_in;
+  a.t = t;
+  a.ldt = m->NF;
+  launch_pass_a<double>(m, op, ngroups, a, s);
+  launch_pass_b<double>(m, 1, 0, nrhs, t, m->NF, no_skip(), s);
+  PassCArgs<double> c{};
+  c.t = t;
+  c.ldt = m->NF;
+  c.y = y;
+  c.ldy = ldy;
+  launch_pass_c<double>(m, POST_COPY, nrhs, c, s);
 }
 
 template <class T>

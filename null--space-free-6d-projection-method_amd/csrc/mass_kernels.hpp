@@ -74,7 +74,15 @@ enum PreOp : int {
   PRE_COPY = 0,     // t = x                                  (1 output)
   PRE_MHAT = 1,     // z = invC*x; t = [z, W1 z, W2 z]        (3 outputs)
   PRE_MHAT_CG = 2,  // p = r + beta p (p = r first); then as PRE_MHAT on p
+  // the modewise steps of K_r^{-1} around its Mhat solve, folded into the M_s apply that
+  // follows each (FP64 only; group g reads x + g*ldx):
+  PRE_V1H = 3,  // t = V1^{-*} [x1; x2] (x2 at x1 + NF)      (2 outputs)
+  PRE_ZW = 4,   // z = invC*x; t = [W1 z, W2 z]             (2 outputs)
 };
+template <int OP>
+constexpr int pre_outputs() {
+  return OP == PRE_COPY ? 1 : (OP == PRE_V1H || OP == PRE_ZW) ? 2 : 3;
+}
 enum PostOp : int {
   POST_COPY = 0,     // y = s
   POST_MHAT = 1,     // y = invC*(s0 + W1 s1 + W2 s2)  [W real]
@@ -103,6 +111,11 @@ struct PassAArgs {
   CgState* st_next;
   const double* rr;
   int nrr;
+  // PRE_V1H: the 2x2 blocks [V11 V12; V21 V22] of group g at V11 + g*ldq, ...
+  const double* V11;
+  const double* V12;
+  const double* V21;
+  const double* V22;
 };
 
 template <class T>
@@ -239,10 +252,30 @@ __device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, do
     return true;
   }
   const int tid = threadIdx.x;
+  const double* rr = a.rr + (long long)g * a.nrr;
   double v = 0.0;
-  if (tid < kCgSumThreads)
-    for (int i = tid; i < a.nrr; i += kCgSumThreads) v += a.rr[(long long)g * a.nrr + i];
-  v = block_sum<THREADS>(v, red);
+  if constexpr (THREADS >= kCgSumThreads) {
+    if (tid < kCgSumThreads)
+      for (int i = tid; i < a.nrr; i += kCgSumThreads) v += rr[i];
+    v = block_sum<THREADS>(v, red);
+  } else {
+    // fewer threads than k_cg_beta has: each takes the lanes tid, tid + THREADS, ... of a
+    // kCgSumThreads-wide sum (in LDS of their own: red may hold fewer doubles), and the
+    // first wave adds them up as block_sum does
+    static_assert(THREADS >= 64, "one full wave");
+    __shared__ double lane[kCgSumThreads];
+    for (int l = tid; l < kCgSumThreads; l += THREADS) {
+      double u = 0.0;
+      for (int i = l; i < a.nrr; i += kCgSumThreads) u += rr[i];
+      lane[l] = u;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      for (int i = tid; i < kCgSumThreads; i += 64) v += lane[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    }
+  }
   if (tid == 0) {
     const CgState* o = a.st + g;
     const bool was_done = o->done != 0;
@@ -346,7 +379,7 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a(PassAArgs<T> a) {
 template <class T, int n, int BPW, int OP>
 __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
   constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
-  constexpr int NOUT = (OP == PRE_COPY) ? 1 : 3;
+  constexpr int NOUT = pre_outputs<OP>();
   constexpr int THREADS = BPW * P2;
   __shared__ cpx<T> lds[BPW * S];
   const int g = blockIdx.y;
@@ -362,6 +395,11 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
   cpx<T> pn[n];
 #pragma unroll
   for (int k = 0; k < n; ++k) pn[k] = xg[P2 * k];
+  cpx<T> pn2[OP == PRE_V1H ? n : 1];  // second component (x2) for PRE_V1H
+  if constexpr (OP == PRE_V1H) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) pn2[k] = xg[(long long)S * S + P2 * k];
+  }
   if constexpr (OP == PRE_MHAT_CG) {
     cpx<T>* pg = a.p + g * a.ldx + off + pp;
     if (a.st[g].iter > 0) {
@@ -379,6 +417,20 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
     if constexpr (OP == PRE_COPY) {
 #pragma unroll
       for (int k = 0; k < n; ++k) u[k] = pn[k];
+    } else if constexpr (OP == PRE_V1H) {
+      // y1 = (V22 x1 - V21 x2) / det, y2 = (V11 x2 - V12 x1) / det (solveV1H_blockdiag.m:7-8)
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        const long long j = qoff + P2 * k;
+        const double v11 = a.V11[j], v12 = a.V12[j], v21 = a.V21[j], v22 = a.V22[j];
+        const double det = v11 * v22 - v12 * v21;
+        u[k] = (o == 0) ? (1.0 / det) * (v22 * pn[k] - v21 * pn2[k]) : (1.0 / det) * (v11 * pn2[k] - v12 * pn[k]);
+      }
+    } else if constexpr (OP == PRE_ZW) {
+      // z = invC u, [W1 z; W2 z] (make_explicit_krinv_compact.m:60-62)
+      const T* wo = (o == 0) ? a.W1 : a.W2;
+#pragma unroll
+      for (int k = 0; k < n; ++k) u[k] = wo[qoff + P2 * k] * (a.invC[qoff + P2 * k] * pn[k]);
     } else {
       const T* wo = (o == 1) ? a.W1 : a.W2;
 #pragma unroll

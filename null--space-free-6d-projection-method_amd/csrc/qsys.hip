@@ -428,11 +428,21 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   const Skip none = no_skip();
   const int gridm = grid_for(NF);
 
-  // r = V1^{-*} q; mr = M_s r (both families)
-  k_v1h_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(qin, 2 * NF, NF, S->V(0), S->V(1), S->V(2),
-                                                         S->V(3), w.r, 2 * NF, none, qmap);
-  F6_CHECK_LAUNCH();
-  mass_apply<double>(m, w.r, NF, w.mr, NF, 2 * nact, s);
+  // mr = M_s r with r = V1^{-*} q (both families), r formed in the apply's first pass
+  PassAArgs<double> pa{};
+  pa.x = qin;
+  pa.ldx = 2 * NF;
+  pa.V11 = S->V(0);
+  pa.V12 = S->V(1);
+  pa.V21 = S->V(2);
+  pa.V22 = S->V(3);
+  pa.W1 = S->W64(0);
+  pa.W2 = S->W64(1);
+  pa.invC = S->W64(2);
+  pa.ldq = NF;
+  pa.skip = none;
+  pa.qmap = qmap;
+  mass_apply_pre(m, PRE_V1H, nact, pa, w.mr, NF, s);
   // bhat = -invC (W1 mr1 + W2 mr2)
   k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
                                                     w.bhat, NF, none, qmap);
@@ -534,11 +544,11 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     S->stats.last_flag = o.flag;
     S->stats.last_relres = o.relres;
   }
-  // z = invC u; s = M_s [W1 z; W2 z]; y = V1^{-1}(mr + s)
-  k_zw<double><<<dim3(gridm, nact), 256, 0, s>>>(w.u, NF, NF, S->W64(0), S->W64(1), S->W64(2), w.s,
-                                                  2 * NF, none, qmap);
-  F6_CHECK_LAUNCH();
-  mass_apply<double>(m, w.s, NF, w.s, NF, 2 * nact, s);
+  // z = invC u; s = M_s [W1 z; W2 z] (z and W z formed in the apply's first pass);
+  // y = V1^{-1}(mr + s)
+  pa.x = w.u;
+  pa.ldx = NF;
+  mass_apply_pre(m, PRE_ZW, nact, pa, w.s, NF, s);
   k_v1_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, w.s, 2 * NF, NF, S->V(0), S->V(1),
                                                         S->V(2), S->V(3), yout, 2 * NF, none, qmap);
   F6_CHECK_LAUNCH();

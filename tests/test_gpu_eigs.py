@@ -117,6 +117,42 @@ def test_krinv_pass_c_variants_n5(n5):
         n5["m"].set_tiles("single", 0, 0)
 
 
+@pytest.fixture(scope="module")
+def n4(pkg, oracle):
+    torch.cuda.set_device(0)
+    N = 4
+    eps = oracle.eps_reference(N)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, Q_EXP1)
+    yield dict(N=N, eps=eps, m=m, s=s)
+    s.close()
+    m.close()
+
+
+def test_krinv_pass_a_fold_small_workgroups_n4(n4):
+    """From the second iteration of a chunk on, pass A applies the previous pcg step, summing
+    the pass-C ||r||^2 partials in k_cg_beta's order.  At N=4 with one block per workgroup
+    pass A has 192 threads and pass C writes 512 partials, more than one per thread: the
+    iteration counts and solutions agree with those of the 384-thread forms."""
+    print("case: krinv N=4 pass-A fold with 192-thread workgroups", flush=True)
+    rng = np.random.default_rng(14)
+    m = n4["m"]
+    q = torch.as_tensor(rand_c(rng, 2 * m.NF), device="cuda")
+    try:
+        m.set_tiles("double", 0, 0)
+        y0, st0 = n4["s"].krinv(q, 1e-10, 1200)
+        assert st0["last_flag"] == 0
+        # 36: one block per workgroup in passes A and C (512 pass-C partials); + 64: persistent
+        # pass A; + 256: persistent pass A sharing its input through LDS; + 8: persistent pass C
+        for code in (36, 36 + 64, 36 + 256 + 64, 36 + 8, 4):
+            m.set_tiles("double", code, 0)
+            y, st = n4["s"].krinv(q, 1e-10, 1200)
+            assert st["last_flag"] == 0 and abs(st["last_iterations"] - st0["last_iterations"]) <= 1, code
+            assert (torch.linalg.norm(y - y0) / torch.linalg.norm(y0)).item() < 1e-9, code
+    finally:
+        m.set_tiles("double", 0, 0)
+
+
 def test_krinv_fp64_and_mixed(n2, oracle):
     print("case: krinv N=2", flush=True)
     rng = np.random.default_rng(2)
