@@ -343,7 +343,8 @@ FAMILY_DESC = {
     "mhat_pass_b": "Mhat pass B (inverse DFT dims 4-6, *eps/NF, forward DFT dims 6-4, p'q partials; 3 vectors): "
                    "NF*(6*16 + 8) bytes per launch",
     "mhat_pass_c": "Mhat pass C in PCG (forward DFT dims 3-1, q = invC(s0 + W1 s1 + W2 s2) in registers, "
-                   "alpha, x += alpha p, r -= alpha q, ||r||^2 partials): NF*(8*16 + 3*8) bytes per launch",
+                   "alpha, x += alpha p, r -= alpha q, ||r||^2 partials; invC recomputed from W1, W2): "
+                   "NF*(8*16 + 2*8) bytes per launch",
     "lanczos_gemv_c": "Lanczos CGS2 sweep h = Q(:,1:j)^H w: (j+1)*2NF*16 bytes per sweep",
     "lanczos_gemv_n": "Lanczos CGS2 sweep r = w - Q(:,1:j) h (+||r||^2): (j+2)*2NF*16 bytes per sweep",
 }

@@ -23,6 +23,7 @@
 #pragma once
 
 #include <cstddef>
+#include <type_traits>
 
 #include "common.hpp"
 #include "dft.hpp"
@@ -295,6 +296,33 @@ __device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, do
   beta = static_cast<T>(red[2]);
   __syncthreads();  // red is LDS the caller reuses
   return !done;
+}
+
+// invC = 1/sqrt(1 + W1^2 + W2^2) (build_experiment1_q_system_C_compact.m:26-31) recomputed
+// from W1 and W2 with the operations, order and rounding of the host's table (no
+// contraction; IEEE square root and division), so it equals the table entry: the FP64 M-hat
+// pass C, which has W1 and W2 in hand, reads 8 bytes per mode less.  The FP32 passes read
+// their table, whose entries are the FP64 values rounded.  (Pass A keeps the table: there
+// each of its three thread groups would load both W and repeat the square root, which
+// measured no faster.)
+__device__ __forceinline__ double inv_c_from(double w1, double w2) {
+#pragma clang fp contract(off)
+  const double korth = 1.0 + w1 * w1 + w2 * w2;
+  return 1.0 / sqrt(korth);
+}
+// Weight of output o of the M-hat pre-op at diagonal index q: invC, invC W1, invC W2.
+template <class T>
+__device__ __forceinline__ T mhat_weight(const T* invC, const T* W1, const T* W2, int o, long long q) {
+  T w = invC[q];
+  if (o > 0) w *= (o == 1 ? W1 : W2)[q];
+  return w;
+}
+
+// invC at q for the post-op: recomputed from the W1, W2 values in hand in FP64, read in FP32.
+template <class T>
+__device__ __forceinline__ T inv_c_or(const T* invC, T w1, T w2, long long q) {
+  if constexpr (std::is_same<T, double>::value) return inv_c_from(w1, w2);
+  else return invC[q];
 }
 
 // ---------------------------------------------------------------- pass A
@@ -766,14 +794,9 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3(PassAArgs<T> a) {
     }
   }
   const long long qoff = qslot(a.qmap, g) * a.ldq + off + pp;
-  const T* wo = (o == 1) ? a.W1 : a.W2;
   cpx<T> u[n];
 #pragma unroll
-  for (int k = 0; k < n; ++k) {
-    T w = a.invC[qoff + P2 * k];
-    if (o > 0) w *= wo[qoff + P2 * k];
-    u[k] = w * pn[k];
-  }
+  for (int k = 0; k < n; ++k) u[k] = mhat_weight(a.invC, a.W1, a.W2, o, qoff + P2 * k) * pn[k];
   dft<T, n, +1>(u);
 #pragma unroll
   for (int k = 0; k < n; ++k) L[pp + P2 * k] = u[k];
@@ -833,7 +856,6 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
   const cpx<T>* xsrc = a.x + g * a.ldx + boff;
   cpx<T>* psrc = CG ? a.p + g * a.ldx + boff : nullptr;
   const long long qg = qslot(a.qmap, g) * a.ldq + boff;
-  const T* wo = (o == 1) ? a.W1 : a.W2;
   v2 px[n], pq[n];
   auto fetch = [&](int tile) {
     const long long to = (long long)tile * E;
@@ -850,10 +872,7 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3p(PassAArgs<T> a) {
     const long long to = (long long)tile * E;
     T w[n];
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-      w[k] = a.invC[qg + to + P2 * k];
-      if (o > 0) w[k] *= wo[qg + to + P2 * k];
-    }
+    for (int k = 0; k < n; ++k) w[k] = mhat_weight(a.invC, a.W1, a.W2, o, qg + to + P2 * k);
     cpx<T> u[n];
 #pragma unroll
     for (int k = 0; k < n; ++k) u[k] = mk(px[k].x, px[k].y);
@@ -936,7 +955,6 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3s(PassAArgs<T> a) {
   const cpx<T>* xsrc = a.x + g * a.ldx;
   cpx<T>* pdst = CG ? a.p + g * a.ldx : nullptr;
   const long long qg = qslot(a.qmap, g) * a.ldq + (long long)blk * S + pp;
-  const T* wo = (o == 1) ? a.W1 : a.W2;
   v2 px[NE], pq[NE];
   auto fetch = [&](int tile) {
     const long long to = (long long)tile * E;
@@ -955,10 +973,7 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_a3s(PassAArgs<T> a) {
     const long long to = (long long)tile * E;
     T w[n];
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-      w[k] = a.invC[qg + to + P2 * k];
-      if (o > 0) w[k] *= wo[qg + to + P2 * k];
-    }
+    for (int k = 0; k < n; ++k) w[k] = mhat_weight(a.invC, a.W1, a.W2, o, qg + to + P2 * k);
 #pragma unroll
     for (int r = 0; r < NE; ++r) {
       const int e = tid + THREADS * r;
@@ -1025,7 +1040,8 @@ __device__ __forceinline__ void mhat_c_epilogue(const PassCArgs<T>& a, const cpx
     const long long xo = g * a.ldx + wg0;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
-      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      const T w1 = a.W1[q], w2 = a.W2[q];
+      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
       a.x[xo + e] = a.x[xo + e] + al * a.p[xo + e];
       const cpx<T> rn = a.r[xo + e] - al * v;
       a.r[xo + e] = rn;
@@ -1036,7 +1052,8 @@ __device__ __forceinline__ void mhat_c_epilogue(const PassCArgs<T>& a, const cpx
     const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
-      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      const T w1 = a.W1[q], w2 = a.W2[q];
+      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
       yg[e] = v;
       if (dg) {
         const cpx<T> d = dg[e];
@@ -1126,7 +1143,8 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
     const long long xo = g * a.ldx + wg0;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
-      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      const T w1 = a.W1[q], w2 = a.W2[q];
+      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
       a.x[xo + e] = a.x[xo + e] + al * a.p[xo + e];
       const cpx<T> rn = a.r[xo + e] - al * v;
       a.r[xo + e] = rn;
@@ -1137,7 +1155,8 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
     const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
-      const cpx<T> v = a.invC[q] * (lds[e] + a.W1[q] * lds[E + e] + a.W2[q] * lds[2 * E + e]);
+      const T w1 = a.W1[q], w2 = a.W2[q];
+      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
       yg[e] = v;
       if (dg) {
         const cpx<T> d = dg[e];

@@ -313,7 +313,8 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
   const double cs = sizeof(cpx<T>), rs = sizeof(T), gb = (double)NF * nact;
   const double bytesA = gb * (6 * cs + 3 * rs);  // zr, p, invC, W1, W2 in; p, 3 vectors out
   const double bytesB = gb * (6 * cs + rs);      // 3 vectors + eps in; 3 vectors out
-  const double bytesC = gb * (8 * cs + 3 * rs);  // 3 vectors, invC, W1, W2, x, p, r in; x, r out
+  // 3 vectors, W1, W2 (and in FP32 invC; FP64 recomputes it from W1, W2), x, p, r in; x, r out
+  const double bytesC = gb * (8 * cs + (f32 ? 3 : 2) * rs);
   const int timer_first = m->timer.pending;
 
   int chunk = hint_ref > 0 ? hint_ref : 8;
