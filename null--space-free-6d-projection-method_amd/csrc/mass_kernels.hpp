@@ -446,13 +446,20 @@ __global__ __launch_bounds__(BPW* n* n) void k_pass_a_lin(PassAArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < n; ++k) u[k] = pn[k];
     } else if constexpr (OP == PRE_V1H) {
-      // y1 = (V22 x1 - V21 x2) / det, y2 = (V11 x2 - V12 x1) / det (solveV1H_blockdiag.m:7-8)
+      // y1 = (V22 x1 - V21 x2) / det, y2 = (V11 x2 - V12 x1) / det (solveV1H_blockdiag.m:7-8);
+      // both at the first output, y2 kept in pn2 for the second (V is read once)
+      if (o == 0) {
 #pragma unroll
-      for (int k = 0; k < n; ++k) {
-        const long long j = qoff + P2 * k;
-        const double v11 = a.V11[j], v12 = a.V12[j], v21 = a.V21[j], v22 = a.V22[j];
-        const double det = v11 * v22 - v12 * v21;
-        u[k] = (o == 0) ? (1.0 / det) * (v22 * pn[k] - v21 * pn2[k]) : (1.0 / det) * (v11 * pn2[k] - v12 * pn[k]);
+        for (int k = 0; k < n; ++k) {
+          const long long j = qoff + P2 * k;
+          const double v11 = a.V11[j], v12 = a.V12[j], v21 = a.V21[j], v22 = a.V22[j];
+          const double det = v11 * v22 - v12 * v21;
+          u[k] = (1.0 / det) * (v22 * pn[k] - v21 * pn2[k]);
+          pn2[k] = (1.0 / det) * (v11 * pn2[k] - v12 * pn[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < n; ++k) u[k] = pn2[k];
       }
     } else if constexpr (OP == PRE_ZW) {
       // z = invC u, [W1 z; W2 z] (make_explicit_krinv_compact.m:60-62)
