@@ -143,6 +143,7 @@ struct PassCArgs {
   long long ldq;
   const cpx<T>* dotp;  // if set: partial sums of Re(dotp^H y) per workgroup
   long long lddot;
+  const cpx<T>* resb;  // POST_MHAT, if set: y = resb - Mhat x (same layout as y), ||y||^2 partials
   double* partials;  // partials[g*gridDim.x + blockIdx.x]
   Skip skip;
   const int* qmap;
@@ -1056,11 +1057,16 @@ __device__ __forceinline__ void mhat_c_epilogue(const PassCArgs<T>& a, const cpx
     }
   } else {
     cpx<T>* yg = a.y + g * a.ldy + wg0;
-    const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
+    const cpx<T>* dg = (a.partials && a.dotp) ? a.dotp + g * a.lddot + wg0 : nullptr;
+    const cpx<T>* bg = a.resb ? a.resb + g * a.ldy + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
       const T w1 = a.W1[q], w2 = a.W2[q];
-      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      if (bg) {  // the residual b - Mhat x and its ||.||^2 partials
+        v = bg[e] - v;
+        s += static_cast<double>(v.re) * v.re + static_cast<double>(v.im) * v.im;
+      }
       yg[e] = v;
       if (dg) {
         const cpx<T> d = dg[e];
@@ -1159,11 +1165,16 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
     }
   } else {
     cpx<T>* yg = a.y + g * a.ldy + wg0;
-    const cpx<T>* dg = a.partials ? a.dotp + g * a.lddot + wg0 : nullptr;
+    const cpx<T>* dg = (a.partials && a.dotp) ? a.dotp + g * a.lddot + wg0 : nullptr;
+    const cpx<T>* bg = a.resb ? a.resb + g * a.ldy + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
       const T w1 = a.W1[q], w2 = a.W2[q];
-      const cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      if (bg) {  // the residual b - Mhat x and its ||.||^2 partials
+        v = bg[e] - v;
+        s += static_cast<double>(v.re) * v.re + static_cast<double>(v.im) * v.im;
+      }
       yg[e] = v;
       if (dg) {
         const cpx<T> d = dg[e];

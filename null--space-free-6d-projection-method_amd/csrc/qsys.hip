@@ -165,7 +165,8 @@ static const T* Wt(const fame6d_qsys* S, int i) {
 
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
-                hipStream_t s, bool inverse_eps, double* energy_partials) {
+                hipStream_t s, bool inverse_eps, double* energy_partials, const cpx<T>* resb,
+                double* res_partials) {
   fame6d_mass* m = S->m;
   const long long NF = m->NF;
   DevBuf& w = work_buf<T>(m);
@@ -195,6 +196,8 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   c.ldq = NF;
   c.skip = skip;
   c.qmap = qmap;
+  c.resb = resb;
+  c.partials = resb ? res_partials : nullptr;
   launch_pass_c<T>(m, POST_MHAT, nact, c, s);
 }
 
@@ -356,10 +359,9 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       // true residual r = b - Mhat x of the groups that stopped on the recurrence
       k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
       const Skip sk{need, 1};
-      mhat_apply<T>(S, nact, qmap_dev, x, q, sk, s);
-      k_residual<T><<<dim3(grid_for(NF), nact), 256, 0, s>>>(b, q, r, NF, NF, sk);
-      k_dot_partials<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(r, NF, r, NF, NF, part2, nullptr, sk);
-      k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, need, resume, precond ? 1 : 0);
+      // r = b - Mhat x and its ||r||^2 partials in the last pass of the apply
+      mhat_apply<T>(S, nact, qmap_dev, x, r, sk, s, false, nullptr, b, part2);
+      k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, need, resume, precond ? 1 : 0);
       if (precond) {
         // z = P^{-1} r_true and rho = r'z for the groups that resume
         mhat_apply<T>(S, nact, qmap_dev, r, z, Skip{resume, 1}, s, true, part);
@@ -473,7 +475,10 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     S->ref.ensure(sizeof(RefState) * nact);
     RefState* rs = S->ref.as<RefState>();
     const int nbD = red_blocks(NF);
-    S->part2.ensure(sizeof(double) * ((size_t)nbD * nact + 8));
+    // sized for every user up front (the FP64 residual's pass-C partials, the inner FP32
+    // solves' own), so no later ensure moves the buffer under this pointer
+    const int nbP = std::max({nbD, pass_c_blocks(m, false), pass_c_blocks(m, true)});
+    S->part2.ensure(sizeof(double) * ((size_t)nbP * nact + 8));
     double* part2 = S->part2.as<double>();
     k_cg_start<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, w.u, w.res, NF, part2);  // u = 0, res = bhat
     k_ref_init<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD);
@@ -509,10 +514,9 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       }
       k_accumulate_ref<float><<<dim3(gridm, nact), 256, 0, s>>>(NF, x32, NF, rs, w.u, NF);
       // FP64 residual of every group still refining
-      mhat_apply<double>(S, nact, qmap, w.u, w.y, skr, s);
-      k_residual<double><<<dim3(gridm, nact), 256, 0, s>>>(w.bhat, w.y, w.res, NF, NF, skr);
-      k_dot_partials<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.res, NF, w.res, NF, NF, part2, nullptr, skr);
-      k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD, tol);
+      // res = bhat - Mhat u and its ||res||^2 partials in the last pass of the apply
+      mhat_apply<double>(S, nact, qmap, w.u, w.res, skr, s, false, nullptr, w.bhat, part2);
+      k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, pass_c_blocks(m, false), tol);
       F6_CHECK_LAUNCH();
       for (int g = 0; g < nact; ++g)
         if (inner[g].iter > 0) S->stats.mass_applies_fp64 += 3;
@@ -560,9 +564,9 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
 }
 
 template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
-                                 hipStream_t, bool, double*);
+                                 hipStream_t, bool, double*, const cpx<double>*, double*);
 template void mhat_apply<float>(fame6d_qsys*, int, const int*, const cpx<float>*, cpx<float>*, Skip,
-                                hipStream_t, bool, double*);
+                                hipStream_t, bool, double*, const cpx<float>*, double*);
 template void dot_groups<double>(const fame6d_mass*, int, const cpx<double>*, long long,
                                  const cpx<double>*, long long, long long, double*, double*);
 template void dot_groups<float>(const fame6d_mass*, int, const cpx<float>*, long long,
