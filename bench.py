@@ -499,8 +499,9 @@ def probe(torch, pkg, N, tiles):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(Bloch points would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(Bloch points would shard across GPUs, each GPU solving its own points with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import __graft_entry__
