@@ -237,7 +237,7 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 // chunk.  p'q (and r'z) come from pass B as sum eps/NF |G u|^2 (PassBArgs::partials).
 // With `precond` the iteration is preconditioned by P^{-1} = Mhat evaluated with
 // 1/eps (exact inverse for constant eps); the reference's solves are unpreconditioned,
-// and this build uses `precond` only for the FP32 inner solves of the mixed form.
+// and `precond` is used only for the FP32 inner solves of the mixed form.
 // `true_check` false accepts the recurrence residual without the true-residual check (for
 // the inner solves of the mixed form).  `hint` is the first chunk length, updated to the
 // iterations this call needed plus one.
