@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter collection: a 64 MiB device copy (calibration), "
                          "then one FP64 solve of the benchmark configuration")
+    ap.add_argument("--probe-taylor", action="store_true",
+                    help="short run for hardware-counter collection of the Taylor moments contraction: "
+                         "N=4 (NF = 8^6), p = 10 (286 monomials), 216 centers, 4 launches")
     ap.add_argument("--tiles", default=None,
                     help="FP64 tile variants 'ac,b' (fame6d_mass_set_tiles codes) instead of the autotuner; "
                          "counter runs use the choice the timed run printed (its config.tile_variants)")
@@ -496,6 +499,28 @@ def probe(torch, pkg, N, tiles):
                       "outer_steps": out["stats"]["outer_steps"]}), flush=True)
 
 
+def probe_taylor(torch, pkg):
+    """Fixed Taylor-moments workload for profiler runs: the moment GEMM of the reconstruction
+    configuration (N=4, p=10, 6^3 centers) on synthetic projected modes and coefficients."""
+    import numpy as np
+    NF, p, nb = 8 ** 6, 10, 6
+    rng = np.random.default_rng(0)
+    q = torch.as_tensor(rng.uniform(-10, 10, (NF, 3)), device="cuda")
+    coeff = torch.as_tensor(rng.standard_normal(NF) + 1j * rng.standard_normal(NF), device="cuda")
+    cen = rng.uniform(-1, 1, (nb ** 3, 3)) * 0.05
+    L = (p + 1) * (p + 2) * (p + 3) // 6
+    flops = 2.0 * L * NF * 2 * nb ** 3  # complex coefficients against real monomials
+    ts = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pkg.taylor_moments(q, coeff, p, cen)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    print(json.dumps({"probe": "taylor", "NF": NF, "p": p, "monomials": L, "centers": nb ** 3,
+                      "min_ms": min(ts) * 1e3, "TFLOPs": flops / min(ts) / 1e12}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -514,6 +539,9 @@ def main():
     N = a.N
     if a.probe:
         probe(torch, pkg, N, a.tiles)
+        return
+    if a.probe_taylor:
+        probe_taylor(torch, pkg)
         return
     NF = (2 * N) ** 6
     eps, phases = pkg.cosine_material(N, AMP, EPS_SEED)
