@@ -77,6 +77,10 @@ int fame6d_mass_destroy(fame6d_mass* m);
  * checks and optimisation loops over eps.  No reference counterpart (the reference
  * rebuilds the backend, exp1/build_experiment1_common.m:12-18). */
 int fame6d_mass_set_eps(fame6d_mass* m, const double* eps_host);
+/* The same with the NF samples in device memory (float64, column-major): validated and
+ * scaled on the device, so a gradient loop over a device-resident eps moves no samples
+ * through the host.  The permittivity enters the operator as in exp1/applyMij_fft.m:11. */
+int fame6d_mass_set_eps_device(fame6d_mass* m, const double* eps_dev);
 int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
 
 /* Project-written tile autotuner: times every compiled tile variant of the three
@@ -305,6 +309,14 @@ int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, in
  * u = exp(i R q^T) coeff. */
 int fame6d_direct_reconstruct(long long NF, const double* qmodes, const void* coeff, long long M,
                               const double* points, void* u, void* stream);
+
+/* compute_field_errors_exp3 (exp2/compute_field_errors_exp3.m:3-10): relative 2-norm and
+ * max-norm errors of a three-component field against a reference,
+ * e2 = ||t - r||_2 / max(||r||_2, eps), einf = max|t - r| / max(max|r|, eps), the norms taken
+ * over all three components together.  t1..t3, r1..r3: n complex128 each (device);
+ * e2, einf: host scalars (the call synchronises the stream). */
+int fame6d_field_errors(long long n, const void* t1, const void* t2, const void* t3, const void* r1,
+                        const void* r2, const void* r3, double* e2, double* einf, void* stream);
 
 /* ----------------------------------------------------------------- Yee closure */
 

@@ -13,8 +13,8 @@ from .eigs import (QSystem, build_experiment1_q_system_C_compact, compute_fourie
                    krinv_nested, make_explicit_krinv_compact, method_a_top, recover_original_eigenvectors,
                    run_method_A, run_method_B, run_method_C, run_method_C_batch,
                    solve_tracked_modes, solve_tracking_scan)
-from .taylor import (auto_taylor_blocks, block_partition, build_multi_index_list_3d,  # noqa: F401
-                     direct_reconstruct, projected_q_modes, reconstruct_yee_field, taylor_evaluate,
+from .taylor import (auto_taylor_blocks, block_centers, block_partition, build_multi_index_list_3d,  # noqa: F401
+                     compute_field_errors, direct_reconstruct, projected_q_modes, reconstruct_yee_field, taylor_evaluate,
                      taylor_moments, yee_block_centers, yee_points)
 from .yee import (apply_cropped_A, compute_lwrq, compute_yee_lwrq_metrics, crop_plus_field,  # noqa: F401
                   sample_scalar_yee_material)

@@ -113,6 +113,7 @@ SIGNATURES = {
     "fame6d_mass_create": [I, P, P, C.POINTER(P)],
     "fame6d_mass_destroy": [P],
     "fame6d_mass_set_eps": [P, P],
+    "fame6d_mass_set_eps_device": [P, P],
     "fame6d_mass_info": [P, C.POINTER(MassInfo)],
     "fame6d_mass_autotune": [P, I, I],
     "fame6d_mass_set_tiles": [P, I, I, I],
@@ -144,6 +145,7 @@ SIGNATURES = {
     "fame6d_taylor_evaluate": [LL, P, P, I, I, P, P, P, P],
     "fame6d_taylor_evaluate_yee": [I, D, I, I, I, I, P, P, P, P],
     "fame6d_direct_reconstruct": [LL, P, P, LL, P, P, P],
+    "fame6d_field_errors": [LL, P, P, P, P, P, P, P, P, P],
     "fame6d_mass_adjoint_eps": [P, P, P, I, P],
     "fame6d_eig_grad_eps": [P, P, I, P, P, P],
     "fame6d_yee_workspace_bytes": [I],

@@ -21,16 +21,21 @@ from .mass import MassBackend
 class MassApply(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, eps, mass: MassBackend):
-        mass.set_eps(eps.detach().cpu().numpy())
+        # the backend computes with this eps and gets its previous permittivity back afterwards
+        with mass.using_eps(eps.detach()):
+            y = mass.MscalarOp(x.detach())
         ctx.mass = mass
-        ctx.save_for_backward(x)
-        return mass.MscalarOp(x.detach())
+        ctx.save_for_backward(x, eps)
+        return y
 
     @staticmethod
     def backward(ctx, gy):
-        (x,) = ctx.saved_tensors
+        x, eps = ctx.saved_tensors
         mass = ctx.mass
-        gx = mass.MscalarOp(gy.contiguous()) if ctx.needs_input_grad[0] else None
+        gx = None
+        if ctx.needs_input_grad[0]:
+            with mass.using_eps(eps.detach()):  # M_s of the forward's eps, whatever was set since
+                gx = mass.MscalarOp(gy.contiguous())
         geps = None
         if ctx.needs_input_grad[1]:
             geps = mass.adjoint_eps(x.detach().reshape(-1, mass.NF), gy.contiguous().reshape(-1, mass.NF))
@@ -44,26 +49,30 @@ def mass_apply(x, eps, mass):
 class LowestEigenvalues(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps, mass: MassBackend, q, k, opts):
-        mass.set_eps(eps.detach().cpu().numpy())
-        sysq = QSystem(mass, q)
-        out = run_method_C(sysq, k=k, want_vectors=True, **opts)
+        with mass.using_eps(eps.detach()):
+            sysq = QSystem(mass, q)
+            out = run_method_C(sysq, k=k, want_vectors=True, **opts)
         ctx.mass = mass
         ctx.sysq = sysq
         ctx.lam = out["lambda_"].copy()
         ctx.Y = out["Y"]  # checkpoint: reduced Ritz vectors
+        ctx.save_for_backward(eps)
         return torch.as_tensor(out["lambda_"], dtype=torch.float64, device=eps.device)
 
     @staticmethod
     def backward(ctx, glam):
         from . import _lib
         import ctypes as C
+        (eps,) = ctx.saved_tensors
         mass, sysq = ctx.mass, ctx.sysq
-        X, MX, _ = recover_original_eigenvectors(ctx.Y, sysq)
         g = torch.zeros(mass.NF, dtype=torch.float64, device=mass.device)
         gl = np.ascontiguousarray(glam.detach().cpu().numpy().astype(np.float64))
         lam = np.ascontiguousarray(ctx.lam.astype(np.float64))
-        _lib.call("fame6d_eig_grad_eps", mass.handle, _lib.ptr(X), X.shape[0], lam.ctypes.data_as(C.c_void_p),
-                  gl.ctypes.data_as(C.c_void_p), _lib.ptr(g))
+        # the recovery X = M^{-1} V Y and the gradient use the forward's eps
+        with mass.using_eps(eps.detach()):
+            X, MX, _ = recover_original_eigenvectors(ctx.Y, sysq)
+            _lib.call("fame6d_eig_grad_eps", mass.handle, _lib.ptr(X), X.shape[0], lam.ctypes.data_as(C.c_void_p),
+                      gl.ctypes.data_as(C.c_void_p), _lib.ptr(g))
         sysq.close()
         return g.to(glam.device) if glam.device != g.device else g, None, None, None, None
 

@@ -271,6 +271,91 @@ extern "C" {
 const char* fame6d_last_error(void) { return g_last_error.c_str(); }
 int fame6d_version(void) { return 10000; }
 
+// Per-workgroup min, max and sum of the permittivity samples, and a non-finite count
+// (block b writes stats[4b .. 4b+3]); the host combines the kEpsBlocks partials.
+constexpr int kEpsBlocks = 256;
+__global__ void __launch_bounds__(256) k_eps_stats(long long n, const double* __restrict__ eps,
+                                                   double* __restrict__ stats) {
+  double mn = INFINITY, mx = -INFINITY, sum = 0.0, bad = 0.0;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+    const double e = eps[i];
+    if (!isfinite(e)) {
+      bad += 1.0;
+    } else {
+      mn = fmin(mn, e);
+      mx = fmax(mx, e);
+      sum += e;
+    }
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = mn;
+  red[1][threadIdx.x] = mx;
+  red[2][threadIdx.x] = sum;
+  red[3][threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+      red[2][threadIdx.x] += red[2][threadIdx.x + w];
+      red[3][threadIdx.x] += red[3][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) stats[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The four operator tables of the samples: eps/NF and 1/(eps NF), FP64 and FP32.
+__global__ void k_eps_tables(long long n, const double* __restrict__ eps, double* __restrict__ e64,
+                             float* __restrict__ e32, double* __restrict__ i64, float* __restrict__ i32) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double e = eps[i];
+    const double a = e / (double)n;
+    const double b = 1.0 / (e * (double)n);
+    e64[i] = a;
+    e32[i] = (float)a;
+    i64[i] = b;
+    i32[i] = (float)b;
+  }
+}
+
+static void check_eps_stats(fame6d_mass* m, double mn, double mx, double sum, bool finite) {
+  if (!finite) throw Error(FAME6D_ENONFINITE, "permittivity contains NaN/Inf");
+  if (!(mn > 0))
+    throw Error(FAME6D_EARG, "permittivity must be positive: M_s = F diag(eps) F^{-1} is then Hermitian "
+                             "positive definite, which the PCG solves require");
+  m->eps_min = mn;
+  m->eps_max = mx;
+  m->eps_mean = sum / (double)m->NF;
+}
+
+// Device form of upload_eps: the samples are validated and the tables formed on the
+// device; only kEpsBlocks x 4 partial statistics cross to the host.
+static void upload_eps_device(fame6d_mass* m, const double* eps_dev) {
+  const long long NF = m->NF;
+  m->eps64.ensure(sizeof(double) * NF);
+  m->eps32.ensure(sizeof(float) * NF);
+  m->ieps64.ensure(sizeof(double) * NF);
+  m->ieps32.ensure(sizeof(float) * NF);
+  m->scal.ensure(std::max<size_t>(m->scal.bytes, sizeof(double) * 4 * kEpsBlocks));
+  k_eps_stats<<<kEpsBlocks, 256, 0, m->stream>>>(NF, eps_dev, m->scal.as<double>());
+  F6_CHECK_LAUNCH();
+  std::vector<double> st(4 * kEpsBlocks);
+  d2h_sync(m, st.data(), m->scal.ptr, sizeof(double) * 4 * kEpsBlocks);
+  double mn = INFINITY, mx = -INFINITY, sum = 0, bad = 0;
+  for (int b = 0; b < kEpsBlocks; ++b) {
+    mn = std::min(mn, st[4 * b]);
+    mx = std::max(mx, st[4 * b + 1]);
+    sum += st[4 * b + 2];
+    bad += st[4 * b + 3];
+  }
+  check_eps_stats(m, mn, mx, sum, bad == 0);
+  k_eps_tables<<<grid_for(NF), 256, 0, m->stream>>>(NF, eps_dev, m->eps64.as<double>(), m->eps32.as<float>(),
+                                                    m->ieps64.as<double>(), m->ieps32.as<float>());
+  F6_CHECK_LAUNCH();
+}
+
 static void upload_eps(fame6d_mass* m, const double* eps_host) {
   const long long NF = m->NF;
   double mn = INFINITY, mx = -INFINITY, sum = 0;
@@ -281,13 +366,7 @@ static void upload_eps(fame6d_mass* m, const double* eps_host) {
     mx = std::max(mx, e);
     sum += e;
   }
-  if (!(mn > 0))
-    throw Error(FAME6D_EARG, "permittivity must be positive: M_s = F diag(eps) F^{-1} is then Hermitian "
-                             "positive definite, which the PCG solves require");
-  m->eps_min = mn;
-  m->eps_max = mx;
-  m->eps_mean = sum / (double)NF;
-  m->eps_host.assign(eps_host, eps_host + NF);
+  check_eps_stats(m, mn, mx, sum, true);
   std::vector<double> e64(NF), i64(NF);
   std::vector<float> e32(NF), i32(NF);
   for (long long i = 0; i < NF; ++i) {
@@ -340,6 +419,13 @@ int fame6d_mass_set_eps(fame6d_mass* m, const double* eps_host) {
   F6_API_BEGIN
   F6_REQUIRE(m && eps_host, FAME6D_EARG, "null argument");
   upload_eps(m, eps_host);
+  F6_API_END
+}
+
+int fame6d_mass_set_eps_device(fame6d_mass* m, const double* eps_dev) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && eps_dev, FAME6D_EARG, "null argument");
+  upload_eps_device(m, eps_dev);
   F6_API_END
 }
 

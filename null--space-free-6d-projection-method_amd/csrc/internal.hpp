@@ -32,10 +32,29 @@ void set_last_error(const std::string& msg);
 
 #define F6_CHECK_LAUNCH() F6_HIP(hipGetLastError())
 
-// Growable device buffer owned by a context.
+// Growable device buffer owned by a context or a scope; freed by its destructor, so a
+// buffer of a function that throws (ENOCONV, EBUDGET, ...) does not outlive the call.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
   void ensure(size_t need) {
     if (need <= bytes) return;
     if (ptr) F6_HIP(hipFree(ptr));
@@ -150,7 +169,6 @@ struct fame6d_mass {
   fame6d::DevBuf partials;         // reduction partials
   fame6d::DevBuf scal;             // small device scalars
   fame6d::TileChoice tile64, tile32;
-  std::vector<double> eps_host;    // unscaled samples (for adjoint / info)
   fame6d::HostPin pin;             // page-locked staging of small transfers
   mutable fame6d::KernelTimer timer;
 };

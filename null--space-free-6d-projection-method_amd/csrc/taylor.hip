@@ -5,6 +5,7 @@
 // m_alpha(r_c) = i^|alpha|/alpha! * sum_j q_j^alpha exp(i q_j . r_c) c_j
 // (taylor_on_blocks_exp3.m:4-6; reconstruct_component_taylor_batch_exp4.m:14-38).
 // Term order alpha = (a1, a2, a3) with a1 outermost (build_multi_index_list_3d_exp3.m:6-12).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -268,6 +269,51 @@ static void alpha_tables(int p, std::vector<int>& al, std::vector<cpx<double>>& 
       }
 }
 
+// ---------------------------------------------------------------------------- field errors
+// Per-workgroup sums of |t_c - r_c|^2 and |r_c|^2 and maxima of |t_c - r_c| and |r_c| over
+// the three components (compute_field_errors_exp3.m:3-10); the host combines the partials
+// in workgroup order, so the result does not depend on scheduling.
+constexpr int kErrBlocks = 512;
+__global__ void __launch_bounds__(256) k_field_err(long long n, const cpx<double>* __restrict__ t1,
+                                                   const cpx<double>* __restrict__ t2,
+                                                   const cpx<double>* __restrict__ t3,
+                                                   const cpx<double>* __restrict__ r1,
+                                                   const cpx<double>* __restrict__ r2,
+                                                   const cpx<double>* __restrict__ r3, double* __restrict__ part) {
+  double sd = 0.0, sr = 0.0, md = 0.0, mr = 0.0;
+  const cpx<double>* ts[3] = {t1, t2, t3};
+  const cpx<double>* rs[3] = {r1, r2, r3};
+  for (int c = 0; c < 3; ++c) {
+    const cpx<double>* t = ts[c];
+    const cpx<double>* r = rs[c];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+      const cpx<double> a = t[i], b = r[i];
+      const double dr = a.re - b.re, di = a.im - b.im;
+      const double d2 = dr * dr + di * di, r2v = b.re * b.re + b.im * b.im;
+      sd += d2;
+      sr += r2v;
+      md = fmax(md, d2);
+      mr = fmax(mr, r2v);
+    }
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = sd;
+  red[1][threadIdx.x] = sr;
+  red[2][threadIdx.x] = md;
+  red[3][threadIdx.x] = mr;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + w]);
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 }  // namespace fame6d
 
 using namespace fame6d;
@@ -395,6 +441,39 @@ int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, in
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
     cen.release();
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  }
+  return FAME6D_OK;
+}
+
+int fame6d_field_errors(long long n, const void* t1, const void* t2, const void* t3, const void* r1,
+                        const void* r2, const void* r3, double* e2, double* einf, void* stream) {
+  try {
+    if (!t1 || !t2 || !t3 || !r1 || !r2 || !r3 || !e2 || !einf) throw Error(FAME6D_EARG, "null argument");
+    if (n < 0) throw Error(FAME6D_EARG, "negative point count");
+    hipStream_t s = (hipStream_t)stream;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(kErrBlocks, (n + 255) / 256));
+    DevBuf part;
+    part.ensure(sizeof(double) * 4 * blocks);
+    k_field_err<<<blocks, 256, 0, s>>>(n, (const cpx<double>*)t1, (const cpx<double>*)t2, (const cpx<double>*)t3,
+                                       (const cpx<double>*)r1, (const cpx<double>*)r2, (const cpx<double>*)r3,
+                                       part.as<double>());
+    F6_CHECK_LAUNCH();
+    std::vector<double> h(4 * blocks);
+    F6_HIP(hipMemcpyAsync(h.data(), part.ptr, sizeof(double) * 4 * blocks, hipMemcpyDeviceToHost, s));
+    F6_HIP(hipStreamSynchronize(s));
+    double sd = 0, sr = 0, md = 0, mr = 0;
+    for (int b = 0; b < blocks; ++b) {
+      sd += h[4 * b];
+      sr += h[4 * b + 1];
+      md = std::max(md, h[4 * b + 2]);
+      mr = std::max(mr, h[4 * b + 3]);
+    }
+    const double eps = 2.220446049250313e-16;  // MATLAB eps, the floor of both denominators
+    *e2 = std::sqrt(sd) / std::max(std::sqrt(sr), eps);
+    *einf = std::sqrt(md) / std::max(std::sqrt(mr), eps);
   } catch (const Error& e) {
     set_last_error(e.what());
     return e.code;

@@ -96,6 +96,7 @@ class MassBackend:
             raise ValueError("eps must have NF = (2N)^6 = %d samples" % self.NF)
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self._eps = eps
+        self._eps_dev = None
         self._h = C.c_void_p()
         with torch.cuda.device(self.device):
             self.stream = torch.cuda.current_stream(self.device)
@@ -118,12 +119,38 @@ class MassBackend:
             pass
 
     def set_eps(self, eps):
-        """Replaces the permittivity samples (same grid)."""
+        """Replaces the permittivity samples (same grid).  A tensor on the backend's device is
+        validated and scaled on the device (fame6d_mass_set_eps_device); anything else is
+        taken as host samples."""
+        if isinstance(eps, torch.Tensor) and eps.device == self.device:
+            e = eps.detach().reshape(-1).to(torch.float64).contiguous()
+            if e.numel() != self.NF:
+                raise ValueError("eps must have NF samples")
+            e = e.clone() if e.data_ptr() == eps.data_ptr() else e  # the backend keeps its own copy
+            call("fame6d_mass_set_eps_device", self._h, ptr(e))
+            self._eps_dev = e
+            self._eps = None
+            return
+        if isinstance(eps, torch.Tensor):
+            eps = eps.detach().cpu().numpy()
         eps = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(-1))
         if eps.size != self.NF:
             raise ValueError("eps must have NF samples")
         call("fame6d_mass_set_eps", self._h, eps.ctypes.data_as(C.c_void_p))
         self._eps = eps
+        self._eps_dev = None
+
+    def eps_tensor(self):
+        """The current permittivity samples as a float64 tensor on the backend's device."""
+        if self._eps_dev is None:
+            self._eps_dev = torch.as_tensor(self._eps, device=self.device)
+        return self._eps_dev
+
+    def using_eps(self, eps):
+        """Context manager: the backend computes with `eps` inside the block and gets its
+        previous permittivity back on exit, so callers that share one backend see no side
+        effect."""
+        return _EpsScope(self, eps)
 
     @property
     def handle(self):
@@ -131,6 +158,8 @@ class MassBackend:
 
     @property
     def eps(self):
+        if self._eps is None:
+            self._eps = self._eps_dev.cpu().numpy()
         return self._eps
 
     @property
@@ -225,6 +254,20 @@ class MassBackend:
         g = torch.zeros(self.NF, dtype=torch.float64, device=self.device)
         call("fame6d_mass_adjoint_eps", self._h, ptr(x), ptr(gy), nrhs, ptr(g))
         return g
+
+
+class _EpsScope:
+    def __init__(self, mass, eps):
+        self.mass, self.eps, self.prev = mass, eps, None
+
+    def __enter__(self):
+        self.prev = self.mass.eps_tensor()
+        self.mass.set_eps(self.eps)
+        return self.mass
+
+    def __exit__(self, *exc):
+        self.mass.set_eps(self.prev)
+        return False
 
 
 def build_mass_backend_large_scalar(mat, fourier, massCfg=None, gpuInfo=None):

@@ -3,8 +3,9 @@
 Mirrors reconstruct_component_taylor_batch_exp4 (exp3/reconstruct_component_taylor_batch_exp4.m:1-43),
 build_multi_index_list_3d_exp3 (exp2/build_multi_index_list_3d_exp3.m:1-15),
 build_yee_edge_points_plus (exp2/build_yee_edge_points_plus.m:1-36),
-build_yee_block_partition_exp3 (exp2/build_yee_block_partition_exp3.m:1-31) and
-reconstruct_field_direct_blocked_exp3 (exp2/reconstruct_field_direct_blocked_exp3.m:1-29).
+build_yee_block_partition_exp3 (exp2/build_yee_block_partition_exp3.m:1-31),
+reconstruct_field_direct_blocked_exp3 (exp2/reconstruct_field_direct_blocked_exp3.m:1-29) and
+compute_field_errors_exp3 (exp2/compute_field_errors_exp3.m:1-11).
 """
 import ctypes as C
 
@@ -89,6 +90,24 @@ def direct_reconstruct(qmodes, coeff, points, device="cuda"):
     call("fame6d_direct_reconstruct", c.numel(), ptr(q), ptr(c), M, ptr(pts), ptr(u),
          C.c_void_p(torch.cuda.current_stream().cuda_stream))
     return u
+
+
+def compute_field_errors(test, ref):
+    """Relative 2-norm and max-norm errors of a three-component field, the norms taken over
+    all three components together (exp2/compute_field_errors_exp3.m:3-10): test and ref are
+    sequences (u1, u2, u3) of equally long complex128 device tensors.  Returns
+    {'e2': ..., 'eInf': ...}."""
+    t = [u.reshape(-1).to(torch.complex128).contiguous() for u in test]
+    r = [u.reshape(-1).to(torch.complex128).contiguous() for u in ref]
+    if len(t) != 3 or len(r) != 3:
+        raise ValueError("test and ref must each hold three components")
+    n = t[0].numel()
+    if any(u.numel() != n for u in t + r):
+        raise ValueError("all six components must have the same length")
+    e2, einf = C.c_double(), C.c_double()
+    call("fame6d_field_errors", n, *[ptr(u) for u in t + r], C.byref(e2), C.byref(einf),
+         C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    return dict(e2=e2.value, eInf=einf.value)
 
 
 def projected_q_modes(N, k_lift, P):

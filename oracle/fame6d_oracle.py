@@ -683,6 +683,17 @@ def direct_reconstruct(qmodes, coeff, points, block=256):
     return u
 
 
+def compute_field_errors(test, ref):
+    """e2 = ||t - r||_2 / max(||r||_2, eps), eInf = max|t - r| / max(max|r|, eps) over the three
+    components together (exp2/compute_field_errors_exp3.m:3-10)."""
+    d = [np.asarray(t) - np.asarray(r) for t, r in zip(test, ref)]
+    num2 = math.sqrt(sum(float(np.sum(np.abs(x) ** 2)) for x in d))
+    den2 = math.sqrt(sum(float(np.sum(np.abs(np.asarray(r)) ** 2)) for r in ref))
+    num_inf = max(float(np.max(np.abs(x))) for x in d)
+    den_inf = max(float(np.max(np.abs(np.asarray(r)))) for r in ref)
+    return dict(e2=num2 / max(den2, EPS), eInf=num_inf / max(den_inf, EPS))
+
+
 def yee_points(n, h, comp):
     """Plus-grid Yee points of component comp (0,1,2): h*(I + 0.5 e_comp), I in -(n+1)..n+1
     (exp2/build_yee_edge_points_plus.m:11-17)."""

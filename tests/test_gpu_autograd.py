@@ -56,3 +56,82 @@ def test_eigenvalue_gradient_fd(pkg, oracle):
         fd = float(((lp - lm) / (2 * h) * wts).sum())
         assert abs(fd - g @ d) < 1e-5 * abs(fd) + 1e-9, (fd, g @ d)
     mass.close()
+
+
+def test_autograd_leaves_backend_eps(pkg, oracle):
+    """Two forwards at different permittivities, then the first one's backward: its gradient
+    equals that of a fresh forward at the same permittivity, and the shared backend keeps the
+    permittivity it was created with (autograd.py uses it only inside scoped blocks)."""
+    from importlib import import_module
+    ag = import_module("fame6d_amd.autograd")
+    torch.cuda.set_device(0)
+    N = 2
+    q = (0.13, 0.21, 0.17)
+    print("case: autograd eps scoping N=2", flush=True)
+    eps_a = oracle.eps_reference(N)
+    eps_b = oracle.eps_reference(N, epsC=3.0, alpha=0.8)
+    eps_home = np.full(eps_a.size, 2.5)
+    mass = pkg.MassBackend(N, eps_home)
+    opts = dict(p=40, tol=1e-12, max_restart=60, tol_cg=1e-13, maxit_cg=5000)
+    wts = torch.tensor([1.0, -0.5, 0.25], dtype=torch.float64, device="cuda")
+    # eigenvalues: forward a, forward b, backward a
+    ea = torch.as_tensor(eps_a, device="cuda").requires_grad_(True)
+    eb = torch.as_tensor(eps_b, device="cuda").requires_grad_(True)
+    la = ag.lowest_eigenvalues(ea, mass, q, k=3, **opts)
+    lb = ag.lowest_eigenvalues(eb, mass, q, k=3, **opts)
+    assert np.allclose(mass.eps, eps_home, rtol=0, atol=0)
+    (la * wts).sum().backward()
+    g_late = ea.grad.clone()
+    fresh_mass = pkg.MassBackend(N, eps_a)
+    ef = torch.as_tensor(eps_a, device="cuda").requires_grad_(True)
+    (ag.lowest_eigenvalues(ef, fresh_mass, q, k=3, **opts) * wts).sum().backward()
+    rel = float(torch.linalg.norm(g_late - ef.grad) / torch.linalg.norm(ef.grad))
+    print("  eigenvalue gradient, late backward vs fresh: rel %.2e" % rel, flush=True)
+    assert rel < 1e-7
+    assert np.allclose(mass.eps, eps_home, rtol=0, atol=0)
+    del lb
+    # mass apply: forward a, forward b, backward a (gx = M_s(eps_a) gy)
+    rng = np.random.default_rng(7)
+    x = torch.as_tensor(rng.standard_normal(mass.NF) + 1j * rng.standard_normal(mass.NF),
+                        device="cuda").requires_grad_(True)
+    w = torch.as_tensor(rng.standard_normal(mass.NF) + 1j * rng.standard_normal(mass.NF), device="cuda")
+    ya = ag.mass_apply(x, torch.as_tensor(eps_a, device="cuda"), mass)
+    yb = ag.mass_apply(x, torch.as_tensor(eps_b, device="cuda"), mass)
+    assert np.allclose(ya.detach().cpu().numpy(), oracle.mass_apply(eps_a, x.detach().cpu().numpy(), N), atol=1e-12)
+    assert np.allclose(yb.detach().cpu().numpy(), oracle.mass_apply(eps_b, x.detach().cpu().numpy(), N), atol=1e-12)
+    torch.real(torch.sum(torch.conj(w) * ya)).backward()
+    assert np.allclose(x.grad.cpu().numpy(), oracle.mass_apply(eps_a, w.cpu().numpy(), N), atol=1e-12)
+    assert np.allclose(mass.eps, eps_home, rtol=0, atol=0)
+    fresh_mass.close()
+    mass.close()
+
+
+def test_set_eps_device_matches_host(pkg, oracle):
+    """fame6d_mass_set_eps_device gives the operator of the host upload, and rejects
+    non-positive and non-finite samples with the host path's status codes."""
+    from importlib import import_module
+    lib = import_module("fame6d_amd._lib")
+    torch.cuda.set_device(0)
+    N = 3
+    print("case: set_eps_device N=3", flush=True)
+    eps0 = oracle.eps_reference(N)
+    mass = pkg.MassBackend(N, np.ones(eps0.size))
+    mass.set_eps(torch.as_tensor(eps0, device="cuda"))
+    info = mass.info
+    assert abs(info["epsMin"] - eps0.min()) <= 1e-15 * eps0.max()
+    assert abs(info["epsMean"] - eps0.mean()) <= 1e-13 * eps0.mean()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(mass.NF) + 1j * rng.standard_normal(mass.NF)
+    y = mass.MscalarOp(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    ref = oracle.mass_apply(eps0, x, N)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
+    bad = eps0.copy()
+    bad[17] = -1.0
+    with pytest.raises(lib.Fame6dError) as e:
+        mass.set_eps(torch.as_tensor(bad, device="cuda"))
+    assert e.value.code == 7
+    bad[17] = np.nan
+    with pytest.raises(lib.Fame6dError) as e:
+        mass.set_eps(torch.as_tensor(bad, device="cuda"))
+    assert e.value.code == 2
+    mass.close()

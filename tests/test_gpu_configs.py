@@ -1,0 +1,282 @@
+"""The benchmark workloads at their own sizes on the GPU, pinned to the oracle's fixtures.
+
+The fixtures in tests/golden/ hold the oracle's FP64 eigenvalues of each workload
+(tests/golden/make_eigs_fixtures.py); every mode of the library's FP64 and mixed solves is
+compared with them, as the reference checks its held eigenvalues per mode
+(exp3/solve_tracked_modes_exp4.m:37-42).  Tolerances: FP64 1e-9 relative (both solves stop at
+Ritz residual 1e-10 with inner PCG to 1e-10); mixed 1e-8 relative (the accuracy target of
+the mixed-precision form).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LANCZOS = dict(p=40, tol=1e-10, max_restart=50, tol_cg=1e-10, maxit_cg=1200)
+TOL_FP64 = 1e-9
+TOL_MIXED = 1e-8
+
+
+def load_fixture(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
+
+def material(pkg, fx):
+    eps, phases = pkg.cosine_material(fx["N"], fx["amp"], fx["eps_seed"])
+    assert np.array_equal(phases, np.array(fx["eps_phases"]))  # the fixture's own inputs
+    return eps
+
+
+def per_mode(lam, ref):
+    return np.abs(np.asarray(lam) - ref) / np.abs(ref)
+
+
+def check_case(pkg, name, precisions=("double", "mixed")):
+    fx = load_fixture(name)
+    ref = np.array(fx["lambda"])
+    mass = pkg.MassBackend(fx["N"], material(pkg, fx))
+    sysq = pkg.QSystem(mass, fx["q"])
+    try:
+        v0 = pkg.start_vector(sysq.nr, fx["v0_seed"])
+        out = {}
+        for prec in precisions:
+            o = pkg.run_method_C(sysq, k=fx["k"], v0=v0, precision=prec, **LANCZOS)
+            err = per_mode(o["lambda_"], ref)
+            print("  %s %s per-mode rel err vs oracle: %s" % (name, prec, " ".join("%.1e" % e for e in err)),
+                  flush=True)
+            assert o["stats"]["converged"]
+            assert np.all(err < (TOL_FP64 if prec == "double" else TOL_MIXED)), (prec, err.max())
+            out[prec] = o
+        return out
+    finally:
+        sysq.close()
+        mass.close()
+
+
+def test_config1_n3_fp64_and_mixed(pkg):
+    """Plain method at N = 3 (46,656 modes, 93,312 unknowns), amplitude 0.4."""
+    print("case: N=3 amplitude 0.4 vs oracle fixture", flush=True)
+    check_case(pkg, "eigs_N3_a04")
+
+
+def test_config2_n5_fp64_and_mixed(pkg):
+    """The headline workload: N = 5 (10^6 modes, 2e6 unknowns), amplitude 0.4; the ten
+    lowest eigenvalues of the FP64 and the mixed solve each within tolerance of the oracle."""
+    print("case: N=5 amplitude 0.4 vs oracle fixture", flush=True)
+    check_case(pkg, "eigs_N5_a04")
+
+
+def test_config3_n5_amplitude_08(pkg):
+    print("case: N=5 amplitude 0.8 vs oracle fixture", flush=True)
+    check_case(pkg, "eigs_N5_a08")
+
+
+def test_config3_n6_amplitude_08(pkg):
+    """The largest mixed-precision size: N = 6 (12^6 = 2,985,984 modes), amplitude 0.8."""
+    print("case: N=6 amplitude 0.8 vs oracle fixture", flush=True)
+    check_case(pkg, "eigs_N6_a08")
+
+
+# ----------------------------------------------------------------- batched Bloch path, N = 4
+Q_END = np.array([0.5, 0.4, 0.3])
+PATH_Q = np.array([Q_END * t for t in np.arange(1, 13) / 12.0])  # q = 0 excluded (Gamma point)
+
+
+@pytest.fixture(scope="module")
+def n4path(pkg):
+    torch.cuda.set_device(0)
+    fx = load_fixture("eigs_N4_a04_t12")
+    mass = pkg.MassBackend(4, material(pkg, fx))
+    yield dict(mass=mass, fx=fx)
+    mass.close()
+
+
+def test_config4_batched_twelve_points_n4(pkg, n4path):
+    """All 12 Bloch points of the path in one batched solve at N = 4 (k = 10, p = 40): every
+    point's eigenvalues equal its own single-point solve, each K_r^{-1} ended with pcg flag 0,
+    and the first and last points match the oracle's fixtures."""
+    print("case: batched 12-point path N=4 k=10 p=40", flush=True)
+    mass = n4path["mass"]
+    nr = 2 * mass.NF
+    v0 = np.stack([pkg.start_vector(nr, 100 + i) for i in range(12)])
+    outb = pkg.run_method_C_batch(mass, PATH_Q, v0=v0, precision="double", k=10, **LANCZOS)
+    for i in range(12):
+        st = outb["stats"][i]
+        assert st["converged"], i
+        assert st["krinv"]["last_flag"] == 0 and st["krinv"]["max_relres"] <= 1e-9, (i, st["krinv"])
+        s = pkg.QSystem(mass, PATH_Q[i])
+        one = pkg.run_method_C(s, k=10, v0=v0[i], precision="double", **LANCZOS)
+        s.close()
+        d = per_mode(outb["lambda_"][i], one["lambda_"]).max()
+        assert d < 1e-10, (i, d)
+    for i, name in ((0, "eigs_N4_a04_t01"), (11, "eigs_N4_a04_t12")):
+        fx = load_fixture(name)
+        assert np.allclose(fx["q"], PATH_Q[i], rtol=0, atol=1e-15)
+        err = per_mode(outb["lambda_"][i], np.array(fx["lambda"]))
+        print("  point %d per-mode rel err vs oracle: %.1e" % (i + 1, err.max()), flush=True)
+        assert np.all(err < TOL_FP64)
+
+
+def test_config4_batched_mixed_n4(pkg, n4path):
+    """The batched path in the mixed form: every point within 1e-8 of the oracle-pinned FP64
+    eigenvalues (first and last point) and of its own FP64 solve."""
+    print("case: batched 12-point path N=4 mixed", flush=True)
+    mass = n4path["mass"]
+    nr = 2 * mass.NF
+    v0 = np.stack([pkg.start_vector(nr, 100 + i) for i in range(12)])
+    outm = pkg.run_method_C_batch(mass, PATH_Q, v0=v0, precision="mixed", k=10, **LANCZOS)
+    out64 = pkg.run_method_C_batch(mass, PATH_Q, v0=v0, precision="double", k=10, **LANCZOS)
+    assert np.max(np.abs(outm["lambda_"] - out64["lambda_"]) / out64["lambda_"]) < TOL_MIXED
+    for i, name in ((0, "eigs_N4_a04_t01"), (11, "eigs_N4_a04_t12")):
+        assert np.all(per_mode(outm["lambda_"][i], np.array(load_fixture(name)["lambda"])) < TOL_MIXED)
+
+
+# ----------------------------------------------------------------- Taylor reconstruction, N = 4
+P_T, NB, H_T, N_YEE = 10, 6, 0.0125, 40
+
+
+@pytest.fixture(scope="module")
+def mode10(pkg, oracle, n4path):
+    """Mode 10 of the last path point, recovered in the original 3-component variables
+    (exp2/recover_original_eigenvectors_exp2.m:18-51), and the projected wave vectors."""
+    mass = n4path["mass"]
+    s = pkg.QSystem(mass, Q_END)
+    out = pkg.run_method_C(s, k=10, v0=pkg.start_vector(s.nr, 20260905), precision="double", want_vectors=True,
+                           **LANCZOS)
+    X, MX, info = pkg.recover_original_eigenvectors(out["Y"][9:10], s)
+    qm = pkg.projected_q_modes(4, s.info["kLift"], pkg.P_DEFAULT)
+    s.close()
+    return dict(X=X[0], qm=qm, NF=mass.NF)
+
+
+def _component(mode10, comp):
+    NF = mode10["NF"]
+    return mode10["X"][comp * NF:(comp + 1) * NF].contiguous()
+
+
+def test_config4_taylor_moments_n4_p10_c216(pkg, oracle, mode10):
+    """The moments contraction at its configured size (N = 4, 262,144 modes; p = 10, 286
+    monomials; 6^3 = 216 centres of the n = 40 Yee window) against the oracle's moments,
+    column by column for 24 of the centres and all three components."""
+    qm = mode10["qm"]
+    rng = np.random.default_rng(40)
+    cols = np.sort(rng.choice(NB ** 3, 24, replace=False))
+    for comp in range(3):
+        print("case: taylor moments N=4 p=10 C=216 component %d" % comp, flush=True)
+        cen = pkg.yee_block_centers(N_YEE, H_T, comp, NB)
+        coeff = _component(mode10, comp)
+        mom = pkg.taylor_moments(qm, coeff, P_T, cen).cpu().numpy()
+        assert mom.shape == (286, 216)
+        ref = oracle.taylor_moments(qm, coeff.cpu().numpy(), P_T, cen[cols])
+        err = np.abs(mom[:, cols] - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, (comp, err)
+
+
+def test_config4_yee_evaluation_n40(pkg, oracle, mode10):
+    """Yee evaluation on the (2n+3)^3 = 571,787 plus-grid points per component at n = 40,
+    h = 0.0125: at 400 sampled points it equals the oracle's Taylor sum of the same moments,
+    and the reconstructed field agrees with the direct Fourier sum (taken on the GPU and
+    checked against the oracle's direct sum) to the Taylor truncation error of p = 10; the
+    three-component errors come from the device reduction of compute_field_errors_exp3."""
+    import ctypes as C
+    from importlib import import_module
+    lib = import_module("fame6d_amd._lib")
+    qm = mode10["qm"]
+    m = 2 * N_YEE + 3
+    bid = pkg.block_partition(m, NB)
+    rng = np.random.default_rng(41)
+    sample = np.sort(rng.choice(m ** 3, 400, replace=False))
+    taylor_s, direct_s = [], []
+    for comp in range(3):
+        print("case: taylor yee evaluation n=40 component %d" % comp, flush=True)
+        cen = np.ascontiguousarray(pkg.yee_block_centers(N_YEE, H_T, comp, NB))
+        coeff = _component(mode10, comp)
+        mom = pkg.taylor_moments(qm, coeff, P_T, cen)
+        u = torch.empty(m ** 3, dtype=torch.complex128, device="cuda")
+        lib.call("fame6d_taylor_evaluate_yee", N_YEE, H_T, comp, NB, P_T, NB ** 3, cen.ctypes.data_as(C.c_void_p),
+                 lib.ptr(mom.T.contiguous()), lib.ptr(u), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        pts = pkg.yee_points(N_YEE, H_T, comp)[sample]
+        us = u[torch.as_tensor(sample, device="cuda")]
+        ref = oracle.taylor_evaluate(pts, bid[sample], cen, mom.cpu().numpy(), P_T)
+        assert np.abs(us.cpu().numpy() - ref).max() < 1e-11 * np.abs(ref).max()
+        ud = pkg.direct_reconstruct(qm, coeff, pts)
+        ud_o = oracle.direct_reconstruct(qm, coeff.cpu().numpy(), pts[:100])
+        assert np.abs(ud[:100].cpu().numpy() - ud_o).max() < 1e-10 * np.abs(ud_o).max()
+        taylor_s.append(us)
+        direct_s.append(ud)
+    errs = pkg.compute_field_errors(taylor_s, direct_s)
+    errs_o = oracle.compute_field_errors([t.cpu().numpy() for t in taylor_s], [d.cpu().numpy() for d in direct_s])
+    print("  Taylor vs direct at the sampled points: e2 %.2e eInf %.2e" % (errs["e2"], errs["eInf"]), flush=True)
+    assert abs(errs["e2"] - errs_o["e2"]) <= 1e-12 * errs_o["e2"] + 1e-300
+    assert abs(errs["eInf"] - errs_o["eInf"]) <= 1e-12 * errs_o["eInf"] + 1e-300
+    assert errs["e2"] < 1e-6 and errs["eInf"] < 1e-6
+
+
+def test_field_errors_device_vs_oracle(pkg, oracle):
+    """compute_field_errors' device reduction on ragged sizes (one point, one workgroup's
+    worth, many workgroups) and a zero reference (the eps floor)."""
+    rng = np.random.default_rng(8)
+    for n in (1, 255, 256, 1000, 300001):
+        print("case: field errors n=%d" % n, flush=True)
+        ref = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)]
+        test = [r + 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for r in ref]
+        got = pkg.compute_field_errors([torch.as_tensor(t, device="cuda") for t in test],
+                                       [torch.as_tensor(r, device="cuda") for r in ref])
+        exp = oracle.compute_field_errors(test, ref)
+        assert abs(got["e2"] - exp["e2"]) <= 1e-13 * exp["e2"]
+        assert got["eInf"] == pytest.approx(exp["eInf"], rel=1e-15)
+    z = torch.zeros(10, dtype=torch.complex128, device="cuda")
+    t = torch.full((10,), 1e-20, dtype=torch.complex128, device="cuda")
+    got = pkg.compute_field_errors([t, t, t], [z, z, z])
+    assert got["e2"] == pytest.approx(math.sqrt(30) * 1e-20 / np.finfo(float).eps, rel=1e-14)
+
+
+# ----------------------------------------------------------------- differentiation, N = 5
+def test_config5_gradient_n5_64_directions(pkg):
+    """Gradients of the five lowest eigenvalues at N = 5 with respect to all 10^6 permittivity
+    samples (Hellmann-Feynman through the adjoint mass kernel, behind torch.autograd) against
+    central differences with step 1e-6 on 64 seeded random directions of a weighted sum of
+    the eigenvalues.  Bound: the differences are accurate to about 1e-5 of the typical
+    directional derivative (solve tolerance 1e-12 over step 1e-6 plus O(h^2) curvature), so
+    each deviation must stay below 1e-4 of the RMS derivative and the median below 2e-5."""
+    from importlib import import_module
+    ag = import_module("fame6d_amd.autograd")
+    torch.cuda.set_device(0)
+    N, k, h = 5, 5, 1e-6
+    q = (0.05, 0.04, 0.03)
+    print("case: eigenvalue gradient N=5 k=5, 64 FD directions", flush=True)
+    eps0, _ = pkg.cosine_material(N, 0.4, 0)
+    mass = pkg.MassBackend(N, eps0)
+    opts = dict(p=40, tol=1e-12, max_restart=60, tol_cg=1e-13, maxit_cg=5000)
+    wts = torch.tensor([1.0, 0.5, -0.25, 0.125, 2.0], dtype=torch.float64, device="cuda")
+    eps = torch.as_tensor(eps0, device="cuda").requires_grad_(True)
+    lam = ag.lowest_eigenvalues(eps, mass, q, k=k, **opts)
+    fx = load_fixture("eigs_N5_a04")
+    assert np.all(per_mode(lam.detach().cpu().numpy(), np.array(fx["lambda"][:k])) < TOL_FP64)
+    (lam * wts).sum().backward()
+    g = eps.grad.cpu().numpy()
+    rng = np.random.default_rng(64)
+    fds, devs = [], []
+    for i in range(64):
+        d = rng.standard_normal(mass.NF)
+        lp = ag.lowest_eigenvalues(torch.as_tensor(eps0 + h * d, device="cuda"), mass, q, k=k, **opts)
+        lm = ag.lowest_eigenvalues(torch.as_tensor(eps0 - h * d, device="cuda"), mass, q, k=k, **opts)
+        fd = float(((lp - lm) / (2 * h) * wts).sum())
+        fds.append(fd)
+        devs.append(abs(fd - float(g @ d)))
+        if i % 16 == 15:
+            print("  %d directions done" % (i + 1), flush=True)
+    mass.close()
+    fds, devs = np.array(fds), np.array(devs)
+    rms = float(np.sqrt(np.mean(fds ** 2)))
+    print("  max deviation / rms %.2e, median relative deviation %.2e"
+          % (devs.max() / rms, np.median(devs / np.abs(fds))), flush=True)
+    assert devs.max() < 1e-4 * rms
+    assert np.median(devs / rms) < 2e-5

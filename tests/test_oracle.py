@@ -86,6 +86,42 @@ def test_qsystem_closed_forms(oracle):
     assert np.allclose(s["invC"], 1 / np.sqrt(1 + s["W1"] ** 2 + s["W2"] ** 2))
 
 
+def test_qsystem_singular_branch_closed_forms(oracle):
+    """q = (c, c, c) puts the modes xi = (m, m, m, 0, 0, 0) on the (1,1,1) line, where the regular
+    basis formula divides by 3|q|^2 - (q1+q2+q3)^2 = 0; the singular branch
+    (exp1/assemble_K_and_nullspace_free_basis.m:75-91) takes a = e_i for the smallest |qhat_i|
+    (the three tie up to rounding, so i is whichever rounding makes smallest),
+    z1 = (a - qhat qhat^T a)/|.| = (3 e_i - (1,1,1))/sqrt6 and z2 = qhat x z1.
+    Every mode, singular or not, keeps V V^T = K, V^T q = 0 and V^T V = |q|^2 I."""
+    N, c = 2, 0.1
+    s = oracle.qsystem(N, (c, c, c))
+    assert s["nsing"] == 2 * N
+    q = np.stack([s["q1"], s["q2"], s["q3"]])
+    V = np.array([[s["V11"], s["V12"]], [s["V21"], s["V22"]], [s["V31"], s["V32"]]])
+    Qq = s["Qq"]
+    for i in range(3):
+        for j in range(3):
+            kij = (Qq if i == j else 0) - q[i] * q[j]
+            assert np.allclose(np.einsum("ak,ak->k", V[i], V[j]), kij, atol=1e-12)
+    assert np.allclose(np.einsum("iak,ik->ak", V, q), 0, atol=1e-12)
+    VtV = np.einsum("iak,ibk->abk", V, V)
+    assert np.allclose(VtV[0, 0], Qq, rtol=1e-12) and np.allclose(VtV[1, 1], Qq, rtol=1e-12)
+    assert np.allclose(VtV[0, 1], 0, atol=1e-12)
+    n = 2 * N
+    f = oracle.fourier_freq(N)
+    for m in range(-N, N):
+        j = int(np.nonzero(f == m)[0][0])
+        lin = j + n * j + n * n * j
+        sq = math.sqrt(3.0) * abs(m + c)
+        qv = q[:, lin]
+        hq = qv / np.linalg.norm(qv)
+        i = int(np.argmin(np.abs(hq)))
+        z1 = (3.0 * np.eye(3)[i] - 1.0) / math.sqrt(6)
+        assert np.allclose(V[:, 0, lin], sq * z1, atol=1e-14)
+        assert np.allclose(V[:, 1, lin], sq * np.cross(np.sign(m + c) * np.ones(3) / math.sqrt(3), z1), atol=1e-14)
+    assert np.all(np.abs(s["det"]) > 1e-12)
+
+
 def test_qsystem_gamma_point_rejected(oracle):
     with pytest.raises(ValueError):
         oracle.qsystem(1, (0.0, 0.0, 0.0))
