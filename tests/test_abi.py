@@ -42,3 +42,20 @@ def test_errors_without_gpu_are_loud(pkg):
         assert "GPU" in str(e)
     else:
         raise AssertionError("MassBackend must refuse to run without a GPU")
+
+
+def test_mex_gateway_compiles_against_header():
+    """The MATLAB MEX gateway (mex/fame6d_mex.c) type-checks against include/fame6d.h: a C
+    compiler in syntax-only mode, with a stub of the MEX API calls the gateway makes
+    (tests/mex_stub/mex.h) since MATLAB is not installed.  A change of an entry point's
+    signature that the gateway does not follow fails here."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    assert cc, "no C compiler"
+    src = os.path.join(ROOT, "null--space-free-6d-projection-method_amd", "mex", "fame6d_mex.c")
+    cmd = [cc, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-std=c11", "-D__HIP_PLATFORM_AMD__",
+           "-I" + os.path.join(ROOT, "tests", "mex_stub"), "-I" + os.path.join(ROOT, "include"),
+           "-I/opt/rocm/include", src]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-4000:]
