@@ -116,6 +116,16 @@ int fame6d_mass_kernel_times(fame6d_mass* m, fame6d_kernel_time_t* out /* FAME6D
 /* mass.MscalarOp: applyMij_rank1_exact_lowmem (exp1/applyMij_rank1_exact_lowmem.m:8-22),
  * applyMij_fft (exp1/applyMij_fft.m:9-23).  x, y: NF x nrhs.  y may alias x. */
 int fame6d_mass_apply_scalar(fame6d_mass* m, int precision, const void* x, void* y, int nrhs);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * operator of fame6d_mass_apply_scalar, with its pass intermediates in a fixed 64 MiB device
+ * array of the library (one copy per GPU, held until the process ends) instead of the
+ * handle's growable workspace, so the call allocates nothing.  Limit: nrhs * NF * 16 bytes
+ * (FP64) or nrhs * NF * 8 bytes (FP32) <= 64 MiB, else FAME6D_ESIZE (N = 5: up to 4 FP64
+ * columns; N = 6: 1).  The array is fully overwritten by each call before it is read.  All
+ * calls on one GPU share it: calls that overlap on one GPU (several streams or host threads)
+ * race. */
+int fame6d_mass_apply_scalar_noalloc(fame6d_mass* m, int precision, const void* x, void* y, int nrhs);
 /* mass.Mop: applyM_component_first_rank1_scalar (exp1/applyM_component_first_rank1_scalar.m:4-19).
  * U, MU: 3NF x nrhs. */
 int fame6d_mass_apply_3comp(fame6d_mass* m, int precision, const void* U, void* MU, int nrhs);

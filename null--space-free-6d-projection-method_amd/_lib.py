@@ -120,6 +120,7 @@ SIGNATURES = {
     "fame6d_mass_profile": [P, I],
     "fame6d_mass_kernel_times": [P, C.POINTER(KernelTime)],
     "fame6d_mass_apply_scalar": [P, I, P, P, I],
+    "fame6d_mass_apply_scalar_noalloc": [P, I, P, P, I],
     "fame6d_mass_apply_3comp": [P, I, P, P, I],
     "fame6d_mass_apply_cols12": [P, I, P, P, I],
     "fame6d_mass_apply_col3": [P, I, P, P, I],

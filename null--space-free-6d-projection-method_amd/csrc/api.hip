@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "internal.hpp"
 #include "krylov_kernels.hpp"
@@ -271,6 +272,39 @@ extern "C" {
 const char* fame6d_last_error(void) { return g_last_error.c_str(); }
 int fame6d_version(void) { return 10000; }
 
+// ---------------------------------------------------------------- allocation-free apply
+// Pass intermediates of fame6d_mass_apply_scalar_noalloc: a fixed array of the library, one
+// copy per GPU, held until the process ends.  Host code never uses the name itself (on the
+// host it is only a stand-in for the symbol); it uses the device address found once per
+// device by hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {nullptr};
+static hipError_t g_scratch_status[kMaxDevices] = {hipSuccess};
+static bool g_scratch_done[kMaxDevices] = {false};
+
+// Device address of `scratch` on `device` (the current device), looked up on the first call
+// per device.  Returns nullptr and sets *status on failure.
+static void* scratch_address(int device, hipError_t* status) {
+  if (device < 0 || device >= kMaxDevices) {
+    *status = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_status[device] = e;
+    g_scratch_ptr[device] = (e == hipSuccess) ? p : nullptr;
+    g_scratch_done[device] = true;
+  }
+  *status = g_scratch_status[device];
+  return g_scratch_ptr[device];
+}
+
 // Per-workgroup min, max and sum of the permittivity samples, and a non-finite count
 // (block b writes stats[4b .. 4b+3]); the host combines the kEpsBlocks partials.
 constexpr int kEpsBlocks = 256;
@@ -403,6 +437,12 @@ int fame6d_mass_create(int N, const double* eps_host, void* stream, fame6d_mass*
     F6_HIP(hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, m->device));
     if (m->cus <= 0) throw Error(FAME6D_EHIP, "device reports no compute units");
     m->pin.init(1 << 16);
+    {
+      // the allocation-free apply's array on this device (the current one)
+      hipError_t st = hipSuccess;
+      m->noalloc_scratch = scratch_address(m->device, &st);
+      m->noalloc_status = (int)st;
+    }
     upload_eps(m, eps_host);
   } catch (...) {
     m->eps64.release();
@@ -531,6 +571,31 @@ int fame6d_mass_apply_scalar(fame6d_mass* m, int precision, const void* x, void*
     mass_apply<float>(m, (const cpx<float>*)x, m->NF, (cpx<float>*)y, m->NF, nrhs, m->stream);
   else
     throw Error(FAME6D_EARG, "precision must be FAME6D_FP64 or FAME6D_FP32");
+  F6_API_END
+}
+
+int fame6d_mass_apply_scalar_noalloc(fame6d_mass* m, int precision, const void* x, void* y, int nrhs) {
+  F6_API_BEGIN
+  F6_REQUIRE(m && x && y, FAME6D_EARG, "null argument");
+  F6_REQUIRE(nrhs >= 0, FAME6D_ESIZE, "nrhs must be >= 0");
+  F6_REQUIRE(precision == FAME6D_FP64 || precision == FAME6D_FP32, FAME6D_EARG,
+             "precision must be FAME6D_FP64 or FAME6D_FP32");
+  if (!m->noalloc_scratch)
+    throw Error(FAME6D_EHIP, std::string("hipGetSymbolAddress of the allocation-free apply's array failed: ") +
+                                 hipGetErrorString((hipError_t)m->noalloc_status));
+  const size_t per = (precision == FAME6D_FP64 ? sizeof(cpx<double>) : sizeof(cpx<float>)) * (size_t)m->NF;
+  if ((size_t)nrhs * per > kNoAllocScratchBytes)
+    throw Error(FAME6D_ESIZE, "fame6d_mass_apply_scalar_noalloc: nrhs * NF * " + std::to_string(per / m->NF) +
+                                  " bytes = " + std::to_string((size_t)nrhs * per) +
+                                  " exceeds its fixed 64 MiB intermediate array (at most " +
+                                  std::to_string(kNoAllocScratchBytes / per) + " columns at this N and precision)");
+  if (precision == FAME6D_FP64)
+    mass_apply_with<double>(m, (const cpx<double>*)x, m->NF, (cpx<double>*)y, m->NF, nrhs,
+                            (cpx<double>*)m->noalloc_scratch, m->stream);
+  else
+    mass_apply_with<float>(m, (const cpx<float>*)x, m->NF, (cpx<float>*)y, m->NF, nrhs,
+                           (cpx<float>*)m->noalloc_scratch, m->stream);
+  F6_CHECK_LAUNCH();
   F6_API_END
 }
 

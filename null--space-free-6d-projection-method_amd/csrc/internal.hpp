@@ -168,6 +168,8 @@ struct fame6d_mass {
   fame6d::DevBuf io64;             // staging for component wrappers
   fame6d::DevBuf partials;         // reduction partials
   fame6d::DevBuf scal;             // small device scalars
+  void* noalloc_scratch = nullptr;  // this device's copy of the allocation-free apply's array
+  int noalloc_status = 0;           // hipGetSymbolAddress status of that lookup
   fame6d::TileChoice tile64, tile32;
   fame6d::HostPin pin;             // page-locked staging of small transfers
   mutable fame6d::KernelTimer timer;
@@ -195,6 +197,12 @@ void tile_variant_desc(int n, bool fp32, bool pass_b, int v, int* param, int* th
 template <class T>
 void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long long ldy, int nrhs,
                 hipStream_t s);
+
+// The same with the nrhs * NF pass intermediates in a caller-supplied buffer t (nothing is
+// allocated).
+template <class T>
+void mass_apply_with(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long long ldy, int nrhs,
+                     cpx<T>* t, hipStream_t s);
 
 template <class T>
 DevBuf& work_buf(fame6d_mass* m);

@@ -278,7 +278,13 @@ void mass_apply(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long 
   if (nrhs <= 0) return;
   DevBuf& w = work_buf<T>(m);
   w.ensure(sizeof(cpx<T>) * (size_t)m->NF * (size_t)nrhs);
-  cpx<T>* t = w.as<cpx<T>>();
+  mass_apply_with(m, x, ldx, y, ldy, nrhs, w.as<cpx<T>>(), s);
+}
+
+template <class T>
+void mass_apply_with(fame6d_mass* m, const cpx<T>* x, long long ldx, cpx<T>* y, long long ldy, int nrhs,
+                     cpx<T>* t, hipStream_t s) {
+  if (nrhs <= 0) return;
   PassAArgs<T> a{};
   a.x = x;
   a.ldx = ldx;
@@ -383,6 +389,8 @@ void KernelTimer::destroy() {
   template void launch_pass_c<T>(const fame6d_mass*, int, int, const PassCArgs<T>&, hipStream_t);  \
   template void mass_apply<T>(fame6d_mass*, const cpx<T>*, long long, cpx<T>*, long long, int,     \
                               hipStream_t);                                                        \
+  template void mass_apply_with<T>(fame6d_mass*, const cpx<T>*, long long, cpx<T>*, long long, int, \
+                                   cpx<T>*, hipStream_t);                                          \
   template DevBuf& work_buf<T>(fame6d_mass*);
 F6_INST(double)
 F6_INST(float)
