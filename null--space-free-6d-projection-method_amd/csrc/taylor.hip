@@ -231,6 +231,77 @@ __global__ void k_taylor_eval_yee(int n, double h, int comp, int nb, int p, int 
   }
 }
 
+// The same evaluation factored along x-rows.  Points of one plus-grid row (fixed J, K) share
+// y and z, so for each block bI of the row and each a1 the partial sum
+//   B[bI][a1] = sum_{a2, a3} dy^a2 dz^a3 m_c[a1, a2, a3],  c = bI + nb bJ + nb^2 bK,
+// is common to all its points, and a point needs only the Horner sum over a1 of
+// dx^a1 B[bI(I)][a1] (p + 1 terms instead of L).  One wave per row: its lanes form the
+// nb (p + 1) partial sums (each over the contiguous run of terms with that a1, which the
+// alpha order a1-outermost makes consecutive in the moment column), stage them in LDS, then
+// evaluate the row's m points in order (coalesced stores).
+constexpr int kEvalRowItems = 256;  // nb * (p + 1) at most (else the point-wise kernel runs)
+__global__ __launch_bounds__(256) void k_taylor_eval_yee_rows(int n, double h, int comp, int nb, int p, int L,
+                                                              int C, const double* __restrict__ centers,
+                                                              const cpx<double>* __restrict__ mom,
+                                                              cpx<double>* __restrict__ u) {
+  __shared__ cpx<double> Bs[4][kEvalRowItems];
+  const int m = 2 * n + 3;
+  const int base = m / nb, rem = m % nb;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long row = (long long)blockIdx.x * 4 + wave;  // row = J + m K
+  const bool live = row < (long long)m * m;
+  const int J = live ? (int)(row % m) : 0, K = live ? (int)(row / m) : 0;
+  const int cJK = nb * block_of(J, base, rem) + nb * nb * block_of(K, base, rem);
+  const double y = h * ((J - (n + 1)) + (comp == 1 ? 0.5 : 0.0));
+  const double z = h * ((K - (n + 1)) + (comp == 2 ? 0.5 : 0.0));
+  const int P1 = p + 1, items = nb * P1;
+  if (live) {
+    for (int it = lane; it < items; it += 64) {
+      const int bI = it / P1, a1 = it - bI * P1;
+      const int c = bI + cJK;
+      cpx<double> acc = mk<double>(NAN, NAN);
+      if (c < C) {
+        const double dy = y - centers[3 * c + 1], dz = z - centers[3 * c + 2];
+        // first term of a1: sum over a < a1 of (p - a + 1)(p - a + 2) / 2
+        int t = 0;
+        for (int a = 0; a < a1; ++a) t += (p - a + 1) * (p - a + 2) / 2;
+        const cpx<double>* mc = mom + (long long)c * L + t;
+        const int q = p - a1;
+        acc = mk(0.0, 0.0);
+        double y2 = 1.0;
+        for (int a2 = 0; a2 <= q; ++a2) {
+          cpx<double> acc3 = mk(0.0, 0.0);
+          double z3 = 1.0;
+          for (int a3 = 0; a3 <= q - a2; ++a3) {
+            acc3 = acc3 + z3 * mc[a3];
+            z3 *= dz;
+          }
+          mc += q - a2 + 1;
+          acc = acc + y2 * acc3;
+          y2 *= dy;
+        }
+      }
+      Bs[wave][it] = acc;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  cpx<double>* ur = u + row * m;
+  for (int I = lane; I < m; I += 64) {
+    const int bI = block_of(I, base, rem);
+    const int c = bI + cJK;
+    if (c >= C) {
+      ur[I] = mk<double>(NAN, NAN);
+      continue;
+    }
+    const double dx = h * ((I - (n + 1)) + (comp == 0 ? 0.5 : 0.0)) - centers[3 * c];
+    const cpx<double>* B = Bs[wave] + bI * P1;
+    cpx<double> acc = B[p];
+    for (int a1 = p - 1; a1 >= 0; --a1) acc = dx * acc + B[a1];
+    ur[I] = acc;
+  }
+}
+
 // Direct sum u(r) = sum_j c_j exp(i q_j . r) (brute force, one thread per point).
 __global__ void k_direct(long long NF, const double* qx, const double* qy, const double* qz,
                          const cpx<double>* coeff, long long M, const double* px, const double* py,
@@ -435,9 +506,16 @@ int fame6d_taylor_evaluate_yee(int n, double h, int comp, int nblocks, int p, in
     cen.ensure(sizeof(double) * 3 * C);
     F6_HIP(hipMemcpyAsync(cen.ptr, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
     const long long m = 2LL * n + 3;
-    k_taylor_eval_yee<<<grid_for(m * m * m), 256, 0, s>>>(n, h, comp, nblocks, p, taylor_terms(p), C,
-                                                           cen.as<double>(), (const cpx<double>*)moments,
-                                                           (cpx<double>*)u);
+    if (nblocks * (p + 1) <= kEvalRowItems) {
+      const long long rows = m * m;
+      k_taylor_eval_yee_rows<<<(unsigned)((rows + 3) / 4), 256, 0, s>>>(
+          n, h, comp, nblocks, p, taylor_terms(p), C, cen.as<double>(), (const cpx<double>*)moments,
+          (cpx<double>*)u);
+    } else {
+      k_taylor_eval_yee<<<grid_for(m * m * m), 256, 0, s>>>(n, h, comp, nblocks, p, taylor_terms(p), C,
+                                                             cen.as<double>(), (const cpx<double>*)moments,
+                                                             (cpx<double>*)u);
+    }
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
     cen.release();
