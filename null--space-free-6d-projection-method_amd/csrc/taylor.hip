@@ -38,6 +38,7 @@ __host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * 
 // fit that cap without spilling and run one workgroup per CU.
 constexpr int kTChunk = 64;
 constexpr int kTZeroRow = kTaylorMaxP + 1;  // row of zeros in the power tables
+constexpr int kTRow = kTChunk + 4;          // LDS row stride of the power tables
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // Workgroups per CU an instance is compiled for: two when its accumulators leave room in
@@ -51,10 +52,16 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
                                                              const unsigned char* alist, long long span,
                                                              int Cpad2, double* part) {
   constexpr int NCOL = 16 * CT;  // real columns of this workgroup
+  // Padded LDS strides.  A wave reads pw[.][s * kTRow + k4 + kl] for up to 11 powers s and 4
+  // values of kl: with a stride of kTChunk doubles every s fell in the same banks (up to
+  // 11-way conflicts); kTChunk + 4 spreads them over 8 bank pairs per kl.  B is read as
+  // bt[kk * BROW + c * 16 + nl]: a row stride of NCOL + 16 doubles puts consecutive kk in the
+  // other half of the banks.
+  constexpr int BROW = NCOL + 16;
   const int Lpad = (L + 15) & ~15;
   const int T = Lpad / 16;
-  __shared__ double pw[3][(kTZeroRow + 1) * kTChunk];
-  __shared__ double bt[kTChunk * NCOL];
+  __shared__ double pw[3][(kTZeroRow + 1) * kTRow];
+  __shared__ double bt[kTChunk * BROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kl = lane >> 4, nl = lane & 15;
   const int cg = blockIdx.x;  // column group (8 CT centers)
@@ -66,11 +73,11 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
   for (int t = 0; t < TPW; ++t) {
     const int alpha = (wave + 4 * t) * 16 + nl;
     const bool ok = alpha < L;
-    o0[t] = (ok ? (int)alist[3 * alpha] : kTZeroRow) * kTChunk;
-    o1[t] = (ok ? (int)alist[3 * alpha + 1] : 0) * kTChunk;
-    o2[t] = (ok ? (int)alist[3 * alpha + 2] : 0) * kTChunk;
+    o0[t] = (ok ? (int)alist[3 * alpha] : kTZeroRow) * kTRow;
+    o1[t] = (ok ? (int)alist[3 * alpha + 1] : 0) * kTRow;
+    o2[t] = (ok ? (int)alist[3 * alpha + 2] : 0) * kTRow;
   }
-  for (int i = tid; i < kTChunk; i += 256) pw[0][kTZeroRow * kTChunk + i] = 0.0;
+  for (int i = tid; i < kTChunk; i += 256) pw[0][kTZeroRow * kTRow + i] = 0.0;
 
   double4_t acc[TPW][CT];
 #pragma unroll
@@ -86,9 +93,9 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
       const double a = ok ? qx[j] : 0.0, b = ok ? qy[j] : 0.0, d = ok ? qz[j] : 0.0;
       double x = ok ? 1.0 : 0.0, y = x, z = x;
       for (int s = 0; s <= p; ++s) {
-        pw[0][s * kTChunk + jj] = x;
-        pw[1][s * kTChunk + jj] = y;
-        pw[2][s * kTChunk + jj] = z;
+        pw[0][s * kTRow + jj] = x;
+        pw[1][s * kTRow + jj] = y;
+        pw[2][s * kTRow + jj] = z;
         x *= a;
         y *= b;
         z *= d;
@@ -106,8 +113,8 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
         re = v.re;
         im = v.im;
       }
-      bt[jj * NCOL + 2 * cl] = re;
-      bt[jj * NCOL + 2 * cl + 1] = im;
+      bt[jj * BROW + 2 * cl] = re;
+      bt[jj * BROW + 2 * cl + 1] = im;
     }
     __syncthreads();
 #pragma unroll 1
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
       const int kk = k4 + kl;
       double bv[CT];
 #pragma unroll
-      for (int c = 0; c < CT; ++c) bv[c] = bt[kk * NCOL + c * 16 + nl];
+      for (int c = 0; c < CT; ++c) bv[c] = bt[kk * BROW + c * 16 + nl];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
         const double av = pw[0][o0[t] + kk] * pw[1][o1[t] + kk] * pw[2][o2[t] + kk];
@@ -144,18 +151,32 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
 // Final moments: sum the mode slices, form re + i*im and multiply by i^|alpha|/alpha!.
 __global__ void k_taylor_finish(int L, int C, int Cpad2, int nslices, const double* part, const cpx<double>* calpha,
                                 cpx<double>* mom) {
+  // consecutive threads take consecutive centres of one alpha row, so every slice's row is
+  // read in order (16 bytes per thread); four running sums keep four loads in flight
   const int Lpad = (L + 15) & ~15;
   const long long tot = (long long)L * C;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
        i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i / L), a = (int)(i % L);
-    double re = 0.0, im = 0.0;
-    for (int z = 0; z < nslices; ++z) {
-      const double* row = part + ((long long)z * Lpad + a) * Cpad2;
-      re += row[2 * c];
-      im += row[2 * c + 1];
+    const int a = (int)(i / C), c = (int)(i % C);
+    const double* col = part + (long long)a * Cpad2 + 2 * c;
+    const long long zs = (long long)Lpad * Cpad2;
+    double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+    int z = 0;
+    for (; z + 4 <= nslices; z += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double2 v = *reinterpret_cast<const double2*>(col + (z + u) * zs);
+        re[u] += v.x;
+        im[u] += v.y;
+      }
     }
-    mom[(long long)c * L + a] = calpha[a] * mk(re, im);  // column-major L x C
+    for (; z < nslices; ++z) {
+      const double2 v = *reinterpret_cast<const double2*>(col + z * zs);
+      re[0] += v.x;
+      im[0] += v.y;
+    }
+    const double sr = (re[0] + re[1]) + (re[2] + re[3]), si = (im[0] + im[1]) + (im[2] + im[3]);
+    mom[(long long)c * L + a] = calpha[a] * mk(sr, si);  // column-major L x C
   }
 }
 
