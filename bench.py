@@ -115,7 +115,9 @@ def cpu_baseline(pkg, eps, N, gpu_stats, n3_gpu=None):
     kr = gpu_stats["krinv"]
     calls = kr["calls"]
     its = kr["total_fp64_iterations"]
-    checks = max(kr["mass_applies_fp64"] - 4 * calls - 3 * its, 0) / 3.0
+    # the oracle follows the reference: 4 scalar applies around each K_r^-1 call, one Mhat apply
+    # per PCG iteration and one true-residual check per call
+    checks = calls
     est = calls * 4 * t_apply + (its + checks) * t_iter
     # a complete solve, not extrapolated, at the plain-method size N = 3 (about half a minute)
     full = None

@@ -295,10 +295,12 @@ fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P)
 // energy_partials (optional): pass-B partials whose per-group sum is x' Mhat x
 // (3 * pass_b_blocks per group).  With resb set, y = resb - Mhat x instead, and res_partials
 // receive ||y||^2 (pass_c_blocks per group).
+// zw (optional, POST form): the pass-C outputs M_s(W1 z), M_s(W2 z), z = invC x, of group g
+// at zw + 2g*NF and zw + (2g+1)*NF.
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
                 hipStream_t s, bool inverse_eps = false, double* energy_partials = nullptr,
-                const cpx<T>* resb = nullptr, double* res_partials = nullptr);
+                const cpx<T>* resb = nullptr, double* res_partials = nullptr, cpx<T>* zw = nullptr);
 // Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).
 void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                   const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,

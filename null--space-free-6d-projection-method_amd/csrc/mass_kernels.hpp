@@ -144,6 +144,9 @@ struct PassCArgs {
   const cpx<T>* dotp;  // if set: partial sums of Re(dotp^H y) per workgroup
   long long lddot;
   const cpx<T>* resb;  // POST_MHAT, if set: y = resb - Mhat x (same layout as y), ||y||^2 partials
+  // POST_MHAT, optional: the transformed outputs 1 and 2 before they are combined,
+  // M_s(W1 z) and M_s(W2 z) with z = invC x, at zw + (2g)*ldy and zw + (2g+1)*ldy
+  cpx<T>* zw;
   double* partials;  // partials[g*gridDim.x + blockIdx.x]
   Skip skip;
   const int* qmap;
@@ -1059,15 +1062,21 @@ __device__ __forceinline__ void mhat_c_epilogue(const PassCArgs<T>& a, const cpx
     cpx<T>* yg = a.y + g * a.ldy + wg0;
     const cpx<T>* dg = (a.partials && a.dotp) ? a.dotp + g * a.lddot + wg0 : nullptr;
     const cpx<T>* bg = a.resb ? a.resb + g * a.ldy + wg0 : nullptr;
+    cpx<T>* zg = a.zw ? a.zw + 2 * g * a.ldy + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
       const T w1 = a.W1[q], w2 = a.W2[q];
-      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      const cpx<T> s1 = lds[E + e], s2 = lds[2 * E + e];
+      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * s1 + w2 * s2);
       if (bg) {  // the residual b - Mhat x and its ||.||^2 partials
         v = bg[e] - v;
         s += static_cast<double>(v.re) * v.re + static_cast<double>(v.im) * v.im;
       }
       yg[e] = v;
+      if (zg) {
+        zg[e] = s1;
+        zg[a.ldy + e] = s2;
+      }
       if (dg) {
         const cpx<T> d = dg[e];
         s += static_cast<double>(d.re * v.re + d.im * v.im);
@@ -1167,15 +1176,21 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
     cpx<T>* yg = a.y + g * a.ldy + wg0;
     const cpx<T>* dg = (a.partials && a.dotp) ? a.dotp + g * a.lddot + wg0 : nullptr;
     const cpx<T>* bg = a.resb ? a.resb + g * a.ldy + wg0 : nullptr;
+    cpx<T>* zg = a.zw ? a.zw + 2 * g * a.ldy + wg0 : nullptr;
     for (int e = tid; e < E; e += THREADS) {
       const long long q = qbase + e;
       const T w1 = a.W1[q], w2 = a.W2[q];
-      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * lds[E + e] + w2 * lds[2 * E + e]);
+      const cpx<T> s1 = lds[E + e], s2 = lds[2 * E + e];
+      cpx<T> v = inv_c_or(a.invC, w1, w2, q) * (lds[e] + w1 * s1 + w2 * s2);
       if (bg) {  // the residual b - Mhat x and its ||.||^2 partials
         v = bg[e] - v;
         s += static_cast<double>(v.re) * v.re + static_cast<double>(v.im) * v.im;
       }
       yg[e] = v;
+      if (zg) {
+        zg[e] = s1;
+        zg[a.ldy + e] = s2;
+      }
       if (dg) {
         const cpx<T> d = dg[e];
         s += static_cast<double>(d.re * v.re + d.im * v.im);

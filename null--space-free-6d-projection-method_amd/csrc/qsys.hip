@@ -166,7 +166,7 @@ static const T* Wt(const fame6d_qsys* S, int i) {
 template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
                 hipStream_t s, bool inverse_eps, double* energy_partials, const cpx<T>* resb,
-                double* res_partials) {
+                double* res_partials, cpx<T>* zw) {
   fame6d_mass* m = S->m;
   const long long NF = m->NF;
   DevBuf& w = work_buf<T>(m);
@@ -198,6 +198,7 @@ void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<
   c.qmap = qmap;
   c.resb = resb;
   c.partials = resb ? res_partials : nullptr;
+  c.zw = zw;
   launch_pass_c<T>(m, POST_MHAT, nact, c, s);
 }
 
@@ -241,10 +242,13 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 // `true_check` false accepts the recurrence residual without the true-residual check (for
 // the inner solves of the mixed form).  `hint` is the first chunk length, updated to the
 // iterations this call needed plus one.
+// `zw` (optional, with true_check): every true-residual apply also stores M_s(W1 z), M_s(W2 z)
+// of the checked x (z = invC x) there, so a solve that ends on a passing check (flag 0) leaves
+// them for its final x.
 template <class T>
 static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
                      cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
-                     bool true_check, int& hint_ref, std::vector<PcgOut>& out) {
+                     bool true_check, int& hint_ref, std::vector<PcgOut>& out, cpx<T>* zw = nullptr) {
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const long long NF = m->NF;
@@ -360,7 +364,7 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
       const Skip sk{need, 1};
       // r = b - Mhat x and its ||r||^2 partials in the last pass of the apply
-      mhat_apply<T>(S, nact, qmap_dev, x, r, sk, s, false, nullptr, b, part2);
+      mhat_apply<T>(S, nact, qmap_dev, x, r, sk, s, false, nullptr, b, part2, zw);
       k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, need, resume, precond ? 1 : 0);
       if (precond) {
         // z = P^{-1} r_true and rho = r'z for the groups that resume
@@ -453,9 +457,15 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   S->stats.mass_applies_fp64 += 2 * nact;
 
   std::vector<PcgOut> res;
+  bool zw_ready = false;  // w.s holds M_s [W1 z; W2 z] of the final u
   if (precision == FAME6D_FP64) {
     pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, true,
-                     S->chunk_hint, res);
+                     S->chunk_hint, res, w.s);
+    // M_s [W1 z; W2 z] of the final u came with the passing true-residual check (flag 0 after
+    // at least one check); otherwise it is formed below
+    zw_ready = true;
+    for (int g = 0; g < nact; ++g)
+      if (res[g].flag != 0 || res[g].checks < 1) zw_ready = false;
     for (int g = 0; g < nact; ++g) {
       S->stats.total_iterations += res[g].iter;
       S->stats.total_fp64_iterations += res[g].iter;
@@ -515,7 +525,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       k_accumulate_ref<float><<<dim3(gridm, nact), 256, 0, s>>>(NF, x32, NF, rs, w.u, NF);
       // FP64 residual of every group still refining
       // res = bhat - Mhat u and its ||res||^2 partials in the last pass of the apply
-      mhat_apply<double>(S, nact, qmap, w.u, w.res, skr, s, false, nullptr, w.bhat, part2);
+      mhat_apply<double>(S, nact, qmap, w.u, w.res, skr, s, false, nullptr, w.bhat, part2, w.s);
       k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, pass_c_blocks(m, false), tol);
       F6_CHECK_LAUNCH();
       for (int g = 0; g < nact; ++g)
@@ -529,9 +539,13 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       }
       if (!any) break;
     }
+    // every group's last FP64 residual apply was at its final u (converged groups are
+    // skipped afterwards), so M_s [W1 z; W2 z] of the final u is in w.s
+    zw_ready = true;
     for (int g = 0; g < nact; ++g) {
       res[g].flag = 0;
       res[g].relres = hr[g].nb > 0 ? hr[g].nr / hr[g].nb : 0.0;
+      if (!(hr[g].nb > 0)) zw_ready = false;
     }
     S->stats.refinement_cycles += cycles;
   } else {
@@ -549,24 +563,26 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     S->stats.last_flag = o.flag;
     S->stats.last_relres = o.relres;
   }
-  // z = invC u; s = M_s [W1 z; W2 z] (z and W z formed in the apply's first pass);
-  // y = V1^{-1}(mr + s)
-  pa.x = w.u;
-  pa.ldx = NF;
-  mass_apply_pre(m, PRE_ZW, nact, pa, w.s, NF, s);
+  // z = invC u; s = M_s [W1 z; W2 z] (z and W z formed in the apply's first pass, unless the
+  // solve's last residual apply left s); y = V1^{-1}(mr + s)
+  if (!zw_ready) {
+    pa.x = w.u;
+    pa.ldx = NF;
+    mass_apply_pre(m, PRE_ZW, nact, pa, w.s, NF, s);
+  }
   k_v1_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, w.s, 2 * NF, NF, S->V(0), S->V(1),
                                                         S->V(2), S->V(3), yout, 2 * NF, none, qmap);
   F6_CHECK_LAUNCH();
-  S->stats.mass_applies_fp64 += 2 * nact;
+  if (!zw_ready) S->stats.mass_applies_fp64 += 2 * nact;
   S->stats.calls += nact;
   if (outs) *outs = res;
   S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
-                                 hipStream_t, bool, double*, const cpx<double>*, double*);
+                                 hipStream_t, bool, double*, const cpx<double>*, double*, cpx<double>*);
 template void mhat_apply<float>(fame6d_qsys*, int, const int*, const cpx<float>*, cpx<float>*, Skip,
-                                hipStream_t, bool, double*, const cpx<float>*, double*);
+                                hipStream_t, bool, double*, const cpx<float>*, double*, cpx<float>*);
 template void dot_groups<double>(const fame6d_mass*, int, const cpx<double>*, long long,
                                  const cpx<double>*, long long, long long, double*, double*);
 template void dot_groups<float>(const fame6d_mass*, int, const cpx<float>*, long long,
