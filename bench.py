@@ -543,8 +543,9 @@ def probe(torch, pkg, N, tiles):
 
 
 def probe_taylor(torch, pkg):
-    """Fixed Taylor-moments workload for profiler runs: the moment GEMM of the reconstruction
-    configuration (N=4, p=10, 6^3 centers) on synthetic projected modes and coefficients."""
+    """Fixed Taylor workload for profiler runs: the moment GEMM of the reconstruction
+    configuration (N=4, p=10, 6^3 centers) on synthetic projected modes and coefficients, four
+    times, then four Yee evaluations of one component on the n = 200 window."""
     import numpy as np
     NF, p, nb = 8 ** 6, 10, 6
     rng = np.random.default_rng(0)
@@ -560,8 +561,27 @@ def probe_taylor(torch, pkg):
         pkg.taylor_moments(q, coeff, p, cen)
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
+    # the Yee evaluation of one component on the n = 200 window (403^3 plus-grid points,
+    # h = 0.0125) with the moments of that window's 6^3 block centres
+    n, h = 200, 0.0125
+    m = 2 * n + 3
+    cen_y = np.ascontiguousarray(pkg.yee_block_centers(n, h, 0, nb))
+    mom = pkg.taylor_moments(q, coeff, p, cen_y)
+    u = torch.empty(m ** 3, dtype=torch.complex128, device="cuda")
+    from importlib import import_module
+    import ctypes as C
+    lib = import_module("fame6d_amd._lib")
+    te = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lib.call("fame6d_taylor_evaluate_yee", n, h, 0, nb, p, nb ** 3, cen_y.ctypes.data_as(C.c_void_p),
+                 lib.ptr(mom.T.contiguous()), lib.ptr(u), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        te.append(time.perf_counter() - t0)
     print(json.dumps({"probe": "taylor", "NF": NF, "p": p, "monomials": L, "centers": nb ** 3,
-                      "min_ms": min(ts) * 1e3, "TFLOPs": flops / min(ts) / 1e12}), flush=True)
+                      "min_ms": min(ts) * 1e3, "TFLOPs": flops / min(ts) / 1e12,
+                      "yee_eval_n": n, "yee_points": m ** 3, "yee_eval_min_ms": min(te) * 1e3}), flush=True)
 
 
 def main():
