@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
       bt[jj * BROW + 2 * cl + 1] = im;
     }
     __syncthreads();
-#pragma unroll 1
+    // two k-steps per trip, so the LDS reads of one overlap the matrix-core ops of the other
+#pragma unroll 2
     for (int k4 = 0; k4 < kTChunk; k4 += 4) {
       const int kk = k4 + kl;
       double bv[CT];
@@ -125,6 +126,9 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
       for (int c = 0; c < CT; ++c) bv[c] = bt[kk * BROW + c * 16 + nl];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
+        // row tiles past the last alpha (the fourth wave's last tile at p = 10) issue no
+        // matrix-core ops; the test is uniform per wave
+        if (wave + 4 * t >= T) continue;
         const double av = pw[0][o0[t] + kk] * pw[1][o1[t] + kk] * pw[2][o2[t] + kk];
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[t][c], 0, 0, 0);
