@@ -252,16 +252,19 @@ int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_resta
                          double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
                          double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
 
-/* Method A statistics (make_A_eigs_top_preconditioned.m:53-54: totalMINRESIterations etc.). */
+/* Method A statistics (make_A_eigs_top_preconditioned.m:53-54 and the per-solve info of
+ * :562-580: totalMINRESIterations, stagnationRestarts, residualCorrections, ...). */
 typedef struct {
   long long solves;      /* shifted systems solved (topVectorApplications) */
-  long long iterations;  /* MINRES iterations over all segments */
-  long long segments;    /* MINRES segments (totalMINRESCalls) */
-  long long restarts;    /* segments started from the current iterate after a true-residual miss */
-  int max_iterations;
-  int plateau_accepts;   /* solves accepted at the 1e-9 plateau ceiling */
+  long long iterations;  /* MINRES iterations over all calls, correction calls included */
+  long long segments;    /* MINRES calls (segmentCount; totalMINRESCalls) */
+  long long restarts;    /* stagnation restarts and MAXIT continuations from the current iterate */
+  int max_iterations;    /* most iterations of one shifted solve */
+  int plateau_accepts;   /* solves accepted at the 1e-9 plateau / guard ceiling */
   double max_relres;     /* largest true relative residual accepted */
   double seconds;
+  long long corrections;            /* residual-correction cycles (residualCorrections) */
+  long long correction_iterations;  /* their MINRES iterations (correctionMINRESIterations) */
 } fame6d_minres_stats_t;
 /* Method A Top (make_A_eigs_top_preconditioned.m:73-113): y = (K - sigma M)^{-1} M x by
  * preconditioned MINRES with P = K + tau I, tau = sigma * mean(eps)

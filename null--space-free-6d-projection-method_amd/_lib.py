@@ -69,7 +69,8 @@ class RecoverStats(C.Structure):
 class MinresStats(C.Structure):
     _fields_ = [("solves", C.c_longlong), ("iterations", C.c_longlong), ("segments", C.c_longlong),
                 ("restarts", C.c_longlong), ("max_iterations", C.c_int), ("plateau_accepts", C.c_int),
-                ("max_relres", C.c_double), ("seconds", C.c_double)]
+                ("max_relres", C.c_double), ("seconds", C.c_double), ("corrections", C.c_longlong),
+                ("correction_iterations", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -25,6 +25,7 @@
 #include "internal.hpp"
 #include "krylov_kernels.hpp"
 #include "minres_kernels.hpp"
+#include "shift_schedule.hpp"
 
 namespace fame6d {
 
@@ -932,16 +933,18 @@ static void mass3(fame6d_mass* m, const cpx<double>* x, cpx<double>* y, Skip ski
   launch_pass_c<double>(m, POST_COPY, 3, c, s);
 }
 
-// Workspace of one MINRES solve (3NF complex vectors) and its device scalars.
+// Workspace of one MINRES solve (3NF complex vectors) and its device scalars.  x is the
+// solution of the shifted system; dx and rc are the iterate and right-hand side of a
+// residual-correction call.
 struct MrWs {
-  cpx<double>*x, *v, *vn, *Mv, *r1, *r2, *y, *z, *w, *w1, *w2, *res;
+  cpx<double>*x, *v, *vn, *Mv, *r1, *r2, *y, *z, *w, *w1, *w2, *res, *dx, *rc;
   MrState* st;
   double* part;
   double* red;  // reduced scalars
 };
 
 struct MrAcc {
-  long long solves = 0, iterations = 0, segments = 0, restarts = 0;
+  long long solves = 0, iterations = 0, segments = 0, restarts = 0, corrections = 0, correction_iterations = 0;
   int max_iterations = 0, plateau_accepts = 0;
   double max_relres = 0.0, seconds = 0.0;
 };
@@ -967,131 +970,184 @@ static void mr_rotate(MrWs& W) {
   std::swap(W.v, W.vn);
 }
 
-// Solves (K - sigma M) x = b by preconditioned MINRES (make_A_eigs_top_preconditioned.m:184-330).
-// The Paige-Saunders recurrence stops when its estimate of the P^{-1}-norm residual falls to
-// tol times its start; the true residual ||b - A x||/||b|| is then computed.  That norm weighs
-// the gradient directions by 1/tau, so the estimate can meet its target while the 2-norm
-// residual has not: a miss continues the same recurrence with the target lowered tenfold
-// (MATLAB minres likewise keeps iterating past a failed true-residual check).  A recurrence
-// that ends on its iteration budget or on breakdown restarts from the current iterate.  The
-// true residual has a floor near eps*max|q|^2*||x||/||b||, since the gradient part of x grows
-// like 1/sigma; a solve whose residual stops decreasing is accepted at the reference's
-// plateau ceiling 1e-9 (make_config_exp1bc.m:61-72) and otherwise, after 50 recovery events
-// or three checks without progress, ends with FAME6D_ENOCONV.
-static void minres_solve(fame6d_qsys* S, double sigma, double tau, const cpx<double>* b, double tol, int maxit,
-                         MrWs& W, int& hint, MrAcc& acc) {
+// ||b - A x|| / bn with A = K - sigma M; leaves b - A x in out.
+static double mr_explicit_relres(fame6d_qsys* S, double sigma, const cpx<double>* b, double bn,
+                                 const cpx<double>* x, cpx<double>* out, MrWs& W) {
+  fame6d_mass* m = S->m;
+  const long long NF = m->NF;
+  const int nb = red_blocks(NF);
+  mass3(m, x, W.Mv, no_skip(), m->stream);
+  k_mr_shifted<<<nb, kRedThreads, 0, m->stream>>>(S->Qm(0), S->Qm(1), S->Qm(2), sigma, x, W.Mv, nullptr, nullptr,
+                                                   0, out, b, NF, W.part);
+  F6_CHECK_LAUNCH();
+  const double rel = std::sqrt(reduce_read(m, W.part, nb, W.red)) / bn;
+  if (!std::isfinite(rel)) throw Error(FAME6D_ENONFINITE, "MINRES true residual is NaN/Inf.");
+  return rel;
+}
+
+// One call of MATLAB minres(A, b, tol, maxit, P, [], x0) as the reference makes it
+// (make_A_eigs_top_preconditioned.m:284-286 from x0, :619-620 from zero), with A = K - sigma M
+// and P = K + tau I.  x holds x0 (from_x) or is cleared, and receives the iterate.  A call
+// whose start already meets tol returns flag 0 after no iterations.  The Paige-Saunders
+// recurrence runs until its estimate of the P^{-1}-norm residual reaches tol * ||b||_{P^{-1}};
+// the explicit residual ||b - A x|| / ||b|| is then checked.  On a miss the recurrence goes on
+// one step at a time with a check after each, five checks in all (MATLAB's continuation past
+// a failed true-residual check), and then returns flag 3.  Flag 1: the maxit budget ran out;
+// flag 4: the Lanczos process broke down (beta = 0) without meeting the check.
+static MrCallResult mr_call(fame6d_qsys* S, double sigma, double tau, const cpx<double>* b, double bn, double tol,
+                            int maxit, bool from_x, cpx<double>* x, MrWs& W, int& hint) {
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
   const long long NF = m->NF, n3 = 3 * NF;
   const double *q1 = S->Qm(0), *q2 = S->Qm(1), *q3 = S->Qm(2);
   const int nb = red_blocks(NF);
   const int gr = grid_for(n3);
+  constexpr int kChecks = 5;
+  MrCallResult out;
+  double tol_abs = -1.0;  // target of the estimate: tol * beta1 from zero, else tol * ||b||_P^{-1}
+  if (!from_x) {
+    k_fill_zero<double><<<gr, 256, 0, s>>>(n3, x);
+    F6_HIP(hipMemcpyAsync(W.r1, b, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+  } else {
+    k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, const_cast<cpx<double>*>(b), nullptr, nullptr, W.z,
+                                            NF, W.part);
+    F6_CHECK_LAUNCH();
+    tol_abs = tol * std::sqrt(std::max(reduce_read(m, W.part, nb, W.red), 0.0));
+    out.relres = mr_explicit_relres(S, sigma, b, bn, x, W.r1, W);
+    if (out.relres <= tol) return out;
+  }
+  k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.r1, nullptr, nullptr, W.z, NF, W.part);
+  k_mr_init<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb, tol, tol_abs, maxit);
+  F6_HIP(hipMemcpyAsync(W.r2, W.r1, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w);
+  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w2);
+  k_mr_first_v<<<gr, 256, 0, s>>>(W.st, W.z, W.v, n3);
+  F6_CHECK_LAUNCH();
+  const Skip sk = skip_mr(W.st);
+  int issued = 0;
+  // issues `count` iterations; those after the one that ends the recurrence are skipped on
+  // the device, and the buffer roles are put back to where that iteration left them
+  auto issue = [&](int count) {
+    const MrWs W0 = W;
+    const int base = issued;
+    for (int c = 0; c < count; ++c, ++issued) {
+      mass3(m, W.v, W.Mv, sk, s);
+      k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.v, W.Mv, W.r1, W.st, 1, W.y, nullptr, NF,
+                                               W.part);
+      k_mr_alpha<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
+      k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.y, W.r2, W.st, W.z, NF, W.part);
+      k_mr_givens<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
+      mr_rotate(W);  // W.w is now the free slot; W.w1, W.w2 hold the two previous directions
+      k_mr_update<<<gr, 256, 0, s>>>(W.st, issued, W.vn, W.w1, W.w2, W.w, x, W.z, W.v, n3);
+      F6_CHECK_LAUNCH();
+    }
+    MrState hs{};
+    d2h_sync(m, &hs, W.st, sizeof(MrState));
+    if (hs.flag == -1) throw Error(FAME6D_ENONFINITE, "MINRES produced nonfinite scalars.");
+    if (hs.iter < issued) {
+      W = W0;
+      for (int j = base; j < hs.iter; ++j) mr_rotate(W);
+      issued = hs.iter;
+    }
+    return hs;
+  };
+  MrState hs{};
+  for (int chunk = std::max(4, hint); !hs.done; chunk = std::max(4, chunk / 2)) hs = issue(chunk);
+  if (!from_x) hint = std::max(4, hs.iter + 1);
+  for (int check = 1;; ++check) {
+    out.iter = hs.iter;
+    out.relres = mr_explicit_relres(S, sigma, b, bn, x, W.res, W);
+    if (out.relres <= tol) {
+      out.flag = 0;
+      return out;
+    }
+    if (hs.flag == 1) {
+      out.flag = 1;
+      return out;
+    }
+    if (!(hs.beta > 0.0)) {
+      out.flag = 4;
+      return out;
+    }
+    if (hs.iter >= maxit) {
+      out.flag = 1;
+      return out;
+    }
+    if (check == kChecks) {
+      out.flag = 3;
+      return out;
+    }
+    k_mr_rearm<<<1, 1, 0, s>>>(W.st, 0.0);  // no estimate target: one step, then the next check
+    F6_CHECK_LAUNCH();
+    hs = issue(1);
+  }
+}
+
+// Solves (K - sigma M) x = b to the true relative residual tol under the reference's
+// recovery schedule (make_A_eigs_top_preconditioned.m:203-560, run_shift_schedule):
+// MINRES calls from zero and then from the current iterate, residual-correction cycles
+// when a call stalls, and acceptance at the 1e-9 plateau or guard ceilings.  The explicit
+// residual has a floor near eps*max|q|^2*||x||/||b|| because the gradient part of x grows
+// like 1/sigma: a shifted system whose floor lies above 1e-9 ends with FAME6D_ENOCONV,
+// as the reference's controlled stop.
+static void minres_solve(fame6d_qsys* S, double sigma, double tau, const cpx<double>* b, double tol, int maxit,
+                         MrWs& W, int& hint, MrAcc& acc) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long n3 = 3 * m->NF;
   const auto t0 = std::chrono::steady_clock::now();
   k_dot_partials<double><<<dim3(red_blocks(n3), 1), kRedThreads, 0, s>>>(b, n3, b, n3, n3, W.part, nullptr);
   const double bn = std::sqrt(reduce_read(m, W.part, red_blocks(n3), W.red));
   if (!std::isfinite(bn)) throw Error(FAME6D_ENONFINITE, "Invalid shifted-system right-hand side.");
-  k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.x);
-  F6_CHECK_LAUNCH();
   acc.solves += 1;
-  if (!(bn > 0)) return;
-  double rel_prev = INFINITY, rel = INFINITY;
-  int events = 0, plateau = 0, stalls = 0, total = 0;
-  bool first = true, accepted = false;
-  while (!accepted) {
-    // r1 = b - A x, z = P^{-1} r1, beta1
-    if (first) {
-      F6_HIP(hipMemcpyAsync(W.r1, b, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
-    } else {
-      mass3(m, W.x, W.Mv, no_skip(), s);
-      k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.x, W.Mv, nullptr, nullptr, 0, W.r1, b, NF,
-                                               nullptr);
-    }
-    k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.r1, nullptr, nullptr, W.z, NF, W.part);
-    k_mr_init<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb, tol, std::max(0, maxit - total));
-    F6_HIP(hipMemcpyAsync(W.r2, W.r1, sizeof(cpx<double>) * n3, hipMemcpyDeviceToDevice, s));
-    k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w);
-    k_fill_zero<double><<<gr, 256, 0, s>>>(n3, W.w2);
-    k_mr_first_v<<<gr, 256, 0, s>>>(W.st, W.z, W.v, n3);
+  if (!(bn > 0)) {
+    k_fill_zero<double><<<grid_for(n3), 256, 0, s>>>(n3, W.x);
     F6_CHECK_LAUNCH();
-    const Skip sk = skip_mr(W.st);
-    const MrWs W0 = W;  // buffer roles at the segment start
-    int issued = 0, chunk = std::max(4, hint);
-    MrState hs{};
-    acc.segments += 1;
-    while (true) {  // recurrence, continued while its true residual misses
-      while (true) {
-        for (int c = 0; c < chunk; ++c, ++issued) {
-          mass3(m, W.v, W.Mv, sk, s);
-          k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.v, W.Mv, W.r1, W.st, 1, W.y, nullptr, NF,
-                                                   W.part);
-          k_mr_alpha<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
-          k_mr_precond<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, tau, W.y, W.r2, W.st, W.z, NF, W.part);
-          k_mr_givens<<<1, kRedThreads, 0, s>>>(W.st, W.part, nb);
-          mr_rotate(W);  // W.w is now the free slot; W.w1, W.w2 hold the two previous directions
-          k_mr_update<<<gr, 256, 0, s>>>(W.st, issued, W.vn, W.w1, W.w2, W.w, W.x, W.z, W.v, n3);
-          F6_CHECK_LAUNCH();
-        }
-        d2h_sync(m, &hs, W.st, sizeof(MrState));
-        if (hs.done) break;
-        chunk = std::max(4, chunk / 2);
-      }
-      if (hs.flag == -1) throw Error(FAME6D_ENONFINITE, "MINRES produced nonfinite scalars.");
-      // Iterations issued after the one that set `done` were skipped on the device: put the
-      // buffer roles back to where that iteration left them.
-      W = W0;
-      for (int j = 0; j < hs.iter; ++j) mr_rotate(W);
-      issued = hs.iter;
-      if (first) hint = hs.iter + 1;
-      // true residual
-      mass3(m, W.x, W.Mv, no_skip(), s);
-      k_mr_shifted<<<nb, kRedThreads, 0, s>>>(q1, q2, q3, sigma, W.x, W.Mv, nullptr, nullptr, 0, W.res, b, NF,
-                                               W.part);
-      rel = std::sqrt(reduce_read(m, W.part, nb, W.red)) / bn;
-      if (!std::isfinite(rel)) throw Error(FAME6D_ENONFINITE, "MINRES true residual is NaN/Inf.");
-      if (rel <= tol) {
-        accepted = true;
-        break;
-      }
-      ++events;
-      if (std::isfinite(rel_prev) && rel >= rel_prev * (1.0 - 1e-2)) ++plateau;
-      else plateau = 0;
-      if (std::isfinite(rel_prev) && !(rel < rel_prev * (1.0 - 1e-8))) ++stalls;
-      else stalls = 0;
-      rel_prev = rel;
-      const bool guard = events > 50 || stalls >= 3 || total + hs.iter >= maxit;
-      if ((plateau >= 2 || guard) && rel <= 1e-9) {
-        acc.plateau_accepts += 1;
-        accepted = true;
-        break;
-      }
-      if (guard) {
-        char msg[160];
-        std::snprintf(msg, sizeof msg,
-                      "Method A MINRES did not converge: true relres %.3e after %d segments, %d iterations", rel,
-                      events + 1, total + hs.iter);
-        throw Error(FAME6D_ENOCONV, msg);
-      }
-      if (hs.flag != 0 || !(hs.beta > 0.0)) break;  // budget or breakdown: restart from x
-      k_mr_rearm<<<1, 1, 0, s>>>(W.st, 0.1);
-      F6_CHECK_LAUNCH();
-      acc.restarts += 1;
-      chunk = 4;
-    }
-    total += hs.iter;
-    first = false;
+    return;
   }
-  acc.max_relres = std::max(acc.max_relres, rel);
-  acc.iterations += total;
-  acc.max_iterations = std::max(acc.max_iterations, total);
+  double rc_norm = 0.0;
+  auto call = [&](bool from_current) {
+    return mr_call(S, sigma, tau, b, bn, tol, maxit, from_current, W.x, W, hint);
+  };
+  auto residual = [&]() {
+    const double rel = mr_explicit_relres(S, sigma, b, bn, W.x, W.rc, W);
+    rc_norm = rel * bn;
+    return rel;
+  };
+  auto correct = [&](double corr_tol) {
+    int chint = 4;
+    MrCallResult r = mr_call(S, sigma, tau, W.rc, rc_norm, corr_tol, maxit, false, W.dx, W, chint);
+    k_accumulate<double><<<grid_for(n3), 256, 0, s>>>(n3, W.dx, 1.0, W.x);
+    F6_CHECK_LAUNCH();
+    r.relres = mr_explicit_relres(S, sigma, b, bn, W.x, W.res, W);
+    return r;
+  };
+  const ShiftOutcome o = run_shift_schedule(tol, ShiftRecovery{}, call, residual, correct, [](int) {});
+  acc.iterations += o.iterations;
+  acc.segments += o.calls;
+  acc.restarts += o.stagnation_restarts + o.maxit_restarts;
+  acc.corrections += o.corrections;
+  acc.correction_iterations += o.correction_iterations;
+  acc.max_iterations = std::max(acc.max_iterations, o.iterations);
   acc.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (o.end == kShiftFailed) {
+    char msg[280];
+    std::snprintf(msg, sizeof msg, "Method A MINRES did not converge: %s (%d calls, %d iterations)", o.message,
+                  o.calls, o.iterations);
+    throw Error(FAME6D_ENOCONV, msg);
+  }
+  if (o.end == kShiftPlateau) acc.plateau_accepts += 1;
+  acc.max_relres = std::max(acc.max_relres, o.relres);
 }
 
 static void mr_alloc(fame6d_mass* m, DevBuf& vec, DevBuf& aux, MrWs& W) {
   const size_t n3 = (size_t)3 * m->NF;
-  vec.ensure(sizeof(cpx<double>) * n3 * 12);
+  constexpr int kSlots = 14;
+  vec.ensure(sizeof(cpx<double>) * n3 * kSlots);
   cpx<double>* b0 = vec.as<cpx<double>>();
-  cpx<double>** slots[12] = {&W.x, &W.v, &W.vn, &W.Mv, &W.r1, &W.r2, &W.y, &W.z, &W.w, &W.w1, &W.w2, &W.res};
-  for (int i = 0; i < 12; ++i) *slots[i] = b0 + n3 * i;
+  cpx<double>** slots[kSlots] = {&W.x, &W.v,  &W.vn, &W.Mv, &W.r1,  &W.r2,  &W.y,
+                                 &W.z, &W.w,  &W.w1, &W.w2, &W.res, &W.dx, &W.rc};
+  for (int i = 0; i < kSlots; ++i) *slots[i] = b0 + n3 * i;
   const int nb = red_blocks((long long)n3);
   aux.ensure(sizeof(MrState) + sizeof(double) * ((size_t)nb + 64));
   W.st = aux.as<MrState>();
@@ -1107,6 +1163,8 @@ static void mr_stats_out(const MrAcc& a, fame6d_minres_stats_t* o) {
   o->restarts = a.restarts;
   o->max_iterations = a.max_iterations;
   o->plateau_accepts = a.plateau_accepts;
+  o->corrections = a.corrections;
+  o->correction_iterations = a.correction_iterations;
   o->max_relres = a.max_relres;
   o->seconds = a.seconds;
 }

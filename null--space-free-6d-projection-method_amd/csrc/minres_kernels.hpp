@@ -133,9 +133,10 @@ __device__ __forceinline__ double mr_sum(const double* partials, int nb, double*
   return block_sum<kRedThreads>(s, red);
 }
 
-// Start of a segment: beta1^2 = r0' P^{-1} r0 from the partials.
+// Start of a MINRES call: beta1^2 = r0' P^{-1} r0 from the partials; the recurrence stops
+// when |phibar| <= tol_abs, or <= tol_rel * beta1 when tol_abs < 0.
 static __global__ __launch_bounds__(kRedThreads) void k_mr_init(MrState* st, const double* partials, int nb,
-                                                                double tol_rel, int maxit) {
+                                                                double tol_rel, double tol_abs, int maxit) {
   __shared__ double red[kRedThreads];
   const double b2 = mr_sum(partials, nb, red);
   if (threadIdx.x == 0) {
@@ -146,7 +147,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_mr_init(MrState* st, con
     c.phibar = c.beta1;
     c.cs = -1.0;
     c.sn = 0.0;
-    c.tolb = tol_rel * c.beta1;
+    c.tolb = tol_abs >= 0.0 ? tol_abs : tol_rel * c.beta1;
     c.maxit = maxit;
     c.flag = isfinite(b2) ? 0 : -1;
     c.done = (!(c.beta1 > 0.0) || !isfinite(b2) || maxit <= 0) ? 1 : 0;
@@ -213,8 +214,8 @@ static __global__ void k_mr_update(const MrState* st, int issued_iter, const cpx
   }
 }
 
-// Continues a converged recurrence: the stopping threshold becomes factor*|phibar| and the
-// iteration kernels run again from the state the last iteration left.
+// Continues an ended recurrence: the stopping threshold becomes factor*|phibar| (0: no
+// threshold) and the iteration kernels run again from the state the last iteration left.
 static __global__ void k_mr_rearm(MrState* st, double factor) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->tolb = fabs(st->phibar) * factor;

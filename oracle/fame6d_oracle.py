@@ -16,8 +16,9 @@ normalisation, stacking) are those of the reference:
 Parity pins (see tests/test_oracle.py): the reference-held eigenvalues of
 exp3/make_config_experiment4.m:40, the rank-1 lattice identity of
 exp1/smoke_test_rank1_lowmem.m:18-23, dense brute force at N = 1, closed forms
-(constant eps, the inverse mass with 1/eps), and the toy-operator Lanczos test of
-exp1/smoke_test_rank1_lowmem.m:44-54.
+(constant eps, the inverse mass with 1/eps), the toy-operator Lanczos test of
+exp1/smoke_test_rank1_lowmem.m:44-54, and action traces of the Method-A recovery schedule
+derived by hand from exp1/make_A_eigs_top_preconditioned.m (tests/test_shift_schedule.py).
 """
 import math
 
@@ -526,25 +527,43 @@ def shift_precond_apply(sys, tau, x):
     return np.concatenate([inv_t * x1 + a * cqx, inv_t * x2 + b * cqx, inv_t * x3 + c * cqx])
 
 
-def minres(A, b, tol, maxit, Minv=None, x0=None):
-    """Preconditioned MINRES (Paige-Saunders; MATLAB minres as called at
-    exp1/make_A_eigs_top_preconditioned.m:284-286) for Hermitian A and HPD Minv, stopping
-    when the recurrence residual estimate phibar <= tol * beta1.  Returns (x, iterations)."""
-    x = np.zeros_like(b) if x0 is None else x0.copy()
+def minres_call(A, b, tol, maxit, Minv=None, x0=None, checks=5):
+    """One MATLAB minres(A, b, tol, maxit, P, [], x0) call as exp1/make_A_eigs_top_preconditioned.m
+    makes it (:284-286 from x0, :619-620 from zero), for Hermitian A and HPD Minv = P^{-1}.
+    Returns (x, flag, relres, iter) with relres = ||b - A x|| / ||b|| of the returned x.
+
+    MATLAB's minres is closed source; this restates its documented behaviour.  A start that
+    already meets tol returns flag 0 after no iterations.  The Paige-Saunders recurrence runs
+    until its estimate |phibar| of the P^{-1}-norm residual reaches tol * beta1 (from zero) or
+    tol * ||b||_{P^{-1}} (from x0); the explicit residual is then checked, and on a miss the
+    iteration continues with a check after every step, `checks` checks in all, before flag 3.
+    Flag 1: maxit ran out; flag 4: beta = 0 (breakdown) without meeting the check."""
     Minv = (lambda v: v) if Minv is None else Minv
-    r1 = b - A(x) if x0 is not None else b.copy()
+    nb = np.linalg.norm(b)
+    if nb == 0.0:
+        return np.zeros_like(b), 0, 0.0, 0
+    if x0 is None:
+        x = np.zeros_like(b)
+        r1 = b.copy()
+        tolb = None
+    else:
+        x = x0.copy()
+        r1 = b - A(x)
+        rel = np.linalg.norm(r1) / nb
+        if rel <= tol:
+            return x, 0, rel, 0
+        tolb = tol * math.sqrt(max(np.real(np.vdot(b, Minv(b))), 0.0))
     y = Minv(r1)
     beta1 = math.sqrt(max(np.real(np.vdot(r1, y)), 0.0))
-    if beta1 == 0.0:
-        return x, 0
+    if tolb is None:
+        tolb = tol * beta1
     oldb, beta, dbar, epsln, phibar = 0.0, beta1, 0.0, 0.0, beta1
     cs, sn = -1.0, 0.0
     w = np.zeros_like(b)
     w2 = np.zeros_like(b)
     r2 = r1.copy()
-    it = 0
-    while it < maxit:
-        it += 1
+    done_checks = 0
+    for it in range(1, maxit + 1):
         v = y / beta
         y = A(v)
         if it >= 2:
@@ -565,32 +584,176 @@ def minres(A, b, tol, maxit, Minv=None, x0=None):
         w1, w2 = w2, w
         w = (v - oldeps * w1 - delta * w2) / gamma
         x = x + phi * w
-        if abs(phibar) <= tol * beta1 or beta == 0.0:
-            break
-    return x, it
+        if done_checks > 0 or abs(phibar) <= tolb or beta == 0.0:
+            rel = np.linalg.norm(b - A(x)) / nb
+            done_checks += 1
+            if rel <= tol:
+                return x, 0, rel, it
+            if beta == 0.0:
+                return x, 4, rel, it
+            if done_checks >= checks:
+                return x, 3, rel, it
+    rel = np.linalg.norm(b - A(x)) / nb
+    return x, (0 if rel <= tol else 1), rel, maxit
 
 
-def method_a_top(sys, eps, x, N, sigma=1e-4, tol=1e-10, maxit=4000):
+# cfg.shift recovery settings of exp1/make_config_exp1bc.m:50-72
+SHIFT_RECOVERY = dict(max_recovery_events=50, max_corrections=8, max_zero_progress=2, min_progress=1e-8,
+                      corr_safety=0.25, corr_max_tol=1e-1, corr_min_tol=1e-6, plateau_ceiling=1e-9,
+                      plateau_min_events=2, plateau_reduction=1e-2, guard_ceiling=1e-9)
+
+
+def shift_recovery_schedule(tol, call, residual, correct, log=None, cfg=None):
+    """The solve loop of one Method-A shifted system (exp1/make_A_eigs_top_preconditioned.m:251-560)
+    with its residual-correction cycle (:583-660).
+
+    call(from_current) -> (flag, iter, relres): MINRES on A y = rhs from zero or from y;
+    residual() -> relres of y, keeping r = rhs - A y; correct(corr_tol) -> (flag, iter, relres):
+    MINRES on A d = r from zero, y += d, relres of the new y.  log(action) sees 'call',
+    'stagnation', 'maxit' and 'correct'.  Returns a dict with 'end' in {'converged', 'plateau',
+    'failed'}, the final 'relres' and the counters of the reference's info struct.  The two
+    ceilings are raised to tol when tol exceeds them (the reference rejects that setting)."""
+    c = dict(SHIFT_RECOVERY if cfg is None else cfg)
+    c["guard_ceiling"] = max(c["guard_ceiling"], tol)
+    c["plateau_ceiling"] = max(c["plateau_ceiling"], c["guard_ceiling"])
+    log = log or (lambda a: None)
+    info = dict(iterations=0, calls=0, stagnation_restarts=0, maxit_restarts=0, corrections=0,
+                correction_iterations=0, recovery_events=0, plateau_events=0)
+    prev = math.inf                # previousExplicitRelres
+    final = math.inf               # finalExplicitRelres
+    zero_events = 0                # zeroProgressEvents
+    x0_current = False
+
+    def end(kind, rel):
+        return dict(info, end=kind, relres=rel)
+
+    def plateau_update(rel, ref):
+        info["plateau_events"] = info["plateau_events"] + 1 if rel >= ref * (1 - c["plateau_reduction"]) else 0
+
+    def plateau_detected(rel):
+        return info["plateau_events"] >= c["plateau_min_events"] and math.isfinite(rel) \
+            and rel <= c["plateau_ceiling"]
+
+    def near_guard(rel):
+        return math.isfinite(rel) and rel <= c["guard_ceiling"]
+
+    while True:
+        if info["recovery_events"] > c["max_recovery_events"]:
+            return end("plateau" if plateau_detected(final) or near_guard(final) else "failed", final)
+        log("call")
+        flag, it, rel = call(x0_current)
+        info["calls"] += 1
+        info["iterations"] += it
+        final = rel
+        if not math.isfinite(rel):
+            return end("failed", rel)
+        if flag == 0 or rel <= tol:
+            return end("converged", rel)
+        made_progress = math.isfinite(prev) and rel < prev * (1 - c["min_progress"])
+        first_termination = not math.isfinite(prev)
+        if flag == 3:
+            if it > 0 and (first_termination or made_progress):
+                if math.isfinite(prev):
+                    plateau_update(rel, prev)
+                    if plateau_detected(rel):
+                        return end("plateau", rel)
+                log("stagnation")
+                info["stagnation_restarts"] += 1
+                info["recovery_events"] += 1
+                zero_events = 0
+                prev = rel
+                x0_current = True
+                continue
+            if info["corrections"] >= c["max_corrections"]:
+                return end("plateau" if plateau_detected(rel) or near_guard(rel) else "failed", rel)
+            log("correct")
+            before = residual()
+            if not math.isfinite(before):
+                return end("failed", before)
+            if before <= tol:
+                cflag, cit, after = 0, 0, before
+            else:
+                ctol = c["corr_safety"] * tol / max(before, np.finfo(float).tiny)
+                ctol = min(c["corr_max_tol"], max(c["corr_min_tol"], ctol))
+                cflag, cit, after = correct(ctol)
+            info["corrections"] += 1
+            info["correction_iterations"] += cit
+            info["calls"] += 1
+            info["iterations"] += cit
+            info["recovery_events"] += 1
+            final = after
+            if not math.isfinite(after):
+                return end("failed", after)
+            if after <= tol:
+                return end("converged", after)
+            plateau_update(after, rel)
+            if plateau_detected(after):
+                return end("plateau", after)
+            if after < rel * (1 - c["min_progress"]):
+                zero_events = 0
+                prev = after
+                x0_current = True
+                continue
+            zero_events += 1
+            if zero_events >= c["max_zero_progress"]:
+                return end("plateau" if plateau_detected(after) or near_guard(after) else "failed", after)
+            prev = after
+            x0_current = True
+            continue
+        if flag == 1:
+            if it <= 0:
+                return end("plateau" if near_guard(rel) else "failed", rel)
+            if math.isfinite(prev):
+                plateau_update(rel, prev)
+                if plateau_detected(rel):
+                    return end("plateau", rel)
+            log("maxit")
+            info["maxit_restarts"] += 1
+            info["recovery_events"] += 1
+            prev = rel
+            x0_current = True
+            continue
+        return end("plateau" if near_guard(rel) else "failed", rel)
+
+
+def method_a_top(sys, eps, x, N, sigma=1e-4, tol=1e-10, maxit=4000, info=None):
     """T x = (K - sigma M)^{-1} M x (exp1/make_A_eigs_top_preconditioned.m:59-60,98-113) with
-    P_A = K + tau I, tau = sigma * mean(eps); MINRES segments restart from the current
-    iterate until the true relative residual meets tol."""
+    P_A = K + tau I, tau = sigma * mean(eps), solved under the reference's recovery schedule
+    (shift_recovery_schedule).  Returns (y, iterations); `info` (a dict) receives the
+    schedule's outcome.  A solve that ends above the acceptance ceilings raises."""
     tau = sigma * float(np.mean(eps))
 
     def A(v):
         return apply_K(sys, v) - sigma * mass3_apply(eps, v, N)
 
+    def Pinv(v):
+        return shift_precond_apply(sys, tau, v)
+
     b = mass3_apply(eps, x, N)
     nb = np.linalg.norm(b)
-    y, its = minres(A, b, tol, maxit, lambda v: shift_precond_apply(sys, tau, v))
-    seg_tol = tol
-    for _ in range(20):
-        rel = np.linalg.norm(b - A(y)) / nb
-        if rel <= tol:
-            break
-        seg_tol = 0.25 * tol / rel
-        y, k = minres(A, b, seg_tol, maxit, lambda v: shift_precond_apply(sys, tau, v), x0=y)
-        its += k
-    return y, its
+    st = {"y": np.zeros_like(b), "r": None}
+
+    def call(from_current):
+        st["y"], flag, rel, it = minres_call(A, b, tol, maxit, Pinv, x0=st["y"] if from_current else None)
+        return flag, it, rel
+
+    def residual():
+        st["r"] = b - A(st["y"])
+        return np.linalg.norm(st["r"]) / nb
+
+    def correct(ctol):
+        d, flag, _, it = minres_call(A, st["r"], ctol, maxit, Pinv)
+        st["y"] = st["y"] + d
+        return flag, it, np.linalg.norm(b - A(st["y"])) / nb
+
+    if nb == 0.0:
+        return st["y"], 0
+    out = shift_recovery_schedule(tol, call, residual, correct)
+    if info is not None:
+        info.update(out)
+    if out["end"] == "failed":
+        raise RuntimeError("Method A MINRES did not converge: true relres %.3e" % out["relres"])
+    return st["y"], out["iterations"]
 
 
 def method_a_dense_lambda(sys, eps, N, sigma, k):

@@ -280,3 +280,30 @@ def test_config5_gradient_n5_64_directions(pkg):
           % (devs.max() / rms, np.median(devs / np.abs(fds))), flush=True)
     assert devs.max() < 1e-4 * rms
     assert np.median(devs / rms) < 2e-5
+
+
+@pytest.mark.parametrize("name,tol", [("eigs_N3_a04", TOL_FP64), ("eigs_N5_a04", TOL_MIXED)])
+def test_method_a_on_benchmark_material(pkg, name, tol):
+    """Method A (MINRES shift-invert on the full 3NF space, sigma = 1e-4) on the benchmark's
+    cosine material against the oracle's Method-C eigenvalues.  At N = 5 some shifted systems
+    stall near 1e-9 and finish through the reference's residual-correction cycles and plateau
+    acceptance (exp1/make_A_eigs_top_preconditioned.m:389-500); every accepted solve stays
+    within the 1e-9 ceiling, and the eigenvalues within `tol` (1e-8 at N = 5: plateau-accepted
+    solves carry residuals up to 1e-9 into the Ritz values)."""
+    print("case: Method A %s vs oracle fixture" % name, flush=True)
+    fx = load_fixture(name)
+    ref = np.array(fx["lambda"])
+    mass = pkg.MassBackend(fx["N"], material(pkg, fx))
+    sysq = pkg.QSystem(mass, fx["q"])
+    try:
+        a = pkg.run_method_A(sysq, k=fx["k"], p=40, tol=1e-10, max_restart=300, sigma=1e-4, tol_minres=1e-10,
+                             maxit_minres=4000, v0=pkg.start_vector(sysq.ndof, 20260906))
+        err = per_mode(a["lambda_"], ref)
+        print("  Method A per-mode rel err vs oracle: %s; minres %s" % (" ".join("%.1e" % e for e in err),
+                                                                      a["minres"]), flush=True)
+        assert a["stats"]["converged"] == 1
+        assert a["minres"]["max_relres"] <= 1e-9
+        assert np.all(err < tol), err.max()
+    finally:
+        sysq.close()
+        mass.close()
