@@ -640,6 +640,7 @@ int fame6d_qsys_destroy(fame6d_qsys* s) {
   for (DevBuf* b : {&s->Vd, &s->Wd, &s->Wf, &s->Qd, &s->kr64, &s->kr32, &s->st, &s->part, &s->part2,
                     &s->ints, &s->sc, &s->ref})
     b->release();
+  if (s->cg_pin) (void)hipHostFree(s->cg_pin);
   delete s;
   F6_API_END
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -259,6 +260,18 @@ struct PcgOut {
   int checks = 0;  // true-residual checks (one Mhat apply each)
 };
 
+// A one-group FP64 K_r^{-1} whose first PCG chunk, true-residual check and tail are enqueued
+// as if that check passes; its state is copied to the host with the caller's own reads, and
+// krinv_complete confirms it or finishes the solve.
+struct KrinvDeferred {
+  bool pending = false;
+  int cur = 0, issued = 0, rounds = 0, timer_first = 0, maxit = 0;
+  double tol = 0.0;
+  cpx<double>* yout = nullptr;
+  const CgState* state = nullptr;  // device state after the check (valid: finished, flag 0, checked)
+  std::chrono::steady_clock::time_point t0;
+};
+
 }  // namespace fame6d
 
 // Opaque handle of build_experiment1_q_system_C_compact for G Bloch points
@@ -280,6 +293,7 @@ struct fame6d_qsys {
   int chunk_hint = 0;                 // first chunk of the FP64 PCG (iterations of the last call + 1)
   std::vector<int> chunk_hint32 = std::vector<int>(4, 0);  // per refinement cycle of the mixed form
   std::vector<int> qmap_uploaded;  // q-slot map last copied to sc (skips repeated uploads)
+  void* cg_pin = nullptr;          // page-locked host copy of a deferred solve's CgState
   fame6d_krinv_stats_t stats{};
   const double* V(int i) const { return Vd.as<double>() + (size_t)i * G * NF; }
   const double* W64(int i) const { return Wd.as<double>() + (size_t)i * G * NF; }
@@ -301,10 +315,14 @@ template <class T>
 void mhat_apply(fame6d_qsys* S, int nact, const int* qmap, const cpx<T>* x, cpx<T>* y, Skip skip,
                 hipStream_t s, bool inverse_eps = false, double* energy_partials = nullptr,
                 const cpx<T>* resb = nullptr, double* res_partials = nullptr, cpx<T>* zw = nullptr);
-// Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).
+// Batched K_r^{-1} of make_explicit_krinv_compact (one call per group slot).  With `def`
+// (FP64, one group) the call returns without synchronising: the caller synchronises the
+// stream and then calls krinv_complete, which returns false when the enqueued tail was not
+// valid and has been formed again (the consumers of yout enqueued meanwhile must be redone).
 void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                   const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,
-                  std::vector<PcgOut>* outs);
+                  std::vector<PcgOut>* outs, KrinvDeferred* def = nullptr);
+bool krinv_complete(fame6d_qsys* S, KrinvDeferred* def);
 template <class T>
 void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, const cpx<T>* b,
                 long long ldb, long long n, double* out_re_host, double* out_im_host);

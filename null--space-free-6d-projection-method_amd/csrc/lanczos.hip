@@ -46,6 +46,21 @@ __global__ void k_scale_rsqrt_dev(long long n, cpx<double>* y, const cpx<double>
     y[i] = f * x[i];
 }
 
+// v = r / ||r|| with ||r||^2 = s[0], enqueued before the host has read ||r||: it writes nothing
+// when the step breaks down (||r|| <= 0 or < 1e-14, where the host ends the cycle instead), or,
+// with `cg`, when the K_r^{-1} solve behind r has not passed its true-residual check (the
+// host then finishes that solve and forms the step again).
+__global__ void k_scale_next(long long n, cpx<double>* y, const cpx<double>* x, const double* s,
+                             const CgState* cg) {
+  if (cg && !(cg->finished && cg->flag == 0 && cg->checks >= 1)) return;
+  const double b2 = s[0];
+  if (!(b2 > 0.0) || sqrt(b2) < 1e-14) return;
+  const double f = 1.0 / sqrt(b2);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = f * x[i];
+}
+
 // Kop: y = K x with K_xi = |q|^2 I - q q^T (assemble_K_and_nullspace_free_basis.m:110-119)
 __global__ void k_apply_k(const double* q1, const double* q2, const double* q3, const cpx<double>* x,
                           cpx<double>* y, long long n) {
@@ -384,32 +399,44 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
       slots[i] = Lg.slot;
     }
+    // a single FP64 run defers the solve's confirmation to the step's one read of scalars
+    const bool defer = na == 1 && !g_nested.on && precision == FAME6D_FP64;
+    KrinvDeferred kd;
     if (g_nested.on) {
       for (int i = 0; i < na; ++i)
         krinv_nested(S, kin + (size_t)n2 * i, qout + (size_t)n2 * i, tol_cg, g_nested.tolM, maxit_cg,
                      g_nested.maxitM, g_nested.epsC, nullptr);
     } else {
-      krinv_groups(S, precision, na, slots.data(), kin, qout, tol_cg, maxit_cg, nullptr);
+      krinv_groups(S, precision, na, slots.data(), kin, qout, tol_cg, maxit_cg, nullptr, defer ? &kd : nullptr);
     }
     ++steps;
-    for (int i = 0; i < na; ++i) {
-      LzState& Lg = L[act[i]];
-      const Basis& B = Lg.B;
-      const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
-      const cpx<double>* w = qout + (size_t)n2 * i;
-      if (Lg.justRestarted) {
-        // w couples to every kept Ritz vector: CGS2 over the whole basis (alpha = Re h1[j])
-        cgs2_step(m, n2, B, jj, w, s);
-        Lg.alpha_at = kScH + jj - 1;
-      } else {
-        lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s);
-        Lg.alpha_at = 0;
+    // where v = r / beta goes: the column it takes next, or B.v when the basis is full and a
+    // restart follows
+    auto next_dst = [&](LzState& Lg) { return Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v; };
+    auto orth = [&](const CgState* gate) {
+      for (int i = 0; i < na; ++i) {
+        LzState& Lg = L[act[i]];
+        const Basis& B = Lg.B;
+        const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
+        const cpx<double>* w = qout + (size_t)n2 * i;
+        if (Lg.justRestarted) {
+          // w couples to every kept Ritz vector: CGS2 over the whole basis (alpha = Re h1[j])
+          cgs2_step(m, n2, B, jj, w, s);
+          Lg.alpha_at = kScH + jj - 1;
+        } else {
+          lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s);
+          Lg.alpha_at = 0;
+        }
+        if (defer) {
+          k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(Lg), B.r, B.sc + 1, gate);
+          F6_CHECK_LAUNCH();
+        }
       }
-      Lg.justRestarted = false;
-    }
+    };
     // alpha = Re h1[jj-1] and beta^2 of every active run: one staged read, one sync
     std::vector<double> ab(2 * na);
-    {
+    std::vector<int> second(na, 0);  // the DGKS second sweep ran (steps away from a restart)
+    auto read_scalars = [&]() {
       const size_t per = sizeof(double) * (kScH + p);
       char* stage = static_cast<char*>(m->pin.take(per * na, s));
       for (int i = 0; i < na; ++i)
@@ -421,12 +448,24 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         const double* v = reinterpret_cast<const double*>(stage + per * i);
         ab[2 * i] = v[L[act[i]].alpha_at];
         ab[2 * i + 1] = v[1];
+        second[i] = 0;
         if (L[act[i]].alpha_at == 0) {
-          int second = 0;
-          std::memcpy(&second, v + 4, sizeof(int));
-          L[act[i]].second_sweeps += second ? 0 : 1;
+          int skipped2 = 0;
+          std::memcpy(&skipped2, v + 4, sizeof(int));
+          second[i] = skipped2 ? 0 : 1;
         }
       }
+    };
+    orth(defer ? kd.state : nullptr);
+    read_scalars();
+    if (defer && !krinv_complete(S, &kd)) {
+      // the solve needed more iterations: its tail has been formed again, and so is the step
+      orth(nullptr);
+      read_scalars();
+    }
+    for (int i = 0; i < na; ++i) {
+      L[act[i]].second_sweeps += second[i];
+      L[act[i]].justRestarted = false;
     }
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
@@ -440,11 +479,11 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         end_cycle = true;
       } else {
         Lg.beta = std::sqrt(beta2);
-        // v = r / beta goes straight into the column it takes next, or into B.v when the
-        // basis is full and a restart follows
-        cpx<double>* dst = Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v;
-        k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, dst, Lg.B.r, Lg.B.sc + 1);
-        F6_CHECK_LAUNCH();
+        cpx<double>* dst = next_dst(Lg);
+        if (!defer) {
+          k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, dst, Lg.B.r, Lg.B.sc + 1);
+          F6_CHECK_LAUNCH();
+        }
         Lg.vloc = dst;
         Lg.Alpha.push_back(alpha);
         Lg.Beta.push_back(Lg.beta);

@@ -246,89 +246,108 @@ void dot_groups(const fame6d_mass* m, int G, const cpx<T>* a, long long lda, con
 // of the checked x (z = invC x) there, so a solve that ends on a passing check (flag 0) leaves
 // them for its final x.
 template <class T>
-static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
-                     cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
-                     bool true_check, int& hint_ref, std::vector<PcgOut>& out, cpx<T>* zw = nullptr) {
-  fame6d_mass* m = S->m;
-  hipStream_t s = m->stream;
-  const long long NF = m->NF;
-  constexpr bool f32 = std::is_same<T, float>::value;
-  const int nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
-  const int nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
-  const int nbD = red_blocks(NF);
-  S->st.ensure(sizeof(CgState) * nact * 2);  // two slots: the unpreconditioned loop alternates
-  S->part.ensure(sizeof(double) * (size_t)nbE * nact);
-  S->part2.ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
-  S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
-  CgState* const st0 = S->st.as<CgState>();
-  int cur = 0;  // slot of the current state
-  auto slot = [&](int k) { return st0 + (size_t)k * nact; };
-  CgState* st = slot(0);
-  double* part = S->part.as<double>();
-  double* part2 = S->part2.as<double>();
-  int* need = S->ints.as<int>();
-  int* resume = need + nact;
-  DevBuf& w = work_buf<T>(m);
-  w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
-  cpx<T>* t = w.as<cpx<T>>();
-  const cpx<T>* zr = precond ? z : r;  // the vector the search direction is built from
-  const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
-
-  k_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, x, r, NF, part2);  // x = 0, r = b, ||b||^2
-  if (precond) {
-    mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, part);
-    k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbE, tol, maxit, maxmsteps);
-  } else {
-    k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, tol, maxit, maxmsteps);
-  }
-  F6_CHECK_LAUNCH();
-
+struct PcgRun {
+  fame6d_qsys* S;
+  int nact;
+  const int* qmap_dev;
+  bool precond, true_check;
+  int maxit;
+  cpx<T>* zw;
+  fame6d_mass* m;
+  hipStream_t s;
+  long long NF;
+  int nbE, nbC, nbD;
+  CgState* st0;
+  int cur = 0;  // slot of the current state (the unpreconditioned loop alternates two)
+  double *part, *part2;
+  int *need, *resume;
+  cpx<T> *r, *z;
   PassAArgs<T> pa{};
-  pa.x = zr;
-  pa.ldx = NF;
-  pa.t = t;
-  pa.ldt = NF;
-  pa.W1 = Wt<T>(S, 0);
-  pa.W2 = Wt<T>(S, 1);
-  pa.invC = Wt<T>(S, 2);
-  pa.ldq = NF;
-  pa.p = p;
-  pa.st = st;
-  pa.skip = skip_done(st);
-  pa.qmap = qmap_dev;
   PassCArgs<T> pc{};
-  pc.t = t;
-  pc.ldt = NF;
-  pc.y = q;
-  pc.ldy = NF;
-  pc.W1 = pa.W1;
-  pc.W2 = pa.W2;
-  pc.invC = pa.invC;
-  pc.ldq = NF;
-  pc.skip = skip_done(st);
-  pc.qmap = qmap_dev;
-  pc.partials = part2;  // ||r_new||^2
-  pc.cg = st;
-  pc.pq_partials = part;
-  pc.npq = nbE;
-  pc.x = x;
-  pc.p = p;
-  pc.r = r;
-  pc.ldx = NF;
+  double bytesA, bytesB, bytesC;
+  int timer_first = 0, issued = 0, rounds = 0;
 
-  // algorithmic HBM bytes per launch (cs: complex element, rs: real element)
-  const double cs = sizeof(cpx<T>), rs = sizeof(T), gb = (double)NF * nact;
-  const double bytesA = gb * (6 * cs + 3 * rs);  // zr, p, invC, W1, W2 in; p, 3 vectors out
-  const double bytesB = gb * (6 * cs + rs);      // 3 vectors + eps in; 3 vectors out
-  // 3 vectors, W1, W2 (and in FP32 invC; FP64 recomputes it from W1, W2), x, p, r in; x, r out
-  const double bytesC = gb * (8 * cs + (f32 ? 3 : 2) * rs);
-  const int timer_first = m->timer.pending;
+  PcgRun(fame6d_qsys* S_, int nact_, const int* qmap_, cpx<T>* x, cpx<T>* r_, cpx<T>* p, cpx<T>* q, cpx<T>* z_,
+         bool precond_, int maxit_, bool true_check_, cpx<T>* zw_)
+      : S(S_), nact(nact_), qmap_dev(qmap_), precond(precond_), true_check(true_check_), maxit(maxit_), zw(zw_),
+        m(S_->m), s(S_->m->stream), NF(S_->m->NF), r(r_), z(z_) {
+    constexpr bool f32 = std::is_same<T, float>::value;
+    nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
+    nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
+    nbD = red_blocks(NF);
+    S->st.ensure(sizeof(CgState) * nact * 2);
+    S->part.ensure(sizeof(double) * (size_t)nbE * nact);
+    S->part2.ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
+    S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
+    st0 = S->st.as<CgState>();
+    part = S->part.as<double>();
+    part2 = S->part2.as<double>();
+    need = S->ints.as<int>();
+    resume = need + nact;
+    DevBuf& w = work_buf<T>(m);
+    w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
+    cpx<T>* t = w.as<cpx<T>>();
+    CgState* st = slot(0);
+    pa.x = precond ? z : r;  // the vector the search direction is built from
+    pa.ldx = NF;
+    pa.t = t;
+    pa.ldt = NF;
+    pa.W1 = Wt<T>(S, 0);
+    pa.W2 = Wt<T>(S, 1);
+    pa.invC = Wt<T>(S, 2);
+    pa.ldq = NF;
+    pa.p = p;
+    pa.st = st;
+    pa.skip = skip_done(st);
+    pa.qmap = qmap_dev;
+    pc.t = t;
+    pc.ldt = NF;
+    pc.y = q;
+    pc.ldy = NF;
+    pc.W1 = pa.W1;
+    pc.W2 = pa.W2;
+    pc.invC = pa.invC;
+    pc.ldq = NF;
+    pc.skip = skip_done(st);
+    pc.qmap = qmap_dev;
+    pc.partials = part2;  // ||r_new||^2
+    pc.cg = st;
+    pc.pq_partials = part;
+    pc.npq = nbE;
+    pc.x = x;
+    pc.p = p;
+    pc.r = r;
+    pc.ldx = NF;
+    // algorithmic HBM bytes per launch (cs: complex element, rs: real element)
+    const double cs = sizeof(cpx<T>), rs = sizeof(T), gb = (double)NF * nact;
+    bytesA = gb * (6 * cs + 3 * rs);  // zr, p, invC, W1, W2 in; p, 3 vectors out
+    bytesB = gb * (6 * cs + rs);      // 3 vectors + eps in; 3 vectors out
+    // 3 vectors, W1, W2 (and in FP32 invC; FP64 recomputes it from W1, W2), x, p, r in; x, r out
+    bytesC = gb * (8 * cs + (f32 ? 3 : 2) * rs);
+  }
+  CgState* slot(int k) const { return st0 + (size_t)k * nact; }
+  CgState* state() const { return slot(cur); }
 
-  int chunk = hint_ref > 0 ? hint_ref : 8;
-  std::vector<CgState> hs(nact);
-  int issued = 0, rounds = 0;
-  while (true) {
+  // x = 0, r = b, ||b||, the initial states (MATLAB pcg with x0 = 0)
+  void start(const cpx<T>* b, double tol) {
+    const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
+    CgState* st = slot(0);
+    k_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, pc.x, r, NF, part2);
+    if (precond) {
+      mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, part);
+      k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbE, tol, maxit, maxmsteps);
+    } else {
+      k_cg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, tol, maxit, maxmsteps);
+    }
+    F6_CHECK_LAUNCH();
+    timer_first = m->timer.pending;
+  }
+
+  // `chunk` iterations, the pcg step of the last, then the true-residual check of the groups
+  // that stopped on the recurrence (or their acceptance without it)
+  void issue(int chunk) {
     if (++rounds > maxit + 8) throw Error(FAME6D_ENOCONV, "Mhat CG control loop did not terminate");
+    CgState* st = state();
     for (int c = 0; c < chunk; ++c, ++issued) {
       // unpreconditioned: from the second iteration of a chunk on, pass A applies the pcg step
       // of the previous iteration (reading slot cur, writing slot cur^1)
@@ -342,11 +361,11 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
       timer_end(m, bytesA, issued);
       if (fold) cur ^= 1;
-      st = slot(cur);
+      st = state();
       pc.cg = st;
       pc.skip = skip_done(st);
       timer_begin(m, TS_MHAT_PASS_B);
-      launch_pass_b<T>(m, 3, 0, nact, t, NF, skip_done(st), s, false, part);
+      launch_pass_b<T>(m, 3, 0, nact, pa.t, NF, skip_done(st), s, false, part);
       timer_end(m, bytesB, issued);
       timer_begin(m, TS_MHAT_PASS_C);
       launch_pass_c<T>(m, POST_MHAT_CG, nact, pc, s);  // alpha, x += alpha p, r -= alpha q
@@ -364,7 +383,7 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
       const Skip sk{need, 1};
       // r = b - Mhat x and its ||r||^2 partials in the last pass of the apply
-      mhat_apply<T>(S, nact, qmap_dev, x, r, sk, s, false, nullptr, b, part2, zw);
+      mhat_apply<T>(S, nact, qmap_dev, pc.x, r, sk, s, false, nullptr, bcheck, part2, zw);
       k_cg_true_check<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, need, resume, precond ? 1 : 0);
       if (precond) {
         // z = P^{-1} r_true and rho = r'z for the groups that resume
@@ -375,30 +394,64 @@ static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>
       k_cg_accept<<<1, 64, 0, s>>>(st, nact);
     }
     F6_CHECK_LAUNCH();
-    d2h_sync(m, hs.data(), st, sizeof(CgState) * nact);
-    bool all = true, any_check = false;
+  }
+  const cpx<T>* bcheck = nullptr;  // right-hand side of the true-residual check
+
+  // length of the next chunk after a read that found unfinished groups
+  static int next_chunk(const std::vector<CgState>& hs) {
+    for (const CgState& c : hs)
+      if (!c.finished && c.check_every) return 1;
+    return 2;
+  }
+  static bool all_finished(const std::vector<CgState>& hs) {
+    for (const CgState& c : hs)
+      if (!c.finished) return false;
+    return true;
+  }
+  // reads the current states (one synchronisation); true when every group has finished
+  bool read(std::vector<CgState>& hs) {
+    hs.resize(nact);
+    d2h_sync(m, hs.data(), state(), sizeof(CgState) * nact);
+    return all_finished(hs);
+  }
+  void collect(const std::vector<CgState>& hs, std::vector<PcgOut>& out, int& hint_ref) {
+    out.assign(nact, PcgOut{});
+    int max_iter = 0;
     for (int g = 0; g < nact; ++g) {
-      if (hs[g].finished) continue;
-      all = false;
-      if (hs[g].check_every) any_check = true;
+      const CgState& c = hs[g];
+      out[g].flag = c.flag;
+      out[g].iter = c.iter;
+      out[g].checks = c.checks;
+      const double nr = c.checks > 0 ? c.normr_true : c.normr;
+      out[g].relres = c.n2b > 0 ? nr / c.n2b : (std::isnan(c.n2b) ? c.n2b : 0.0);
+      max_iter = std::max(max_iter, c.iter);
     }
-    if (all) break;
-    chunk = any_check ? 1 : 2;
+    // iterations issued after the last group finished ran as skipped launches
+    m->timer.discard_tags(timer_first, max_iter);
+    hint_ref = std::max(2, max_iter + 1);
   }
-  out.assign(nact, PcgOut{});
-  int max_iter = 0;
-  for (int g = 0; g < nact; ++g) {
-    const CgState& c = hs[g];
-    out[g].flag = c.flag;
-    out[g].iter = c.iter;
-    out[g].checks = c.checks;
-    const double nr = c.checks > 0 ? c.normr_true : c.normr;
-    out[g].relres = c.n2b > 0 ? nr / c.n2b : (std::isnan(c.n2b) ? c.n2b : 0.0);
-    max_iter = std::max(max_iter, c.iter);
+  // issues chunks until every group has finished, starting from the host copy hs
+  void finish(std::vector<CgState>& hs, int chunk) {
+    while (!all_finished(hs)) {
+      issue(chunk);
+      read(hs);
+      chunk = next_chunk(hs);
+    }
   }
-  // iterations issued after the last group finished ran as skipped launches
-  m->timer.discard_tags(timer_first, max_iter);
-  hint_ref = std::max(2, max_iter + 1);
+};
+
+template <class T>
+static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
+                     cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
+                     bool true_check, int& hint_ref, std::vector<PcgOut>& out, cpx<T>* zw = nullptr) {
+  PcgRun<T> R(S, nact, qmap_dev, x, r, p, q, z, precond, maxit, true_check, zw);
+  R.bcheck = b;
+  R.start(b, tol);
+  std::vector<CgState> hs;
+  R.issue(hint_ref > 0 ? hint_ref : 8);
+  R.read(hs);
+  R.finish(hs, PcgRun<T>::next_chunk(hs));
+  R.collect(hs, out, hint_ref);
 }
 
 // ------------------------------------------------------------------ K_r^{-1}
@@ -406,9 +459,87 @@ struct KrinvWs64 {
   cpx<double>*r, *mr, *s, *bhat, *u, *cr, *cp, *cq, *res, *y;
 };
 
+// page-locked host copy of one CgState (allocated on first use, freed with the q-system)
+static CgState* cg_stage(fame6d_qsys* S) {
+  if (!S->cg_pin) F6_HIP(hipHostMalloc(&S->cg_pin, sizeof(CgState), hipHostMallocDefault));
+  return static_cast<CgState*>(S->cg_pin);
+}
+
+static KrinvWs64 krinv_ws(fame6d_qsys* S, int nact) {
+  const size_t gn = (size_t)S->m->NF * nact;
+  // FP64 workspace: r, mr, s (2NF each) + bhat, u, cr, cp, cq, res, y (NF each)
+  S->kr64.ensure(sizeof(cpx<double>) * gn * 13);
+  cpx<double>* base = S->kr64.as<cpx<double>>();
+  return KrinvWs64{base,          base + 2 * gn, base + 4 * gn,  base + 6 * gn,  base + 7 * gn,
+                   base + 8 * gn, base + 9 * gn, base + 10 * gn, base + 11 * gn, base + 12 * gn};
+}
+
+// Pass-A arguments of the modewise steps around the Mhat solve (V1 solves, W products).
+static PassAArgs<double> krinv_pre_args(fame6d_qsys* S, const int* qmap) {
+  PassAArgs<double> pa{};
+  pa.V11 = S->V(0);
+  pa.V12 = S->V(1);
+  pa.V21 = S->V(2);
+  pa.V22 = S->V(3);
+  pa.W1 = S->W64(0);
+  pa.W2 = S->W64(1);
+  pa.invC = S->W64(2);
+  pa.ldq = S->m->NF;
+  pa.skip = no_skip();
+  pa.qmap = qmap;
+  return pa;
+}
+
+// Counts the iterations of an FP64 solve and checks its outcome (make_explicit_krinv_compact.m:
+// "Mhat CG failed" unless flag 0 or the relative residual within 10 tol).
+static void krinv_account(fame6d_qsys* S, const std::vector<PcgOut>& res, double tol, bool fp64_solve) {
+  for (const PcgOut& o : res) {
+    if (fp64_solve) {
+      S->stats.total_iterations += o.iter;
+      S->stats.total_fp64_iterations += o.iter;
+      S->stats.mass_applies_fp64 += 3LL * (o.iter + o.checks);
+    }
+    if (!std::isfinite(o.relres) || o.flag < 0 || (o.flag != 0 && o.relres > 10 * tol))
+      throw Error(FAME6D_ENOCONV, "Mhat CG failed: flag=" + std::to_string(o.flag) +
+                                      ", relres=" + std::to_string(o.relres) +
+                                      ", iter=" + std::to_string(o.iter) + ".");
+    S->stats.max_iterations = std::max(S->stats.max_iterations, o.iter);
+    S->stats.max_relres = std::max(S->stats.max_relres, o.relres);
+    S->stats.last_iterations = o.iter;
+    S->stats.last_flag = o.flag;
+    S->stats.last_relres = o.relres;
+  }
+}
+
+// z = invC u; s = M_s [W1 z; W2 z] (z and W z formed in the apply's first pass, unless the
+// solve's last residual apply left s); y = V1^{-1}(mr + s)
+static void krinv_tail(fame6d_qsys* S, int nact, const int* qmap, const KrinvWs64& w, bool zw_ready,
+                       cpx<double>* yout) {
+  fame6d_mass* m = S->m;
+  const long long NF = m->NF;
+  if (!zw_ready) {
+    PassAArgs<double> pa = krinv_pre_args(S, qmap);
+    pa.x = w.u;
+    pa.ldx = NF;
+    mass_apply_pre(m, PRE_ZW, nact, pa, w.s, NF, m->stream);
+    S->stats.mass_applies_fp64 += 2 * nact;
+  }
+  k_v1_solve<double><<<dim3(grid_for(NF), nact), 256, 0, m->stream>>>(
+      w.mr, w.s, 2 * NF, NF, S->V(0), S->V(1), S->V(2), S->V(3), yout, 2 * NF, no_skip(), qmap);
+  F6_CHECK_LAUNCH();
+}
+
+// M_s [W1 z; W2 z] of the final u came with the passing true-residual check (flag 0 after at
+// least one check); otherwise the tail forms it
+static bool zw_from_check(const std::vector<PcgOut>& res) {
+  for (const PcgOut& o : res)
+    if (o.flag != 0 || o.checks < 1) return false;
+  return true;
+}
+
 void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                   const cpx<double>* qin, cpx<double>* yout, double tol, int maxit,
-                  std::vector<PcgOut>* outs) {
+                  std::vector<PcgOut>* outs, KrinvDeferred* def) {
   if (nact <= 0) return;
   fame6d_mass* m = S->m;
   hipStream_t s = m->stream;
@@ -422,55 +553,55 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   for (int v : qm)
     if (v < 0 || v >= S->G) throw Error(FAME6D_EARG, "bad q-system slot");
   if (nact > 64) throw Error(FAME6D_EARG, "at most 64 Bloch points per batched K_r^{-1}");
+  if (def && (precision != FAME6D_FP64 || nact != 1))
+    throw Error(FAME6D_EARG, "a deferred K_r^{-1} is FP64 with one group");
   if (S->qmap_uploaded != qm) {
     h2d_async(m, qmap, qm.data(), sizeof(int) * nact);
     S->qmap_uploaded = qm;
   }
 
-  // FP64 workspace: r, mr, s (2NF each) + bhat, u, cr, cp, cq, res, y (NF each)
-  S->kr64.ensure(sizeof(cpx<double>) * gn * 13);
-  cpx<double>* base = S->kr64.as<cpx<double>>();
-  KrinvWs64 w{base, base + 2 * gn, base + 4 * gn, base + 6 * gn, base + 7 * gn,
-              base + 8 * gn, base + 9 * gn, base + 10 * gn, base + 11 * gn, base + 12 * gn};
-  const Skip none = no_skip();
+  const KrinvWs64 w = krinv_ws(S, nact);
   const int gridm = grid_for(NF);
 
   // mr = M_s r with r = V1^{-*} q (both families), r formed in the apply's first pass
-  PassAArgs<double> pa{};
+  PassAArgs<double> pa = krinv_pre_args(S, qmap);
   pa.x = qin;
   pa.ldx = 2 * NF;
-  pa.V11 = S->V(0);
-  pa.V12 = S->V(1);
-  pa.V21 = S->V(2);
-  pa.V22 = S->V(3);
-  pa.W1 = S->W64(0);
-  pa.W2 = S->W64(1);
-  pa.invC = S->W64(2);
-  pa.ldq = NF;
-  pa.skip = none;
-  pa.qmap = qmap;
   mass_apply_pre(m, PRE_V1H, nact, pa, w.mr, NF, s);
   // bhat = -invC (W1 mr1 + W2 mr2)
   k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
-                                                    w.bhat, NF, none, qmap);
+                                                    w.bhat, NF, no_skip(), qmap);
   F6_CHECK_LAUNCH();
   S->stats.mass_applies_fp64 += 2 * nact;
 
   std::vector<PcgOut> res;
   bool zw_ready = false;  // w.s holds M_s [W1 z; W2 z] of the final u
+  if (def) {
+    // first chunk and its check, then the tail as if the check passed; the caller reads the
+    // state together with its own scalars and calls krinv_complete
+    PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, maxit, true, w.s);
+    R.bcheck = w.bhat;
+    R.start(w.bhat, tol);
+    R.issue(S->chunk_hint > 0 ? S->chunk_hint : 8);
+    F6_HIP(hipMemcpyAsync(cg_stage(S), R.state(), sizeof(CgState), hipMemcpyDeviceToHost, s));
+    krinv_tail(S, 1, qmap, w, true, yout);
+    def->pending = true;
+    def->cur = R.cur;
+    def->issued = R.issued;
+    def->rounds = R.rounds;
+    def->timer_first = R.timer_first;
+    def->tol = tol;
+    def->maxit = maxit;
+    def->yout = yout;
+    def->state = R.state();
+    def->t0 = t0;
+    return;
+  }
   if (precision == FAME6D_FP64) {
     pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, true,
                      S->chunk_hint, res, w.s);
-    // M_s [W1 z; W2 z] of the final u came with the passing true-residual check (flag 0 after
-    // at least one check); otherwise it is formed below
-    zw_ready = true;
-    for (int g = 0; g < nact; ++g)
-      if (res[g].flag != 0 || res[g].checks < 1) zw_ready = false;
-    for (int g = 0; g < nact; ++g) {
-      S->stats.total_iterations += res[g].iter;
-      S->stats.total_fp64_iterations += res[g].iter;
-      S->stats.mass_applies_fp64 += 3LL * (res[g].iter + res[g].checks);
-    }
+    zw_ready = zw_from_check(res);
+    krinv_account(S, res, tol, true);
   } else if (precision == FAME6D_MIXED) {
     // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.  The refinement
     // state (||b||, ||r||, converged) lives on the device; the host reads it once the
@@ -548,35 +679,38 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       if (!(hr[g].nb > 0)) zw_ready = false;
     }
     S->stats.refinement_cycles += cycles;
+    krinv_account(S, res, tol, false);
   } else {
     throw Error(FAME6D_EARG, "K_r^{-1} precision must be FAME6D_FP64 or FAME6D_MIXED");
   }
-  for (int g = 0; g < nact; ++g) {
-    const PcgOut& o = res[g];
-    if (!std::isfinite(o.relres) || o.flag < 0 || (o.flag != 0 && o.relres > 10 * tol))
-      throw Error(FAME6D_ENOCONV, "Mhat CG failed: flag=" + std::to_string(o.flag) +
-                                      ", relres=" + std::to_string(o.relres) +
-                                      ", iter=" + std::to_string(o.iter) + ".");
-    S->stats.max_iterations = std::max(S->stats.max_iterations, o.iter);
-    S->stats.max_relres = std::max(S->stats.max_relres, o.relres);
-    S->stats.last_iterations = o.iter;
-    S->stats.last_flag = o.flag;
-    S->stats.last_relres = o.relres;
-  }
-  // z = invC u; s = M_s [W1 z; W2 z] (z and W z formed in the apply's first pass, unless the
-  // solve's last residual apply left s); y = V1^{-1}(mr + s)
-  if (!zw_ready) {
-    pa.x = w.u;
-    pa.ldx = NF;
-    mass_apply_pre(m, PRE_ZW, nact, pa, w.s, NF, s);
-  }
-  k_v1_solve<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, w.s, 2 * NF, NF, S->V(0), S->V(1),
-                                                        S->V(2), S->V(3), yout, 2 * NF, none, qmap);
-  F6_CHECK_LAUNCH();
-  if (!zw_ready) S->stats.mass_applies_fp64 += 2 * nact;
+  krinv_tail(S, nact, qmap, w, zw_ready, yout);
   S->stats.calls += nact;
   if (outs) *outs = res;
   S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bool krinv_complete(fame6d_qsys* S, KrinvDeferred* def) {
+  if (!def || !def->pending) throw Error(FAME6D_EARG, "no deferred K_r^{-1} to complete");
+  def->pending = false;
+  std::vector<CgState> hs(1);
+  std::memcpy(hs.data(), cg_stage(S), sizeof(CgState));
+  const int* qmap = S->sc.as<int>();
+  const KrinvWs64 w = krinv_ws(S, 1);
+  PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, def->maxit, true, w.s);
+  R.bcheck = w.bhat;
+  R.cur = def->cur;
+  R.issued = def->issued;
+  R.rounds = def->rounds;
+  R.timer_first = def->timer_first;
+  const bool valid = PcgRun<double>::all_finished(hs) && hs[0].flag == 0 && hs[0].checks >= 1;
+  std::vector<PcgOut> res;
+  if (!valid) R.finish(hs, PcgRun<double>::next_chunk(hs));
+  R.collect(hs, res, S->chunk_hint);
+  krinv_account(S, res, def->tol, true);
+  if (!valid) krinv_tail(S, 1, qmap, w, zw_from_check(res), def->yout);
+  S->stats.calls += 1;
+  S->stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - def->t0).count();
+  return valid;
 }
 
 template void mhat_apply<double>(fame6d_qsys*, int, const int*, const cpx<double>*, cpx<double>*, Skip,
