@@ -638,7 +638,7 @@ int fame6d_qsys_destroy(fame6d_qsys* s) {
   if (!s) return FAME6D_OK;
   (void)hipStreamSynchronize(s->m->stream);
   for (DevBuf* b : {&s->Vd, &s->Wd, &s->Wf, &s->Qd, &s->kr64, &s->kr32, &s->st, &s->part, &s->part2,
-                    &s->ints, &s->sc, &s->ref})
+                    &s->ints, &s->sc, &s->ref, &s->kr64_1, &s->st_1, &s->part_1, &s->part2_1, &s->ints_1})
     b->release();
   if (s->cg_pin) (void)hipHostFree(s->cg_pin);
   delete s;

@@ -132,9 +132,12 @@ struct KernelTimer {
   void end(double alg_bytes, int tag = -1);
   hipEvent_t ev0() const { return ev[2 * pending]; }
   hipEvent_t ev1() const { return ev[2 * pending + 1]; }
-  // discards pending pairs from index `first` on whose tag is >= `threshold` (launches
-  // that the device skipped, e.g. CG iterations after convergence)
-  void discard_tags(int first, int threshold);
+  // discards pending pairs from index `first` on whose tag belongs to `owner` and whose
+  // iteration is >= `threshold` (launches that the device skipped, e.g. CG iterations after
+  // convergence); tags are owner * kTagSpan + iteration
+  static constexpr int kTagSpan = 1 << 16;
+  void discard_tags(int first, int owner, int threshold);
+  int next_owner = 0;
   void harvest();  // waits for the pending events and accumulates
   void reset();
   void destroy();
@@ -265,7 +268,9 @@ struct PcgOut {
 // krinv_complete confirms it or finishes the solve.
 struct KrinvDeferred {
   bool pending = false;
-  int cur = 0, issued = 0, rounds = 0, timer_first = 0, maxit = 0;
+  int set = 0;                  // workspace set (fame6d_qsys::*_set)
+  CgState* host_stage = nullptr;  // page-locked destination of the state copy (null: the q-system's own)
+  int cur = 0, issued = 0, rounds = 0, timer_first = 0, timer_owner = 0, maxit = 0;
   double tol = 0.0;
   cpx<double>* yout = nullptr;
   const CgState* state = nullptr;  // device state after the check (valid: finished, flag 0, checked)
@@ -294,6 +299,14 @@ struct fame6d_qsys {
   std::vector<int> chunk_hint32 = std::vector<int>(4, 0);  // per refinement cycle of the mixed form
   std::vector<int> qmap_uploaded;  // q-slot map last copied to sc (skips repeated uploads)
   void* cg_pin = nullptr;          // page-locked host copy of a deferred solve's CgState
+  // second workspace set of the one-group FP64 K_r^{-1}: a solve enqueued while the one before
+  // it is still unconfirmed works in the other set, so that one can still be continued
+  fame6d::DevBuf kr64_1, st_1, part_1, part2_1, ints_1;
+  fame6d::DevBuf& kr64_set(int k) { return k ? kr64_1 : kr64; }
+  fame6d::DevBuf& st_set(int k) { return k ? st_1 : st; }
+  fame6d::DevBuf& part_set(int k) { return k ? part_1 : part; }
+  fame6d::DevBuf& part2_set(int k) { return k ? part2_1 : part2; }
+  fame6d::DevBuf& ints_set(int k) { return k ? ints_1 : ints; }
   fame6d_krinv_stats_t stats{};
   const double* V(int i) const { return Vd.as<double>() + (size_t)i * G * NF; }
   const double* W64(int i) const { return Wd.as<double>() + (size_t)i * G * NF; }

@@ -320,6 +320,128 @@ void set_nested(bool on, double tolM, int maxitM, double epsC) {
   g_nested.epsC = epsC;
 }
 
+// The Lanczos loop of one FP64 run with the host one step ahead of the device.  Each step's
+// K_r^{-1} solve is enqueued with its first PCG chunk, true-residual check and tail (the
+// deferred form of krinv_groups), followed by the orthogonalisation and the scaling of the
+// next Lanczos vector into its column, which does nothing unless the solve passed its check
+// and the step did not break down.  While the cycle continues, the next step's solve is
+// enqueued at once on that column (in the other workspace set) before the host waits for
+// this step's scalars, so the device never waits for the host between steps.  A solve that
+// needed more iterations is continued from its own workspace set and its step is formed
+// again; the next step enqueued meanwhile read a column that was not yet written and is
+// dropped, as it is after a breakdown.  The arithmetic is that of the one-read loop.
+static void lanczos_run_ahead(fame6d_qsys* S, LzState& Lg, int k0, int p, double tol, int maxRestart,
+                              double tol_cg, int maxit_cg, cpx<double>* qout, DevBuf& ubuf, int& steps) {
+  fame6d_mass* m = S->m;
+  hipStream_t s = m->stream;
+  const long long n2 = 2 * m->NF;
+  const int gr = grid_for(n2);
+  struct Stage {
+    CgState cg;
+    double sc[kScH + kLzMaxP];
+  };
+  struct HostStages {
+    Stage* p = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~HostStages() {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+      if (p) (void)hipHostFree(p);
+    }
+  } hst;
+  F6_HIP(hipHostMalloc(reinterpret_cast<void**>(&hst.p), sizeof(Stage) * 2, hipHostMallocDefault));
+  for (hipEvent_t& e : hst.ev) F6_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  KrinvDeferred kd[2];
+  auto enqueue_solve = [&](int k, const cpx<double>* vin) {
+    kd[k] = KrinvDeferred{};
+    kd[k].set = k;
+    kd[k].host_stage = &hst.p[k].cg;
+    int slot = Lg.slot;
+    krinv_groups(S, FAME6D_FP64, 1, &slot, vin, qout, tol_cg, maxit_cg, nullptr, &kd[k]);
+  };
+  auto next_dst = [&]() { return Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v; };
+  auto enqueue_orth = [&](const CgState* gate) {
+    const int jj = Lg.jj;
+    if (Lg.justRestarted) {
+      // w couples to every kept Ritz vector: CGS2 over the whole basis (alpha = Re h1[j])
+      cgs2_step(m, n2, Lg.B, jj, qout, s);
+      Lg.alpha_at = kScH + jj - 1;
+    } else {
+      lanczos_orth_step(m, n2, Lg.B, jj, qout, jj >= 2 ? Lg.beta : 0.0, s);
+      Lg.alpha_at = 0;
+    }
+    k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(), Lg.B.r, Lg.B.sc + 1, gate);
+    F6_CHECK_LAUNCH();
+  };
+  auto stage_scalars = [&](int k) {
+    F6_HIP(hipMemcpyAsync(hst.p[k].sc, Lg.B.sc, sizeof(double) * (kScH + Lg.jj), hipMemcpyDeviceToHost, s));
+    F6_HIP(hipEventRecord(hst.ev[k], s));
+  };
+
+  int k = 0;
+  bool queued = false;  // this step's solve is already in the stream (slot k)
+  while (!Lg.done) {
+    const int jj = Lg.jj;  // columns 0..jj-1 of Q are set, column jj-1 is the current q_j
+    if (!queued) {
+      cpx<double>* Qj = Lg.B.Q + (size_t)n2 * (jj - 1);
+      if (Lg.vloc != Qj)  // after the start and after restarts
+        F6_HIP(hipMemcpyAsync(Qj, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      enqueue_solve(k, Qj);
+    }
+    enqueue_orth(kd[k].state);
+    stage_scalars(k);
+    // the next step's solve, on the column this step's scaling writes
+    bool ahead = jj < p;
+    if (ahead) enqueue_solve(k ^ 1, Lg.B.Q + (size_t)n2 * jj);
+    F6_HIP(hipEventSynchronize(hst.ev[k]));
+    ++steps;
+    if (!krinv_complete(S, &kd[k])) {
+      // the solve was continued and its tail formed again: so is the step
+      enqueue_orth(nullptr);
+      stage_scalars(k);
+      F6_HIP(hipEventSynchronize(hst.ev[k]));
+      ahead = false;
+    }
+    const double* v = hst.p[k].sc;
+    const double alpha = v[Lg.alpha_at], beta2 = v[1];
+    if (Lg.alpha_at == 0) {
+      int skipped2 = 0;
+      std::memcpy(&skipped2, v + 4, sizeof(int));
+      Lg.second_sweeps += skipped2 ? 0 : 1;
+    }
+    Lg.justRestarted = false;
+    if (!std::isfinite(alpha) || !std::isfinite(beta2))
+      throw Error(FAME6D_ENONFINITE, "Lanczos inverse action returned invalid data.");
+    bool end_cycle = false;
+    if (beta2 <= 0 || std::sqrt(beta2) < 1e-14) {  // the scaling wrote nothing
+      Lg.beta = std::sqrt(std::max(beta2, 0.0));
+      Lg.breakdown = true;
+      end_cycle = true;
+      ahead = false;
+    } else {
+      Lg.beta = std::sqrt(beta2);
+      Lg.vloc = next_dst();
+      Lg.Alpha.push_back(alpha);
+      Lg.Beta.push_back(Lg.beta);
+      Lg.kk += 1;
+      if (Lg.jj >= p) end_cycle = true;
+      else Lg.jj += 1;
+    }
+    if (!ahead) kd[k ^ 1].pending = false;  // dropped: its kernels only touch workspace set k^1
+    if (end_cycle) {
+      // the restart rewrites the basis columns in place: the newest v must be in B.v
+      if (Lg.vloc != Lg.B.v) {
+        F6_HIP(hipMemcpyAsync(Lg.B.v, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+        Lg.vloc = Lg.B.v;
+      }
+      if (restart_step(Lg, k0, p, tol, maxRestart, n2, ubuf, s)) Lg.done = true;
+    }
+    queued = ahead;
+    k ^= 1;
+  }
+  F6_HIP(hipStreamSynchronize(s));
+}
+
 void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int maxRestart,
                    double tol_cg, int maxit_cg, const cpx<double>* v0, double* lambda, cpx<double>* Y,
                    fame6d_lanczos_stats_t* stats, const RitzSelect* sel) {
@@ -379,6 +501,8 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   }
 
   int steps = 0;
+  if (G == 1 && !g_nested.on && precision == FAME6D_FP64)
+    lanczos_run_ahead(S, L[0], k0, p, tol, maxRestart, tol_cg, maxit_cg, qout, ubuf, steps);
   while (true) {
     std::vector<int> act;
     for (int g = 0; g < G; ++g)

@@ -349,9 +349,9 @@ void KernelTimer::end(double b, int tg) {
   armed = fired = false;
 }
 
-void KernelTimer::discard_tags(int first, int threshold) {
+void KernelTimer::discard_tags(int first, int owner, int threshold) {
   for (int i = std::max(first, 0); i < pending; ++i)
-    if (tag[i] >= threshold) bytes_pending[i] = -1.0;
+    if (tag[i] >= 0 && tag[i] / kTagSpan == owner && tag[i] % kTagSpan >= threshold) bytes_pending[i] = -1.0;
 }
 
 void KernelTimer::harvest() {

@@ -266,23 +266,25 @@ struct PcgRun {
   PassCArgs<T> pc{};
   double bytesA, bytesB, bytesC;
   int timer_first = 0, issued = 0, rounds = 0;
+  int owner;  // timer tag owner of this run's launches
 
   PcgRun(fame6d_qsys* S_, int nact_, const int* qmap_, cpx<T>* x, cpx<T>* r_, cpx<T>* p, cpx<T>* q, cpx<T>* z_,
-         bool precond_, int maxit_, bool true_check_, cpx<T>* zw_)
+         bool precond_, int maxit_, bool true_check_, cpx<T>* zw_, int set = 0)
       : S(S_), nact(nact_), qmap_dev(qmap_), precond(precond_), true_check(true_check_), maxit(maxit_), zw(zw_),
         m(S_->m), s(S_->m->stream), NF(S_->m->NF), r(r_), z(z_) {
+    owner = m->timer.next_owner++ & 0x3fff;
     constexpr bool f32 = std::is_same<T, float>::value;
     nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
     nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
     nbD = red_blocks(NF);
-    S->st.ensure(sizeof(CgState) * nact * 2);
-    S->part.ensure(sizeof(double) * (size_t)nbE * nact);
-    S->part2.ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
-    S->ints.ensure(sizeof(int) * (size_t)nact * 2 + 64);
-    st0 = S->st.as<CgState>();
-    part = S->part.as<double>();
-    part2 = S->part2.as<double>();
-    need = S->ints.as<int>();
+    S->st_set(set).ensure(sizeof(CgState) * nact * 2);
+    S->part_set(set).ensure(sizeof(double) * (size_t)nbE * nact);
+    S->part2_set(set).ensure(sizeof(double) * ((size_t)std::max(nbC, nbD) * nact + 8));
+    S->ints_set(set).ensure(sizeof(int) * (size_t)nact * 2 + 64);
+    st0 = S->st_set(set).as<CgState>();
+    part = S->part_set(set).as<double>();
+    part2 = S->part2_set(set).as<double>();
+    need = S->ints_set(set).as<int>();
     resume = need + nact;
     DevBuf& w = work_buf<T>(m);
     w.ensure(sizeof(cpx<T>) * (size_t)NF * 3 * nact);
@@ -326,6 +328,7 @@ struct PcgRun {
     bytesC = gb * (8 * cs + (f32 ? 3 : 2) * rs);
   }
   CgState* slot(int k) const { return st0 + (size_t)k * nact; }
+  int tag() const { return owner * KernelTimer::kTagSpan + std::min(issued, KernelTimer::kTagSpan - 1); }
   CgState* state() const { return slot(cur); }
 
   // x = 0, r = b, ||b||, the initial states (MATLAB pcg with x0 = 0)
@@ -359,17 +362,17 @@ struct PcgRun {
       pa.nrr = nbC;
       timer_begin(m, TS_MHAT_PASS_A);
       launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
-      timer_end(m, bytesA, issued);
+      timer_end(m, bytesA, tag());
       if (fold) cur ^= 1;
       st = state();
       pc.cg = st;
       pc.skip = skip_done(st);
       timer_begin(m, TS_MHAT_PASS_B);
       launch_pass_b<T>(m, 3, 0, nact, pa.t, NF, skip_done(st), s, false, part);
-      timer_end(m, bytesB, issued);
+      timer_end(m, bytesB, tag());
       timer_begin(m, TS_MHAT_PASS_C);
       launch_pass_c<T>(m, POST_MHAT_CG, nact, pc, s);  // alpha, x += alpha p, r -= alpha q
-      timer_end(m, bytesC, issued);
+      timer_end(m, bytesC, tag());
       if (precond) {
         mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
         k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, part, nbE);
@@ -427,7 +430,7 @@ struct PcgRun {
       max_iter = std::max(max_iter, c.iter);
     }
     // iterations issued after the last group finished ran as skipped launches
-    m->timer.discard_tags(timer_first, max_iter);
+    m->timer.discard_tags(timer_first, owner, max_iter);
     hint_ref = std::max(2, max_iter + 1);
   }
   // issues chunks until every group has finished, starting from the host copy hs
@@ -465,11 +468,11 @@ static CgState* cg_stage(fame6d_qsys* S) {
   return static_cast<CgState*>(S->cg_pin);
 }
 
-static KrinvWs64 krinv_ws(fame6d_qsys* S, int nact) {
+static KrinvWs64 krinv_ws(fame6d_qsys* S, int nact, int set = 0) {
   const size_t gn = (size_t)S->m->NF * nact;
   // FP64 workspace: r, mr, s (2NF each) + bhat, u, cr, cp, cq, res, y (NF each)
-  S->kr64.ensure(sizeof(cpx<double>) * gn * 13);
-  cpx<double>* base = S->kr64.as<cpx<double>>();
+  S->kr64_set(set).ensure(sizeof(cpx<double>) * gn * 13);
+  cpx<double>* base = S->kr64_set(set).as<cpx<double>>();
   return KrinvWs64{base,          base + 2 * gn, base + 4 * gn,  base + 6 * gn,  base + 7 * gn,
                    base + 8 * gn, base + 9 * gn, base + 10 * gn, base + 11 * gn, base + 12 * gn};
 }
@@ -560,7 +563,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     S->qmap_uploaded = qm;
   }
 
-  const KrinvWs64 w = krinv_ws(S, nact);
+  const KrinvWs64 w = krinv_ws(S, nact, def ? def->set : 0);
   const int gridm = grid_for(NF);
 
   // mr = M_s r with r = V1^{-*} q (both families), r formed in the apply's first pass
@@ -579,17 +582,19 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   if (def) {
     // first chunk and its check, then the tail as if the check passed; the caller reads the
     // state together with its own scalars and calls krinv_complete
-    PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, maxit, true, w.s);
+    PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, maxit, true, w.s, def->set);
     R.bcheck = w.bhat;
     R.start(w.bhat, tol);
     R.issue(S->chunk_hint > 0 ? S->chunk_hint : 8);
-    F6_HIP(hipMemcpyAsync(cg_stage(S), R.state(), sizeof(CgState), hipMemcpyDeviceToHost, s));
+    F6_HIP(hipMemcpyAsync(def->host_stage ? def->host_stage : cg_stage(S), R.state(), sizeof(CgState),
+                          hipMemcpyDeviceToHost, s));
     krinv_tail(S, 1, qmap, w, true, yout);
     def->pending = true;
     def->cur = R.cur;
     def->issued = R.issued;
     def->rounds = R.rounds;
     def->timer_first = R.timer_first;
+    def->timer_owner = R.owner;
     def->tol = tol;
     def->maxit = maxit;
     def->yout = yout;
@@ -693,10 +698,11 @@ bool krinv_complete(fame6d_qsys* S, KrinvDeferred* def) {
   if (!def || !def->pending) throw Error(FAME6D_EARG, "no deferred K_r^{-1} to complete");
   def->pending = false;
   std::vector<CgState> hs(1);
-  std::memcpy(hs.data(), cg_stage(S), sizeof(CgState));
+  std::memcpy(hs.data(), def->host_stage ? def->host_stage : cg_stage(S), sizeof(CgState));
   const int* qmap = S->sc.as<int>();
-  const KrinvWs64 w = krinv_ws(S, 1);
-  PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, def->maxit, true, w.s);
+  const KrinvWs64 w = krinv_ws(S, 1, def->set);
+  PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, def->maxit, true, w.s, def->set);
+  R.owner = def->timer_owner;
   R.bcheck = w.bhat;
   R.cur = def->cur;
   R.issued = def->issued;
