@@ -726,13 +726,22 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
 // Norm bookkeeping of the re-orthogonalisation (one workgroup).  sc[1] = ||r||^2 of the
 // current vector, sc[2] = ||r||^2 before the first sweep, the int at sc + 4 = 1 when the
 // second sweep is not needed.  stage 0: after the three-term update; stage 1: after the
-// first sweep, which decides the second one by the DGKS test ||r_after|| < ||r_before||/sqrt(2);
-// stage 2: after the second sweep (taken only when it ran).
+// first sweep, which decides the second one by the DGKS test ||r_after|| < ||r_before||/sqrt(2)
+// (with `pre`, the stage-0 sum of the same step first); stage 2: after the second sweep (taken
+// only when it ran).
 static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, const double* partials, int nb,
-                                                                int stage) {
+                                                                int stage, const double* pre = nullptr,
+                                                                int npre = 0) {
   __shared__ double red[kRedThreads];
   int* skip = reinterpret_cast<int*>(sc + 4);
   if (stage == 2 && *skip) return;
+  if (pre) {  // stage 0 in the same launch: ||r||^2 before the sweep
+    double s0 = 0.0;
+    for (int i = threadIdx.x; i < npre; i += kRedThreads) s0 += pre[i];
+    s0 = block_sum<kRedThreads>(s0, red);
+    if (threadIdx.x == 0) sc[2] = s0;
+    __syncthreads();
+  }
   double s = 0.0;
   for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
   s = block_sum<kRedThreads>(s, red);

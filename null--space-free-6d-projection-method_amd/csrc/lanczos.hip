@@ -47,12 +47,14 @@ __global__ void k_scale_rsqrt_dev(long long n, cpx<double>* y, const cpx<double>
 }
 
 // v = r / ||r|| with ||r||^2 = s[0], enqueued before the host has read ||r||: it writes nothing
-// when the step breaks down (||r|| <= 0 or < 1e-14, where the host ends the cycle instead), or,
+// when the step breaks down (||r|| <= 0 or < 1e-14, where the host ends the cycle instead);
 // with `cg`, when the K_r^{-1} solve behind r has not passed its true-residual check (the
-// host then finishes that solve and forms the step again).
+// host then finishes that solve and forms the step again); with `no_second`, when the DGKS
+// test asked for a second sweep (the host then runs it and scales again).
 __global__ void k_scale_next(long long n, cpx<double>* y, const cpx<double>* x, const double* s,
-                             const CgState* cg) {
+                             const CgState* cg, const int* no_second) {
   if (cg && !(cg->finished && cg->flag == 0 && cg->checks >= 1)) return;
+  if (no_second && *no_second == 0) return;  // a second Gram-Schmidt sweep comes first
   const double b2 = s[0];
   if (!(b2 > 0.0) || sqrt(b2) < 1e-14) return;
   const double f = 1.0 / sqrt(b2);
@@ -152,6 +154,16 @@ static void cgs2_step(fame6d_mass* m, long long n, const Basis& B, int jj, const
   F6_CHECK_LAUNCH();
 }
 
+// The second sweep of lanczos_orth_step, r -= Q (Q^H r) with ||r||^2 into sc[1]; with `sk`
+// taken only when the DGKS test of the first sweep asked for it (the int at sc + 4 is 0).
+static void lanczos_second_sweep(fame6d_mass* m, long long n, const Basis& B, int jj, hipStream_t s, Skip sk) {
+  const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
+  gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s, sk);
+  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n, B.part, sk);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 2);
+  F6_CHECK_LAUNCH();
+}
+
 // A Lanczos step away from a restart (Lanczos_hybrid_identity_gpuop.m:55-63): alpha =
 // Re q_j^H w into sc[0]; the three-term update r = w - alpha q_j - beta q_{j-1}; one
 // classical Gram-Schmidt sweep r -= Q (Q^H r) over the whole basis; and a second sweep only
@@ -161,7 +173,7 @@ static void cgs2_step(fame6d_mass* m, long long n, const Basis& B, int jj, const
 // needs repeating; the reference always runs two modified Gram-Schmidt sweeps.
 // ||r||^2 ends in sc[1].
 static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* w,
-                              double beta, hipStream_t s) {
+                              double beta, hipStream_t s, bool second = true) {
   const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
   const double vb = 16.0 * n;
   const cpx<double>* qj = B.Q + (size_t)n * (jj - 1);
@@ -172,19 +184,14 @@ static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int j
   double* npart = B.part + (size_t)2 * nb * kLzMaxP;
   const ThreeTerm tt{B.sc, beta, npart};
   gemv_c(m, n, B, jj, w, B.sc + kScH, B.sc + kScHim, s, no_skip(), &tt);
-  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, npart, nb, 0);
-  F6_CHECK_LAUNCH();
   timer_begin(m, TS_GEMV_N);
   launch_k(m, k_gemv_n_out, dim3(gn), dim3(256), s, B.Q, n, jj, B.sc + kScH, B.sc + kScHim, w, B.r, n,
            B.part, no_skip(), (const double*)B.sc, beta);
   timer_end(m, (jj + 2) * vb);
-  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1);
+  // ||r||^2 before (the three-term update) and after the sweep, and the DGKS test
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1, npart, nb);
   F6_CHECK_LAUNCH();
-  const Skip sk{reinterpret_cast<const int*>(B.sc + 4), 0};
-  gemv_c(m, n, B, jj, B.r, B.sc + kScH2, B.sc + kScH2im, s, sk);
-  k_gemv_n_out<<<gn, 256, 0, s>>>(B.Q, n, jj, B.sc + kScH2, B.sc + kScH2im, B.r, B.r, n, B.part, sk);
-  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 2);
-  F6_CHECK_LAUNCH();
+  if (second) lanczos_second_sweep(m, n, B, jj, s, Skip{reinterpret_cast<const int*>(B.sc + 4), 0});
 }
 
 // Host-side state of one Lanczos run.
@@ -360,6 +367,8 @@ static void lanczos_run_ahead(fame6d_qsys* S, LzState& Lg, int k0, int p, double
     krinv_groups(S, FAME6D_FP64, 1, &slot, vin, qout, tol_cg, maxit_cg, nullptr, &kd[k]);
   };
   auto next_dst = [&]() { return Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v; };
+  // the second DGKS sweep is not enqueued ahead: the host runs it when the test asked for it
+  const int* no_second = reinterpret_cast<const int*>(Lg.B.sc + 4);
   auto enqueue_orth = [&](const CgState* gate) {
     const int jj = Lg.jj;
     if (Lg.justRestarted) {
@@ -367,10 +376,10 @@ static void lanczos_run_ahead(fame6d_qsys* S, LzState& Lg, int k0, int p, double
       cgs2_step(m, n2, Lg.B, jj, qout, s);
       Lg.alpha_at = kScH + jj - 1;
     } else {
-      lanczos_orth_step(m, n2, Lg.B, jj, qout, jj >= 2 ? Lg.beta : 0.0, s);
+      lanczos_orth_step(m, n2, Lg.B, jj, qout, jj >= 2 ? Lg.beta : 0.0, s, false);
       Lg.alpha_at = 0;
     }
-    k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(), Lg.B.r, Lg.B.sc + 1, gate);
+    k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(), Lg.B.r, Lg.B.sc + 1, gate, Lg.alpha_at == 0 ? no_second : nullptr);
     F6_CHECK_LAUNCH();
   };
   auto stage_scalars = [&](int k) {
@@ -402,13 +411,22 @@ static void lanczos_run_ahead(fame6d_qsys* S, LzState& Lg, int k0, int p, double
       F6_HIP(hipEventSynchronize(hst.ev[k]));
       ahead = false;
     }
-    const double* v = hst.p[k].sc;
-    const double alpha = v[Lg.alpha_at], beta2 = v[1];
     if (Lg.alpha_at == 0) {
       int skipped2 = 0;
-      std::memcpy(&skipped2, v + 4, sizeof(int));
-      Lg.second_sweeps += skipped2 ? 0 : 1;
+      std::memcpy(&skipped2, hst.p[k].sc + 4, sizeof(int));
+      if (!skipped2) {
+        // the DGKS test asked for the second sweep: run it, then scale
+        Lg.second_sweeps += 1;
+        lanczos_second_sweep(m, n2, Lg.B, Lg.jj, s, no_skip());
+        k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(), Lg.B.r, Lg.B.sc + 1, nullptr, nullptr);
+        F6_CHECK_LAUNCH();
+        stage_scalars(k);
+        F6_HIP(hipEventSynchronize(hst.ev[k]));
+        ahead = false;
+      }
     }
+    const double* v = hst.p[k].sc;
+    const double alpha = v[Lg.alpha_at], beta2 = v[1];
     Lg.justRestarted = false;
     if (!std::isfinite(alpha) || !std::isfinite(beta2))
       throw Error(FAME6D_ENONFINITE, "Lanczos inverse action returned invalid data.");
@@ -552,7 +570,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
           Lg.alpha_at = 0;
         }
         if (defer) {
-          k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(Lg), B.r, B.sc + 1, gate);
+          k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(Lg), B.r, B.sc + 1, gate, nullptr);
           F6_CHECK_LAUNCH();
         }
       }
