@@ -42,6 +42,31 @@ __global__ void k_bhat(const cpx<T>* mr, long long ldm, long long n, const T* W1
   }
 }
 
+// k_bhat and the start of the conjugate-gradient solve on bhat in one pass: bhat as k_bhat,
+// x = 0, r = bhat and the partials of ||bhat||^2 in k_cg_start's element order and sums.
+// grid (nb, groups) with kRedThreads threads, as k_cg_start.
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_bhat_cg_start(const cpx<T>* mr, long long ldm, long long n,
+                                                               const T* W1, const T* W2, const T* invC,
+                                                               const int* qmap, cpx<T>* bhat, cpx<T>* x,
+                                                               cpx<T>* r, double* partials) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.y;
+  const long long go = g * n, qs = qslot(qmap, g) * n;
+  double sr = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const long long j = qs + i;
+    const cpx<T> u = (-invC[j]) * (W1[j] * mr[g * ldm + i] + W2[j] * mr[g * ldm + n + i]);
+    bhat[go + i] = u;
+    x[go + i] = mk(T(0), T(0));
+    r[go + i] = u;
+    sr += static_cast<double>(u.re) * u.re + static_cast<double>(u.im) * u.im;
+  }
+  sr = block_sum<kRedThreads>(sr, red);
+  if (threadIdx.x == 0) partials[(long long)g * gridDim.x + blockIdx.x] = sr;
+}
+
 // y = V1^{-1} [mr1 + s1; mr2 + s2]   (make_explicit_krinv_compact.m:61-64, solveV1_blockdiag.m:7-8)
 template <class T>
 __global__ void k_v1_solve(const cpx<T>* mr, const cpx<T>* s, long long ld, long long n,
@@ -298,14 +323,7 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_update(cpx<T>* x, const cpx<
 }
 
 static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, const double* partials,
-                                                         int nb) {
-  static_assert(kRedThreads == kCgSumThreads, "the pass-A fold sums the partials in the same order");
-  __shared__ double red[kRedThreads];
-  const int g = blockIdx.x;
-  if (st[g].done) return;
-  const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
-  if (threadIdx.x == 0) cg_store_step(st + g, cg_beta_step(st + g, rho_new));
-}
+                                                         int nb, int* need = nullptr);
 
 // True-residual check of MATLAB pcg, on the device.  k_cg_need_check marks the groups
 // that stopped on the recurrence with flag 0 (need[g] = 0 means "check g"; every other
@@ -317,6 +335,28 @@ static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, con
 // (for the preconditioned form, which needs z = P^{-1} r first); 1 for all others.
 static __global__ void k_cg_need_check(CgState* st, int G, int* need) {
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+    CgState& c = st[g];
+    int skip = 1;
+    if (!c.finished && c.done) {
+      if (c.flag == 0) skip = 0;
+      else c.finished = 1;
+    }
+    need[g] = skip;
+  }
+}
+
+// The pcg step after the last iteration of a chunk; with `need`, k_cg_need_check of the
+// same group in the same launch.
+static __global__ __launch_bounds__(kRedThreads) void k_cg_beta(CgState* st, const double* partials,
+                                                         int nb, int* need) {
+  static_assert(kRedThreads == kCgSumThreads, "the pass-A fold sums the partials in the same order");
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.x;
+  if (!st[g].done) {  // uniform in the workgroup
+    const double rho_new = sum_partials(partials + (long long)g * nb, nb, red);
+    if (threadIdx.x == 0) cg_store_step(st + g, cg_beta_step(st + g, rho_new));
+  }
+  if (need && threadIdx.x == 0) {
     CgState& c = st[g];
     int skip = 1;
     if (!c.finished && c.done) {

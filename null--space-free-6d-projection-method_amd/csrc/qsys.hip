@@ -331,11 +331,16 @@ struct PcgRun {
   int tag() const { return owner * KernelTimer::kTagSpan + std::min(issued, KernelTimer::kTagSpan - 1); }
   CgState* state() const { return slot(cur); }
 
-  // x = 0, r = b, ||b||, the initial states (MATLAB pcg with x0 = 0)
-  void start(const cpx<T>* b, double tol) {
+  // x = 0, r = b, ||b||, the initial states (MATLAB pcg with x0 = 0); with `mr` (FP64 K_r^{-1}),
+  // b = bhat is formed from mr in the same pass and stored at b
+  void start(const cpx<T>* b, double tol, const cpx<T>* mr = nullptr) {
     const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
     CgState* st = slot(0);
-    k_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, pc.x, r, NF, part2);
+    if (mr)
+      k_bhat_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(mr, 2 * NF, NF, pa.W1, pa.W2, pa.invC, qmap_dev,
+                                                                 const_cast<cpx<T>*>(b), pc.x, r, part2);
+    else
+      k_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(b, pc.x, r, NF, part2);
     if (precond) {
       mhat_apply<T>(S, nact, qmap_dev, r, z, no_skip(), s, true, part);
       k_pcg_init<<<nact, kRedThreads, 0, s>>>(st, part2, nbD, part, nbE, tol, maxit, maxmsteps);
@@ -380,10 +385,11 @@ struct PcgRun {
       F6_CHECK_LAUNCH();
     }
     // the pcg step of the chunk's last iteration (a group that stopped earlier is skipped)
-    if (!precond) k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC);
+    if (!precond) k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, true_check ? need : nullptr);
     if (true_check) {
-      // true residual r = b - Mhat x of the groups that stopped on the recurrence
-      k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
+      // true residual r = b - Mhat x of the groups that stopped on the recurrence (marked in
+      // the beta launch when unpreconditioned)
+      if (precond) k_cg_need_check<<<1, 64, 0, s>>>(st, nact, need);
       const Skip sk{need, 1};
       // r = b - Mhat x and its ||r||^2 partials in the last pass of the apply
       mhat_apply<T>(S, nact, qmap_dev, pc.x, r, sk, s, false, nullptr, bcheck, part2, zw);
@@ -446,10 +452,11 @@ struct PcgRun {
 template <class T>
 static void pcg_mhat(fame6d_qsys* S, int nact, const int* qmap_dev, const cpx<T>* b, cpx<T>* x,
                      cpx<T>* r, cpx<T>* p, cpx<T>* q, cpx<T>* z, bool precond, double tol, int maxit,
-                     bool true_check, int& hint_ref, std::vector<PcgOut>& out, cpx<T>* zw = nullptr) {
+                     bool true_check, int& hint_ref, std::vector<PcgOut>& out, cpx<T>* zw = nullptr,
+                     const cpx<T>* mr = nullptr) {
   PcgRun<T> R(S, nact, qmap_dev, x, r, p, q, z, precond, maxit, true_check, zw);
   R.bcheck = b;
-  R.start(b, tol);
+  R.start(b, tol, mr);
   std::vector<CgState> hs;
   R.issue(hint_ref > 0 ? hint_ref : 8);
   R.read(hs);
@@ -571,11 +578,8 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   pa.x = qin;
   pa.ldx = 2 * NF;
   mass_apply_pre(m, PRE_V1H, nact, pa, w.mr, NF, s);
-  // bhat = -invC (W1 mr1 + W2 mr2)
-  k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
-                                                    w.bhat, NF, no_skip(), qmap);
-  F6_CHECK_LAUNCH();
   S->stats.mass_applies_fp64 += 2 * nact;
+  // bhat = -invC (W1 mr1 + W2 mr2): formed with the FP64 solve's start, or here for the mixed form
 
   std::vector<PcgOut> res;
   bool zw_ready = false;  // w.s holds M_s [W1 z; W2 z] of the final u
@@ -584,7 +588,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     // state together with its own scalars and calls krinv_complete
     PcgRun<double> R(S, 1, qmap, w.u, w.cr, w.cp, w.cq, nullptr, false, maxit, true, w.s, def->set);
     R.bcheck = w.bhat;
-    R.start(w.bhat, tol);
+    R.start(w.bhat, tol, w.mr);
     R.issue(S->chunk_hint > 0 ? S->chunk_hint : 8);
     F6_HIP(hipMemcpyAsync(def->host_stage ? def->host_stage : cg_stage(S), R.state(), sizeof(CgState),
                           hipMemcpyDeviceToHost, s));
@@ -604,13 +608,15 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
   }
   if (precision == FAME6D_FP64) {
     pcg_mhat<double>(S, nact, qmap, w.bhat, w.u, w.cr, w.cp, w.cq, nullptr, false, tol, maxit, true,
-                     S->chunk_hint, res, w.s);
+                     S->chunk_hint, res, w.s, w.mr);
     zw_ready = zw_from_check(res);
     krinv_account(S, res, tol, true);
   } else if (precision == FAME6D_MIXED) {
     // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.  The refinement
     // state (||b||, ||r||, converged) lives on the device; the host reads it once the
     // cycles that cannot have converged yet are done (after every cycle from then on).
+    k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
+                                                      w.bhat, NF, no_skip(), qmap);
     S->kr32.ensure(sizeof(cpx<float>) * gn * 6);
     cpx<float>* b32 = S->kr32.as<cpx<float>>();
     cpx<float>* x32 = b32 + gn;
