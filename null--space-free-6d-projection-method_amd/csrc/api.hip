@@ -641,6 +641,7 @@ int fame6d_qsys_destroy(fame6d_qsys* s) {
                     &s->ints, &s->sc, &s->ref, &s->kr64_1, &s->st_1, &s->part_1, &s->part2_1, &s->ints_1})
     b->release();
   if (s->cg_pin) (void)hipHostFree(s->cg_pin);
+  if (s->inner_pin) (void)hipHostFree(s->inner_pin);
   delete s;
   F6_API_END
 }

@@ -299,6 +299,8 @@ struct fame6d_qsys {
   std::vector<int> chunk_hint32 = std::vector<int>(4, 0);  // per refinement cycle of the mixed form
   std::vector<int> qmap_uploaded;  // q-slot map last copied to sc (skips repeated uploads)
   void* cg_pin = nullptr;          // page-locked host copy of a deferred solve's CgState
+  void* inner_pin = nullptr;       // page-locked copies of the mixed form's inner-solve states
+  size_t inner_pin_bytes = 0;
   // second workspace set of the one-group FP64 K_r^{-1}: a solve enqueued while the one before
   // it is still unconfirmed works in the other set, so that one can still be continued
   fame6d::DevBuf kr64_1, st_1, part_1, part2_1, ints_1;

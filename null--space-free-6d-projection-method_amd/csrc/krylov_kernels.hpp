@@ -495,6 +495,31 @@ __global__ void k_convert_ref(long long n, const cpx<Ti>* x, long long ldx, cons
   }
 }
 
+// k_convert_ref into the start of the inner solve on y in one pass: r = x / nr (0 for the
+// groups no longer refining), z = 0 and the partials of ||r||^2 in k_cg_start's element order
+// and sums.  grid (nb, groups) with kRedThreads threads, as k_cg_start.
+template <class To, class Ti>
+__global__ __launch_bounds__(kRedThreads) void k_ref_cg_start(long long n, const cpx<Ti>* x, long long ldx,
+                                                              const RefState* rs, cpx<To>* z, cpx<To>* r,
+                                                              double* partials) {
+  __shared__ double red[kRedThreads];
+  const int g = blockIdx.y;
+  const bool on = !rs[g].inactive;
+  const double scale = on ? 1.0 / rs[g].nr : 0.0;
+  const long long go = g * n;
+  double sr = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const cpx<Ti> v = on ? x[g * ldx + i] : mk<Ti>(0, 0);
+    const cpx<To> u = mk(static_cast<To>(scale * v.re), static_cast<To>(scale * v.im));
+    z[go + i] = mk(To(0), To(0));
+    r[go + i] = u;
+    sr += static_cast<double>(u.re) * u.re + static_cast<double>(u.im) * u.im;
+  }
+  sr = block_sum<kRedThreads>(sr, red);
+  if (threadIdx.x == 0) partials[(long long)g * gridDim.x + blockIdx.x] = sr;
+}
+
 // acc += nr * x for the groups still refining (grid.y = group).
 template <class Ti>
 __global__ void k_accumulate_ref(long long n, const cpx<Ti>* x, long long ldx, const RefState* rs,

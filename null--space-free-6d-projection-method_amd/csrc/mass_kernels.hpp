@@ -112,6 +112,10 @@ struct PassAArgs {
   CgState* st_next;
   const double* rr;
   int nrr;
+  // ... and for the preconditioned form (the work of k_pcg_beta) also the nrz partials of
+  // r'z (rz + g*nrz), which become rho; ||r|| is then taken from rr alone.
+  const double* rz;
+  int nrz;
   // PRE_V1H: the 2x2 blocks [V11 V12; V21 V22] of group g at V11 + g*ldq, ...
   const double* V11;
   const double* V12;
@@ -213,9 +217,10 @@ struct CgStep {
   int iter, done, flag;
   double normr, beta, rho, rho_prev;
 };
-__device__ __forceinline__ CgStep cg_beta_step(const CgState* st, double rho_new) {
-  CgStep c{st->iter + 1, st->done, st->flag, sqrt(rho_new), st->beta, rho_new, st->rho};
-  if (!isfinite(rho_new)) {
+// The pcg step after an iteration from rr = ||r||^2 and rho_new = r'z (= rr unpreconditioned).
+__device__ __forceinline__ CgStep cg_pbeta_step(const CgState* st, double rr, double rho_new) {
+  CgStep c{st->iter + 1, st->done, st->flag, sqrt(rr), st->beta, rho_new, st->rho};
+  if (!isfinite(rr) || !isfinite(rho_new)) {
     c.flag = -1;
     c.done = 1;
   } else if (c.normr <= st->tolb || st->check_every) {
@@ -233,6 +238,9 @@ __device__ __forceinline__ CgStep cg_beta_step(const CgState* st, double rho_new
     c.beta = beta;
   }
   return c;
+}
+__device__ __forceinline__ CgStep cg_beta_step(const CgState* st, double rho_new) {
+  return cg_pbeta_step(st, rho_new, rho_new);
 }
 __device__ __forceinline__ void cg_store_step(CgState* d, const CgStep& c) {
   d->iter = c.iter;
@@ -257,35 +265,41 @@ __device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, do
     return true;
   }
   const int tid = threadIdx.x;
-  const double* rr = a.rr + (long long)g * a.nrr;
-  double v = 0.0;
-  if constexpr (THREADS >= kCgSumThreads) {
-    if (tid < kCgSumThreads)
-      for (int i = tid; i < a.nrr; i += kCgSumThreads) v += rr[i];
-    v = block_sum<THREADS>(v, red);
-  } else {
-    // fewer threads than k_cg_beta has: each takes the lanes tid, tid + THREADS, ... of a
-    // kCgSumThreads-wide sum (in LDS of their own: red may hold fewer doubles), and the
-    // first wave adds them up as block_sum does
-    static_assert(THREADS >= 64, "one full wave");
-    __shared__ double lane[kCgSumThreads];
-    for (int l = tid; l < kCgSumThreads; l += THREADS) {
-      double u = 0.0;
-      for (int i = l; i < a.nrr; i += kCgSumThreads) u += rr[i];
-      lane[l] = u;
-    }
-    __syncthreads();
-    if (tid < 64) {
-      for (int i = tid; i < kCgSumThreads; i += 64) v += lane[i];
+  // k_cg_beta's (k_pcg_beta's) fixed-order sum of the nb partials at src, on every workgroup
+  auto fold_sum = [&](const double* src, int nb) {
+    double v = 0.0;
+    if constexpr (THREADS >= kCgSumThreads) {
+      if (tid < kCgSumThreads)
+        for (int i = tid; i < nb; i += kCgSumThreads) v += src[i];
+      v = block_sum<THREADS>(v, red);
+    } else {
+      // fewer threads than k_cg_beta has: each takes the lanes tid, tid + THREADS, ... of a
+      // kCgSumThreads-wide sum (in LDS of their own: red may hold fewer doubles), and the
+      // first wave adds them up as block_sum does
+      static_assert(THREADS >= 64, "one full wave");
+      __shared__ double lane[kCgSumThreads];
+      for (int l = tid; l < kCgSumThreads; l += THREADS) {
+        double u = 0.0;
+        for (int i = l; i < nb; i += kCgSumThreads) u += src[i];
+        lane[l] = u;
+      }
+      __syncthreads();
+      if (tid < 64) {
+        for (int i = tid; i < kCgSumThreads; i += 64) v += lane[i];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      }
     }
-  }
+    __syncthreads();  // red / lane free for the next sum
+    return v;
+  };
+  const double v = fold_sum(a.rr + (long long)g * a.nrr, a.nrr);
+  const double vz = a.rz ? fold_sum(a.rz + (long long)g * a.nrz, a.nrz) : v;
   if (tid == 0) {
     const CgState* o = a.st + g;
     const bool was_done = o->done != 0;
     const CgStep c = was_done ? CgStep{o->iter, o->done, o->flag, o->normr, o->beta, o->rho, o->rho_prev}
-                              : cg_beta_step(o, v);
+                              : cg_pbeta_step(o, v, vz);
     if (blockIdx.x == 0) {
       a.st_next[g] = *o;
       if (!was_done) cg_store_step(a.st_next + g, c);

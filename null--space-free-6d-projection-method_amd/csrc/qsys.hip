@@ -333,10 +333,15 @@ struct PcgRun {
 
   // x = 0, r = b, ||b||, the initial states (MATLAB pcg with x0 = 0); with `mr` (FP64 K_r^{-1}),
   // b = bhat is formed from mr in the same pass and stored at b
-  void start(const cpx<T>* b, double tol, const cpx<T>* mr = nullptr) {
+  // with `rs` (the inner solves of the mixed form), b = res / ||res|| is formed from the FP64
+  // refinement residual `res` in the same pass (b itself is not stored)
+  void start(const cpx<T>* b, double tol, const cpx<T>* mr = nullptr, const RefState* rs = nullptr,
+             const cpx<double>* res = nullptr) {
     const int maxmsteps = (int)std::min<long long>({NF / 50, 5, std::max<long long>(NF - maxit, 0)});
     CgState* st = slot(0);
-    if (mr)
+    if (rs)
+      k_ref_cg_start<T, double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(NF, res, NF, rs, pc.x, r, part2);
+    else if (mr)
       k_bhat_cg_start<T><<<dim3(nbD, nact), kRedThreads, 0, s>>>(mr, 2 * NF, NF, pa.W1, pa.W2, pa.invC, qmap_dev,
                                                                  const_cast<cpx<T>*>(b), pc.x, r, part2);
     else
@@ -357,14 +362,17 @@ struct PcgRun {
     if (++rounds > maxit + 8) throw Error(FAME6D_ENOCONV, "Mhat CG control loop did not terminate");
     CgState* st = state();
     for (int c = 0; c < chunk; ++c, ++issued) {
-      // unpreconditioned: from the second iteration of a chunk on, pass A applies the pcg step
-      // of the previous iteration (reading slot cur, writing slot cur^1)
-      const bool fold = !precond && c > 0;
+      // from the second iteration of a chunk on, pass A applies the pcg step of the previous
+      // iteration (reading slot cur, writing slot cur^1); preconditioned, with the r'z
+      // partials of the previous preconditioner apply
+      const bool fold = c > 0;
       pa.st = slot(cur);
       pa.skip = skip_done(slot(cur));
       pa.st_next = fold ? slot(cur ^ 1) : nullptr;
       pa.rr = part2;
       pa.nrr = nbC;
+      pa.rz = precond ? part : nullptr;
+      pa.nrz = nbE;
       timer_begin(m, TS_MHAT_PASS_A);
       launch_pass_a<T>(m, PRE_MHAT_CG, nact, pa, s);
       timer_end(m, bytesA, tag());
@@ -378,14 +386,12 @@ struct PcgRun {
       timer_begin(m, TS_MHAT_PASS_C);
       launch_pass_c<T>(m, POST_MHAT_CG, nact, pc, s);  // alpha, x += alpha p, r -= alpha q
       timer_end(m, bytesC, tag());
-      if (precond) {
-        mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
-        k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, part, nbE);
-      }
+      if (precond) mhat_apply<T>(S, nact, qmap_dev, r, z, skip_done(st), s, true, part);
       F6_CHECK_LAUNCH();
     }
     // the pcg step of the chunk's last iteration (a group that stopped earlier is skipped)
-    if (!precond) k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, true_check ? need : nullptr);
+    if (precond) k_pcg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, part, nbE);
+    else k_cg_beta<<<nact, kRedThreads, 0, s>>>(st, part2, nbC, true_check ? need : nullptr);
     if (true_check) {
       // true residual r = b - Mhat x of the groups that stopped on the recurrence (marked in
       // the beta launch when unpreconditioned)
@@ -473,6 +479,20 @@ struct KrinvWs64 {
 static CgState* cg_stage(fame6d_qsys* S) {
   if (!S->cg_pin) F6_HIP(hipHostMalloc(&S->cg_pin, sizeof(CgState), hipHostMallocDefault));
   return static_cast<CgState*>(S->cg_pin);
+}
+
+// page-locked host copies of the inner solves' states of one mixed K_r^{-1}: one CgState per
+// group and refinement cycle (allocated on first use, freed with the q-system)
+static CgState* inner_stage(fame6d_qsys* S, int nact) {
+  const size_t need = sizeof(CgState) * (size_t)nact * std::max(1, S->mixed_max_cycles);
+  if (S->inner_pin_bytes < need) {
+    if (S->inner_pin) F6_HIP(hipHostFree(S->inner_pin));
+    S->inner_pin = nullptr;
+    S->inner_pin_bytes = 0;
+    F6_HIP(hipHostMalloc(&S->inner_pin, need, hipHostMallocDefault));
+    S->inner_pin_bytes = need;
+  }
+  return static_cast<CgState*>(S->inner_pin);
 }
 
 static KrinvWs64 krinv_ws(fame6d_qsys* S, int nact, int set = 0) {
@@ -643,37 +663,63 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
                           : 1;
     const Skip skr = skip_inactive(rs);
     std::vector<RefState> hr(nact);
-    int cycles = 0;
+    struct InnerRun {
+      int budget, slot, timer_first, owner;
+    };
+    std::vector<InnerRun> runs;
+    int cycles = 0, seen = 0;
     while (true) {
       if (cycles >= S->mixed_max_cycles)
         throw Error(FAME6D_ENOCONV, "Mhat iterative refinement did not converge");
-      // b32 = r / ||r|| (0 for converged groups, whose inner solve then stops at once)
-      k_convert_ref<float, double><<<dim3(gridm, nact), 256, 0, s>>>(NF, w.res, NF, rs, b32, NF);
-      F6_CHECK_LAUNCH();
-      std::vector<PcgOut> inner;
-      int& hint = S->chunk_hint32[std::min(cycles, (int)S->chunk_hint32.size() - 1)];
-      pcg_mhat<float>(S, nact, qmap, b32, x32, r32, p32, q32, z32, S->mixed_precond, tol_in, maxit, false,
-                      hint, inner);
-      ++cycles;
-      for (int g = 0; g < nact; ++g) {
-        if (inner[g].flag != 0 && inner[g].flag != 1 && inner[g].flag != 3)
-          throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(inner[g].flag) + ")");
-        S->stats.total_iterations += inner[g].iter;
-        if (inner[g].iter > 0)
-          S->stats.mass_applies_fp32 += 3LL * ((S->mixed_precond ? 2 : 1) * inner[g].iter + inner[g].checks +
-                                               (S->mixed_precond ? 1 : 0));
-        res[g].iter += inner[g].iter;
+      // The inner solve: one chunk of `budget` iterations and no host read (an inner solve
+      // that stops short of tol_in still improves u, and the FP64 residual decides).  Its
+      // right-hand side r / ||r|| is formed in its start pass (0 for converged groups, whose
+      // inner solve then stops at once); its states are copied out for the host to read with
+      // the refinement state.
+      const int ci = std::min(cycles, (int)S->chunk_hint32.size() - 1);
+      const int budget = std::max(2, std::min(maxit, S->chunk_hint32[ci] > 0 ? S->chunk_hint32[ci] : 16));
+      {
+        PcgRun<float> R(S, nact, qmap, x32, r32, p32, q32, z32, S->mixed_precond, budget, false, nullptr);
+        R.start(nullptr, tol_in, nullptr, rs, w.res);
+        R.issue(budget);
+        F6_HIP(hipMemcpyAsync(inner_stage(S, nact) + (size_t)cycles * nact, R.state(), sizeof(CgState) * nact,
+                              hipMemcpyDeviceToHost, s));
+        runs.push_back({budget, ci, R.timer_first, R.owner});
       }
+      ++cycles;
       k_accumulate_ref<float><<<dim3(gridm, nact), 256, 0, s>>>(NF, x32, NF, rs, w.u, NF);
       // FP64 residual of every group still refining
       // res = bhat - Mhat u and its ||res||^2 partials in the last pass of the apply
       mhat_apply<double>(S, nact, qmap, w.u, w.res, skr, s, false, nullptr, w.bhat, part2, w.s);
       k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, pass_c_blocks(m, false), tol);
       F6_CHECK_LAUNCH();
-      for (int g = 0; g < nact; ++g)
-        if (inner[g].iter > 0) S->stats.mass_applies_fp64 += 3;
       if (cycles < blind) continue;
       d2h_sync(m, hr.data(), rs, sizeof(RefState) * nact);
+      // the inner solves of the cycles since the last read
+      for (; seen < cycles; ++seen) {
+        const CgState* hs = inner_stage(S, nact) + (size_t)seen * nact;
+        const InnerRun& ir = runs[seen];
+        int need = 2, max_iter = 0;
+        for (int g = 0; g < nact; ++g) {
+          const CgState& c = hs[g];
+          if (c.flag != 0 && c.flag != 1 && c.flag != 3)
+            throw Error(FAME6D_ENOCONV, "FP32 Mhat CG broke down (flag " + std::to_string(c.flag) + ")");
+          S->stats.total_iterations += c.iter;
+          if (c.iter > 0) {
+            S->stats.mass_applies_fp32 +=
+                3LL * ((S->mixed_precond ? 2 : 1) * c.iter + c.checks + (S->mixed_precond ? 1 : 0));
+            S->stats.mass_applies_fp64 += 3;
+          }
+          res[g].iter += c.iter;
+          max_iter = std::max(max_iter, c.iter);
+          // the next budget of this cycle: the iterations it took plus one, or two more when
+          // it ran out of budget before tol_in
+          need = std::max(need, c.flag == 1 ? ir.budget + 2 : c.iter + 1);
+        }
+        // launches after the last group stopped ran as skipped launches
+        m->timer.discard_tags(ir.timer_first, ir.owner, max_iter);
+        S->chunk_hint32[ir.slot] = need;
+      }
       bool any = false;
       for (int g = 0; g < nact; ++g) {
         if (hr[g].nonfinite) throw Error(FAME6D_ENONFINITE, "Mhat refinement residual is NaN/Inf.");
