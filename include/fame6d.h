@@ -163,6 +163,12 @@ int fame6d_qsys_diagonals(const fame6d_qsys* s, double* V11, double* V12, double
  * the reference solves in FP64 only (cfg.gpu.precision = 'double'). */
 int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles, int precondition);
 
+/* Re-orthogonalisation threshold of the Method-C Lanczos steps: after the first classical
+ * Gram-Schmidt sweep a second one runs when ||r_after|| < eta ||r_before|| (the DGKS test;
+ * default eta = 1/sqrt(2)); eta >= 1 runs it at every step, as the reference's two fixed
+ * modified Gram-Schmidt sweeps (exp1/Lanczos_hybrid_identity_gpuop.m:55-63). */
+int fame6d_qsys_set_reorth(fame6d_qsys* s, double eta);
+
 /* apply_Mhat_C_scalar_compact (exp1/apply_Mhat_C_scalar_compact.m:8-21): x, y NF. */
 int fame6d_mhat_apply(fame6d_qsys* s, int precision, const void* x, void* y);
 

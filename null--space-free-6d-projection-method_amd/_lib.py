@@ -130,6 +130,7 @@ SIGNATURES = {
     "fame6d_qsys_info": [P, C.POINTER(QsysInfo)],
     "fame6d_qsys_diagonals": [P, P, P, P, P, P, P, P, P, P],
     "fame6d_qsys_set_mixed": [P, D, I, I],
+    "fame6d_qsys_set_reorth": [P, D],
     "fame6d_mhat_apply": [P, I, P, P],
     "fame6d_apply_v": [P, P, P],
     "fame6d_apply_k": [P, P, P],

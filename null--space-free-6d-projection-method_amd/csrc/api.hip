@@ -733,6 +733,13 @@ int fame6d_qsys_set_mixed(fame6d_qsys* s, double inner_tol, int max_cycles, int 
   F6_API_END
 }
 
+int fame6d_qsys_set_reorth(fame6d_qsys* s, double eta) {
+  F6_API_BEGIN
+  F6_REQUIRE(s && eta > 0 && eta <= 1e3, FAME6D_EARG, "bad re-orthogonalisation threshold");
+  s->dgks_eta2 = eta * eta;
+  F6_API_END
+}
+
 int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart,
                          double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host, void* Y,
                          fame6d_lanczos_stats_t* stats) {

@@ -294,6 +294,7 @@ struct fame6d_qsys {
   fame6d::DevBuf kr64, kr32, st, part, part2, ints, sc, ref;
   double mixed_inner_tol = 3e-5;
   bool mixed_precond = true;  // FP32 inner PCG preconditioned by Mhat evaluated with 1/eps
+  double dgks_eta2 = 0.5;     // squared DGKS threshold of the Lanczos re-orthogonalisation
   int mixed_max_cycles = 30;
   int chunk_hint = 0;                 // first chunk of the FP64 PCG (iterations of the last call + 1)
   std::vector<int> chunk_hint32 = std::vector<int>(4, 0);  // per refinement cycle of the mixed form

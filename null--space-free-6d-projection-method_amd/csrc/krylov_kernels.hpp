@@ -791,12 +791,13 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
 // Norm bookkeeping of the re-orthogonalisation (one workgroup).  sc[1] = ||r||^2 of the
 // current vector, sc[2] = ||r||^2 before the first sweep, the int at sc + 4 = 1 when the
 // second sweep is not needed.  stage 0: after the three-term update; stage 1: after the
-// first sweep, which decides the second one by the DGKS test ||r_after|| < ||r_before||/sqrt(2)
+// first sweep, which decides the second one by the DGKS test ||r_after||^2 < eta2 ||r_before||^2
+// (eta2 = 1/2 by default)
 // (with `pre`, the stage-0 sum of the same step first); stage 2: after the second sweep (taken
 // only when it ran).
 static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, const double* partials, int nb,
                                                                 int stage, const double* pre = nullptr,
-                                                                int npre = 0) {
+                                                                int npre = 0, double eta2 = 0.5) {
   __shared__ double red[kRedThreads];
   int* skip = reinterpret_cast<int*>(sc + 4);
   if (stage == 2 && *skip) return;
@@ -815,7 +816,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, cons
       sc[2] = s;
     } else {
       sc[1] = s;
-      if (stage == 1) skip[0] = (s < 0.5 * sc[2]) ? 0 : 1;
+      if (stage == 1) skip[0] = (eta2 >= 1.0 || s < eta2 * sc[2]) ? 0 : 1;  // eta >= 1: always
     }
   }
 }

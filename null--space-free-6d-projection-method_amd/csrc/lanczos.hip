@@ -167,13 +167,13 @@ static void lanczos_second_sweep(fame6d_mass* m, long long n, const Basis& B, in
 // A Lanczos step away from a restart (Lanczos_hybrid_identity_gpuop.m:55-63): alpha =
 // Re q_j^H w into sc[0]; the three-term update r = w - alpha q_j - beta q_{j-1}; one
 // classical Gram-Schmidt sweep r -= Q (Q^H r) over the whole basis; and a second sweep only
-// when the first removed more than 1 - 1/sqrt(2) of ||r|| (the DGKS test, decided on the
-// device, so the step never waits for the host).  The three-term update removes the two
-// large components of w, so the full sweep only cleans rounding-level ones and seldom
-// needs repeating; the reference always runs two modified Gram-Schmidt sweeps.
-// ||r||^2 ends in sc[1].
+// when the first left less than eta of ||r|| (the DGKS test, eta^2 = eta2, 1/2 by default),
+// decided on the device (`second`) or left to the caller.  The three-term update removes the
+// two large components of w, so the full sweep only cleans rounding-level ones and seldom
+// needs repeating; the reference always runs two modified Gram-Schmidt sweeps (eta >= 1
+// here).  ||r||^2 ends in sc[1].
 static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int jj, const cpx<double>* w,
-                              double beta, hipStream_t s, bool second = true) {
+                              double beta, hipStream_t s, bool second = true, double eta2 = 0.5) {
   const int gn = (int)std::min<long long>(2048, (n + 255) / 256);
   const double vb = 16.0 * n;
   const cpx<double>* qj = B.Q + (size_t)n * (jj - 1);
@@ -189,7 +189,7 @@ static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int j
            B.part, no_skip(), (const double*)B.sc, beta);
   timer_end(m, (jj + 2) * vb);
   // ||r||^2 before (the three-term update) and after the sweep, and the DGKS test
-  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1, npart, nb);
+  k_lz_dgks<<<1, kRedThreads, 0, s>>>(B.sc, B.part, gn, 1, npart, nb, eta2);
   F6_CHECK_LAUNCH();
   if (second) lanczos_second_sweep(m, n, B, jj, s, Skip{reinterpret_cast<const int*>(B.sc + 4), 0});
 }
@@ -376,7 +376,7 @@ static void lanczos_run_ahead(fame6d_qsys* S, LzState& Lg, int k0, int p, double
       cgs2_step(m, n2, Lg.B, jj, qout, s);
       Lg.alpha_at = kScH + jj - 1;
     } else {
-      lanczos_orth_step(m, n2, Lg.B, jj, qout, jj >= 2 ? Lg.beta : 0.0, s, false);
+      lanczos_orth_step(m, n2, Lg.B, jj, qout, jj >= 2 ? Lg.beta : 0.0, s, false, S->dgks_eta2);
       Lg.alpha_at = 0;
     }
     k_scale_next<<<gr, 256, 0, s>>>(n2, next_dst(), Lg.B.r, Lg.B.sc + 1, gate, Lg.alpha_at == 0 ? no_second : nullptr);
@@ -566,7 +566,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
           cgs2_step(m, n2, B, jj, w, s);
           Lg.alpha_at = kScH + jj - 1;
         } else {
-          lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s);
+          lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s, true, S->dgks_eta2);
           Lg.alpha_at = 0;
         }
         if (defer) {

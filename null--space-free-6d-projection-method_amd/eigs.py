@@ -64,6 +64,11 @@ class QSystem:
     def set_mixed(self, inner_tol=3e-5, max_cycles=30, precondition=True):
         call("fame6d_qsys_set_mixed", self._h, float(inner_tol), int(max_cycles), int(bool(precondition)))
 
+    def set_reorth(self, eta=2 ** -0.5):
+        """DGKS threshold of the Lanczos re-orthogonalisation: a second Gram-Schmidt sweep runs
+        when the first leaves less than eta of ||r||; eta >= 1 runs it at every step."""
+        call("fame6d_qsys_set_reorth", self._h, float(eta))
+
     def _dev(self, x, rows, dtype=torch.complex128):
         if not isinstance(x, torch.Tensor):
             x = torch.as_tensor(np.asarray(x))

@@ -195,6 +195,27 @@ def test_eigs_golden_n2(n2_eigs):
     assert 0 <= out["stats"]["second_sweeps"] <= out["stats"]["outer_steps"]
 
 
+def test_eigs_every_step_second_sweep(pkg, n2, n2_eigs):
+    """Re-orthogonalisation threshold eta = 1 (exp1/Lanczos_hybrid_identity_gpuop.m:55-63 runs
+    two sweeps at every step): the DGKS test asks for the second sweep at every step away from
+    a restart, which the single-run loop issues from the host after reading the step and then
+    forms the step's next vector again; the eigenvalues and the basis equal the default run's."""
+    print("case: eigs N=2 second sweep at every step", flush=True)
+    s = n2["s"]
+    try:
+        s.set_reorth(1.0)
+        out = pkg.run_method_C(s, k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000,
+                               v0=pkg.start_vector(s.nr, 20260831), want_vectors=True)
+    finally:
+        s.set_reorth()
+    st = out["stats"]
+    assert st["converged"] and st["second_sweeps"] >= st["outer_steps"] - st["restarts"] - 1 > 0
+    ref = n2_eigs["lambda_"]
+    assert np.max(np.abs(out["lambda_"] - ref) / ref) < 1e-12
+    Y = out["Y"].cpu().numpy()
+    assert np.max(np.abs(Y.conj() @ Y.T - np.eye(10))) < 1e-12
+
+
 def test_eigs_vs_oracle_and_mixed(pkg, oracle, n2):
     print("case: eigs oracle + mixed N=2", flush=True)
     v0 = pkg.start_vector(n2["s"].nr, 5)
