@@ -7,6 +7,8 @@
 // Term order alpha = (a1, a2, a3) with a1 outermost (build_multi_index_list_3d_exp3.m:6-12).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -16,7 +18,7 @@ namespace fame6d {
 
 constexpr int kTaylorMaxP = 16;
 
-__host__ __device__ inline int taylor_terms(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ constexpr int taylor_terms(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 
 // ---------------------------------------------------------------------------- moments
 // moments = coeff_alpha .* (Q_alpha^T B), B(j, c) = exp(i q_j . r_c) coeff_j: a real
@@ -45,12 +47,71 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // 256 registers per lane (TPW = 10, CT = 2 would spill at two).
 constexpr int taylor_minb(int tpw, int ct) { return (tpw * ct <= 20 && tpw < 10) ? 2 : 1; }
 
+// Per-axis phase factors of centres that form a tensor grid, c = ix + nx (iy + ny iz) with
+// centre (vx[ix], vy[iy], vz[iz]): exp(i q.r_c) = exp(i qx vx) exp(i qy vy) exp(i qz vz), so a
+// mode needs nx + ny + nz phases instead of one per centre (216 -> 18 for the 6^3 block
+// centres of a Yee window).  tab holds, per mode j, the nx x-factors times coeff_j, then the
+// ny y-factors and the nz z-factors; tab == nullptr selects one phase per (mode, centre).
+struct AxisPhases {
+  const cpx<double>* tab;
+  int nx, ny, na;  // na = nx + ny + nz
+};
+
+__global__ void k_taylor_axis_phases(long long NF, const double* __restrict__ qx, const double* __restrict__ qy,
+                                     const double* __restrict__ qz, const cpx<double>* __restrict__ coeff, int nx,
+                                     int ny, int na, const double* __restrict__ vals, cpx<double>* __restrict__ tab) {
+  const long long tot = NF * na;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long j = e / na;
+    const int k = (int)(e - j * na);
+    const double q = k < nx ? qx[j] : (k < nx + ny ? qy[j] : qz[j]);
+    double sn, cs;
+    sincos(q * vals[k], &sn, &cs);
+    cpx<double> v = mk(cs, sn);
+    if (k < nx) v = v * coeff[j];
+    tab[e] = v;
+  }
+}
+
+// B operand of one chunk of kTChunk modes for the workgroup's 8 CT centres (column group cg):
+// bt[jj * BROW + 2 cl + {0, 1}] = re/im of exp(i q_j . r_c) coeff_j, c = cg 8 CT + cl, zero
+// for modes past the slice and centres past C.
+template <int CT, int BROW>
+__device__ __forceinline__ void taylor_build_b(double* bt, long long j0, long long j_end, int cg, int C,
+                                               const double* qx, const double* qy, const double* qz,
+                                               const cpx<double>* coeff, const double* centers, AxisPhases ax) {
+  for (int e = threadIdx.x; e < kTChunk * 8 * CT; e += 256) {
+    const int jj = e / (8 * CT), cl = e - jj * (8 * CT);
+    const long long j = j0 + jj;
+    const int c = cg * 8 * CT + cl;
+    double re = 0.0, im = 0.0;
+    if (j < j_end && c < C) {
+      cpx<double> v;
+      if (ax.tab) {
+        const int ix = c % ax.nx, r = c / ax.nx;
+        const int iy = r % ax.ny, iz = r / ax.ny;
+        const cpx<double>* t = ax.tab + j * ax.na;
+        v = t[ix] * t[ax.nx + iy] * t[ax.nx + ax.ny + iz];
+      } else {
+        double sn, cs;
+        sincos(qx[j] * centers[3 * c] + qy[j] * centers[3 * c + 1] + qz[j] * centers[3 * c + 2], &sn, &cs);
+        v = mk(cs, sn) * coeff[j];
+      }
+      re = v.re;
+      im = v.im;
+    }
+    bt[jj * BROW + 2 * cl] = re;
+    bt[jj * BROW + 2 * cl + 1] = im;
+  }
+}
+
 template <int TPW, int CT, int MINB>
 __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF, const double* qx, const double* qy,
                                                              const double* qz, const cpx<double>* coeff, int p,
                                                              int L, int C, const double* centers,
-                                                             const unsigned char* alist, long long span,
-                                                             int Cpad2, double* part) {
+                                                             AxisPhases ax, const unsigned char* alist,
+                                                             long long span, int Cpad2, double* part) {
   constexpr int NCOL = 16 * CT;  // real columns of this workgroup
   // Padded LDS strides.  A wave reads pw[.][s * kTRow + k4 + kl] for up to 11 powers s and 4
   // values of kl: with a stride of kTChunk doubles every s fell in the same banks (up to
@@ -101,24 +162,11 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
         z *= d;
       }
     }
-    for (int e = tid; e < kTChunk * 8 * CT; e += 256) {
-      const int jj = e / (8 * CT), cl = e - jj * (8 * CT);
-      const long long j = j0 + jj;
-      const int c = cg * 8 * CT + cl;
-      double re = 0.0, im = 0.0;
-      if (j < j_end && c < C) {
-        double sn, cs;
-        sincos(qx[j] * centers[3 * c] + qy[j] * centers[3 * c + 1] + qz[j] * centers[3 * c + 2], &sn, &cs);
-        const cpx<double> v = mk(cs, sn) * coeff[j];
-        re = v.re;
-        im = v.im;
-      }
-      bt[jj * BROW + 2 * cl] = re;
-      bt[jj * BROW + 2 * cl + 1] = im;
-    }
+    taylor_build_b<CT, BROW>(bt, j0, j_end, cg, C, qx, qy, qz, coeff, centers, ax);
     __syncthreads();
     // two k-steps per trip, so the LDS reads of one overlap the matrix-core ops of the other
-#pragma unroll 2
+    // (one where the two sets of operands would not fit the registers of two workgroups)
+#pragma unroll MINB == 2 && TPW * CT > 16 ? 1 : 2
     for (int k4 = 0; k4 < kTChunk; k4 += 4) {
       const int kk = k4 + kl;
       double bv[CT];
@@ -152,9 +200,174 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_mfma(long long NF,
   }
 }
 
+// Balanced form for 4 column tiles (32 centres per workgroup) and up to 32 row tiles: the
+// T = 4 TF + R row tiles of 16 alphas are split so every wave issues the same number of
+// matrix-core ops per k-step.  Wave w owns the full row tiles w, w + 4, ..., w + 4 (TF - 1)
+// over all 4 column tiles, and R of the 4 R (row tile, column tile) pairs of the last R row
+// tiles: pairs g = w R .. w R + R - 1, row tile 4 TF + g / 4, column tile g % 4.  With the
+// round-robin form (wave w owns tiles w, w + 4, ...) the waves of the last partial round of
+// tiles sat idle at the chunk barrier (at p = 10, 18 tiles: two waves did 5 tiles, two did 4).
+template <int TF, int R>
+constexpr int taylor_bal_minb() {
+  return TF * 4 + R <= 20 ? 2 : 1;
+}
+
+// Chunk loop of k_taylor_moments_bal for CTL live column tiles of the workgroup's 4 (the
+// remainder pairs run whatever their column).
+template <int TF, int R, int CTL, int BROW, int TFA, int RA>
+__device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], double4_t (&accr)[RA],
+                                                  double (&pw)[3][(kTZeroRow + 1) * kTRow], double* bt,
+                                                  const int (&o0)[TFA], const int (&o1)[TFA], const int (&o2)[TFA],
+                                                  const int (&r0)[RA], const int (&r1)[RA], const int (&r2)[RA],
+                                                  const int (&rc)[RA], long long j_begin, long long j_end, int p,
+                                                  int cg, int C, const double* qx, const double* qy,
+                                                  const double* qz, const cpx<double>* coeff, const double* centers,
+                                                  AxisPhases ax) {
+  const int tid = threadIdx.x, kl = (tid & 63) >> 4;
+  for (long long j0 = j_begin; j0 < j_end; j0 += kTChunk) {
+    __syncthreads();  // previous chunk fully consumed
+    // powers: thread (axis, mode) of 3 x kTChunk
+    for (int e = tid; e < 3 * kTChunk; e += 256) {
+      const int d = e / kTChunk, jj = e - d * kTChunk;
+      const long long j = j0 + jj;
+      const bool ok = j < j_end;
+      const double* qd = d == 0 ? qx : (d == 1 ? qy : qz);
+      const double a = ok ? qd[j] : 0.0;
+      double x = ok ? 1.0 : 0.0;
+      for (int s = 0; s <= p; ++s) {
+        pw[d][s * kTRow + jj] = x;
+        x *= a;
+      }
+    }
+    taylor_build_b<4, BROW>(bt, j0, j_end, cg, C, qx, qy, qz, coeff, centers, ax);
+    __syncthreads();
+    const int nl = tid & 15;
+#pragma unroll 1
+    for (int k4 = 0; k4 < kTChunk; k4 += 4) {
+      const int kk = k4 + kl;
+      double bv[CTL];
+#pragma unroll
+      for (int c = 0; c < CTL; ++c) bv[c] = bt[kk * BROW + c * 16 + nl];
+      if constexpr (TF > 0) {
+#pragma unroll
+        for (int t = 0; t < TF; ++t) {
+          const double av = pw[0][o0[t] + kk] * pw[1][o1[t] + kk] * pw[2][o2[t] + kk];
+#pragma unroll
+          for (int c = 0; c < CTL; ++c) acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[t][c], 0, 0, 0);
+        }
+      }
+      if constexpr (R > 0) {
+        if constexpr (R == 2) {
+          // both pairs of a wave lie in one row tile (g = 2w, 2w + 1)
+          const double av = pw[0][r0[0] + kk] * pw[1][r1[0] + kk] * pw[2][r2[0] + kk];
+#pragma unroll
+          for (int r = 0; r < 2; ++r)
+            accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bt[kk * BROW + rc[r]], accr[r], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double av = pw[0][r0[r] + kk] * pw[1][r1[r] + kk] * pw[2][r2[r] + kk];
+            accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bt[kk * BROW + rc[r]], accr[r], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int TF, int R, int MINB>
+__global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, const double* qx, const double* qy,
+                                                                  const double* qz, const cpx<double>* coeff, int p,
+                                                                  int L, int C, const double* centers,
+                                                                  AxisPhases ax, const unsigned char* alist,
+                                                                  int gf, int nsl, long long span,
+                                                                  long long span_last, int Cpad2, double* part) {
+  constexpr int CT = 4;
+  constexpr int NCOL = 16 * CT;
+  constexpr int BROW = NCOL + 16;  // padded row stride of B (see k_taylor_moments_mfma)
+  constexpr int TFA = TF > 0 ? TF : 1, RA = R > 0 ? R : 1;
+  const int Lpad = (L + 15) & ~15;
+  __shared__ double pw[3][(kTZeroRow + 1) * kTRow];
+  __shared__ double bt[kTChunk * BROW];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = tid >> 6;
+  const int kl = lane >> 4, nl = lane & 15;
+  // One-dimensional grid: workgroups 0 .. gf nsl - 1 take column group id % gf and mode slice
+  // id / gf of length span; the rest (when the last column group holds fewer than 4 live
+  // column tiles) take that group, gf, and slices of length span_last, so that they carry as
+  // much work as the others.  The grid holds no idle workgroups: with the round-robin
+  // assignment of workgroups to the XCDs, a grid of at most 8 x (resident per XCD)
+  // workgroups runs in one round.
+  const int id = blockIdx.x;
+  const bool last = id >= gf * nsl;
+  const int cg = last ? gf : id % gf;
+  const long long sp = last ? span_last : span;
+  const int slice = last ? id - gf * nsl : id / gf;
+  const long long j_begin = (long long)slice * sp;
+  const long long j_end = min(NF, j_begin + sp);
+
+  // power-row offsets of this lane's alpha in each owned row tile (alphas past L read the
+  // all-zero row)
+  auto offsets = [&](int tile, int& a0, int& a1, int& a2) {
+    const int alpha = tile * 16 + nl;
+    const bool ok = alpha < L;
+    a0 = (ok ? (int)alist[3 * alpha] : kTZeroRow) * kTRow;
+    a1 = (ok ? (int)alist[3 * alpha + 1] : 0) * kTRow;
+    a2 = (ok ? (int)alist[3 * alpha + 2] : 0) * kTRow;
+  };
+  int o0[TFA], o1[TFA], o2[TFA];
+#pragma unroll
+  for (int t = 0; t < TFA; ++t) offsets(wave + 4 * t, o0[t], o1[t], o2[t]);
+  int r0[RA], r1[RA], r2[RA], rc[RA];
+#pragma unroll
+  for (int r = 0; r < RA; ++r) {
+    const int g = wave * R + r;
+    offsets(4 * TF + g / 4, r0[r], r1[r], r2[r]);
+    rc[r] = (g % 4) * 16 + nl;
+  }
+  for (int i = tid; i < kTChunk; i += 256) pw[0][kTZeroRow * kTRow + i] = 0.0;
+
+  double4_t acc[TFA][CT], accr[RA];
+#pragma unroll
+  for (int t = 0; t < TFA; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[t][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < RA; ++r) accr[r] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  // column tiles holding a centre: the last column group can hold fewer than 4, and its
+  // full row tiles then issue no matrix-core ops for the empty ones
+  const int ctl = min(CT, (C - cg * 8 * CT + 7) / 8);
+  if (ctl >= 4)
+    taylor_bal_chunks<TF, R, 4, BROW>(acc, accr, pw, bt, o0, o1, o2, r0, r1, r2, rc, j_begin, j_end, p, cg, C, qx,
+                                      qy, qz, coeff, centers, ax);
+  else
+    taylor_bal_chunks<TF, R, 3, BROW>(acc, accr, pw, bt, o0, o1, o2, r0, r1, r2, rc, j_begin, j_end, p, cg, C, qx,
+                                      qy, qz, coeff, centers, ax);
+  // partial sums of this mode slice: part[(slice*Lpad + alpha)*Cpad2 + col]
+  double* ps = part + (long long)slice * Lpad * Cpad2 + cg * NCOL;
+  if constexpr (TF > 0) {
+#pragma unroll
+    for (int t = 0; t < TF; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ps[(long long)((wave + 4 * t) * 16 + kl + 4 * q) * Cpad2 + c * 16 + nl] = acc[t][c][q];
+  }
+  if constexpr (R > 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int g = wave * R + r;
+      const int tile = 4 * TF + g / 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ps[(long long)(tile * 16 + kl + 4 * q) * Cpad2 + rc[r]] = accr[r][q];
+    }
+  }
+}
+
 // Final moments: sum the mode slices, form re + i*im and multiply by i^|alpha|/alpha!.
-__global__ void k_taylor_finish(int L, int C, int Cpad2, int nslices, const double* part, const cpx<double>* calpha,
-                                cpx<double>* mom) {
+__global__ void k_taylor_finish(int L, int C, int Cpad2, int nslices_all, int c_last, int nsl_last,
+                                const double* part, const cpx<double>* calpha, cpx<double>* mom) {
   // consecutive threads take consecutive centres of one alpha row, so every slice's row is
   // read in order (16 bytes per thread); four running sums keep four loads in flight
   const int Lpad = (L + 15) & ~15;
@@ -162,6 +375,7 @@ __global__ void k_taylor_finish(int L, int C, int Cpad2, int nslices, const doub
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < tot;
        i += (long long)gridDim.x * blockDim.x) {
     const int a = (int)(i / C), c = (int)(i % C);
+    const int nslices = c >= c_last ? nsl_last : nslices_all;  // slices of c's column group
     const double* col = part + (long long)a * Cpad2 + 2 * c;
     const long long zs = (long long)Lpad * Cpad2;
     double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
@@ -346,6 +560,44 @@ __global__ void k_direct(long long NF, const double* qx, const double* qy, const
   }
 }
 
+// Device buffers of the moments call (tables, per-slice partial moments, per-axis phases),
+// kept per device between calls and only grown, so repeated calls do not pay hipMalloc and
+// hipFree; the mutex serialises the calls (each synchronises its stream before it returns,
+// when the buffers are free again).  The buffers live until the process ends.
+struct MomentsWorkspace {
+  DevBuf tabs, part, axt;
+};
+constexpr int kMaxDevices = 64;
+static std::mutex g_mom_mu;
+static MomentsWorkspace* g_mom_ws[kMaxDevices];
+
+// Largest per-axis phase table (NF x (nx + ny + nz) complex) the moments call allocates.
+constexpr double kAxisTableMaxBytes = 2.0e9;
+
+// Do the centres form a tensor grid in the order c = ix + nx (iy + ny iz), centre c =
+// (vx[ix], vy[iy], vz[iz])?  On success vals = vx, vy, vz (nx + ny + nz values).  Every
+// centre is compared exactly, so the per-axis factorisation of exp(i q . r_c) holds for each.
+static bool tensor_grid(const double* cen, int C, int& nx, int& ny, int& nz, std::vector<double>& vals) {
+  nx = 1;
+  while (nx < C && cen[3 * nx + 1] == cen[1] && cen[3 * nx + 2] == cen[2]) ++nx;
+  if (C % nx) return false;
+  ny = 1;
+  while ((long long)ny * nx < C && cen[3 * ny * nx + 2] == cen[2]) ++ny;
+  if (C % (nx * ny)) return false;
+  nz = C / (nx * ny);
+  for (int c = 0; c < C; ++c) {
+    const int ix = c % nx, iy = (c / nx) % ny, iz = c / (nx * ny);
+    if (cen[3 * c] != cen[3 * ix] || cen[3 * c + 1] != cen[3 * iy * nx + 1] ||
+        cen[3 * c + 2] != cen[3 * iz * nx * ny + 2])
+      return false;
+  }
+  vals.clear();
+  for (int i = 0; i < nx; ++i) vals.push_back(cen[3 * i]);
+  for (int i = 0; i < ny; ++i) vals.push_back(cen[3 * i * nx + 1]);
+  for (int i = 0; i < nz; ++i) vals.push_back(cen[3 * i * nx * ny + 2]);
+  return true;
+}
+
 static void alpha_tables(int p, std::vector<int>& al, std::vector<cpx<double>>& ca) {
   al.clear();
   ca.clear();
@@ -428,65 +680,126 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
     const int L = (int)ca.size();
     const int Lpad = (L + 15) & ~15;
     const int T = Lpad / 16;
-    const int tpw = (T + 3) / 4;           // row tiles per wave
-    const int ct = tpw <= 8 ? 4 : 2;       // 16-column tiles per workgroup (tpw * ct <= 32 accumulators)
+    const bool balanced = T <= 32;         // 4 column tiles, balanced row tiles (p <= 12)
+    const int tpw = (T + 3) / 4;           // row tiles per wave of the round-robin form
+    const int ct = balanced ? 4 : 2;       // 16-column tiles per workgroup
     const int cgroups = (C + 8 * ct - 1) / (8 * ct);
     const int Cpad2 = cgroups * 16 * ct;
     // mode slices, each a multiple of the chunk: the grid is kept at whole rounds of the
-    // resident workgroups (at most four rounds) rather than just over a multiple, which would
-    // add a round for a few stragglers
+    // resident workgroups rather than just over a multiple, which would add a round for a
+    // few stragglers
+    int dev = 0;
+    F6_HIP(hipGetDevice(&dev));
     int cus = 256;
     {
-      int dev = 0;
-      hipDeviceProp_t prop;
-      if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-          prop.multiProcessorCount > 0)
-        cus = prop.multiProcessorCount;
+      int n = 0;
+      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
     }
-    const int minb = taylor_minb(tpw, ct);  // workgroups per CU of the instance below
+    const int minb = balanced ? (T / 4 * 4 + T % 4 <= 20 ? 2 : 1) : taylor_minb(tpw, ct);
     const int resident = cus * minb;
     const long long chunks = (NF + kTChunk - 1) / kTChunk;
-    long long target = std::max<long long>(1, (4LL * resident) / cgroups);
+    // one round for the balanced form (fewest partial sums: at N = 4, p = 10, 216 centres one
+    // and two rounds measured the same and four 6% slower), four for the round-robin form
+    const int rounds = balanced ? 1 : 4;
+    long long target = std::max<long long>(1, ((long long)rounds * resident) / cgroups);
     int nslices = (int)std::max<long long>(1, std::min<long long>(chunks, target));
     long long span = ((chunks + nslices - 1) / nslices) * kTChunk;
     nslices = (int)((NF + span - 1) / span);
+    // balanced form with a partly filled last column group (it runs 3 column tiles): its
+    // slices are 4/3 as long, and the span is the shortest for which all workgroups fit the
+    // rounds of resident workgroups
+    long long span_last = span;
+    int nsl_last = nslices;
+    int gf = cgroups;  // column groups of the balanced grid that walk slices of length span
+    const int ctl_last = std::min(4, (C - (cgroups - 1) * 32 + 7) / 8);
+    if (balanced && cgroups > 1 && ctl_last < 4) {
+      gf = cgroups - 1;
+      const long long cap = (long long)rounds * resident;
+      for (long long sc = 1; sc <= chunks; ++sc) {
+        const long long nf = (chunks + sc - 1) / sc, scl = sc * 4 / 3, nl = (chunks + scl - 1) / scl;
+        if ((cgroups - 1) * nf + nl <= cap || sc == chunks) {
+          span = sc * kTChunk;
+          span_last = scl * kTChunk;
+          nslices = (int)((NF + span - 1) / span);
+          nsl_last = (int)((NF + span_last - 1) / span_last);
+          break;
+        }
+      }
+    }
+    // centres on a tensor grid: per-axis phase tables
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<double> axv;
+    const bool axis = tensor_grid(centers_host, C, nx, ny, nz, axv) && nx + ny + nz < C &&
+                      (double)NF * (nx + ny + nz) * sizeof(cpx<double>) <= kAxisTableMaxBytes;
     std::vector<unsigned char> alb(al.begin(), al.end());
-    DevBuf tabs, part;
+    if (dev < 0 || dev >= kMaxDevices) throw Error(FAME6D_EARG, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> lock(g_mom_mu);
+    if (!g_mom_ws[dev]) g_mom_ws[dev] = new MomentsWorkspace();
+    DevBuf& tabs = g_mom_ws[dev]->tabs;
+    DevBuf& part = g_mom_ws[dev]->part;
+    DevBuf& axt = g_mom_ws[dev]->axt;
     const size_t off_ca = ((alb.size() + 255) / 256) * 256;
     const size_t off_cen = off_ca + sizeof(cpx<double>) * ca.size();
-    tabs.ensure(off_cen + sizeof(double) * 3 * C);
+    const size_t off_axv = off_cen + sizeof(double) * 3 * C;
+    tabs.ensure(off_axv + sizeof(double) * axv.size());
     char* b = tabs.as<char>();
     F6_HIP(hipMemcpyAsync(b, alb.data(), alb.size(), hipMemcpyHostToDevice, s));
     F6_HIP(hipMemcpyAsync(b + off_ca, ca.data(), sizeof(cpx<double>) * ca.size(), hipMemcpyHostToDevice, s));
     F6_HIP(hipMemcpyAsync(b + off_cen, centers_host, sizeof(double) * 3 * C, hipMemcpyHostToDevice, s));
     part.ensure(sizeof(double) * (size_t)nslices * Lpad * Cpad2);
     dim3 grid(cgroups, nslices);
+    const int grid_bal = gf * nslices + (cgroups - gf) * nsl_last;
     const auto* cf = (const cpx<double>*)coeff;
     const auto* cen = (const double*)(b + off_cen);
     const auto* alp = (const unsigned char*)b;
-    switch (tpw) {
+    AxisPhases ax{nullptr, 1, 1, 3};
+    if (axis) {
+      const int na = nx + ny + nz;
+      F6_HIP(hipMemcpyAsync(b + off_axv, axv.data(), sizeof(double) * axv.size(), hipMemcpyHostToDevice, s));
+      axt.ensure(sizeof(cpx<double>) * (size_t)NF * na);
+      k_taylor_axis_phases<<<grid_for(NF * na), 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, nx, ny,
+                                                             na, (const double*)(b + off_axv), axt.as<cpx<double>>());
+      F6_CHECK_LAUNCH();
+      ax = AxisPhases{axt.as<cpx<double>>(), nx, ny, na};
+    }
+    if (balanced) {
+      switch (p) {
+#define F6_TAYLOR_BAL(P)                                                                                          \
+  case P: {                                                                                                      \
+    constexpr int TT = (taylor_terms(P) + 15) / 16;                                                              \
+    k_taylor_moments_bal<TT / 4, TT % 4, taylor_bal_minb<TT / 4, TT % 4>()><<<grid_bal, 256, 0, s>>>(           \
+        NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C, cen, ax, alp, gf, nslices, span, span_last, Cpad2, \
+        part.as<double>());                                                                                     \
+    break;                                                                                                       \
+  }
+        F6_TAYLOR_BAL(0) F6_TAYLOR_BAL(1) F6_TAYLOR_BAL(2) F6_TAYLOR_BAL(3) F6_TAYLOR_BAL(4) F6_TAYLOR_BAL(5)
+        F6_TAYLOR_BAL(6) F6_TAYLOR_BAL(7) F6_TAYLOR_BAL(8) F6_TAYLOR_BAL(9) F6_TAYLOR_BAL(10) F6_TAYLOR_BAL(11)
+        F6_TAYLOR_BAL(12)
+#undef F6_TAYLOR_BAL
+        default:
+          throw Error(FAME6D_EARG, "Taylor order without a balanced instance");
+      }
+    } else {
+      switch (tpw) {
 #define F6_TAYLOR_CASE(TP, CTV)                                                                                   \
   case TP:                                                                                                       \
     k_taylor_moments_mfma<TP, CTV, taylor_minb(TP, CTV)><<<grid, 256, 0, s>>>(                                  \
-        NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C,                                                   \
-                                                        cen, alp, span, Cpad2, part.as<double>());               \
+        NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C, cen, ax, alp, span, Cpad2, part.as<double>());   \
     break;
-      F6_TAYLOR_CASE(1, 4) F6_TAYLOR_CASE(2, 4) F6_TAYLOR_CASE(3, 4) F6_TAYLOR_CASE(4, 4)
-      F6_TAYLOR_CASE(5, 4) F6_TAYLOR_CASE(6, 4) F6_TAYLOR_CASE(7, 4) F6_TAYLOR_CASE(8, 4)
-      F6_TAYLOR_CASE(9, 2) F6_TAYLOR_CASE(10, 2) F6_TAYLOR_CASE(11, 2) F6_TAYLOR_CASE(12, 2)
-      F6_TAYLOR_CASE(13, 2) F6_TAYLOR_CASE(14, 2) F6_TAYLOR_CASE(15, 2) F6_TAYLOR_CASE(16, 2)
+        F6_TAYLOR_CASE(9, 2) F6_TAYLOR_CASE(10, 2) F6_TAYLOR_CASE(11, 2) F6_TAYLOR_CASE(12, 2)
+        F6_TAYLOR_CASE(13, 2) F6_TAYLOR_CASE(14, 2) F6_TAYLOR_CASE(15, 2) F6_TAYLOR_CASE(16, 2)
 #undef F6_TAYLOR_CASE
-      default:
-        throw Error(FAME6D_EARG, "Taylor order too large");
+        default:
+          throw Error(FAME6D_EARG, "Taylor order too large");
+      }
     }
     F6_CHECK_LAUNCH();
-    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, Cpad2, nslices, part.as<double>(),
+    k_taylor_finish<<<grid_for((long long)L * C), 256, 0, s>>>(L, C, Cpad2, nslices, gf * 8 * ct, nsl_last,
+                                                               part.as<double>(),
                                                                (const cpx<double>*)(b + off_ca),
                                                                (cpx<double>*)moments);
     F6_CHECK_LAUNCH();
     F6_HIP(hipStreamSynchronize(s));
-    tabs.release();
-    part.release();
   } catch (const Error& e) {
     set_last_error(e.what());
     return e.code;

@@ -73,3 +73,25 @@ def test_yee_evaluation_vs_oracle_and_direct(pkg, oracle, setup):
         ud_ref = oracle.direct_reconstruct(setup["qm"], setup["coeff"], pts)
         assert np.abs(ud - ud_ref).max() < 1e-10 * np.abs(ud_ref).max()
         assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-6
+
+
+@pytest.mark.parametrize("N,p,nb", [(2, 0, 2), (2, 3, 3), (3, 10, 3), (3, 10, 6), (3, 14, 4)])
+def test_moments_tensor_grid_centres(pkg, oracle, N, p, nb):
+    """Centres on a tensor grid (the block centres of a Yee window, c = i + nb j + nb^2 k)
+    take the per-axis phase tables; the moments equal the oracle's one-phase-per-centre sum.
+    The same centres with one of them moved off the grid take the per-centre phases and
+    also match.  p = 14 runs the round-robin tile form, the others the balanced one."""
+    print("case: taylor moments tensor-grid centres N=%d p=%d nb=%d" % (N, p, nb), flush=True)
+    rng = np.random.default_rng(100 + 10 * N + p)
+    qm = oracle.projected_q_modes(N, (0.05, 0.04, 0.03))
+    NF = qm.shape[0]
+    coeff = (rng.standard_normal(NF) + 1j * rng.standard_normal(NF)) * np.exp(-0.5 * (qm ** 2).sum(1))
+    cen = np.ascontiguousarray(pkg.yee_block_centers(4 * nb, 0.05, 1, nb))
+    for moved in (False, True):
+        c = cen.copy()
+        if moved:
+            c[nb ** 3 // 2, 0] += 0.01
+        mom = pkg.taylor_moments(qm, coeff, p, c).cpu().numpy()
+        ref = oracle.taylor_moments(qm, coeff, p, c)
+        err = np.abs(mom - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, (moved, err)
