@@ -72,18 +72,20 @@ def cpu_model():
     return platform.processor()
 
 
-def time_applies(torch, mass, dtype, reps):
-    """Average duration of one scalar M_s apply (3 pass kernels) from HIP events recorded
-    on the stream the library launches on."""
-    x = torch.randn(mass.NF, dtype=dtype, device=mass.device)
+def time_applies(torch, mass, dtype, reps, comps=1):
+    """Average duration of one scalar M_s apply (3 pass kernels; comps=3: the three-component
+    Mop, whose passes take the three vectors together) from HIP events recorded on the stream
+    the library launches on."""
+    x = torch.randn(comps * mass.NF, dtype=dtype, device=mass.device)
+    op = mass.MscalarOp if comps == 1 else mass.Mop
     for _ in range(3):
-        y = mass.MscalarOp(x)
+        y = op(x)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(mass.stream)
     for _ in range(reps):
-        y = mass.MscalarOp(x)
+        y = op(x)
     e1.record(mass.stream)
     e1.synchronize()
     del y
@@ -632,6 +634,7 @@ def main():
     # ---- M-apply roofline (FP64 and FP32), HIP events on the library stream
     t64 = time_applies(torch, mass, torch.complex128, a.apply_reps)
     t32 = time_applies(torch, mass, torch.complex64, a.apply_reps)
+    t64_3 = time_applies(torch, mass, torch.complex128, a.apply_reps, comps=3)
     bytes64 = 40.0 * NF
     bytes32 = 20.0 * NF
     gbs64 = bytes64 / t64 / 1e9
@@ -738,6 +741,11 @@ def main():
                                       "frac": gbs64 / HBM_PEAK_GBS},
                      "m_apply_fp32": {"us": t32 * 1e6, "algorithmic_bytes": bytes32, "achieved": gbs32,
                                       "frac": gbs32 / HBM_PEAK_GBS},
+                     "m_apply_3comp_fp64": {"us": t64_3 * 1e6, "algorithmic_bytes": 3 * bytes64,
+                                            "achieved": 3 * bytes64 / t64_3 / 1e9,
+                                            "frac": 3 * bytes64 / t64_3 / 1e9 / HBM_PEAK_GBS,
+                                            "what": "Mop: M_s on the three components of a 3NF vector, "
+                                                    "one set of pass launches for the three"},
                      "pass_operand_bandwidth": pass_operand,
                      "timing": "start/stop HIP events of hipExtLaunchKernel on the library stream, every %dth "
                                "launch of each kernel family during the timed steps; the M_s apply from events "
