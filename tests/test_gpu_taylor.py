@@ -95,3 +95,35 @@ def test_moments_tensor_grid_centres(pkg, oracle, N, p, nb):
         ref = oracle.taylor_moments(qm, coeff, p, c)
         err = np.abs(mom - ref).max() / np.abs(ref).max()
         assert err < 1e-12, (moved, err)
+
+
+def test_reconstruct_yee_field_vs_oracle(pkg, oracle, setup):
+    """reconstruct_yee_field (exp3/reconstruct_tracked_modes_exp4.m:10-30): the three plus-grid
+    components of a 6D field on the n = 4 window, with the automatic partition and with two
+    blocks per axis, equal the oracle's moments and Taylor sums on the oracle's Yee points,
+    block ids and centres; with the small phase radius of this window they also equal the
+    direct Fourier sum."""
+    rng = np.random.default_rng(77)
+    qm, NF = setup["qm"], setup["NF"]
+    X = (rng.standard_normal(3 * NF) + 1j * rng.standard_normal(3 * NF)) * np.tile(np.exp(-0.5 * (qm ** 2).sum(1)), 3)
+    n, h, p = 4, 0.0125, 10
+    m = 2 * n + 3
+    Xd = torch.as_tensor(X, device="cuda")
+    for nb_req in (None, 2):
+        print("case: reconstruct_yee_field n=%d nblocks=%s" % (n, nb_req), flush=True)
+        comps, nb = pkg.reconstruct_yee_field(Xd, qm, n, h, p=p, nblocks=nb_req)
+        if nb_req is not None:
+            assert nb == nb_req
+        assert len(comps) == 3
+        for comp in range(3):
+            coeff = X[comp * NF:(comp + 1) * NF]
+            pts = oracle.yee_points(n, h, comp)
+            bid = oracle.block_partition(m, nb)
+            cen = oracle.block_centers(pts, bid, nb ** 3)
+            mom = oracle.taylor_moments(qm, coeff, p, cen)
+            ref = oracle.taylor_evaluate(pts, bid, cen, mom, p)
+            u = comps[comp].cpu().numpy()
+            assert u.shape == (m ** 3,)
+            assert np.abs(u - ref).max() < 1e-11 * np.abs(ref).max(), (nb, comp)
+            ud = oracle.direct_reconstruct(qm, coeff, pts)
+            assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-6, (nb, comp)
