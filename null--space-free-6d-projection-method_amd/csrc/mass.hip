@@ -15,25 +15,28 @@ namespace fame6d {
 template <int n, class T>
 struct Tiles;
 // Mhat passes (three thread groups): BPW3 blocks per workgroup, threads = 3*BPW3*n^2.
-#define F6_TILES(N_, T_, BPW0, BPW1, A0, A1, B30, B31)        \
-  template <>                                                 \
-  struct Tiles<N_, T_> {                                      \
-    static constexpr int bpw[2] = {BPW0, BPW1};               \
-    static constexpr int a[2] = {A0, A1};                     \
-    static constexpr int bpw3[2] = {B30, B31};                \
+// Pass B variant v: A = a[v] inner positions per workgroup, vp[v] of them per thread (2: the
+// FP32 form with 16-byte accesses, k_pass_b2).
+#define F6_TILES(N_, T_, BPW0, BPW1, A0, A1, B30, B31, VP0, VP1) \
+  template <>                                                   \
+  struct Tiles<N_, T_> {                                        \
+    static constexpr int bpw[2] = {BPW0, BPW1};                 \
+    static constexpr int a[2] = {A0, A1};                       \
+    static constexpr int bpw3[2] = {B30, B31};                  \
+    static constexpr int vp[2] = {VP0, VP1};                    \
   };
-F6_TILES(2, double, 8, 8, 8, 8, 8, 8)
-F6_TILES(2, float, 8, 8, 8, 8, 8, 8)
-F6_TILES(4, double, 16, 8, 16, 32, 8, 4)
-F6_TILES(4, float, 16, 8, 16, 32, 8, 4)
-F6_TILES(6, double, 8, 6, 8, 24, 4, 2)
-F6_TILES(6, float, 8, 6, 8, 24, 4, 2)
-F6_TILES(8, double, 4, 8, 8, 4, 2, 1)
-F6_TILES(8, float, 4, 8, 8, 16, 2, 4)
-F6_TILES(10, double, 2, 1, 4, 2, 1, 2)
-F6_TILES(10, float, 2, 1, 4, 2, 1, 2)
-F6_TILES(12, double, 2, 2, 4, 2, 1, 1)
-F6_TILES(12, float, 2, 2, 4, 6, 1, 2)
+F6_TILES(2, double, 8, 8, 8, 8, 8, 8, 1, 1)
+F6_TILES(2, float, 8, 8, 8, 8, 8, 8, 1, 1)
+F6_TILES(4, double, 16, 8, 16, 32, 8, 4, 1, 1)
+F6_TILES(4, float, 16, 8, 16, 32, 8, 4, 1, 1)
+F6_TILES(6, double, 8, 6, 8, 24, 4, 2, 1, 1)
+F6_TILES(6, float, 8, 6, 8, 24, 4, 2, 1, 1)
+F6_TILES(8, double, 4, 8, 8, 4, 2, 1, 1, 1)
+F6_TILES(8, float, 4, 8, 8, 16, 2, 4, 1, 1)
+F6_TILES(10, double, 2, 1, 4, 2, 1, 2, 1, 1)
+F6_TILES(10, float, 2, 1, 4, 8, 1, 2, 1, 2)
+F6_TILES(12, double, 2, 2, 4, 2, 1, 1, 1, 1)
+F6_TILES(12, float, 2, 2, 6, 8, 1, 2, 1, 2)
 #undef F6_TILES
 
 template <class F>
@@ -138,16 +141,29 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
-    auto go = [&](auto ac) {
+    auto go = [&](auto ac, auto vc) {
       constexpr int A = decltype(ac)::value;
-      dim3 grid(S / A, ngroups * nr), block(A * n * n);
-      if (mode == 0 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 0>, grid, block, s, a);
-      else if (mode == 0 && nr == 3) launch_k(m, k_pass_b<T, n, A, 3, 0>, grid, block, s, a);
-      else if (mode == 1 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 1>, grid, block, s, a);
-      else throw Error(FAME6D_EARG, "bad pass-B configuration");
+      constexpr int VP = decltype(vc)::value;
+      dim3 grid(S / A, ngroups * nr);
+      if constexpr (VP == 2 && std::is_same<T, float>::value) {
+        dim3 block(A / 2 * n * n);
+        if (mode == 0 && nr == 1) launch_k(m, k_pass_b2<n, A, 1, 0>, grid, block, s, a);
+        else if (mode == 0 && nr == 3) launch_k(m, k_pass_b2<n, A, 3, 0>, grid, block, s, a);
+        else if (mode == 1 && nr == 1) launch_k(m, k_pass_b2<n, A, 1, 1>, grid, block, s, a);
+        else throw Error(FAME6D_EARG, "bad pass-B configuration");
+      } else {
+        static_assert(VP == 1, "two positions per thread only in FP32");
+        dim3 block(A * n * n);
+        if (mode == 0 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 0>, grid, block, s, a);
+        else if (mode == 0 && nr == 3) launch_k(m, k_pass_b<T, n, A, 3, 0>, grid, block, s, a);
+        else if (mode == 1 && nr == 1) launch_k(m, k_pass_b<T, n, A, 1, 1>, grid, block, s, a);
+        else throw Error(FAME6D_EARG, "bad pass-B configuration");
+      }
     };
-    if (v == 0) go(std::integral_constant<int, Tiles<n, T>::a[0]>{});
-    else go(std::integral_constant<int, Tiles<n, T>::a[1]>{});
+    if (v == 0)
+      go(std::integral_constant<int, Tiles<n, T>::a[0]>{}, std::integral_constant<int, Tiles<n, T>::vp[0]>{});
+    else
+      go(std::integral_constant<int, Tiles<n, T>::a[1]>{}, std::integral_constant<int, Tiles<n, T>::vp[1]>{});
   });
   F6_CHECK_LAUNCH();
 }
@@ -235,11 +251,15 @@ int num_tile_variants(int n, bool fp32, bool pass_b) {
 void tile_variant_desc(int nn, bool fp32, bool pass_b, int v, int* param, int* threads) {
   with_n(nn, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
-    int p;
-    if (pass_b) p = fp32 ? Tiles<n, float>::a[v] : Tiles<n, double>::a[v];
-    else p = fp32 ? Tiles<n, float>::bpw[v] : Tiles<n, double>::bpw[v];
+    int p, per = 1;  // per: inner positions per thread of pass B
+    if (pass_b) {
+      p = fp32 ? Tiles<n, float>::a[v] : Tiles<n, double>::a[v];
+      per = fp32 ? Tiles<n, float>::vp[v] : Tiles<n, double>::vp[v];
+    } else {
+      p = fp32 ? Tiles<n, float>::bpw[v] : Tiles<n, double>::bpw[v];
+    }
     *param = p;
-    *threads = p * n * n;
+    *threads = p * n * n / per;
   });
 }
 

@@ -617,6 +617,129 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
   }
 }
 
+// Pass B with two adjacent inner positions per thread, FP32 only: a thread's pencil element
+// is one 16-byte load or store (re, im of positions 2t and 2t + 1), where the one-position
+// form issues 8-byte accesses, and a tile of A positions is a run of 8 A bytes at stride S
+// with half the threads of the one-position form.  Same arithmetic per pencil.
+template <int n, int A, int NR, int MODE>
+__global__ __launch_bounds__(A / 2 * n * n) void k_pass_b2(PassBArgs<float> a) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  constexpr int P2 = Grid<n>::P2, S = Grid<n>::S;
+  constexpr int AT = A / 2;  // threads per pencil column
+  constexpr int THREADS = AT * P2;
+  static_assert(A % 2 == 0, "pairs of inner positions");
+  __shared__ cpx<float> L[A * S];  // l = al + A*(k4 + n*k5 + n^2*k6)
+  const int nt = gridDim.x, nv = gridDim.y;
+  const int logical = xcd_swizzle(blockIdx.x + nt * blockIdx.y, nt * nv);
+  const int tile = logical / nv, vidx = logical - tile * nv;
+  if (skipped(a.skip, vidx / NR)) return;
+  const int tid = threadIdx.x;
+  const int ap = tid % AT, qq = tid / AT;  // qq in [0, n^2)
+  const int al = 2 * ap;
+  const long long a0 = (long long)tile * A + al;
+  cpx<float>* v = a.t + vidx * a.ldt + a0;
+  auto ld2 = [&](long long off, cpx<float>& x, cpx<float>& y) {
+    const f4 u = *reinterpret_cast<const f4*>(v + off);
+    x = mk(u.x, u.y);
+    y = mk(u.z, u.w);
+  };
+  auto st2 = [&](long long off, const cpx<float>& x, const cpx<float>& y) {
+    *reinterpret_cast<f4*>(v + off) = (f4){x.re, x.im, y.re, y.im};
+  };
+  auto lds2 = [&](int l, cpx<float>& x, cpx<float>& y) {
+    const f4 u = *reinterpret_cast<const f4*>(L + l);
+    x = mk(u.x, u.y);
+    y = mk(u.z, u.w);
+  };
+  auto sts2 = [&](int l, const cpx<float>& x, const cpx<float>& y) {
+    *reinterpret_cast<f4*>(L + l) = (f4){x.re, x.im, y.re, y.im};
+  };
+  {  // dim 4 inverse: (k5, k6) = qq
+    cpx<float> w0[n], w1[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) ld2((long long)S * (k + n * qq), w0[k], w1[k]);
+    dft<float, n, +1>(w0);
+    dft<float, n, +1>(w1);
+#pragma unroll
+    for (int k = 0; k < n; ++k) sts2(al + A * (k + n * qq), w0[k], w1[k]);
+  }
+  __syncthreads();
+  const int b5 = al + A * ((qq % n) + P2 * (qq / n));  // (k4, k6) = qq
+  {
+    cpx<float> w0[n], w1[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) lds2(b5 + A * n * k, w0[k], w1[k]);
+    dft<float, n, +1>(w0);
+    dft<float, n, +1>(w1);
+#pragma unroll
+    for (int k = 0; k < n; ++k) sts2(b5 + A * n * k, w0[k], w1[k]);
+  }
+  __syncthreads();
+  double en = 0.0;  // as in k_pass_b, over both positions
+  {  // dim 6: inverse, eps, forward
+    const int b6 = al + A * qq;
+    cpx<float> w0[n], w1[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) lds2(b6 + A * P2 * k, w0[k], w1[k]);
+    dft<float, n, +1>(w0);
+    dft<float, n, +1>(w1);
+    if constexpr (MODE == 1) {
+#pragma unroll
+      for (int k = 0; k < n; ++k) st2((long long)S * (qq + P2 * k), w0[k], w1[k]);
+      return;
+    } else {
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        const f2 e = *reinterpret_cast<const f2*>(a.eps + a0 + (long long)S * (qq + P2 * k));
+        w0[k] = e.x * w0[k];
+        w1[k] = e.y * w1[k];
+      }
+      dft<float, n, -1>(w0);
+      dft<float, n, -1>(w1);
+      if (a.partials != nullptr) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          cpx<float> z0, z1;
+          lds2(b6 + A * P2 * k, z0, z1);
+          en += static_cast<double>(z0.re) * w0[k].re + static_cast<double>(z0.im) * w0[k].im;
+          en += static_cast<double>(z1.re) * w1[k].re + static_cast<double>(z1.im) * w1[k].im;
+          sts2(b6 + A * P2 * k, w0[k], w1[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < n; ++k) sts2(b6 + A * P2 * k, w0[k], w1[k]);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    cpx<float> w0[n], w1[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) lds2(b5 + A * n * k, w0[k], w1[k]);
+    dft<float, n, -1>(w0);
+    dft<float, n, -1>(w1);
+#pragma unroll
+    for (int k = 0; k < n; ++k) sts2(b5 + A * n * k, w0[k], w1[k]);
+  }
+  __syncthreads();
+  {  // dim 4 forward, store
+    cpx<float> w0[n], w1[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) lds2(al + A * (k + n * qq), w0[k], w1[k]);
+    dft<float, n, -1>(w0);
+    dft<float, n, -1>(w1);
+#pragma unroll
+    for (int k = 0; k < n; ++k) st2((long long)S * (k + n * qq), w0[k], w1[k]);
+  }
+  if (a.partials != nullptr) {
+    static_assert(sizeof(cpx<float>) * A * S >= sizeof(double) * THREADS, "LDS reuse for the reduction");
+    __syncthreads();  // every thread has read L
+    en = block_sum<THREADS>(en, reinterpret_cast<double*>(L));
+    if (tid == 0) a.partials[(long long)vidx * nt + tile] = en;
+  }
+}
+
 // ---------------------------------------------------------------- pass C
 template <class T, int n, int BPW, int OP, int NOUT>
 __global__ __launch_bounds__(BPW* n* n) void k_pass_c(PassCArgs<T> a) {

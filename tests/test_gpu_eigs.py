@@ -89,6 +89,49 @@ def test_mhat_pass_c_variants_n5(n5, oracle):
         n5["m"].set_tiles("single", 0, 0)
 
 
+@pytest.mark.parametrize("N", [5, 6])
+def test_mhat_fp32_pass_b_two_positions(pkg, oracle, N):
+    """FP32 pass B with two inner positions per thread (16-byte accesses; pass-B variant 1 in
+    FP32 at N = 5 and 6) against the oracle: the three-vector Mhat apply (variant code vb = 1)
+    and the single-vector scalar apply (vb = 2), and the mixed K_r^-1 of the same system
+    against the FP64 one."""
+    print("case: FP32 pass B two positions per thread N=%d" % N, flush=True)
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(50 + N)
+    eps = oracle.eps_reference(N)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, Q_EXP1)
+    try:
+        x = rand_c(rng, m.NF)
+        xd = torch.as_tensor(x, device="cuda")
+        ref = oracle.mhat_apply(oracle.qsystem(N, Q_EXP1), eps, x, N)
+        for vb in (1, 2, 3):
+            m.set_tiles("single", 0, vb)
+            y32 = s.Mhat(xd.to(torch.complex64), "single").cpu().numpy()
+            assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 5e-6, vb
+            ys = m.MscalarOp(xd.to(torch.complex64)).cpu().numpy()
+            rs = oracle.mass_apply(eps, x, N)
+            assert np.linalg.norm(ys - rs) / np.linalg.norm(rs) < 2e-6, vb
+        # mixed K_r^-1 with either FP32 pass B against FP64: both solve Mhat u = bhat to 1e-10
+        # relative residual, so they differ from the FP64 solution by the same order (the
+        # reference material at N = 6 has a permittivity contrast of e^2.4)
+        q = torch.as_tensor(rand_c(rng, 2 * m.NF), device="cuda")
+        y64, st64 = s.krinv(q, 1e-10, 1200, "double")
+        assert st64["last_flag"] == 0
+        errs = []
+        for vb in (0, 1):
+            m.set_tiles("single", 0, vb)
+            ymx, stmx = s.krinv(q, 1e-10, 1200, "mixed")
+            assert stmx["last_flag"] == 0, vb
+            errs.append((torch.linalg.norm(ymx - y64) / torch.linalg.norm(y64)).item())
+        print("  mixed vs FP64 K_r^-1, FP32 pass-B variants 0 / 1: %.2e / %.2e" % tuple(errs), flush=True)
+        assert max(errs) < 2e-7, errs
+        assert errs[1] < 10 * errs[0] + 1e-9, errs
+    finally:
+        s.close()
+        m.close()
+
+
 def test_krinv_pass_c_variants_n5(n5):
     """The CG form of the persistent pass C (alpha from the pass-B partials, x and r updated
     over several tiles per workgroup, ||r||^2 partials per workgroup) against the
