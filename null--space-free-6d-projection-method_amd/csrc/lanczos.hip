@@ -566,7 +566,10 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
           cgs2_step(m, n2, B, jj, w, s);
           Lg.alpha_at = kScH + jj - 1;
         } else {
-          lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s, true, S->dgks_eta2);
+          // the deferred single run keeps the device-side skip of the second sweep (its
+          // k_scale_next waits on the same flag); otherwise the host issues the second sweep
+          // after the step's read of scalars, only when the DGKS test asked for it
+          lanczos_orth_step(m, n2, B, jj, w, jj >= 2 ? Lg.beta : 0.0, s, defer, S->dgks_eta2);
           Lg.alpha_at = 0;
         }
         if (defer) {
@@ -604,6 +607,20 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       // the solve needed more iterations: its tail has been formed again, and so is the step
       orth(nullptr);
       read_scalars();
+    }
+    if (!defer) {
+      // second Gram-Schmidt sweeps the DGKS tests asked for, then beta^2 read again
+      bool any = false;
+      for (int i = 0; i < na; ++i)
+        if (second[i]) {
+          lanczos_second_sweep(m, n2, L[act[i]].B, L[act[i]].jj, s, no_skip());
+          any = true;
+        }
+      if (any) {
+        const std::vector<int> asked = second;
+        read_scalars();
+        second = asked;
+      }
     }
     for (int i = 0; i < na; ++i) {
       L[act[i]].second_sweeps += second[i];

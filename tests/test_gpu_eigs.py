@@ -257,6 +257,25 @@ def test_eigs_every_step_second_sweep(pkg, n2, n2_eigs):
     assert np.max(np.abs(out["lambda_"] - ref) / ref) < 1e-12
     Y = out["Y"].cpu().numpy()
     assert np.max(np.abs(Y.conj() @ Y.T - np.eye(10))) < 1e-12
+    # the mixed form steps through the general loop, where the host also issues the second
+    # sweep after the step's read: with eta = 1 at every step, with the default only when asked
+    print("case: eigs N=2 mixed, second sweep at every step and by the DGKS test", flush=True)
+    v0 = pkg.start_vector(s.nr, 20260831)
+    outm = pkg.run_method_C(s, k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000, v0=v0,
+                            precision="mixed", want_vectors=True)
+    try:
+        s.set_reorth(1.0)
+        outm1 = pkg.run_method_C(s, k=10, p=40, tol=1e-10, max_restart=30, tol_cg=1e-10, maxit_cg=3000, v0=v0,
+                                 precision="mixed", want_vectors=True)
+    finally:
+        s.set_reorth()
+    st1 = outm1["stats"]
+    assert st1["converged"] and st1["second_sweeps"] >= st1["outer_steps"] - st1["restarts"] - 1 > 0
+    assert outm["stats"]["second_sweeps"] <= st1["second_sweeps"]
+    for o in (outm, outm1):
+        assert np.max(np.abs(o["lambda_"] - ref) / ref) < 1e-8
+        Ym = o["Y"].cpu().numpy()
+        assert np.max(np.abs(Ym.conj() @ Ym.T - np.eye(10))) < 1e-10
 
 
 def test_eigs_vs_oracle_and_mixed(pkg, oracle, n2):
