@@ -212,36 +212,102 @@ constexpr int taylor_bal_minb() {
   return TF * 4 + R <= 20 ? 2 : 1;
 }
 
+// Staged inputs of the balanced form: per-axis phase entries per mode (nx + ny + nz at most)
+// and the chunk's three q components, copied to LDS by asynchronous global-to-LDS loads one
+// chunk ahead.
+constexpr int kStageNa = 18;
+
 // Chunk loop of k_taylor_moments_bal for CTL live column tiles of the workgroup's 4 (the
-// remainder pairs run whatever their column).
-template <int TF, int R, int CTL, int BROW, int TFA, int RA>
-__device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], double4_t (&accr)[RA],
-                                                  double (&pw)[3][(kTZeroRow + 1) * kTRow], double* bt,
+// remainder pairs run whatever their column).  pw holds the power tables of the three axes
+// PWN doubles apart.
+//
+// STAGE (centres on a tensor grid with nx + ny + nz <= kStageNa, whole chunks): the phase
+// entries and q components of chunk i + 1 are loaded straight into LDS
+// (global_load_lds_dwordx4, no registers) while the matrix cores work on chunk i; the table
+// build of chunk i + 1 then reads LDS only.  Otherwise the build loads its inputs from global
+// memory after the chunk barrier and waits for them there.
+template <int TF, int R, int CTL, int BROW, int TFA, int RA, int PWN, bool STAGE>
+__device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], double4_t (&accr)[RA], double* pw,
+                                                  double* bt, cpx<double>* tabs, double* qs,
                                                   const int (&o0)[TFA], const int (&o1)[TFA], const int (&o2)[TFA],
                                                   const int (&r0)[RA], const int (&r1)[RA], const int (&r2)[RA],
                                                   const int (&rc)[RA], long long j_begin, long long j_end, int p,
                                                   int cg, int C, const double* qx, const double* qy,
                                                   const double* qz, const cpx<double>* coeff, const double* centers,
                                                   AxisPhases ax) {
-  const int tid = threadIdx.x, kl = (tid & 63) >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, kl = lane >> 4;
+  // STAGE: this thread's table column (every B element a thread builds has centre
+  // cg 32 + (tid & 31)) and its three entries of each mode
+  const int cl = tid & 31;
+  int kx = 0, ky = 0, kz = 0;
+  if constexpr (STAGE) {
+    const int c = min(cg * 32 + cl, C - 1);
+    const int ix = c % ax.nx, r = c / ax.nx;
+    kx = ix;
+    ky = ax.nx + r % ax.ny;
+    kz = ax.nx + ax.ny + r / ax.ny;
+  }
+  // asynchronous copy of chunk j0's q components (waves 0, 1) and phase rows to LDS
+  auto stage = [&](long long j0) {
+    if (wave == 0)
+      __builtin_amdgcn_global_load_lds((lane < 32 ? qx + j0 + 2 * lane : qy + j0 + 2 * (lane - 32)), qs, 16, 0, 0);
+    else if (wave == 1 && lane < 32)
+      __builtin_amdgcn_global_load_lds(qz + j0 + 2 * lane, qs + 2 * kTChunk, 16, 0, 0);
+    const int tot = kTChunk * ax.na;
+    const cpx<double>* src = ax.tab + j0 * ax.na;
+    for (int e0 = 0; e0 < tot; e0 += 256)
+      if (e0 + tid < tot) __builtin_amdgcn_global_load_lds(src + e0 + tid, tabs + e0 + wave * 64, 16, 0, 0);
+  };
+  if constexpr (STAGE) {
+    if (j_begin < j_end) stage(j_begin);
+  }
   for (long long j0 = j_begin; j0 < j_end; j0 += kTChunk) {
-    __syncthreads();  // previous chunk fully consumed
-    // powers: thread (axis, mode) of 3 x kTChunk
-    for (int e = tid; e < 3 * kTChunk; e += 256) {
-      const int d = e / kTChunk, jj = e - d * kTChunk;
-      const long long j = j0 + jj;
-      const bool ok = j < j_end;
-      const double* qd = d == 0 ? qx : (d == 1 ? qy : qz);
-      const double a = ok ? qd[j] : 0.0;
-      double x = ok ? 1.0 : 0.0;
-      for (int s = 0; s <= p; ++s) {
-        pw[d][s * kTRow + jj] = x;
-        x *= a;
+    if constexpr (STAGE) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this thread's staged loads
+    __syncthreads();  // previous chunk fully consumed; the staged inputs of this one visible
+    if constexpr (STAGE) {
+      for (int e = tid; e < 3 * kTChunk; e += 256) {
+        const int d = e / kTChunk, jj = e - d * kTChunk;
+        const double a = qs[e];
+        double x = 1.0;
+        for (int s = 0; s <= p; ++s) {
+          pw[d * PWN + s * kTRow + jj] = x;
+          x *= a;
+        }
       }
+      const bool cok = cg * 32 + cl < C;
+#pragma unroll
+      for (int k = 0; k < kTChunk * 32 / 256; ++k) {
+        const int jj = (tid >> 5) + 8 * k;
+        const cpx<double>* t = tabs + jj * ax.na;
+        const cpx<double> v = t[kx] * t[ky] * t[kz];
+        bt[jj * BROW + 2 * cl] = cok ? v.re : 0.0;
+        bt[jj * BROW + 2 * cl + 1] = cok ? v.im : 0.0;
+      }
+    } else {
+      // powers: thread (axis, mode) of 3 x kTChunk
+      for (int e = tid; e < 3 * kTChunk; e += 256) {
+        const int d = e / kTChunk, jj = e - d * kTChunk;
+        const long long j = j0 + jj;
+        const bool ok = j < j_end;
+        const double* qd = d == 0 ? qx : (d == 1 ? qy : qz);
+        const double a = ok ? qd[j] : 0.0;
+        double x = ok ? 1.0 : 0.0;
+        for (int s = 0; s <= p; ++s) {
+          pw[d * PWN + s * kTRow + jj] = x;
+          x *= a;
+        }
+      }
+      taylor_build_b<4, BROW>(bt, j0, j_end, cg, C, qx, qy, qz, coeff, centers, ax);
     }
-    taylor_build_b<4, BROW>(bt, j0, j_end, cg, C, qx, qy, qz, coeff, centers, ax);
     __syncthreads();
+    if constexpr (STAGE) {
+      // the staged buffers are free again: the next chunk's inputs go in flight now
+      if (j0 + kTChunk < j_end) stage(j0 + kTChunk);
+    }
     const int nl = tid & 15;
+    const double* pa = pw;
+    const double* pb = pw + PWN;
+    const double* pc = pw + 2 * PWN;
 #pragma unroll 1
     for (int k4 = 0; k4 < kTChunk; k4 += 4) {
       const int kk = k4 + kl;
@@ -251,7 +317,7 @@ __device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], doub
       if constexpr (TF > 0) {
 #pragma unroll
         for (int t = 0; t < TF; ++t) {
-          const double av = pw[0][o0[t] + kk] * pw[1][o1[t] + kk] * pw[2][o2[t] + kk];
+          const double av = pa[o0[t] + kk] * pb[o1[t] + kk] * pc[o2[t] + kk];
 #pragma unroll
           for (int c = 0; c < CTL; ++c) acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[t][c], 0, 0, 0);
         }
@@ -259,14 +325,14 @@ __device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], doub
       if constexpr (R > 0) {
         if constexpr (R == 2) {
           // both pairs of a wave lie in one row tile (g = 2w, 2w + 1)
-          const double av = pw[0][r0[0] + kk] * pw[1][r1[0] + kk] * pw[2][r2[0] + kk];
+          const double av = pa[r0[0] + kk] * pb[r1[0] + kk] * pc[r2[0] + kk];
 #pragma unroll
           for (int r = 0; r < 2; ++r)
             accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bt[kk * BROW + rc[r]], accr[r], 0, 0, 0);
         } else {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double av = pw[0][r0[r] + kk] * pw[1][r1[r] + kk] * pw[2][r2[r] + kk];
+            const double av = pa[r0[r] + kk] * pb[r1[r] + kk] * pc[r2[r] + kk];
             accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bt[kk * BROW + rc[r]], accr[r], 0, 0, 0);
           }
         }
@@ -275,7 +341,9 @@ __device__ __forceinline__ void taylor_bal_chunks(double4_t (&acc)[TFA][4], doub
   }
 }
 
-template <int TF, int R, int MINB>
+// PMAX: the largest order the instance serves (its power tables hold rows 0..PMAX and an
+// all-zero row PMAX + 1).
+template <int TF, int R, int MINB, int PMAX, bool STAGE>
 __global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, const double* qx, const double* qy,
                                                                   const double* qz, const cpx<double>* coeff, int p,
                                                                   int L, int C, const double* centers,
@@ -286,9 +354,13 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, 
   constexpr int NCOL = 16 * CT;
   constexpr int BROW = NCOL + 16;  // padded row stride of B (see k_taylor_moments_mfma)
   constexpr int TFA = TF > 0 ? TF : 1, RA = R > 0 ? R : 1;
+  constexpr int ZR = PMAX + 1;              // the zero row of the power tables
+  constexpr int PWN = (PMAX + 2) * kTRow;   // doubles per axis
   const int Lpad = (L + 15) & ~15;
-  __shared__ double pw[3][(kTZeroRow + 1) * kTRow];
+  __shared__ double pw[3 * PWN];
   __shared__ double bt[kTChunk * BROW];
+  __shared__ __attribute__((aligned(16))) cpx<double> tabs[STAGE ? kTChunk * kStageNa : 1];
+  __shared__ __attribute__((aligned(16))) double qs[STAGE ? 3 * kTChunk : 1];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = tid >> 6;
   const int kl = lane >> 4, nl = lane & 15;
@@ -311,7 +383,7 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, 
   auto offsets = [&](int tile, int& a0, int& a1, int& a2) {
     const int alpha = tile * 16 + nl;
     const bool ok = alpha < L;
-    a0 = (ok ? (int)alist[3 * alpha] : kTZeroRow) * kTRow;
+    a0 = (ok ? (int)alist[3 * alpha] : ZR) * kTRow;
     a1 = (ok ? (int)alist[3 * alpha + 1] : 0) * kTRow;
     a2 = (ok ? (int)alist[3 * alpha + 2] : 0) * kTRow;
   };
@@ -325,7 +397,7 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, 
     offsets(4 * TF + g / 4, r0[r], r1[r], r2[r]);
     rc[r] = (g % 4) * 16 + nl;
   }
-  for (int i = tid; i < kTChunk; i += 256) pw[0][kTZeroRow * kTRow + i] = 0.0;
+  for (int i = tid; i < kTChunk; i += 256) pw[ZR * kTRow + i] = 0.0;
 
   double4_t acc[TFA][CT], accr[RA];
 #pragma unroll
@@ -339,11 +411,11 @@ __global__ __launch_bounds__(256, MINB) void k_taylor_moments_bal(long long NF, 
   // full row tiles then issue no matrix-core ops for the empty ones
   const int ctl = min(CT, (C - cg * 8 * CT + 7) / 8);
   if (ctl >= 4)
-    taylor_bal_chunks<TF, R, 4, BROW>(acc, accr, pw, bt, o0, o1, o2, r0, r1, r2, rc, j_begin, j_end, p, cg, C, qx,
-                                      qy, qz, coeff, centers, ax);
+    taylor_bal_chunks<TF, R, 4, BROW, TFA, RA, PWN, STAGE>(acc, accr, pw, bt, tabs, qs, o0, o1, o2, r0, r1, r2, rc,
+                                                           j_begin, j_end, p, cg, C, qx, qy, qz, coeff, centers, ax);
   else
-    taylor_bal_chunks<TF, R, 3, BROW>(acc, accr, pw, bt, o0, o1, o2, r0, r1, r2, rc, j_begin, j_end, p, cg, C, qx,
-                                      qy, qz, coeff, centers, ax);
+    taylor_bal_chunks<TF, R, 3, BROW, TFA, RA, PWN, STAGE>(acc, accr, pw, bt, tabs, qs, o0, o1, o2, r0, r1, r2, rc,
+                                                           j_begin, j_end, p, cg, C, qx, qy, qz, coeff, centers, ax);
   // partial sums of this mode slice: part[(slice*Lpad + alpha)*Cpad2 + col]
   double* ps = part + (long long)slice * Lpad * Cpad2 + cg * NCOL;
   if constexpr (TF > 0) {
@@ -762,14 +834,22 @@ int fame6d_taylor_moments(long long NF, const double* qmodes, const void* coeff,
       F6_CHECK_LAUNCH();
       ax = AxisPhases{axt.as<cpx<double>>(), nx, ny, na};
     }
+    // staged inputs: tensor-grid phase tables that fit the LDS staging area, whole chunks
+    const bool stage = axis && ax.na <= kStageNa && NF % kTChunk == 0;
     if (balanced) {
       switch (p) {
 #define F6_TAYLOR_BAL(P)                                                                                          \
   case P: {                                                                                                      \
     constexpr int TT = (taylor_terms(P) + 15) / 16;                                                              \
-    k_taylor_moments_bal<TT / 4, TT % 4, taylor_bal_minb<TT / 4, TT % 4>()><<<grid_bal, 256, 0, s>>>(           \
-        NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C, cen, ax, alp, gf, nslices, span, span_last, Cpad2, \
-        part.as<double>());                                                                                     \
+    constexpr int PM = P < 2 ? 2 : P; /* p = 0, 1, 2 share the one-row-tile instance */                        \
+    if (stage)                                                                                                   \
+      k_taylor_moments_bal<TT / 4, TT % 4, taylor_bal_minb<TT / 4, TT % 4>(), PM, true>                          \
+          <<<grid_bal, 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C, cen, ax, alp, gf,    \
+                                    nslices, span, span_last, Cpad2, part.as<double>());                        \
+    else                                                                                                         \
+      k_taylor_moments_bal<TT / 4, TT % 4, taylor_bal_minb<TT / 4, TT % 4>(), PM, false>                         \
+          <<<grid_bal, 256, 0, s>>>(NF, qmodes, qmodes + NF, qmodes + 2 * NF, cf, p, L, C, cen, ax, alp, gf,    \
+                                    nslices, span, span_last, Cpad2, part.as<double>());                        \
     break;                                                                                                       \
   }
         F6_TAYLOR_BAL(0) F6_TAYLOR_BAL(1) F6_TAYLOR_BAL(2) F6_TAYLOR_BAL(3) F6_TAYLOR_BAL(4) F6_TAYLOR_BAL(5)
