@@ -328,12 +328,15 @@ __device__ __forceinline__ double inv_c_from(double w1, double w2) {
   const double korth = 1.0 + w1 * w1 + w2 * w2;
   return 1.0 / sqrt(korth);
 }
-// Weight of output o of the M-hat pre-op at diagonal index q: invC, invC W1, invC W2.
+// Weight of output o of the M-hat pre-op at diagonal index q: invC, invC W1, invC W2.  Both
+// loads are issued by every thread (output 0 reads invC twice): a wave can hold threads of
+// two outputs, and a load under `if (o > 0)` became a divergent branch that waited for every
+// earlier load before its own, once per pencil element.
 template <class T>
 __device__ __forceinline__ T mhat_weight(const T* invC, const T* W1, const T* W2, int o, long long q) {
-  T w = invC[q];
-  if (o > 0) w *= (o == 1 ? W1 : W2)[q];
-  return w;
+  const T* wp = o == 0 ? invC : (o == 1 ? W1 : W2);
+  const T a = invC[q], b = wp[q];
+  return o > 0 ? a * b : a;
 }
 
 // invC at q for the post-op: recomputed from the W1, W2 values in hand in FP64, read in FP32.
