@@ -594,7 +594,7 @@ struct ThreeTerm {
   double* norm;         // nrb partials of ||r||^2
 };
 template <int KB, bool THREE = false>
-__global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
+__global__ __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
                                                             const cpx<double>* r, long long n, int nrb,
                                                             double* pre, double* pim, Skip sk,
                                                             ThreeTerm tt = ThreeTerm{nullptr, 0.0, nullptr}) {
@@ -608,13 +608,14 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q
   double ar[KB], ai[KB];
 #pragma unroll
   for (int k = 0; k < KB; ++k) ar[k] = ai[k] = 0.0;
+  // every column slot accumulates (a short last chunk re-reads its last column into the
+  // slots past kcount, whose sums are never written): a test per slot became a branch into
+  // which the compiler sank that slot's load, behind a wait for every earlier load
   auto accumulate = [&](const cpx<double>& v, const cpx<double>* q) {
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      if (k < kcount) {
-        ar[k] += q[k].re * v.re + q[k].im * v.im;
-        ai[k] += q[k].re * v.im - q[k].im * v.re;
-      }
+      ar[k] += q[k].re * v.re + q[k].im * v.im;
+      ai[k] += q[k].re * v.im - q[k].im * v.re;
     }
   };
   const long long stride = (long long)nrb * kRedThreads;
@@ -636,17 +637,25 @@ __global__ __launch_bounds__(kRedThreads) void k_gemv_c_all(const cpx<double>* Q
   };
   long long i = rb * (long long)kRedThreads + threadIdx.x;
   for (; i + (RPT - 1) * stride < n; i += RPT * stride) {
-    cpx<double> v[RPT], q[RPT][KB];
+    // every load of the step first (the three-term inputs as loaded, not yet combined: an
+    // update formed between the loads made the column loads wait for it)
+    cpx<double> rv[RPT], av[RPT], bv[RPT], q[RPT][KB];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
-      v[u] = row(i + u * stride);
+      rv[u] = r[i + u * stride];
+      if constexpr (THREE) {
+        av[u] = qa[i + u * stride];
+        bv[u] = qb[i + u * stride];
+      }
 #pragma unroll
       for (int k = 0; k < KB; ++k) q[u][k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i + u * stride];
     }
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
-      accumulate(v[u], q[u]);
-      if constexpr (THREE) nrm += v[u].re * v[u].re + v[u].im * v[u].im;
+      cpx<double> v = rv[u];
+      if constexpr (THREE) v = (v - alpha * av[u]) - beta * bv[u];
+      accumulate(v, q[u]);
+      if constexpr (THREE) nrm += v.re * v.re + v.im * v.im;
     }
   }
   if (i < n) {  // at most one row is left: the loop above stops within RPT strides of n
