@@ -164,7 +164,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const doubl
   __shared__ double red[kRedThreads];
   const int g = blockIdx.x;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[(long long)g * nb + i];
+  s = strided_sum(s, partials + (long long)g * nb, threadIdx.x, nb, kRedThreads);
   s = block_sum<kRedThreads>(s, red);
   if (threadIdx.x == 0) out[g] = s;
 }
@@ -180,7 +180,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_reduce_final_ri(const do
   const int g = im ? blockIdx.x - jj : blockIdx.x;
   const double* partials = im ? pim : pre;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[(long long)g * nb + i];
+  s = strided_sum(s, partials + (long long)g * nb, threadIdx.x, nb, kRedThreads);
   s = block_sum<kRedThreads>(s, red);
   if (threadIdx.x == 0) (im ? out_im : out_re)[g] = s;
 }
@@ -191,8 +191,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_reduce_final_ri(const do
 // alpha = rho/(p'q), stop on the recurrence residual ||r|| <= tol*||b||
 // (the host then verifies with the true residual b - A x).
 __device__ __forceinline__ double sum_partials(const double* partials, int nb, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
+  const double s = strided_sum(0.0, partials, threadIdx.x, nb, kRedThreads);
   return block_sum<kRedThreads>(s, red);
 }
 
@@ -811,14 +810,12 @@ static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, cons
   int* skip = reinterpret_cast<int*>(sc + 4);
   if (stage == 2 && *skip) return;
   if (pre) {  // stage 0 in the same launch: ||r||^2 before the sweep
-    double s0 = 0.0;
-    for (int i = threadIdx.x; i < npre; i += kRedThreads) s0 += pre[i];
+    double s0 = strided_sum(0.0, pre, threadIdx.x, npre, kRedThreads);
     s0 = block_sum<kRedThreads>(s0, red);
     if (threadIdx.x == 0) sc[2] = s0;
     __syncthreads();
   }
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
+  double s = strided_sum(0.0, partials, threadIdx.x, nb, kRedThreads);
   s = block_sum<kRedThreads>(s, red);
   if (threadIdx.x == 0) {
     if (stage == 0) {

@@ -205,6 +205,20 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+// s + src[i0] + src[i0 + stride] + ... over i < n, added in that order (the order of the plain
+// strided loop, so every caller that sums the same partials gets the same value), with the
+// loads issued four at a time: the plain loop waited for each load before the next.
+__device__ __forceinline__ double strided_sum(double s, const double* src, int i0, int n, int stride) {
+  for (int i = i0; i < n; i += 4 * stride) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = i + u * stride < n ? src[i + u * stride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += v[u];
+  }
+  return s;
+}
+
 // MATLAB pcg's bookkeeping after an unpreconditioned iteration: iteration count, recurrence
 // convergence test, maxit, breakdown, beta = rho_new / rho (make_explicit_krinv_compact.m:43
 // calls pcg with x0 = 0).  rho_new = ||r_new||^2 is the fixed-order sum of the pass-C
@@ -270,7 +284,7 @@ __device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, do
     double v = 0.0;
     if constexpr (THREADS >= kCgSumThreads) {
       if (tid < kCgSumThreads)
-        for (int i = tid; i < nb; i += kCgSumThreads) v += src[i];
+        v = strided_sum(v, src, tid, nb, kCgSumThreads);
       v = block_sum<THREADS>(v, red);
     } else {
       // fewer threads than k_cg_beta has: each takes the lanes tid, tid + THREADS, ... of a
@@ -280,7 +294,7 @@ __device__ __forceinline__ bool cg_pass_a_state(const PassAArgs<T>& a, int g, do
       __shared__ double lane[kCgSumThreads];
       for (int l = tid; l < kCgSumThreads; l += THREADS) {
         double u = 0.0;
-        for (int i = l; i < nb; i += kCgSumThreads) u += src[i];
+        u = strided_sum(u, src, l, nb, kCgSumThreads);
         lane[l] = u;
       }
       __syncthreads();
@@ -1282,7 +1296,7 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3(PassCArgs<T> a) {
     double pq = 0.0;
     {
       const double* pp = a.pq_partials + (long long)g * a.npq;
-      for (int i = tid; i < a.npq; i += THREADS) pq += pp[i];
+      pq = strided_sum(pq, pp, tid, a.npq, THREADS);
       pq = block_sum<THREADS>(pq, red);
       if (tid == 0) red[0] = pq;
       __syncthreads();
@@ -1382,7 +1396,7 @@ __global__ __launch_bounds__(3 * BPW * n * n) void k_pass_c3p(PassCArgs<T> a) {
     double pq = 0.0;
     {
       const double* pqp = a.pq_partials + (long long)g * a.npq;
-      for (int i = tid; i < a.npq; i += THREADS) pq += pqp[i];
+      pq = strided_sum(pq, pqp, tid, a.npq, THREADS);
       pq = block_sum<THREADS>(pq, red);
       if (tid == 0) red[0] = pq;
       __syncthreads();

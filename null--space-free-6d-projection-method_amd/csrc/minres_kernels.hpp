@@ -128,8 +128,7 @@ static __global__ __launch_bounds__(kRedThreads) void k_mr_precond(
 }
 
 __device__ __forceinline__ double mr_sum(const double* partials, int nb, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nb; i += kRedThreads) s += partials[i];
+  const double s = strided_sum(0.0, partials, threadIdx.x, nb, kRedThreads);
   return block_sum<kRedThreads>(s, red);
 }
 
