@@ -560,6 +560,16 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
   const int al = tid % A, qq = tid / A;  // qq in [0, n^2)
   const long long a0 = (long long)tile * A + al;
   cpx<T>* v = a.t + vidx * a.ldt + a0;
+  // FP32: the permittivity of this thread's dim-6 pencil is loaded with the input, so it is
+  // in registers when the dim-6 stage needs it (a load there, after two LDS stages and their
+  // barriers, waits its full latency).  FP64 loads it in the stage: held from the start it
+  // made the FP64 kernel 3% slower at N = 5.
+  constexpr bool PRE_EPS = MODE == 0 && std::is_same<T, float>::value;
+  T ep[n];
+  if constexpr (PRE_EPS) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) ep[k] = a.eps[a0 + (long long)S * (qq + P2 * k)];
+  }
   {  // dim 4 inverse: (k5, k6) = qq
     cpx<T> w[n];
 #pragma unroll
@@ -592,7 +602,7 @@ __global__ __launch_bounds__(A* n* n) void k_pass_b(PassBArgs<T> a) {
       return;
     } else {
 #pragma unroll
-      for (int k = 0; k < n; ++k) w[k] = a.eps[a0 + (long long)S * (qq + P2 * k)] * w[k];
+      for (int k = 0; k < n; ++k) w[k] = (PRE_EPS ? ep[k] : a.eps[a0 + (long long)S * (qq + P2 * k)]) * w[k];
       dft<T, n, -1>(w);
       if (a.partials != nullptr) {
         // z (still in LDS) against F6 eps G6 z: no extra registers over the plain apply
@@ -672,6 +682,12 @@ __global__ __launch_bounds__(A / 2 * n * n) void k_pass_b2(PassBArgs<float> a) {
   auto sts2 = [&](int l, const cpx<float>& x, const cpx<float>& y) {
     *reinterpret_cast<f4*>(L + l) = (f4){x.re, x.im, y.re, y.im};
   };
+  // the permittivity of this thread's dim-6 pencils, loaded with the input (see k_pass_b)
+  f2 ep[n];
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) ep[k] = *reinterpret_cast<const f2*>(a.eps + a0 + (long long)S * (qq + P2 * k));
+  }
   {  // dim 4 inverse: (k5, k6) = qq
     cpx<float> w0[n], w1[n];
 #pragma unroll
@@ -708,9 +724,8 @@ __global__ __launch_bounds__(A / 2 * n * n) void k_pass_b2(PassBArgs<float> a) {
     } else {
 #pragma unroll
       for (int k = 0; k < n; ++k) {
-        const f2 e = *reinterpret_cast<const f2*>(a.eps + a0 + (long long)S * (qq + P2 * k));
-        w0[k] = e.x * w0[k];
-        w1[k] = e.y * w1[k];
+        w0[k] = ep[k].x * w0[k];
+        w1[k] = ep[k].y * w1[k];
       }
       dft<float, n, -1>(w0);
       dft<float, n, -1>(w1);
