@@ -11,9 +11,168 @@
 
 namespace fame6d {
 
-// ------------------------------------------------------------------ q-system (host)
+// ------------------------------------------------------------------ q-system
 // Restates assemble_K_and_nullspace_free_basis.m:18-105 and
-// build_experiment1_q_system_C_compact.m:9-31 for real P (6x3), T = 2*pi.
+// build_experiment1_q_system_C_compact.m:9-31 for real P (6x3), T = 2*pi.  The per-mode
+// quantities are formed on the device (one thread per mode, every Bloch point of the set at
+// once); the host forms the lift vectors and checks the per-point minima.
+
+struct QsysBuildArgs {
+  double P[18];          // P (6x3, row-major)
+  const double* klift;   // 6 per point
+  int N, n;
+  long long NF, GN;      // modes per point, G * NF
+  double* V;             // V11, V12, V21, V22, V31, V32 (GN each)
+  double* W;             // W1, W2, invC (GN each)
+  float* Wf;             // W in FP32
+  double* Qm;            // q1, q2, q3 (GN each)
+  double* minqq;         // per workgroup and point: min |q|^2
+  double* mindet;        //                         min |V11 V22 - V12 V21|
+  int* nsing;            //                         modes on the singular branch
+};
+
+constexpr int kQsysThreads = 256, kQsysBlocks = 256;
+
+// One thread per mode of point blockIdx.y: q = P^T (xi + k_lift), the null-space-free basis
+// V = |q| [u1 u2] (the regular branch, or the singular one of
+// assemble_K_and_nullspace_free_basis.m:75-91 where 3|q|^2 - (q1+q2+q3)^2 <= 1e-13), W and
+// invC.  The operations and their order are those of the per-mode formulas in the reference;
+// contraction into fused multiply-adds is off, so every value is the correctly rounded
+// result of the same IEEE operations on any conforming machine.
+__global__ __launch_bounds__(kQsysThreads) void k_qsys_build(QsysBuildArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double rq[kQsysThreads], rd[kQsysThreads];
+  __shared__ int rs[kQsysThreads];
+  const int g = blockIdx.y;
+  const double* kl = a.klift + 6 * g;
+  const long long off = (long long)g * a.NF;
+  double minqq = INFINITY, mindet = INFINITY;
+  int nsing = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.NF;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long t = i;
+    double qv[3] = {0, 0, 0};
+    for (int l = 0; l < 6; ++l) {
+      const int k = (int)(t % a.n);
+      t /= a.n;
+      const double xi = (k < a.N) ? k : k - a.n;  // build_fourier_grid_6d.m:12
+      const double sx = xi + kl[l];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qv[c] += a.P[l * 3 + c] * sx;
+    }
+    const long long j = off + i;
+    a.Qm[j] = qv[0];
+    a.Qm[a.GN + j] = qv[1];
+    a.Qm[2 * a.GN + j] = qv[2];
+    const double Qq = qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2];
+    minqq = fmin(minqq, Qq);
+    const double sq = sqrt(Qq);
+    const double Qs = qv[0] + qv[1] + qv[2];
+    const double den = 3 * Qq - Qs * Qs;
+    double u1[3], u2[3];
+    if (den > 1e-13) {
+      const double s1 = 1.0 / sqrt(Qq * den), s2 = 1.0 / sqrt(den);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u1[c] = (Qq - qv[c] * Qs) * s1;
+      u2[0] = (qv[2] - qv[1]) * s2;
+      u2[1] = (qv[0] - qv[2]) * s2;
+      u2[2] = (qv[1] - qv[0]) * s2;
+    } else {
+      // singular branch (assemble_K_and_nullspace_free_basis.m:75-91)
+      ++nsing;
+      const double h[3] = {qv[0] / sq, qv[1] / sq, qv[2] / sq};
+      int imin = 0;
+      for (int c = 1; c < 3; ++c)
+        if (fabs(h[c]) < fabs(h[imin])) imin = c;
+      double av[3] = {0, 0, 0};
+      av[imin] = 1;
+      const double ha = h[imin];
+      double z1[3] = {av[0] - h[0] * ha, av[1] - h[1] * ha, av[2] - h[2] * ha};
+      double nz = sqrt(z1[0] * z1[0] + z1[1] * z1[1] + z1[2] * z1[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) z1[c] /= nz;
+      const double z2[3] = {h[1] * z1[2] - h[2] * z1[1], h[2] * z1[0] - h[0] * z1[2], h[0] * z1[1] - h[1] * z1[0]};
+      nz = sqrt(z2[0] * z2[0] + z2[1] * z2[1] + z2[2] * z2[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u1[c] = z1[c];
+        u2[c] = z2[c] / nz;
+      }
+    }
+    const double v11 = u1[0] * sq, v21 = u1[1] * sq, v31 = u1[2] * sq;
+    const double v12 = u2[0] * sq, v22 = u2[1] * sq, v32 = u2[2] * sq;
+    a.V[j] = v11;
+    a.V[a.GN + j] = v12;
+    a.V[2 * a.GN + j] = v21;
+    a.V[3 * a.GN + j] = v22;
+    a.V[4 * a.GN + j] = v31;
+    a.V[5 * a.GN + j] = v32;
+    const double d1 = v11 * v22 - v12 * v21;
+    mindet = fmin(mindet, fabs(d1));
+    const double w1 = (v22 * v31 - v21 * v32) / d1;
+    const double w2 = (v11 * v32 - v12 * v31) / d1;
+    const double korth = 1.0 + w1 * w1 + w2 * w2;
+    const double ic = 1.0 / sqrt(korth);
+    a.W[j] = w1;
+    a.W[a.GN + j] = w2;
+    a.W[2 * a.GN + j] = ic;
+    a.Wf[j] = (float)w1;
+    a.Wf[a.GN + j] = (float)w2;
+    a.Wf[2 * a.GN + j] = (float)ic;
+  }
+  const int tid = threadIdx.x;
+  rq[tid] = minqq;
+  rd[tid] = mindet;
+  rs[tid] = nsing;
+  __syncthreads();
+  for (int h = kQsysThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      rq[tid] = fmin(rq[tid], rq[tid + h]);
+      rd[tid] = fmin(rd[tid], rd[tid + h]);
+      rs[tid] += rs[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long long b = (long long)g * gridDim.x + blockIdx.x;
+    a.minqq[b] = rq[0];
+    a.mindet[b] = rd[0];
+    a.nsing[b] = rs[0];
+  }
+}
+
+// Per point: the minima and the count over the workgroups of k_qsys_build (one workgroup
+// per point), into out[3 g .. 3 g + 2] (the count as a double).
+__global__ __launch_bounds__(kQsysThreads) void k_qsys_minima(const double* minqq, const double* mindet,
+                                                              const int* nsing, int nb, double* out) {
+  __shared__ double rq[kQsysThreads], rd[kQsysThreads];
+  __shared__ int rs[kQsysThreads];
+  const int g = blockIdx.x, tid = threadIdx.x;
+  double q = INFINITY, d = INFINITY;
+  int c = 0;
+  for (int b = tid; b < nb; b += kQsysThreads) {
+    q = fmin(q, minqq[(long long)g * nb + b]);
+    d = fmin(d, mindet[(long long)g * nb + b]);
+    c += nsing[(long long)g * nb + b];
+  }
+  rq[tid] = q;
+  rd[tid] = d;
+  rs[tid] = c;
+  __syncthreads();
+  for (int h = kQsysThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      rq[tid] = fmin(rq[tid], rq[tid + h]);
+      rd[tid] = fmin(rd[tid], rd[tid + h]);
+      rs[tid] += rs[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[3 * g] = rq[0];
+    out[3 * g + 1] = rd[0];
+    out[3 * g + 2] = (double)rs[0];
+  }
+}
 fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P) {
   if (G <= 0) throw Error(FAME6D_EARG, "need at least one Bloch point");
   const int N = m->N, n = m->n;
@@ -49,89 +208,49 @@ fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P)
   Ai[8] = (A[0] * A[4] - A[1] * A[3]) / det;
 
   const size_t GN = (size_t)G * NF;
-  std::vector<double> V(6 * GN), W(3 * GN), Qm(3 * GN);
   for (int g = 0; g < G; ++g) {
     const double* qq = q + 3 * g;
-    if (!(std::isfinite(qq[0]) && std::isfinite(qq[1]) && std::isfinite(qq[2])))
+    if (!(std::isfinite(qq[0]) && std::isfinite(qq[1]) && std::isfinite(qq[2]))) {
+      delete S;
       throw Error(FAME6D_ENONFINITE, "Bloch vector contains NaN/Inf");
+    }
     double y[3];
     for (int r = 0; r < 3; ++r) y[r] = Ai[r * 3 + 0] * qq[0] + Ai[r * 3 + 1] * qq[1] + Ai[r * 3 + 2] * qq[2];
     double* kl = &S->klift[6 * g];
     for (int l = 0; l < 6; ++l) kl[l] = P[l * 3 + 0] * y[0] + P[l * 3 + 1] * y[1] + P[l * 3 + 2] * y[2];
-
-    double* V11 = &V[0 * GN + g * NF];
-    double* V12 = &V[1 * GN + g * NF];
-    double* V21 = &V[2 * GN + g * NF];
-    double* V22 = &V[3 * GN + g * NF];
-    double* V31 = &V[4 * GN + g * NF];
-    double* V32 = &V[5 * GN + g * NF];
-    double* W1 = &W[0 * GN + g * NF];
-    double* W2 = &W[1 * GN + g * NF];
-    double* iC = &W[2 * GN + g * NF];
-    double* q1 = &Qm[0 * GN + g * NF];
-    double* q2 = &Qm[1 * GN + g * NF];
-    double* q3 = &Qm[2 * GN + g * NF];
-    double minqq = INFINITY, mindet = INFINITY;
-    int nsing = 0;
-    for (long long i = 0; i < NF; ++i) {
-      long long t = i;
-      double qv[3] = {0, 0, 0};
-      for (int l = 0; l < 6; ++l) {
-        const int k = (int)(t % n);
-        t /= n;
-        const double xi = (k < N) ? k : k - n;  // build_fourier_grid_6d.m:12
-        const double sx = xi + kl[l];
-        for (int c = 0; c < 3; ++c) qv[c] += P[l * 3 + c] * sx;
-      }
-      q1[i] = qv[0];
-      q2[i] = qv[1];
-      q3[i] = qv[2];
-      const double Qq = qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2];
-      minqq = std::min(minqq, Qq);
-      const double sq = std::sqrt(Qq);
-      const double Qs = qv[0] + qv[1] + qv[2];
-      const double den = 3 * Qq - Qs * Qs;
-      double u1[3], u2[3];
-      if (den > 1e-13) {
-        const double s1 = 1.0 / std::sqrt(Qq * den), s2 = 1.0 / std::sqrt(den);
-        for (int c = 0; c < 3; ++c) u1[c] = (Qq - qv[c] * Qs) * s1;
-        u2[0] = (qv[2] - qv[1]) * s2;
-        u2[1] = (qv[0] - qv[2]) * s2;
-        u2[2] = (qv[1] - qv[0]) * s2;
-      } else {
-        // singular branch (assemble_K_and_nullspace_free_basis.m:75-91)
-        ++nsing;
-        double h[3] = {qv[0] / sq, qv[1] / sq, qv[2] / sq};
-        int imin = 0;
-        for (int c = 1; c < 3; ++c)
-          if (std::fabs(h[c]) < std::fabs(h[imin])) imin = c;
-        double a[3] = {0, 0, 0};
-        a[imin] = 1;
-        const double ha = h[imin];
-        double z1[3] = {a[0] - h[0] * ha, a[1] - h[1] * ha, a[2] - h[2] * ha};
-        double nz = std::sqrt(z1[0] * z1[0] + z1[1] * z1[1] + z1[2] * z1[2]);
-        for (int c = 0; c < 3; ++c) z1[c] /= nz;
-        double z2[3] = {h[1] * z1[2] - h[2] * z1[1], h[2] * z1[0] - h[0] * z1[2],
-                        h[0] * z1[1] - h[1] * z1[0]};
-        nz = std::sqrt(z2[0] * z2[0] + z2[1] * z2[1] + z2[2] * z2[2]);
-        for (int c = 0; c < 3; ++c) {
-          u1[c] = z1[c];
-          u2[c] = z2[c] / nz;
-        }
-      }
-      V11[i] = u1[0] * sq;
-      V21[i] = u1[1] * sq;
-      V31[i] = u1[2] * sq;
-      V12[i] = u2[0] * sq;
-      V22[i] = u2[1] * sq;
-      V32[i] = u2[2] * sq;
-      const double d1 = V11[i] * V22[i] - V12[i] * V21[i];
-      mindet = std::min(mindet, std::fabs(d1));
-      W1[i] = (V22[i] * V31[i] - V21[i] * V32[i]) / d1;
-      W2[i] = (V11[i] * V32[i] - V12[i] * V31[i]) / d1;
-      const double korth = 1.0 + W1[i] * W1[i] + W2[i] * W2[i];
-      iC[i] = 1.0 / std::sqrt(korth);
-    }
+  }
+  S->Vd.ensure(sizeof(double) * 6 * GN);
+  S->Wd.ensure(sizeof(double) * 3 * GN);
+  S->Wf.ensure(sizeof(float) * 3 * GN);
+  S->Qd.ensure(sizeof(double) * 3 * GN);
+  const int nb = kQsysBlocks;
+  DevBuf tmp;  // k_lift, then the per-workgroup minima and counts, then the per-point results
+  const size_t o_kl = 0, o_q = o_kl + 6 * G, o_d = o_q + (size_t)nb * G, o_s = o_d + (size_t)nb * G;
+  const size_t o_out = o_s + ((size_t)nb * G + 1) / 2;  // the int counts take half as many doubles
+  tmp.ensure(sizeof(double) * (o_out + 3 * G));
+  double* td = tmp.as<double>();
+  h2d_async(m, td + o_kl, S->klift.data(), sizeof(double) * 6 * G);
+  QsysBuildArgs a{};
+  std::memcpy(a.P, P, sizeof(double) * 18);
+  a.klift = td + o_kl;
+  a.N = N;
+  a.n = n;
+  a.NF = NF;
+  a.GN = (long long)GN;
+  a.V = S->Vd.as<double>();
+  a.W = S->Wd.as<double>();
+  a.Wf = S->Wf.as<float>();
+  a.Qm = S->Qd.as<double>();
+  a.minqq = td + o_q;
+  a.mindet = td + o_d;
+  a.nsing = reinterpret_cast<int*>(td + o_s);
+  k_qsys_build<<<dim3(nb, G), kQsysThreads, 0, m->stream>>>(a);
+  k_qsys_minima<<<G, kQsysThreads, 0, m->stream>>>(a.minqq, a.mindet, a.nsing, nb, td + o_out);
+  F6_CHECK_LAUNCH();
+  std::vector<double> mins(3 * G);
+  d2h_sync(m, mins.data(), td + o_out, sizeof(double) * 3 * G);
+  for (int g = 0; g < G; ++g) {
+    const double minqq = mins[3 * g];
     if (minqq <= 1e-13) {
       delete S;
       throw Error(FAME6D_EGAMMA,
@@ -140,18 +259,9 @@ fame6d_qsys* qsys_build(fame6d_mass* m, int G, const double* q, const double* P)
                       std::to_string(minqq) + ")");
     }
     S->min_qq[g] = minqq;
-    S->min_abs_det[g] = mindet;
-    S->n_singular[g] = nsing;
+    S->min_abs_det[g] = mins[3 * g + 1];
+    S->n_singular[g] = (int)mins[3 * g + 2];
   }
-  std::vector<float> Wf(W.begin(), W.end());
-  S->Vd.ensure(sizeof(double) * V.size());
-  S->Wd.ensure(sizeof(double) * W.size());
-  S->Wf.ensure(sizeof(float) * Wf.size());
-  S->Qd.ensure(sizeof(double) * Qm.size());
-  F6_HIP(hipMemcpyAsync(S->Vd.ptr, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, m->stream));
-  F6_HIP(hipMemcpyAsync(S->Wd.ptr, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, m->stream));
-  F6_HIP(hipMemcpyAsync(S->Wf.ptr, Wf.data(), sizeof(float) * Wf.size(), hipMemcpyHostToDevice, m->stream));
-  F6_HIP(hipMemcpyAsync(S->Qd.ptr, Qm.data(), sizeof(double) * Qm.size(), hipMemcpyHostToDevice, m->stream));
   F6_HIP(hipStreamSynchronize(m->stream));
   return S;
 }

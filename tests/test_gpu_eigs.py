@@ -35,6 +35,31 @@ def test_qsystem_diagonals(n2):
     assert np.allclose(n2["s"].info["kLift"], o["klift"], atol=1e-16)
 
 
+@pytest.mark.parametrize("N,q", [(3, Q_EXP1), (4, Q_EXP4), (5, Q_EXP1)])
+def test_qsystem_device_build(pkg, oracle, N, q):
+    """The q-system diagonals formed on the device (one thread per mode) against the oracle's
+    per-mode formulas at every mode, and the minima the handle reports (min |q|^2, min
+    |det V1|) against the oracle's arrays."""
+    print("case: device q-system build N=%d" % N, flush=True)
+    torch.cuda.set_device(0)
+    eps = oracle.eps_reference(N)
+    m = pkg.MassBackend(N, eps)
+    s = pkg.QSystem(m, q)
+    try:
+        o = oracle.qsystem(N, q)
+        d = s.diagonals()
+        for k in ("V11", "V12", "V21", "V22", "V31", "V32", "W1", "W2", "invC"):
+            assert np.allclose(d[k], o[k], rtol=1e-13, atol=1e-13), k
+        info = s.info
+        assert info["minQq"] == pytest.approx(o["Qq"].min(), rel=1e-14)
+        det = np.abs(o["V11"] * o["V22"] - o["V12"] * o["V21"])
+        assert info["minAbsDetV1"] == pytest.approx(det.min(), rel=1e-12)
+        assert info["nSingular"] == o["nsing"] == 0
+    finally:
+        s.close()
+        m.close()
+
+
 @pytest.mark.parametrize("v3", [0, 1])
 def test_mhat(n2, oracle, v3):
     print("case: Mhat N=2 variant %d" % v3, flush=True)
