@@ -592,17 +592,12 @@ struct ThreeTerm {
   double beta;          // 0 on the first step
   double* norm;         // nrb partials of ||r||^2
 };
-template <int KB, bool THREE = false>
-__global__ __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
-                                                            const cpx<double>* r, long long n, int nrb,
-                                                            double* pre, double* pim, Skip sk,
-                                                            ThreeTerm tt = ThreeTerm{nullptr, 0.0, nullptr}) {
+// The work of block (rb, ch) (row block, column chunk) of k_gemv_c_all.
+template <int KB, bool THREE>
+__device__ __forceinline__ void gemv_c_block(const cpx<double>* Q, long long ld, int jj, const cpx<double>* r,
+                                             long long n, int nrb, int rb, int ch, double* pre, double* pim,
+                                             const ThreeTerm& tt, double* red) {
   constexpr int RPT = 2;
-  __shared__ double red[kRedThreads];
-  if (skipped(sk, 0)) return;
-  const int nch = (jj + KB - 1) / KB;
-  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int rb = logical / nch, ch = logical - rb * nch;
   const int k0 = ch * KB, kcount = min(KB, jj - k0);
   double ar[KB], ai[KB];
 #pragma unroll
@@ -682,6 +677,19 @@ __global__ __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 
       pim[(long long)(k0 + k) * nrb + rb] = t;
     }
   }
+}
+
+template <int KB, bool THREE = false>
+__global__ __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gemv_c_all(const cpx<double>* Q, long long ld, int jj,
+                                                            const cpx<double>* r, long long n, int nrb,
+                                                            double* pre, double* pim, Skip sk,
+                                                            ThreeTerm tt = ThreeTerm{nullptr, 0.0, nullptr}) {
+  __shared__ double red[kRedThreads];
+  if (skipped(sk, 0)) return;
+  const int nch = (jj + KB - 1) / KB;
+  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int rb = logical / nch, ch = logical - rb * nch;
+  gemv_c_block<KB, THREE>(Q, ld, jj, r, n, nrb, rb, ch, pre, pim, tt, red);
 }
 
 // r -= Q(:, 0:j) h   (h complex, on device)
@@ -765,14 +773,11 @@ __global__ __launch_bounds__(256) void k_basis_gemm(const double* __restrict__ Q
 // out = in - Q(:, 0:j) h (h complex, on device); optional ||out||^2 partials per workgroup.
 // With `alpha` set, h_{j-1} += alpha and h_{j-2} += beta (j >= 2): out = in - alpha q_{j-1}
 // - beta q_{j-2} - Q h, the three-term update and the sweep in one pass.
-static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
-                                                           const double* hre, const double* him,
-                                                           const cpx<double>* in, cpx<double>* out,
-                                                           long long n, double* norm_partials, Skip sk,
-                                                           const double* alpha = nullptr, double beta = 0.0) {
-  __shared__ cpx<double> hs[kLzMaxP];
-  __shared__ double red[256];
-  if (skipped(sk, 0)) return;
+// The work of block blk of nblk of k_gemv_n_out.
+__device__ __forceinline__ void gemv_n_block(const cpx<double>* Q, long long ld, int j, const double* hre,
+                                             const double* him, const cpx<double>* in, cpx<double>* out,
+                                             long long n, double* norm_partials, const double* alpha,
+                                             double beta, int blk, int nblk, cpx<double>* hs, double* red) {
   for (int k = threadIdx.x; k < j; k += blockDim.x) {
     cpx<double> h = mk(hre[k], him[k]);
     if (alpha != nullptr) {
@@ -783,8 +788,7 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
   }
   __syncthreads();
   double s = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = blk * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)nblk * blockDim.x) {
     cpx<double> acc = in[i];
     for (int k = 0; k < j; ++k) acc = acc - Q[(long long)k * ld + i] * hs[k];
     out[i] = acc;
@@ -792,8 +796,19 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
   }
   if (norm_partials != nullptr) {
     s = block_sum<256>(s, red);
-    if (threadIdx.x == 0) norm_partials[blockIdx.x] = s;
+    if (threadIdx.x == 0) norm_partials[blk] = s;
   }
+}
+
+static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q, long long ld, int j,
+                                                           const double* hre, const double* him,
+                                                           const cpx<double>* in, cpx<double>* out,
+                                                           long long n, double* norm_partials, Skip sk,
+                                                           const double* alpha = nullptr, double beta = 0.0) {
+  __shared__ cpx<double> hs[kLzMaxP];
+  __shared__ double red[256];
+  if (skipped(sk, 0)) return;
+  gemv_n_block(Q, ld, j, hre, him, in, out, n, norm_partials, alpha, beta, blockIdx.x, gridDim.x, hs, red);
 }
 
 // Norm bookkeeping of the re-orthogonalisation (one workgroup).  sc[1] = ||r||^2 of the
@@ -803,10 +818,8 @@ static __global__ __launch_bounds__(256) void k_gemv_n_out(const cpx<double>* Q,
 // (eta2 = 1/2 by default)
 // (with `pre`, the stage-0 sum of the same step first); stage 2: after the second sweep (taken
 // only when it ran).
-static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, const double* partials, int nb,
-                                                                int stage, const double* pre = nullptr,
-                                                                int npre = 0, double eta2 = 0.5) {
-  __shared__ double red[kRedThreads];
+__device__ __forceinline__ void lz_dgks_block(double* sc, const double* partials, int nb, int stage,
+                                              const double* pre, int npre, double eta2, double* red) {
   int* skip = reinterpret_cast<int*>(sc + 4);
   if (stage == 2 && *skip) return;
   if (pre) {  // stage 0 in the same launch: ||r||^2 before the sweep
@@ -825,6 +838,13 @@ static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, cons
       if (stage == 1) skip[0] = (eta2 >= 1.0 || s < eta2 * sc[2]) ? 0 : 1;  // eta >= 1: always
     }
   }
+}
+
+static __global__ __launch_bounds__(kRedThreads) void k_lz_dgks(double* sc, const double* partials, int nb,
+                                                                int stage, const double* pre = nullptr,
+                                                                int npre = 0, double eta2 = 0.5) {
+  __shared__ double red[kRedThreads];
+  lz_dgks_block(sc, partials, nb, stage, pre, npre, eta2, red);
 }
 
 }  // namespace fame6d

@@ -194,6 +194,114 @@ static void lanczos_orth_step(fame6d_mass* m, long long n, const Basis& B, int j
   if (second) lanczos_second_sweep(m, n, B, jj, s, Skip{reinterpret_cast<const int*>(B.sc + 4), 0});
 }
 
+// ------------------------------------------------------------------ batched steps
+// The orthogonalisation of one Lanczos step for several runs (the Bloch points of a batched
+// eigensolve) with one launch per kernel, blockIdx.y = entry of a per-step table: the same
+// kernels and arithmetic as lanczos_orth_step, each entry with its own basis, column count
+// and scalars.
+struct LzPt {
+  cpx<double>* Q;         // basis (columns of n)
+  cpx<double>* r;         // the orthogonalised vector
+  double* sc;             // device scalars of the run
+  double* part;           // partials of the run
+  const cpx<double>* w;   // the K_r^{-1} output of the step (k_scale_pts: unused)
+  cpx<double>* dst;       // k_scale_pts: where v = r / ||r|| goes
+  double beta;            // beta of the three-term update (0 before the second column)
+  int jj;                 // columns set; column jj - 1 is the current q_j
+  int pad;
+};
+constexpr int kLzPtMax = 64;
+
+// out(:, e) = Q_e(:, jj_e - 1), the K_r^{-1} inputs of the step packed for the batched solve
+__global__ void k_gather_pts(const LzPt* pts, cpx<double>* out, long long n) {
+  const LzPt P = pts[blockIdx.y];
+  const cpx<double>* src = P.Q + (long long)(P.jj - 1) * n;
+  cpx<double>* dst = out + blockIdx.y * n;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// partials of alpha = Re q_j^H w (k_dot_partials' element order and sums)
+__global__ __launch_bounds__(kRedThreads) void k_dot_pts(const LzPt* pts, long long n) {
+  __shared__ double red[kRedThreads];
+  const LzPt P = pts[blockIdx.y];
+  const cpx<double>* a = P.Q + (long long)(P.jj - 1) * n;
+  double sr = 0.0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kRedThreads) {
+    const cpx<double> u = a[i], v = P.w[i];
+    sr += u.re * v.re + u.im * v.im;
+  }
+  sr = block_sum<kRedThreads>(sr, red);
+  if (threadIdx.x == 0) P.part[blockIdx.x] = sr;
+}
+
+// sc[0] = alpha (grid = entries)
+__global__ __launch_bounds__(kRedThreads) void k_reduce_alpha_pts(const LzPt* pts, int nb) {
+  __shared__ double red[kRedThreads];
+  const LzPt P = pts[blockIdx.x];
+  double s = strided_sum(0.0, P.part, threadIdx.x, nb, kRedThreads);
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) P.sc[0] = s;
+}
+
+// h = Q^H r of the three-term update, grid (nrb * nchmax, entries); chunks past an entry's
+// columns return
+template <int KB>
+__global__ __launch_bounds__(kRedThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_gemv_c_pts(
+    const LzPt* pts, long long n, int nrb, int nchmax) {
+  __shared__ double red[kRedThreads];
+  const LzPt P = pts[blockIdx.y];
+  const int logical = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int rb = logical / nchmax, ch = logical - rb * nchmax;
+  if (ch * KB >= P.jj) return;
+  const ThreeTerm tt{P.sc, P.beta, P.part + (size_t)2 * nrb * kLzMaxP};
+  gemv_c_block<KB, true>(P.Q, n, P.jj, P.w, n, nrb, rb, ch, P.part, P.part + (size_t)nrb * kLzMaxP, tt, red);
+}
+
+// sc[kScH + k], sc[kScHim + k] for k < jj, grid (2 * jjmax, entries)
+__global__ __launch_bounds__(kRedThreads) void k_reduce_h_pts(const LzPt* pts, int nb) {
+  __shared__ double red[kRedThreads];
+  const LzPt P = pts[blockIdx.y];
+  if ((int)blockIdx.x >= 2 * P.jj) return;
+  const bool im = (int)blockIdx.x >= P.jj;
+  const int k = im ? blockIdx.x - P.jj : blockIdx.x;
+  const double* partials = (im ? P.part + (size_t)nb * kLzMaxP : P.part) + (size_t)k * nb;
+  double s = strided_sum(0.0, partials, threadIdx.x, nb, kRedThreads);
+  s = block_sum<kRedThreads>(s, red);
+  if (threadIdx.x == 0) P.sc[(im ? kScHim : kScH) + k] = s;
+}
+
+// r = w - Q (h + alpha e_j + beta e_{j-1}) with ||r||^2 partials, grid (gn, entries)
+__global__ __launch_bounds__(256) void k_gemv_n_pts(const LzPt* pts, long long n) {
+  __shared__ cpx<double> hs[kLzMaxP];
+  __shared__ double red[256];
+  const LzPt P = pts[blockIdx.y];
+  gemv_n_block(P.Q, n, P.jj, P.sc + kScH, P.sc + kScHim, P.w, P.r, n, P.part, P.sc, P.beta, blockIdx.x,
+               gridDim.x, hs, red);
+}
+
+// the DGKS test of each entry (k_lz_dgks stage 1), then its scalars sc[0 .. kScH + jj) into
+// out + e * per for one read by the host (grid = entries)
+__global__ __launch_bounds__(kRedThreads) void k_dgks_pts(const LzPt* pts, int gn, int nb, double eta2,
+                                                          double* out, int per) {
+  __shared__ double red[kRedThreads];
+  const LzPt P = pts[blockIdx.x];
+  lz_dgks_block(P.sc, P.part, gn, 1, P.part + (size_t)2 * nb * kLzMaxP, nb, eta2, red);
+  __syncthreads();  // thread 0's scalars are visible to the block
+  for (int t = threadIdx.x; t < kScH + P.jj; t += kRedThreads) out[(size_t)blockIdx.x * per + t] = P.sc[t];
+}
+
+// dst = r / sqrt(sc[1]), grid (blocks, entries)
+__global__ void k_scale_pts(const LzPt* pts, long long n) {
+  const LzPt P = pts[blockIdx.y];
+  const double f = 1.0 / sqrt(P.sc[1]);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    P.dst[i] = f * P.r[i];
+}
+
 // Host-side state of one Lanczos run.
 struct LzState {
   int slot = 0;
@@ -478,6 +586,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
   // device memory: per group Q (n2 x p) + v + r; scalars; partials; packed in/out
   const int nb = red_blocks(n2);
   DevBuf basis, scal, part, io, ubuf;
+  DevBuf ptab, sstage;  // batched steps: per-step tables, the scalars staged for one read
   const int cols = p + 2 + (p > 64 ? p : 0);  // Q, v, r (+ the second basis of wide restarts)
   const size_t part_per = (size_t)std::max(nb * 2 * kLzMaxP, 2048) + nb;
   basis.ensure(sizeof(cpx<double>) * (size_t)n2 * cols * G);
@@ -528,18 +637,28 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     if (act.empty()) break;
     const int na = (int)act.size();
     std::vector<int> slots(na);
-    // K_r^{-1} reads its input only, so a single active run passes its basis column itself
+    // K_r^{-1} reads its input only, so a single active run passes its basis column itself;
+    // several are gathered into qin by one launch over the step's table
     const cpx<double>* kin = qin;
+    const bool batch = na >= 2;
+    std::vector<LzPt> tab(na);
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       cpx<double>* Qj = Lg.B.Q + (size_t)n2 * (Lg.jj - 1);
       if (Lg.vloc != Qj)  // after the start and after restarts; a step's own v is written in place
         F6_HIP(hipMemcpyAsync(Qj, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
-      if (na == 1)
-        kin = Qj;
-      else
-        F6_HIP(hipMemcpyAsync(qin + (size_t)n2 * i, Qj, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
+      if (na == 1) kin = Qj;
+      tab[i] = LzPt{Lg.B.Q, Lg.B.r, Lg.B.sc, Lg.B.part, qout + (size_t)n2 * i, nullptr,
+                    Lg.jj >= 2 ? Lg.beta : 0.0, Lg.jj, 0};
       slots[i] = Lg.slot;
+    }
+    LzPt* tab_d = nullptr;  // three tables of kLzPtMax entries: the step, its batched sweep, the scaling
+    if (batch) {
+      ptab.ensure(sizeof(LzPt) * 3 * kLzPtMax);
+      tab_d = ptab.as<LzPt>();
+      h2d_async(m, tab_d, tab.data(), sizeof(LzPt) * na);
+      k_gather_pts<<<dim3(gr, na), 256, 0, s>>>(tab_d, qin, n2);
+      F6_CHECK_LAUNCH();
     }
     // a single FP64 run defers the solve's confirmation to the step's one read of scalars
     const bool defer = na == 1 && !g_nested.on && precision == FAME6D_FP64;
@@ -555,7 +674,46 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     // where v = r / beta goes: the column it takes next, or B.v when the basis is full and a
     // restart follows
     auto next_dst = [&](LzState& Lg) { return Lg.jj < p ? Lg.B.Q + (size_t)n2 * Lg.jj : Lg.B.v; };
+    const int per = kScH + p;  // doubles of one run's staged scalars
+    std::vector<int> sub_of(na, -1);  // entry of each active run in the batched sweep, or -1
+    int nsub = 0;
+    auto orth_batched = [&]() {
+      // runs just restarted take the full CGS2 sweep, one by one; the others one launch each
+      std::vector<LzPt> sub;
+      for (int i = 0; i < na; ++i) {
+        LzState& Lg = L[act[i]];
+        if (Lg.justRestarted) {
+          cgs2_step(m, n2, Lg.B, Lg.jj, qout + (size_t)n2 * i, s);
+          Lg.alpha_at = kScH + Lg.jj - 1;
+          sub_of[i] = -1;
+        } else {
+          sub_of[i] = (int)sub.size();
+          sub.push_back(tab[i]);
+          Lg.alpha_at = 0;
+        }
+      }
+      nsub = (int)sub.size();
+      if (nsub == 0) return;
+      LzPt* t2 = tab_d + kLzPtMax;
+      h2d_async(m, t2, sub.data(), sizeof(LzPt) * nsub);
+      int jjmax = 0;
+      for (const LzPt& e : sub) jjmax = std::max(jjmax, e.jj);
+      const int nchmax = (jjmax + kGemvKB - 1) / kGemvKB;
+      const int gn = (int)std::min<long long>(2048, (n2 + 255) / 256);
+      sstage.ensure(sizeof(double) * (size_t)kLzPtMax * per);
+      k_dot_pts<<<dim3(nb, nsub), kRedThreads, 0, s>>>(t2, n2);
+      k_reduce_alpha_pts<<<nsub, kRedThreads, 0, s>>>(t2, nb);
+      k_gemv_c_pts<kGemvKB><<<dim3(nb * nchmax, nsub), kRedThreads, 0, s>>>(t2, n2, nb, nchmax);
+      k_reduce_h_pts<<<dim3(2 * jjmax, nsub), kRedThreads, 0, s>>>(t2, nb);
+      k_gemv_n_pts<<<dim3(gn, nsub), 256, 0, s>>>(t2, n2);
+      k_dgks_pts<<<nsub, kRedThreads, 0, s>>>(t2, gn, nb, S->dgks_eta2, sstage.as<double>(), per);
+      F6_CHECK_LAUNCH();
+    };
     auto orth = [&](const CgState* gate) {
+      if (batch) {
+        orth_batched();
+        return;
+      }
       for (int i = 0; i < na; ++i) {
         LzState& Lg = L[act[i]];
         const Basis& B = Lg.B;
@@ -581,16 +739,21 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
     // alpha = Re h1[jj-1] and beta^2 of every active run: one staged read, one sync
     std::vector<double> ab(2 * na);
     std::vector<int> second(na, 0);  // the DGKS second sweep ran (steps away from a restart)
-    auto read_scalars = [&]() {
-      const size_t per = sizeof(double) * (kScH + p);
-      char* stage = static_cast<char*>(m->pin.take(per * na, s));
+    // with `staged`, the batched entries' scalars come from the stage of k_dgks_pts in one copy
+    auto read_scalars = [&](bool staged) {
+      const size_t bper = sizeof(double) * per;
+      char* stage = static_cast<char*>(m->pin.take(bper * na, s));
+      char* sst = staged && nsub > 0 ? static_cast<char*>(m->pin.take(bper * nsub, s)) : nullptr;
+      if (sst) F6_HIP(hipMemcpyAsync(sst, sstage.ptr, bper * nsub, hipMemcpyDeviceToHost, s));
       for (int i = 0; i < na; ++i)
-        F6_HIP(hipMemcpyAsync(stage + per * i, L[act[i]].B.sc, sizeof(double) * (kScH + L[act[i]].jj),
-                              hipMemcpyDeviceToHost, s));
+        if (!sst || sub_of[i] < 0)
+          F6_HIP(hipMemcpyAsync(stage + bper * i, L[act[i]].B.sc, sizeof(double) * (kScH + L[act[i]].jj),
+                                hipMemcpyDeviceToHost, s));
       F6_HIP(hipStreamSynchronize(s));
       m->pin.used = 0;
       for (int i = 0; i < na; ++i) {
-        const double* v = reinterpret_cast<const double*>(stage + per * i);
+        const double* v = reinterpret_cast<const double*>(sst && sub_of[i] >= 0 ? sst + bper * sub_of[i]
+                                                                                 : stage + bper * i);
         ab[2 * i] = v[L[act[i]].alpha_at];
         ab[2 * i + 1] = v[1];
         second[i] = 0;
@@ -602,11 +765,11 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       }
     };
     orth(defer ? kd.state : nullptr);
-    read_scalars();
+    read_scalars(batch);
     if (defer && !krinv_complete(S, &kd)) {
       // the solve needed more iterations: its tail has been formed again, and so is the step
       orth(nullptr);
-      read_scalars();
+      read_scalars(false);
     }
     if (!defer) {
       // second Gram-Schmidt sweeps the DGKS tests asked for, then beta^2 read again
@@ -618,7 +781,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         }
       if (any) {
         const std::vector<int> asked = second;
-        read_scalars();
+        read_scalars(false);
         second = asked;
       }
     }
@@ -626,6 +789,10 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       L[act[i]].second_sweeps += second[i];
       L[act[i]].justRestarted = false;
     }
+    // v = r / beta of every run (one launch when batched, before any restart reads it), then
+    // the restarts of the runs whose cycle ended
+    std::vector<char> ends(na, 0);
+    std::vector<LzPt> scl;
     for (int i = 0; i < na; ++i) {
       LzState& Lg = L[act[i]];
       const double alpha = ab[2 * i], beta2 = ab[2 * i + 1];
@@ -639,7 +806,9 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       } else {
         Lg.beta = std::sqrt(beta2);
         cpx<double>* dst = next_dst(Lg);
-        if (!defer) {
+        if (batch) {
+          scl.push_back(LzPt{nullptr, Lg.B.r, Lg.B.sc, nullptr, nullptr, dst, 0.0, 0, 0});
+        } else if (!defer) {
           k_scale_rsqrt_dev<<<gr, 256, 0, s>>>(n2, dst, Lg.B.r, Lg.B.sc + 1);
           F6_CHECK_LAUNCH();
         }
@@ -650,7 +819,17 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
         if (Lg.jj >= p) end_cycle = true;
         else Lg.jj += 1;
       }
-      if (end_cycle) {
+      ends[i] = end_cycle;
+    }
+    if (!scl.empty()) {
+      LzPt* t3 = tab_d + 2 * kLzPtMax;
+      h2d_async(m, t3, scl.data(), sizeof(LzPt) * scl.size());
+      k_scale_pts<<<dim3(gr, (unsigned)scl.size()), 256, 0, s>>>(t3, n2);
+      F6_CHECK_LAUNCH();
+    }
+    for (int i = 0; i < na; ++i) {
+      LzState& Lg = L[act[i]];
+      if (ends[i]) {
         // the restart rewrites the basis columns in place: the newest v must be in B.v
         if (Lg.vloc != Lg.B.v) {
           F6_HIP(hipMemcpyAsync(Lg.B.v, Lg.vloc, sizeof(cpx<double>) * n2, hipMemcpyDeviceToDevice, s));
