@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--probe-taylor", action="store_true",
                     help="short run for hardware-counter collection of the Taylor moments contraction: "
                          "N=4 (NF = 8^6), p = 10 (286 monomials), 216 centers, 4 launches")
+    ap.add_argument("--probe-batch", action="store_true",
+                    help="short run for the kernel trace of the batched Bloch path: N=4, 12 points, "
+                         "FP64, two batched solves and the same 12 points one after the other")
     ap.add_argument("--tiles", default=None,
                     help="FP64 tile variants 'ac,b' (fame6d_mass_set_tiles codes) instead of the autotuner; "
                          "counter runs use the choice the timed run printed (its config.tile_variants)")
@@ -172,6 +175,39 @@ def gpu_solve_n3(torch, pkg):
     return dict(seconds=t, lam=out["lambda_"])
 
 
+def bloch_path(pkg, N=4, npts=12):
+    """The batched-path workload: N, points t*(0.5,0.4,0.3), t = k/npts, k = 1..npts (q = 0
+    excluded because the null-space-free reduction is undefined there), seeded start vectors."""
+    eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
+    mass = pkg.MassBackend(N, eps)
+    mass.autotune(reps=3)
+    qs = np.array([[0.5 * t, 0.4 * t, 0.3 * t] for t in np.arange(1, npts + 1) / float(npts)])
+    v0 = np.stack([pkg.start_vector(2 * mass.NF, 100 + i) for i in range(npts)])
+    return mass, qs, v0
+
+
+def probe_batch(torch, pkg):
+    mass, qs, v0 = bloch_path(pkg)
+    rows = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = pkg.run_method_C_batch(mass, qs, v0=v0, precision="double", **LANCZOS)
+        torch.cuda.synchronize()
+        rows.append(("batched", time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(qs)):
+        s_i = pkg.QSystem(mass, qs[i])
+        pkg.run_method_C(s_i, v0=v0[i], precision="double", **LANCZOS)
+        s_i.close()
+    torch.cuda.synchronize()
+    rows.append(("sequential", time.perf_counter() - t0))
+    print(json.dumps({"probe": "batch", "N": 4, "points": len(qs), "seconds": rows,
+                      "outer_steps": [st["outer_steps"] for st in out["stats"]]}))
+    mass.close()
+
+
 def extras(torch, pkg):
     """Batched Bloch path (N=4, 12 points on t*(0.5,0.4,0.3), t = k/12, k = 1..12; q = 0
     excluded because the null-space-free reduction is undefined there) and the Taylor
@@ -181,12 +217,7 @@ def extras(torch, pkg):
     from importlib import import_module
     lib = import_module("fame6d_amd._lib")
     N = 4
-    eps, _ = pkg.cosine_material(N, AMP, EPS_SEED)
-    mass = pkg.MassBackend(N, eps)
-    mass.autotune(reps=3)
-    qs = np.array([[0.5 * t, 0.4 * t, 0.3 * t] for t in np.arange(1, 13) / 12.0])
-    nr = 2 * mass.NF
-    v0 = np.stack([pkg.start_vector(nr, 100 + i) for i in range(12)])
+    mass, qs, v0 = bloch_path(pkg, N)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     outb = pkg.run_method_C_batch(mass, qs, v0=v0, precision="double", **LANCZOS)
@@ -622,6 +653,9 @@ def main():
         return
     if a.probe_taylor:
         probe_taylor(torch, pkg)
+        return
+    if a.probe_batch:
+        probe_batch(torch, pkg)
         return
     NF = (2 * N) ** 6
     eps, phases = pkg.cosine_material(N, AMP, EPS_SEED)

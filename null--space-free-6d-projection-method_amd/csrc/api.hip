@@ -100,6 +100,8 @@ static float time_ms(fame6d_mass* m, hipEvent_t e0, hipEvent_t e1, int reps, F&&
   return ms;
 }
 
+constexpr int kTuneGroups = 8;  // groups of the several-group tile choice's timing
+
 template <class T>
 static void autotune_t(fame6d_mass* m, int reps) {
   const long long NF = m->NF;
@@ -181,7 +183,7 @@ static void autotune_t(fame6d_mass* m, int reps) {
   DevBuf cgs, pqp, cpart;
   cgs.ensure(sizeof(CgState));
   F6_HIP(hipMemcpyAsync(cgs.ptr, &hst, sizeof(CgState), hipMemcpyHostToDevice, m->stream));
-  const int npq = 3 * pass_b_blocks(m, std::is_same<T, float>::value);
+  const int npq = 3 * pass_b_blocks(m, std::is_same<T, float>::value, 1);
   const std::vector<double> ones(npq, 1.0);
   pqp.ensure(sizeof(double) * npq);
   F6_HIP(hipMemcpyAsync(pqp.ptr, ones.data(), sizeof(double) * npq, hipMemcpyHostToDevice, m->stream));
@@ -239,6 +241,70 @@ static void autotune_t(fame6d_mass* m, int reps) {
     mass_apply<T>(m, x, NF, x + 2 * NF, NF, 2, m->stream);
     mass_apply<T>(m, x, NF, x + 2 * NF, NF, 1, m->stream);
   });
+  // 3) launches over several groups (batched Bloch points): one tile per workgroup, tuned on
+  // kTuneGroups groups, every pass-A/C blocks-per-workgroup pair with either pass-B tile for the
+  // Mhat passes, then the plain passes on 2 kTuneGroups right-hand sides.  The groups share
+  // q-system slot 0 (zero vectors and weights, as above).
+  {
+    constexpr int GT = kTuneGroups;
+    TileChoice& tg = std::is_same<T, float>::value ? m->tile32g : m->tile64g;
+    DevBuf gvec, gmap;
+    gvec.ensure(sizeof(cpx<T>) * NF * 4 * GT);
+    gmap.ensure(sizeof(int) * GT);
+    cpx<T>* gx = gvec.as<cpx<T>>();
+    k_fill_zero<T><<<grid_for(NF * 4 * GT), 256, 0, m->stream>>>(NF * 4 * GT, gx);
+    F6_HIP(hipMemsetAsync(gmap.ptr, 0, sizeof(int) * GT, m->stream));
+    F6_CHECK_LAUNCH();
+    w.ensure(sizeof(cpx<T>) * NF * 3 * GT);
+    PassAArgs<T> ga = pa;
+    ga.x = gx;
+    ga.t = w.as<cpx<T>>();
+    ga.qmap = gmap.as<int>();
+    PassCArgs<T> gc = pc;
+    gc.t = ga.t;
+    gc.y = gx + NF * GT;
+    gc.qmap = ga.qmap;
+    auto pick_g = [&](std::vector<TileChoice> cnd, auto&& run) {
+      std::vector<float> best(cnd.size(), 1e30f);
+      for (int round = 0; round < kRounds; ++round)
+        for (size_t c = 0; c < cnd.size(); ++c) {
+          tg = cnd[c];
+          best[c] = std::min(best[c], time_ms(m, e0, e1, reps, run));
+        }
+      size_t arg = 0;
+      for (size_t c = 1; c < cnd.size(); ++c)
+        if (best[c] < best[arg]) arg = c;
+      tg = cnd[arg];
+    };
+    std::vector<TileChoice> cg;
+    for (int vb = 0; vb < 2; ++vb)
+      for (int va3 = 0; va3 < 2; ++va3)
+        for (int vc3 = 0; vc3 < 2; ++vc3) {
+          TileChoice c = tc;
+          c.b = vb;
+          c.a3 = va3;
+          c.c3 = vc3;
+          c.a3p = c.c3p = c.a3s = 0;
+          cg.push_back(c);
+        }
+    pick_g(cg, [&]() {
+      launch_pass_a<T>(m, PRE_MHAT, GT, ga, m->stream);
+      launch_pass_b<T>(m, 3, 0, GT, ga.t, NF, no_skip(), m->stream);
+      launch_pass_c<T>(m, POST_MHAT, GT, gc, m->stream);
+    });
+    cg.clear();
+    for (int vio = 0; vio < 2; ++vio)
+      for (int va = 0; va < 2; ++va)
+        for (int vb1 = 0; vb1 < 2; ++vb1) {
+          TileChoice c = tg;
+          c.io = vio;
+          c.a = va;
+          c.b1 = vb1;
+          cg.push_back(c);
+        }
+    pick_g(cg, [&]() { mass_apply<T>(m, gx, NF, gx + 2 * NF * GT, NF, 2 * GT, m->stream); });
+    F6_HIP(hipStreamSynchronize(m->stream));
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   vec.release();

@@ -175,6 +175,9 @@ struct fame6d_mass {
   void* noalloc_scratch = nullptr;  // this device's copy of the allocation-free apply's array
   int noalloc_status = 0;           // hipGetSymbolAddress status of that lookup
   fame6d::TileChoice tile64, tile32;
+  // launches over several groups (batched Bloch points, several right-hand sides): one tile
+  // per workgroup, tuned on several groups
+  fame6d::TileChoice tile64g, tile32g;
   fame6d::HostPin pin;             // page-locked staging of small transfers
   mutable fame6d::KernelTimer timer;
 };
@@ -191,9 +194,9 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s);
 // Number of workgroups pass C uses per group (size of its partials per group).
-int pass_c_blocks(const fame6d_mass* m, bool fp32);
+int pass_c_blocks(const fame6d_mass* m, bool fp32, int ngroups);  // for a launch over ngroups groups
 // Number of pass-B tiles per vector (its partials per vector; NR vectors per group).
-int pass_b_blocks(const fame6d_mass* m, bool fp32);
+int pass_b_blocks(const fame6d_mass* m, bool fp32, int ngroups);  // for a launch over ngroups groups
 int num_tile_variants(int n, bool fp32, bool pass_b);
 void tile_variant_desc(int n, bool fp32, bool pass_b, int v, int* param, int* threads);
 

@@ -52,11 +52,19 @@ void with_n(int n, F&& f) {
   }
 }
 
+// The tile choice of a launch over ngroups groups: launches over several groups take the
+// choice tuned on several groups (one tile per workgroup: the grid is already several times
+// the device, and the persistent forms tuned on one group only add their loop there).
 template <class T>
-static const TileChoice& tile_of(const fame6d_mass* m) {
-  if constexpr (std::is_same<T, double>::value) return m->tile64;
-  else return m->tile32;
+static const TileChoice& tile_of(const fame6d_mass* m, int ngroups) {
+  if constexpr (std::is_same<T, double>::value) return ngroups > 1 ? m->tile64g : m->tile64;
+  else return ngroups > 1 ? m->tile32g : m->tile32;
 }
+
+// The group count that selects the choice: the Mhat passes (three vectors per group) take the
+// several-group choice from two groups on; the plain passes, whose one-group choice is tuned
+// on one and two right-hand sides (the pairs around each K_r^{-1}), from three.
+static int tile_groups(bool mhat, int ngroups) { return mhat || ngroups > 2 ? ngroups : 1; }
 
 // Workgroups per group of the persistent Mhat passes A and C: per_cu per compute unit, at
 // most one per tile.
@@ -67,10 +75,11 @@ static int pass_c3p_blocks(const fame6d_mass* m, int tiles, int per_cu) {
 template <class T>
 void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
-  const int v = tile_of<T>(m).a;
+  const TileChoice& tc = tile_of<T>(m, tile_groups(op == PRE_MHAT || op == PRE_MHAT_CG, ngroups));
+  const int v = tc.a;
   if (op == PRE_MHAT || op == PRE_MHAT_CG) {
-    const int v3 = tile_of<T>(m).a3;
-    const int a3p = tile_of<T>(m).a3p, a3s = tile_of<T>(m).a3s;
+    const int v3 = tc.a3;
+    const int a3p = tc.a3p, a3s = tc.a3s;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
       constexpr int S = n * n * n;
@@ -108,7 +117,7 @@ void launch_pass_a(const fame6d_mass* m, int op, int ngroups, const PassAArgs<T>
         }
         throw Error(FAME6D_EARG, "bad pass-A op");
       }
-      if (tile_of<T>(m).io == 1) {
+      if (tc.io == 1) {
         switch (op) {
           case PRE_COPY: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_COPY>, grid, block, s, a); break;
           case PRE_MHAT: launch_k(m, k_pass_a_lin<T, n, BPW, PRE_MHAT>, grid, block, s, a); break;
@@ -137,7 +146,8 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
   const void* e = std::is_same<T, double>::value ? (inverse_eps ? m->ieps64.ptr : m->eps64.ptr)
                                                  : (inverse_eps ? m->ieps32.ptr : m->eps32.ptr);
   PassBArgs<T> a{t, ldt, (const T*)e, skip, partials};
-  const int v = nr == 1 ? tile_of<T>(m).b1 : tile_of<T>(m).b;  // plain applies have their own tile
+  const TileChoice& tc = tile_of<T>(m, tile_groups(nr == 3, ngroups));
+  const int v = nr == 1 ? tc.b1 : tc.b;  // plain applies have their own tile
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
@@ -171,12 +181,13 @@ void launch_pass_b(const fame6d_mass* m, int nr, int mode, int ngroups, cpx<T>* 
 template <class T>
 void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>& a, hipStream_t s) {
   if (ngroups <= 0) return;
+  const TileChoice& tc = tile_of<T>(m, tile_groups(op == POST_MHAT || op == POST_MHAT_CG, ngroups));
   if (op == POST_MHAT || op == POST_MHAT_CG) {
-    const int v3 = tile_of<T>(m).c3;
+    const int v3 = tc.c3;
     with_n(m->n, [&](auto nc) {
       constexpr int n = decltype(nc)::value;
       constexpr int S = n * n * n;
-      const int c3p = tile_of<T>(m).c3p;
+      const int c3p = tc.c3p;
       auto go = [&](auto bc) {
         constexpr int BPW = decltype(bc)::value;
         if (c3p > 0) {
@@ -195,14 +206,14 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
     F6_CHECK_LAUNCH();
     return;
   }
-  const int v = tile_of<T>(m).a;
+  const int v = tc.a;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
     auto go = [&](auto bc) {
       constexpr int BPW = decltype(bc)::value;
       dim3 grid(S / BPW, ngroups), block(BPW * n * n);
-      const bool lin = tile_of<T>(m).io == 1;
+      const bool lin = tc.io == 1;
       if (op == POST_COPY && lin) launch_k(m, k_pass_c_lin<T, n, BPW, POST_COPY, 1>, grid, block, s, a);
       else if (op == POST_MHAT && lin) launch_k(m, k_pass_c_lin<T, n, BPW, POST_MHAT, 3>, grid, block, s, a);
       else if (op == POST_COPY) launch_k(m, k_pass_c<T, n, BPW, POST_COPY, 1>, grid, block, s, a);
@@ -215,10 +226,10 @@ void launch_pass_c(const fame6d_mass* m, int op, int ngroups, const PassCArgs<T>
   F6_CHECK_LAUNCH();
 }
 
-int pass_c_blocks(const fame6d_mass* m, bool fp32) {
+int pass_c_blocks(const fame6d_mass* m, bool fp32, int ngroups) {
   // workgroups per group of the Mhat pass C (the only pass C that writes partials)
   int out = 0;
-  const TileChoice& tc = fp32 ? m->tile32 : m->tile64;
+  const TileChoice& tc = fp32 ? tile_of<float>(m, ngroups) : tile_of<double>(m, ngroups);
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;
@@ -228,10 +239,10 @@ int pass_c_blocks(const fame6d_mass* m, bool fp32) {
   return out;
 }
 
-int pass_b_blocks(const fame6d_mass* m, bool fp32) {
+int pass_b_blocks(const fame6d_mass* m, bool fp32, int ngroups) {
   // tiles of pass B per vector (size of its partials per vector)
   int out = 0;
-  const int v = fp32 ? m->tile32.b : m->tile64.b;
+  const int v = fp32 ? tile_of<float>(m, ngroups).b : tile_of<double>(m, ngroups).b;
   with_n(m->n, [&](auto nc) {
     constexpr int n = decltype(nc)::value;
     constexpr int S = n * n * n;

@@ -384,8 +384,8 @@ struct PcgRun {
         m(S_->m), s(S_->m->stream), NF(S_->m->NF), r(r_), z(z_) {
     owner = m->timer.next_owner++ & 0x3fff;
     constexpr bool f32 = std::is_same<T, float>::value;
-    nbE = 3 * pass_b_blocks(m, f32);  // pass-B energy partials per group
-    nbC = pass_c_blocks(m, f32);       // pass-C ||r||^2 partials per group
+    nbE = 3 * pass_b_blocks(m, f32, nact);  // pass-B energy partials per group
+    nbC = pass_c_blocks(m, f32, nact);       // pass-C ||r||^2 partials per group
     nbD = red_blocks(NF);
     S->st_set(set).ensure(sizeof(CgState) * nact * 2);
     S->part_set(set).ensure(sizeof(double) * (size_t)nbE * nact);
@@ -759,7 +759,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     const int nbD = red_blocks(NF);
     // sized for every user up front (the FP64 residual's pass-C partials, the inner FP32
     // solves' own), so no later ensure moves the buffer under this pointer
-    const int nbP = std::max({nbD, pass_c_blocks(m, false), pass_c_blocks(m, true)});
+    const int nbP = std::max({nbD, pass_c_blocks(m, false, nact), pass_c_blocks(m, true, nact)});
     S->part2.ensure(sizeof(double) * ((size_t)nbP * nact + 8));
     double* part2 = S->part2.as<double>();
     k_cg_start<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, w.u, w.res, NF, part2);  // u = 0, res = bhat
@@ -801,7 +801,7 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
       // FP64 residual of every group still refining
       // res = bhat - Mhat u and its ||res||^2 partials in the last pass of the apply
       mhat_apply<double>(S, nact, qmap, w.u, w.res, skr, s, false, nullptr, w.bhat, part2, w.s);
-      k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, pass_c_blocks(m, false), tol);
+      k_ref_check<<<nact, kRedThreads, 0, s>>>(rs, part2, pass_c_blocks(m, false, nact), tol);
       F6_CHECK_LAUNCH();
       if (cycles < blind) continue;
       d2h_sync(m, hr.data(), rs, sizeof(RefState) * nact);
