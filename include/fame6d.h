@@ -84,8 +84,11 @@ int fame6d_mass_set_eps_device(fame6d_mass* m, const double* eps_dev);
 int fame6d_mass_info(const fame6d_mass* m, fame6d_mass_info_t* info);
 
 /* Project-written tile autotuner: times every compiled tile variant of the three
- * passes at this N and precision and keeps the fastest (no reference counterpart;
- * the reference has one implicit cuFFT plan). */
+ * passes at this N and precision and keeps the fastest, once for launches over one group
+ * (one Bloch point; one or two right-hand sides of the plain apply) and once, among the
+ * one-tile-per-workgroup forms, for launches over several (batched Bloch points, more
+ * right-hand sides); fame6d_mass_set_tiles sets the one-group choice (no reference
+ * counterpart; the reference has one implicit cuFFT plan). */
 int fame6d_mass_autotune(fame6d_mass* m, int precision, int reps);
 /* Set tile variants explicitly: variant_ac in 0..511 (bit 0: blocks per workgroup of the
  * plain passes A/C, bit 1 set: their dim-1 register-pencil I/O instead of the coalesced
