@@ -31,19 +31,9 @@ __host__ inline int red_blocks(long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;         \
        i += (long long)gridDim.x * blockDim.x)
 
-// bhat = -invC .* (W1 .* mr1 + W2 .* mr2)   (make_explicit_krinv_compact.m:34-35)
-template <class T>
-__global__ void k_bhat(const cpx<T>* mr, long long ldm, long long n, const T* W1, const T* W2,
-                       const T* invC, cpx<T>* bhat, long long ldb, Skip skip, const int* qmap) {
-  F6_MODEWISE_LOOP {
-    const long long j = qs + i;
-    const cpx<T> s = W1[j] * mr[g * ldm + i] + W2[j] * mr[g * ldm + n + i];
-    bhat[g * ldb + i] = (-invC[j]) * s;
-  }
-}
-
-// k_bhat and the start of the conjugate-gradient solve on bhat in one pass: bhat as k_bhat,
-// x = 0, r = bhat and the partials of ||bhat||^2 in k_cg_start's element order and sums.
+// bhat = -invC .* (W1 .* mr1 + W2 .* mr2) (make_explicit_krinv_compact.m:34-35) and the start
+// of the conjugate-gradient solve on it in one pass: x = 0, r = bhat and the partials of
+// ||bhat||^2 in k_cg_start's element order and sums.
 // grid (nb, groups) with kRedThreads threads, as k_cg_start.
 template <class T>
 __global__ __launch_bounds__(kRedThreads) void k_bhat_cg_start(const cpx<T>* mr, long long ldm, long long n,

@@ -745,8 +745,6 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     // FP64 iterative refinement of Mhat u = bhat with FP32 inner PCG solves.  The refinement
     // state (||b||, ||r||, converged) lives on the device; the host reads it once the
     // cycles that cannot have converged yet are done (after every cycle from then on).
-    k_bhat<double><<<dim3(gridm, nact), 256, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1), S->W64(2),
-                                                      w.bhat, NF, no_skip(), qmap);
     S->kr32.ensure(sizeof(cpx<float>) * gn * 6);
     cpx<float>* b32 = S->kr32.as<cpx<float>>();
     cpx<float>* x32 = b32 + gn;
@@ -762,7 +760,9 @@ void krinv_groups(fame6d_qsys* S, int precision, int nact, const int* qmap_host,
     const int nbP = std::max({nbD, pass_c_blocks(m, false, nact), pass_c_blocks(m, true, nact)});
     S->part2.ensure(sizeof(double) * ((size_t)nbP * nact + 8));
     double* part2 = S->part2.as<double>();
-    k_cg_start<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.bhat, w.u, w.res, NF, part2);  // u = 0, res = bhat
+    // bhat, u = 0, res = bhat and the ||bhat||^2 partials in one pass over mr
+    k_bhat_cg_start<double><<<dim3(nbD, nact), kRedThreads, 0, s>>>(w.mr, 2 * NF, NF, S->W64(0), S->W64(1),
+                                                                    S->W64(2), qmap, w.bhat, w.u, w.res, part2);
     k_ref_init<<<nact, kRedThreads, 0, s>>>(rs, part2, nbD);
     F6_CHECK_LAUNCH();
     res.assign(nact, PcgOut{});
