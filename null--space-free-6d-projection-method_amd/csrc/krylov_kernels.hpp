@@ -696,28 +696,43 @@ static __global__ void k_gemv_n_sub(const cpx<double>* Q, long long ld, int j, c
   }
 }
 
-// In-place restart Q(:, 0:kk) = Q(:, 0:m) U with U (m x kk, column-major, real).
-// Each workgroup owns kRows rows and stages them in LDS.
-constexpr int kRestartRows = 32;
-static __global__ __launch_bounds__(256) void k_restart(cpx<double>* Q, long long ld, long long n, int m,
-                                                 int kk, const double* U) {
-  __shared__ cpx<double> tile[kRestartRows * 64];
-  __shared__ double Us[64 * 64];
-  for (int t = threadIdx.x; t < m * kk; t += blockDim.x) Us[t] = U[t];
-  const long long r0 = (long long)blockIdx.x * kRestartRows;
-  for (int t = threadIdx.x; t < kRestartRows * m; t += blockDim.x) {
-    const int rr = t % kRestartRows, c = t / kRestartRows;
-    const long long i = r0 + rr;
-    tile[c * kRestartRows + rr] = (i < n) ? Q[(long long)c * ld + i] : mk(0.0, 0.0);
+// In-place restart Q(:, 0:kk) = Q(:, 0:m) U, m <= 64, kk <= CB, with Q as the real (2n x m)
+// matrix of its interleaved parts: a thread owns one real row, reads all m of its entries
+// (eight loads in flight at a time) before it writes any of its kk outputs, so no other
+// thread's input is overwritten; U (real, m x kk, column-major) is broadcast from LDS.  The
+// sums run over l = 0..m-1 in order.
+template <int CB>
+__global__ __launch_bounds__(256) void k_restart_rows(double* __restrict__ Q, long long ldq, long long nr, int m,
+                                                      int kk, const double* __restrict__ U) {
+  __shared__ double Us[64 * CB];  // Us[l * CB + c] = U(l, c), zero past kk
+  for (int t = threadIdx.x; t < m * CB; t += 256) {
+    const int l = t / CB, c = t - (t / CB) * CB;
+    Us[t] = c < kk ? U[(long long)c * m + l] : 0.0;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kRestartRows * kk; t += blockDim.x) {
-    const int rr = t % kRestartRows, c = t / kRestartRows;
-    const long long i = r0 + rr;
-    cpx<double> acc = mk(0.0, 0.0);
-    for (int l = 0; l < m; ++l) acc = acc + Us[c * m + l] * tile[l * kRestartRows + rr];
-    if (i < n) Q[(long long)c * ld + i] = acc;
+  const long long row = blockIdx.x * 256LL + threadIdx.x;
+  if (row >= nr) return;
+  double acc[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+  int l = 0;
+  for (; l + 8 <= m; l += 8) {
+    double q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = Q[(long long)(l + u) * ldq + row];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[c] += q[u] * Us[(l + u) * CB + c];
   }
+  for (; l < m; ++l) {
+    const double q = Q[(long long)l * ldq + row];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[c] += q * Us[l * CB + c];
+  }
+#pragma unroll
+  for (int c = 0; c < CB; ++c)  // compile-time indices keep acc[] in registers
+    if (c < kk) Q[(long long)c * ldq + row] = acc[c];
 }
 
 // out(:, c) = Q(:, 0:m) U(:, c) for c < kk, out-of-place: complex Q (ld complex rows) times
@@ -746,7 +761,17 @@ __global__ __launch_bounds__(256) void k_basis_gemm(const double* __restrict__ Q
     }
     __syncthreads();
     if (row < nr) {
-      for (int l = 0; l < lc; ++l) {
+      int l = 0;
+      for (; l + 8 <= lc; l += 8) {  // eight loads in flight, then their products in order
+        double q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = Q[(long long)(l0 + l + u) * ldq + row];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int c = 0; c < CB; ++c) acc[c] += q[u] * Us[(l + u) * CB + c];
+      }
+      for (; l < lc; ++l) {
         const double qv = Q[(long long)(l0 + l) * ldq + row];
 #pragma unroll
         for (int c = 0; c < CB; ++c) acc[c] += qv * Us[l * CB + c];

@@ -321,6 +321,33 @@ struct LzState {
   cpx<double>* vloc = nullptr;  // where the newest unit vector is: B.v or the next basis column
 };
 
+// Q(:, 0:k) = Q(:, 0:m) U in place (n complex rows, column stride n; U real m x k, column-major,
+// on the device; m, k <= 64)
+static void restart_in_place(cpx<double>* Q, long long n, int m, int k, const double* U, hipStream_t s) {
+  if (m > 64 || k > 64) throw Error(FAME6D_EARG, "in-place basis product of more than 64 columns");
+  const long long nr = 2 * n;  // Q as a real (2n x m) matrix
+  const int blocks = (int)((nr + 255) / 256);
+  double* Qd = reinterpret_cast<double*>(Q);
+  if (k <= 16) k_restart_rows<16><<<blocks, 256, 0, s>>>(Qd, nr, nr, m, k, U);
+  else if (k <= 32) k_restart_rows<32><<<blocks, 256, 0, s>>>(Qd, nr, nr, m, k, U);
+  else k_restart_rows<64><<<blocks, 256, 0, s>>>(Qd, nr, nr, m, k, U);
+  F6_CHECK_LAUNCH();
+}
+
+// out(:, c) = Q(:, 0:m) U(:, c) for c < kk, out of place (n complex rows, column stride n for Q
+// and out); the accumulator count follows kk
+static void basis_product(const cpx<double>* Q, long long n, int m, int kk, const double* U, cpx<double>* out,
+                          hipStream_t s) {
+  const long long nr = 2 * n;
+  const unsigned gx = (unsigned)((nr + 255) / 256);
+  const double* Qd = reinterpret_cast<const double*>(Q);
+  double* Od = reinterpret_cast<double*>(out);
+  if (kk <= 16) k_basis_gemm<16><<<dim3(gx, 1), 256, 0, s>>>(Qd, nr, nr, m, kk, U, Od, nr);
+  else if (kk <= 32) k_basis_gemm<32><<<dim3(gx, 1), 256, 0, s>>>(Qd, nr, nr, m, kk, U, Od, nr);
+  else k_basis_gemm<64><<<dim3(gx, (unsigned)((kk + 63) / 64)), 256, 0, s>>>(Qd, nr, nr, m, kk, U, Od, nr);
+  F6_CHECK_LAUNCH();
+}
+
 // Builds H, solves it, tests convergence and restarts (Lanczos_hybrid_identity_gpuop.m:79-120).
 // Returns true when the run is finished.
 static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, long long n2,
@@ -394,15 +421,12 @@ static bool restart_step(LzState& L, int k0, int p, double tol, int maxRestart, 
   ubuf.ensure(sizeof(double) * Uk.size());
   F6_HIP(hipMemcpyAsync(ubuf.ptr, Uk.data(), sizeof(double) * Uk.size(), hipMemcpyHostToDevice, s));
   if (m <= 64) {
-    const int blocks = (int)((n2 + kRestartRows - 1) / kRestartRows);
-    k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n2, n2, m, k, ubuf.as<double>());
-    if (L.Z) k_restart<<<blocks, 256, 0, s>>>(L.Z, n2, n2, m, k, ubuf.as<double>());
+    restart_in_place(L.B.Q, n2, m, k, ubuf.as<double>(), s);
+    if (L.Z) restart_in_place(L.Z, n2, m, k, ubuf.as<double>(), s);
   } else {
     // wide basis: out-of-place product into the second basis, then swap the two
     if (!L.B.Qalt || L.Z) throw Error(FAME6D_EARG, "Lanczos restart of a basis wider than 64 needs the second basis");
-    const dim3 grid((unsigned)((2 * n2 + 255) / 256), (unsigned)((k + 63) / 64));
-    k_basis_gemm<64><<<grid, 256, 0, s>>>(reinterpret_cast<const double*>(L.B.Q), 2 * n2, 2 * n2, m, k,
-                                          ubuf.as<double>(), reinterpret_cast<double*>(L.B.Qalt), 2 * n2);
+    basis_product(L.B.Q, n2, m, k, ubuf.as<double>(), L.B.Qalt, s);
     std::swap(L.B.Q, L.B.Qalt);
   }
   F6_CHECK_LAUNCH();
@@ -907,11 +931,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
       }
       ubuf.ensure(sizeof(double) * Uy.size());
       F6_HIP(hipMemcpyAsync(ubuf.ptr, Uy.data(), sizeof(double) * Uy.size(), hipMemcpyHostToDevice, s));
-      const dim3 grid((unsigned)((2 * n2 + 255) / 256), (unsigned)((ny + 63) / 64));
-      k_basis_gemm<64><<<grid, 256, 0, s>>>(reinterpret_cast<const double*>(Lg.B.Q), 2 * n2, 2 * n2, mq, ny,
-                                            ubuf.as<double>(), reinterpret_cast<double*>(Y + (size_t)n2 * ny * g),
-                                            2 * n2);
-      F6_CHECK_LAUNCH();
+      basis_product(Lg.B.Q, n2, mq, ny, ubuf.as<double>(), Y + (size_t)n2 * ny * g, s);
       F6_HIP(hipStreamSynchronize(s));
     }
     if (stats) {
@@ -1677,9 +1697,7 @@ void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, do
     }
     ubuf.ensure(sizeof(double) * Uy.size());
     F6_HIP(hipMemcpyAsync(ubuf.ptr, Uy.data(), sizeof(double) * Uy.size(), hipMemcpyHostToDevice, s));
-    const int blocks = (int)((n3 + kRestartRows - 1) / kRestartRows);
-    k_restart<<<blocks, 256, 0, s>>>(L.B.Q, n3, n3, mq, k0, ubuf.as<double>());
-    F6_CHECK_LAUNCH();
+    restart_in_place(L.B.Q, n3, mq, k0, ubuf.as<double>(), s);
     F6_HIP(hipMemcpyAsync(Y, L.B.Q, sizeof(cpx<double>) * n3 * k0, hipMemcpyDeviceToDevice, s));
   }
   F6_HIP(hipStreamSynchronize(s));
