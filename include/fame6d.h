@@ -217,11 +217,12 @@ typedef struct {
 
 /* run_method_C_fig1bc (exp1/run_method_C_fig1bc.m:4-34) driving
  * Lanczos_hybrid_identity_gpuop (exp1/Lanczos_hybrid_identity_gpuop.m:11-130) with the
- * basis kept on the device.  v0_host: 2NF complex128 start vector (normalised inside).
+ * basis kept on the device.  v0: 2NF complex128 start vector (normalised inside), in host or
+ * device memory (a device start vector is copied device to device).
  * lambda_host: k ascending eigenvalues 1/theta.  Y: optional device 2NF x k complex128
  * reduced Ritz vectors (ordered as lambda). */
 int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart,
-                         double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host,
+                         double tol_cg, int maxit_cg, const void* v0, double* lambda_host,
                          void* Y, fame6d_lanczos_stats_t* stats);
 
 
@@ -235,18 +236,18 @@ int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol
  * rel_res_host (k relative inverse-Ritz residuals, optional), sel_host (the selected indices,
  * capacity sel_capacity), *nsel, Y (device, 2NF x *nsel complex128, optional). */
 int fame6d_eigs_tracking(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart, double tol_cg,
-                         int maxit_cg, const void* v0_host, int nfixed, const int* fixed_host, int ntarget,
+                         int maxit_cg, const void* v0, int nfixed, const int* fixed_host, int ntarget,
                          const double* target_host, int half_width, double* lambda_host, double* rel_res_host,
                          int* sel_host, int sel_capacity, int* nsel, void* Y, fame6d_lanczos_stats_t* stats);
 
 /* The same for nq Bloch points sharing one mass backend, solved together: every
  * Lanczos step issues one batched K_r^{-1} over all still-active points, so each
- * kernel launch covers all of them (q_host: 3*nq; v0_host: nq x 2NF; lambda_host:
+ * kernel launch covers all of them (q_host: 3*nq; v0: nq x 2NF, host or device; lambda_host:
  * nq x k; Y: optional device nq x (2NF x k)).  No reference counterpart: the
  * reference loops over Bloch points sequentially (exp1/run_fig1a_BC_method.m:37-80). */
 int fame6d_eigs_method_c_batch(fame6d_mass* m, int nq, const double* q_host, const double* P_host,
                                int precision, int k, int p, double tol, int max_restart,
-                               double tol_cg, int maxit_cg, const void* v0_host,
+                               double tol_cg, int maxit_cg, const void* v0,
                                double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
 
 /* make_nested_krinv_instrumented applyOne (exp1/make_nested_krinv_instrumented.m:33-132), Method B:
@@ -258,7 +259,7 @@ int fame6d_krinv_nested_apply(fame6d_qsys* s, const void* b, void* x, double tol
 /* run_method_B_fig1bc: the Lanczos of fame6d_eigs_method_c driving the Method-B inverse
  * (exp1/make_nested_krinv_instrumented.m; cfg.nested in exp1/make_config_exp1bc.m:83-87). */
 int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_restart, double tol_kr,
-                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
+                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0,
                          double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats);
 
 /* Method A statistics (make_A_eigs_top_preconditioned.m:53-54 and the per-solve info of
@@ -285,9 +286,9 @@ int fame6d_method_a_apply(fame6d_qsys* s, double sigma, double tol, int maxit, c
  * T = (K - sigma M)^{-1} M, lambda = sigma + 1/theta ascending.  The outer solver is
  * thick-restart Lanczos in the M inner product (T is M-self-adjoint) where the reference
  * calls eigs; k, p, tol, max_restart as cfg.eigs (make_config_exp1bc.m:36-47).
- * v0_host: 3NF start vector; Y (device, 3NF x k) or NULL. */
+ * v0: 3NF start vector, host or device; Y (device, 3NF x k) or NULL. */
 int fame6d_eigs_method_a(fame6d_qsys* s, int k, int p, double tol, int max_restart, double sigma,
-                         double tol_minres, int maxit_minres, const void* v0_host, double* lambda_host, void* Y,
+                         double tol_minres, int maxit_minres, const void* v0, double* lambda_host, void* Y,
                          fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats);
 
 typedef struct {

@@ -807,22 +807,22 @@ int fame6d_qsys_set_reorth(fame6d_qsys* s, double eta) {
 }
 
 int fame6d_eigs_method_c(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart,
-                         double tol_cg, int maxit_cg, const void* v0_host, double* lambda_host, void* Y,
+                         double tol_cg, int maxit_cg, const void* v0, double* lambda_host, void* Y,
                          fame6d_lanczos_stats_t* stats) {
   F6_API_BEGIN
-  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s && v0 && lambda_host, FAME6D_EARG, "null argument");
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
-  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
+  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0,
                 lambda_host, (cpx<double>*)Y, stats);
   F6_API_END
 }
 
 int fame6d_eigs_tracking(fame6d_qsys* s, int precision, int k, int p, double tol, int max_restart, double tol_cg,
-                         int maxit_cg, const void* v0_host, int nfixed, const int* fixed_host, int ntarget,
+                         int maxit_cg, const void* v0, int nfixed, const int* fixed_host, int ntarget,
                          const double* target_host, int half_width, double* lambda_host, double* rel_res_host,
                          int* sel_host, int sel_capacity, int* nsel, void* Y, fame6d_lanczos_stats_t* stats) {
   F6_API_BEGIN
-  F6_REQUIRE(s && v0_host && lambda_host && sel_host && nsel, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s && v0 && lambda_host && sel_host && nsel, FAME6D_EARG, "null argument");
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
   F6_REQUIRE((nfixed > 0 && fixed_host) || (ntarget > 0 && target_host), FAME6D_EARG,
              "Either fixedIndices or targetLambda must be supplied.");
@@ -840,7 +840,7 @@ int fame6d_eigs_tracking(fame6d_qsys* s, int precision, int k, int p, double tol
   sel.idx_capacity = sel_capacity;
   sel.nidx_out = nsel;
   sel.rel_res = rel_res_host;
-  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host, lambda_host,
+  eigs_method_c(s, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0, lambda_host,
                 (cpx<double>*)Y, stats, &sel);
   F6_API_END
 }
@@ -864,15 +864,15 @@ int fame6d_krinv_nested_apply(fame6d_qsys* s, const void* b, void* x, double tol
 }
 
 int fame6d_eigs_method_b(fame6d_qsys* s, int k, int p, double tol, int max_restart, double tol_kr,
-                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0_host,
+                         double tol_m, int maxit_kr, int maxit_m, double eps_c, const void* v0,
                          double* lambda_host, void* Y, fame6d_lanczos_stats_t* stats) {
   F6_API_BEGIN
-  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s && v0 && lambda_host, FAME6D_EARG, "null argument");
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
   F6_REQUIRE(tol_m > 0 && eps_c > 0, FAME6D_EARG, "tol_m and eps_c must be positive");
   set_nested(true, tol_m, maxit_m, eps_c);
   try {
-    eigs_method_c(s, FAME6D_FP64, k, p, tol, max_restart, tol_kr, maxit_kr, (const cpx<double>*)v0_host,
+    eigs_method_c(s, FAME6D_FP64, k, p, tol, max_restart, tol_kr, maxit_kr, (const cpx<double>*)v0,
                   lambda_host, (cpx<double>*)Y, stats);
   } catch (...) {
     set_nested(false, 0, 0, 0);
@@ -893,27 +893,27 @@ int fame6d_method_a_apply(fame6d_qsys* s, double sigma, double tol, int maxit, c
 }
 
 int fame6d_eigs_method_a(fame6d_qsys* s, int k, int p, double tol, int max_restart, double sigma,
-                         double tol_minres, int maxit_minres, const void* v0_host, double* lambda_host, void* Y,
+                         double tol_minres, int maxit_minres, const void* v0, double* lambda_host, void* Y,
                          fame6d_lanczos_stats_t* stats, fame6d_minres_stats_t* mstats) {
   F6_API_BEGIN
-  F6_REQUIRE(s && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(s && v0 && lambda_host, FAME6D_EARG, "null argument");
   F6_REQUIRE(s->G == 1, FAME6D_EARG, "single-point handle expected");
   F6_REQUIRE(sigma > 0 && tol_minres > 0 && maxit_minres > 0, FAME6D_EARG, "bad Method-A parameters");
-  eigs_method_a(s, k, p, tol, max_restart, sigma, tol_minres, maxit_minres, (const cpx<double>*)v0_host,
+  eigs_method_a(s, k, p, tol, max_restart, sigma, tol_minres, maxit_minres, (const cpx<double>*)v0,
                 lambda_host, (cpx<double>*)Y, stats, mstats);
   F6_API_END
 }
 
 int fame6d_eigs_method_c_batch(fame6d_mass* m, int nq, const double* q_host, const double* P_host,
                                int precision, int k, int p, double tol, int max_restart, double tol_cg,
-                               int maxit_cg, const void* v0_host, double* lambda_host, void* Y,
+                               int maxit_cg, const void* v0, double* lambda_host, void* Y,
                                fame6d_lanczos_stats_t* stats) {
   F6_API_BEGIN
-  F6_REQUIRE(m && q_host && P_host && v0_host && lambda_host, FAME6D_EARG, "null argument");
+  F6_REQUIRE(m && q_host && P_host && v0 && lambda_host, FAME6D_EARG, "null argument");
   F6_REQUIRE(nq >= 1 && nq <= 64, FAME6D_EARG, "nq must be in 1..64");
   fame6d_qsys* S = qsys_build(m, nq, q_host, P_host);
   try {
-    eigs_method_c(S, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0_host,
+    eigs_method_c(S, precision, k, p, tol, max_restart, tol_cg, maxit_cg, (const cpx<double>*)v0,
                   lambda_host, (cpx<double>*)Y, stats);
   } catch (...) {
     fame6d_qsys_destroy(S);

@@ -635,7 +635,7 @@ void eigs_method_c(fame6d_qsys* S, int precision, int k0, int p, double tol, int
 
   // v = v0 / ||v0||  (Lanczos_hybrid_identity_gpuop.m:14-16)
   for (int g = 0; g < G; ++g) {
-    F6_HIP(hipMemcpyAsync(L[g].B.v, v0 + (size_t)n2 * g, sizeof(cpx<double>) * n2, hipMemcpyHostToDevice, s));
+    F6_HIP(hipMemcpyAsync(L[g].B.v, v0 + (size_t)n2 * g, sizeof(cpx<double>) * n2, hipMemcpyDefault, s));
     dot_dev(n2, L[g].B.v, L[g].B.v, L[g].B.part, L[g].B.sc + 1, nullptr, s);
   }
   {
@@ -1621,7 +1621,7 @@ void eigs_method_a(fame6d_qsys* S, int k0, int p, double tol, int maxRestart, do
 
   // v = v0 / ||v0||_M, zv = M v  (run_method_A_fig1a.m:12-14 normalises in the 2-norm; the
   // M-norm normalisation here is the same direction)
-  F6_HIP(hipMemcpyAsync(L.B.v, v0, sizeof(cpx<double>) * n3, hipMemcpyHostToDevice, s));
+  F6_HIP(hipMemcpyAsync(L.B.v, v0, sizeof(cpx<double>) * n3, hipMemcpyDefault, s));
   mass3(m, L.B.v, zv, no_skip(), s);
   k_dot_partials<double><<<dim3(nb3, 1), kRedThreads, 0, s>>>(L.B.v, n3, zv, n3, n3, L.B.part, nullptr);
   const double b0 = reduce_read(m, L.B.part, nb3, L.B.sc + 1);

@@ -18,6 +18,20 @@ from ._lib import FP32, FP64, MIXED, call, ptr
 from .mass import P_DEFAULT, MassBackend, precision_code, start_vector
 
 
+def _start_vectors(v0, rows, n):
+    """(owner, pointer) of rows x n complex128 start vectors: a CUDA tensor stays on the
+    device (the library copies it device to device), anything else becomes a host array."""
+    if isinstance(v0, torch.Tensor) and v0.is_cuda:
+        if v0.numel() != rows * n:
+            raise _lib.Fame6dError(1, "start vector must have %d entries" % (rows * n))
+        v = v0.to(torch.complex128).reshape(rows, n).contiguous()
+        return v, ptr(v)
+    v = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    if v.size != rows * n:
+        raise _lib.Fame6dError(1, "start vector must have %d entries" % (rows * n))
+    return v, v.ctypes.data_as(C.c_void_p)
+
+
 class QSystem:
     """Null-space-free system of one Bloch point on a mass backend."""
 
@@ -149,14 +163,12 @@ def run_method_C(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_cg=1e-
     Returns dict(lambda (ascending), Y (reduced Ritz vectors, (k, 2NF) tensor or None), stats)."""
     if v0 is None:
         v0 = start_vector(sys.nr, seed)
-    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
-    if v0.size != sys.nr:
-        raise _lib.Fame6dError(1, "start vector must have 2NF entries")
+    v0, v0p = _start_vectors(v0, 1, sys.nr)
     lam = np.empty(k)
     Y = torch.empty((k, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
     st = _lib.LanczosStats()
     call("fame6d_eigs_method_c", sys.handle, precision_code(precision), int(k), int(p), float(tol),
-         int(max_restart), float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p),
+         int(max_restart), float(tol_cg), int(maxit_cg), v0p,
          lam.ctypes.data_as(C.c_void_p), ptr(Y), C.byref(st))
     return dict(lambda_=lam, Y=Y, stats=st.as_dict())
 
@@ -173,9 +185,7 @@ def solve_tracking_scan(sys: QSystem, k=140, p=None, tol=1e-10, max_restart=30, 
     p = k + 48 if p is None else p
     if v0 is None:
         v0 = start_vector(sys.nr, (20260831 + 1000 * sys.mass.N + k) if seed is None else seed)
-    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
-    if v0.size != sys.nr:
-        raise _lib.Fame6dError(1, "start vector must have 2NF entries")
+    v0, v0p = _start_vectors(v0, 1, sys.nr)
     fixed = None if fixed_indices is None else np.ascontiguousarray(np.asarray(fixed_indices, dtype=np.int32))
     tgt = None if target_lambda is None else np.ascontiguousarray(np.asarray(target_lambda, dtype=np.float64))
     cap = len(fixed) if fixed is not None else len(tgt) * (2 * half_width + 1)
@@ -186,7 +196,7 @@ def solve_tracking_scan(sys: QSystem, k=140, p=None, tol=1e-10, max_restart=30, 
     Y = torch.empty((cap, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
     st = _lib.LanczosStats()
     call("fame6d_eigs_tracking", sys.handle, precision_code(precision), int(k), int(p), float(tol), int(max_restart),
-         float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p), 0 if fixed is None else len(fixed),
+         float(tol_cg), int(maxit_cg), v0p, 0 if fixed is None else len(fixed),
          None if fixed is None else fixed.ctypes.data_as(C.c_void_p), 0 if tgt is None else len(tgt),
          None if tgt is None else tgt.ctypes.data_as(C.c_void_p), int(half_width), lam.ctypes.data_as(C.c_void_p),
          rel.ctypes.data_as(C.c_void_p), sel.ctypes.data_as(C.c_void_p), cap, C.byref(nsel), ptr(Y), C.byref(st))
@@ -241,12 +251,12 @@ def run_method_B(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=50, tol_kr=1e-
     eps_c = sys.mass.info["epsMean"] if eps_c is None else eps_c
     if v0 is None:
         v0 = start_vector(sys.nr, seed)
-    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
+    v0, v0p = _start_vectors(v0, 1, sys.nr)
     lam = np.empty(k)
     Y = torch.empty((k, sys.nr), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
     st = _lib.LanczosStats()
     call("fame6d_eigs_method_b", sys.handle, int(k), int(p), float(tol), int(max_restart), float(tol_kr),
-         float(tol_m), int(maxit_kr), int(maxit_m), float(eps_c), v0.ctypes.data_as(C.c_void_p),
+         float(tol_m), int(maxit_kr), int(maxit_m), float(eps_c), v0p,
          lam.ctypes.data_as(C.c_void_p), ptr(Y), C.byref(st))
     return dict(lambda_=lam, Y=Y, stats=st.as_dict())
 
@@ -268,15 +278,13 @@ def run_method_A(sys: QSystem, k=10, p=40, tol=1e-10, max_restart=300, sigma=1e-
     Returns dict(lambda (ascending), Y ((k, 3NF) Ritz vectors or None), stats, minres)."""
     if v0 is None:
         v0 = start_vector(sys.ndof, seed)
-    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(-1))
-    if v0.size != sys.ndof:
-        raise _lib.Fame6dError(1, "start vector must have 3NF entries")
+    v0, v0p = _start_vectors(v0, 1, sys.ndof)
     lam = np.empty(k)
     Y = torch.empty((k, sys.ndof), dtype=torch.complex128, device=sys.mass.device) if want_vectors else None
     st = _lib.LanczosStats()
     ms = _lib.MinresStats()
     call("fame6d_eigs_method_a", sys.handle, int(k), int(p), float(tol), int(max_restart), float(sigma),
-         float(tol_minres), int(maxit_minres), v0.ctypes.data_as(C.c_void_p), lam.ctypes.data_as(C.c_void_p),
+         float(tol_minres), int(maxit_minres), v0p, lam.ctypes.data_as(C.c_void_p),
          ptr(Y), C.byref(st), C.byref(ms))
     return dict(lambda_=lam, Y=Y, stats=st.as_dict(), minres=ms.as_dict())
 
@@ -290,14 +298,14 @@ def run_method_C_batch(mass: MassBackend, qs, k=10, p=40, tol=1e-10, max_restart
     nr = 2 * mass.NF
     if v0 is None:
         v0 = np.stack([start_vector(nr, seed + i) for i in range(nq)])
-    v0 = np.ascontiguousarray(np.asarray(v0, dtype=np.complex128).reshape(nq, nr))
+    v0, v0p = _start_vectors(v0, nq, nr)
     Pc = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(6, 3))
     lam = np.empty((nq, k))
     Y = torch.empty((nq, k, nr), dtype=torch.complex128, device=mass.device) if want_vectors else None
     st = (_lib.LanczosStats * nq)()
     call("fame6d_eigs_method_c_batch", mass.handle, nq, qs.ctypes.data_as(C.c_void_p),
          Pc.ctypes.data_as(C.c_void_p), precision_code(precision), int(k), int(p), float(tol),
-         int(max_restart), float(tol_cg), int(maxit_cg), v0.ctypes.data_as(C.c_void_p),
+         int(max_restart), float(tol_cg), int(maxit_cg), v0p,
          lam.ctypes.data_as(C.c_void_p), ptr(Y), st)
     return dict(lambda_=lam, Y=Y, stats=[s.as_dict() for s in st])
 
