@@ -60,6 +60,24 @@ def test_qsystem_device_build(pkg, oracle, N, q):
         m.close()
 
 
+def test_start_vector_on_device(pkg, n2):
+    """The eigensolvers take the start vector from host or device memory: the same vector from
+    a CUDA tensor gives the same eigenvalues, bit for bit (single run and batched path)."""
+    print("case: device start vector N=2", flush=True)
+    v0 = pkg.start_vector(n2["s"].nr, 9)
+    kw = dict(k=6, p=30, tol=1e-10, max_restart=40, tol_cg=1e-10, maxit_cg=3000)
+    lh = pkg.run_method_C(n2["s"], v0=v0, **kw)["lambda_"]
+    ld = pkg.run_method_C(n2["s"], v0=torch.as_tensor(v0, device="cuda"), **kw)["lambda_"]
+    assert np.array_equal(lh, ld)
+    qs = np.array([Q_EXP4, Q_EXP1])
+    v2 = np.stack([v0, pkg.start_vector(n2["s"].nr, 10)])
+    bh = pkg.run_method_C_batch(n2["m"], qs, v0=v2, **kw)["lambda_"]
+    bd = pkg.run_method_C_batch(n2["m"], qs, v0=torch.as_tensor(v2, device="cuda"), **kw)["lambda_"]
+    assert np.array_equal(bh, bd)
+    # the batched launches may use another tile choice than the single run: equal to 1e-10
+    assert np.max(np.abs(bh[0] - lh) / lh) < 1e-10
+
+
 @pytest.mark.parametrize("v3", [0, 1])
 def test_mhat(n2, oracle, v3):
     print("case: Mhat N=2 variant %d" % v3, flush=True)
