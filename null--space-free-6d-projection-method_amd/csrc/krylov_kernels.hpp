@@ -582,6 +582,14 @@ struct ThreeTerm {
   double beta;          // 0 on the first step
   double* norm;         // nrb partials of ||r||^2
 };
+// A basis element read once per sweep: a non-temporal load (the sweeps stream a basis larger
+// than the last-level cache, so keeping its lines there only displaces the solve's vectors).
+__device__ __forceinline__ cpx<double> load_stream(const cpx<double>* p) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+  return mk(v.x, v.y);
+}
+
 // The work of block (rb, ch) (row block, column chunk) of k_gemv_c_all.
 template <int KB, bool THREE>
 __device__ __forceinline__ void gemv_c_block(const cpx<double>* Q, long long ld, int jj, const cpx<double>* r,
@@ -632,7 +640,7 @@ __device__ __forceinline__ void gemv_c_block(const cpx<double>* Q, long long ld,
         bv[u] = qb[i + u * stride];
       }
 #pragma unroll
-      for (int k = 0; k < KB; ++k) q[u][k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i + u * stride];
+      for (int k = 0; k < KB; ++k) q[u][k] = load_stream(Q + (long long)(k0 + min(k, kcount - 1)) * ld + i + u * stride);
     }
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -646,7 +654,7 @@ __device__ __forceinline__ void gemv_c_block(const cpx<double>* Q, long long ld,
     cpx<double> q[KB];
     const cpx<double> v = row(i);
 #pragma unroll
-    for (int k = 0; k < KB; ++k) q[k] = Q[(long long)(k0 + min(k, kcount - 1)) * ld + i];
+    for (int k = 0; k < KB; ++k) q[k] = load_stream(Q + (long long)(k0 + min(k, kcount - 1)) * ld + i);
     accumulate(v, q);
     if constexpr (THREE) nrm += v.re * v.re + v.im * v.im;
   }
@@ -805,7 +813,7 @@ __device__ __forceinline__ void gemv_n_block(const cpx<double>* Q, long long ld,
   double s = 0.0;
   for (long long i = blk * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)nblk * blockDim.x) {
     cpx<double> acc = in[i];
-    for (int k = 0; k < j; ++k) acc = acc - Q[(long long)k * ld + i] * hs[k];
+    for (int k = 0; k < j; ++k) acc = acc - load_stream(Q + (long long)k * ld + i) * hs[k];
     out[i] = acc;
     s += acc.re * acc.re + acc.im * acc.im;
   }
