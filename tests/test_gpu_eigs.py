@@ -60,6 +60,29 @@ def test_qsystem_device_build(pkg, oracle, N, q):
         m.close()
 
 
+def test_batched_two_points_n5(pkg, n5):
+    """Two Bloch points solved together at N = 5 (the several-group tile choice, tuned here,
+    and the batched Lanczos steps) against each point's own solve, FP64 to 1e-10."""
+    print("case: batched two points N=5", flush=True)
+    m = n5["m"]
+    m.autotune(reps=2)
+    kw = dict(k=6, p=20, tol=1e-10, max_restart=60, tol_cg=1e-10, maxit_cg=1200)
+    qs = np.array([Q_EXP1, Q_EXP4])
+    v2 = np.stack([pkg.start_vector(2 * m.NF, 31), pkg.start_vector(2 * m.NF, 32)])
+    try:
+        out = pkg.run_method_C_batch(m, qs, v0=torch.as_tensor(v2, device="cuda"), **kw)
+        for i, q in enumerate(qs):
+            s_i = pkg.QSystem(m, q)
+            try:
+                lam = pkg.run_method_C(s_i, v0=v2[i], **kw)["lambda_"]
+            finally:
+                s_i.close()
+            assert np.max(np.abs(out["lambda_"][i] - lam) / lam) < 1e-10, i
+    finally:
+        m.set_tiles("double", 0, 0)
+        m.set_tiles("single", 0, 0)
+
+
 def test_start_vector_on_device(pkg, n2):
     """The eigensolvers take the start vector from host or device memory: the same vector from
     a CUDA tensor gives the same eigenvalues, bit for bit (single run and batched path)."""
